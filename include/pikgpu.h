@@ -1,0 +1,233 @@
+/*
+ * pikgpu.h -- C ABI of the MI355X (gfx950) PIK decoder-side reconstruction.
+ *
+ * The boundary sits where the reference decoder has finished entropy
+ * decoding: the caller hands over the frame-wide quantized coefficients and
+ * side information (exactly what the reference's host decoder holds in
+ * FrameDecCache / GroupDecCache after DecodeDCGroups and DecodeFromBitstream)
+ * and receives the linear-RGB Image3F that the reference's
+ * FinalizeFrameDecoding leaves in `*idct`.
+ *
+ * Reference interfaces this replaces (google/pik):
+ *   DequantImageAC          pik/compressed_image.h:83-87, .cc:827-867
+ *   ReconOpsinImage         pik/compressed_image.h:90-97, .cc:936-1106
+ *   FinalizeFrameDecoding   pik/compressed_image.h:102-110, .cc:1144-1181
+ *   DequantDC + AdaptiveDCReconstruction
+ *                           pik/compressed_dc.cc:181-230,
+ *                           pik/adaptive_reconstruction.cc:300-332
+ * and the per-group driver loop around them in PikPassToPixels
+ * (pik/pik_frame.cc:1106-1149) and RoundtripImage
+ * (pik/adaptive_quantization.cc:580-671).
+ *
+ * Conventions
+ *  - Every function returns PIKGPU_OK (0) or a negative PIKGPU_E* code and
+ *    never aborts on bad input; pikgpu_last_error() describes the failure.
+ *  - A context owns one HIP stream and the device buffers of one frame; it is
+ *    not internally locked (one context per host thread).
+ *  - All image planes are row-major. Strides are in ELEMENTS unless named
+ *    *_bytes.
+ *  - Blocks are 8x8 pixels; xsize_blocks = ceil(xsize/8) etc. Colour tiles
+ *    (CfL map, dequant control field) are 64x64 pixels.
+ */
+#ifndef PIKGPU_H_
+#define PIKGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIKGPU_ABI_VERSION 1
+
+#if defined(__GNUC__) || defined(__clang__)
+#define PIKGPU_API __attribute__((visibility("default")))
+#else
+#define PIKGPU_API
+#endif
+
+enum {
+  PIKGPU_OK = 0,
+  PIKGPU_EINVAL = -1,    /* bad argument / inconsistent frame */
+  PIKGPU_EHIP = -2,      /* HIP runtime error */
+  PIKGPU_ENOMEM = -3,    /* device allocation failed */
+  PIKGPU_ESTATE = -4,    /* call out of order (e.g. run before upload) */
+  PIKGPU_EUNSUPPORTED = -5
+};
+
+/* AC strategy raw types; the per-block strategy byte is (type << 4) | index
+ * of the 8x8 block inside a multi-block transform (pik/ac_strategy.h:55-75,
+ * 303-313). */
+enum {
+  PIKGPU_ACS_DCT = 0,
+  PIKGPU_ACS_IDENTITY = 1,
+  PIKGPU_ACS_DCT2X2 = 2,
+  PIKGPU_ACS_DCT4X4 = 3,
+  PIKGPU_ACS_DCT16X16 = 4,
+  PIKGPU_ACS_DCT32X32 = 5,
+  PIKGPU_ACS_LINES = 6,
+  PIKGPU_ACS_DCT_NOHF = 7,
+  PIKGPU_ACS_DCT4X4_NOHF = 8
+};
+
+/* Gaborish strength, same numbering as GaborishStrength
+ * (pik/pik_params.h:39-46). */
+enum {
+  PIKGPU_GABORISH_OFF = 0,
+  PIKGPU_GABORISH_500 = 1,
+  PIKGPU_GABORISH_750 = 2,
+  PIKGPU_GABORISH_875 = 3,
+  PIKGPU_GABORISH_1000 = 4
+};
+
+/* One dot-dictionary placement (BlockPosition + QuantizedBlock,
+ * pik/block_dictionary.h:26-70). `pixels` holds the int8 patch for the three
+ * channels, xsize*ysize used entries each, row-major. */
+typedef struct pikgpu_dot {
+  uint32_t x, y;          /* top-left corner in pixels */
+  uint32_t xsize, ysize;  /* patch size, 1..8 */
+  int32_t transform;      /* 0: copied as-is; 1: scaled/rotated splat */
+  int32_t dx, dy, width;  /* only if transform: offset of top-right corner,
+                             width in half-pixels */
+  int8_t pixels[3][64];
+} pikgpu_dot;
+
+/* Frame-wide decoder state after entropy decoding. Host pointers. */
+typedef struct pikgpu_frame {
+  uint32_t xsize, ysize; /* image size in pixels (before 8-padding) */
+
+  /* FrameHeader flags (pik/headers.h:182-294). */
+  int32_t gaborish;            /* PIKGPU_GABORISH_* */
+  int32_t predict_lf;          /* bool */
+  int32_t predict_hf;          /* bool */
+  int32_t adaptive_reconstruction; /* bool: EPF + AR clamp */
+  int32_t epf_enable_adaptive; /* bool; only adaptive sigma is supported */
+  int32_t epf_use_sharpened;   /* bool */
+
+  /* Quantizer (pik/quantizer.h:111-389): serialized scalars. */
+  int32_t global_scale; /* 1..32768 */
+  int32_t quant_dc;     /* 1..256 */
+
+  /* Quantized AC: 3 planes of ysize_blocks rows x (xsize_blocks*64) int16,
+   * coefficients in the reference's frame-wide Image3S layout
+   * (pik/compressed_image.cc:827-837). Coefficient 0 of each block is
+   * ignored (DC arrives separately). */
+  const int16_t* ac[3];
+  size_t ac_stride;
+
+  /* Quantized DC: 3 planes of ysize_blocks x xsize_blocks int16 (after DC
+   * prediction has been undone by the host, i.e. GroupDecCache::quantized_dc
+   * of pik/compressed_dc.cc:276-345). */
+  const int16_t* dc[3];
+  size_t dc_stride;
+
+  /* Per-block side information, ysize_blocks x xsize_blocks each. */
+  const int32_t* quant_field; size_t quant_field_stride; /* 1..256 */
+  const uint8_t* ac_strategy; size_t ac_strategy_stride; /* (type<<4)|idx */
+  const uint8_t* sigma_lut_ids; size_t sigma_lut_stride; /* 0 or 1 */
+
+  /* Per 64x64 colour tile: ceil(ysize/64) x ceil(xsize/64). */
+  const uint8_t* dequant_cf; size_t dequant_cf_stride;
+  const int32_t* ytox_map; size_t ytox_stride; /* CfL factors, 0..255 */
+  const int32_t* ytob_map; size_t ytob_stride;
+  int32_t ytox_dc, ytob_dc;
+
+  /* dequant_map[cf][qf-1] -> dequant table index (16 x 256 bytes). */
+  const uint8_t* dequant_map;
+
+  /* DequantMatrices (pik/quant_weights.h:202-266): the float table and the
+   * offsets table indexed [(table * 7 + kind) * 3 + c]. */
+  const float* dequant_table; size_t dequant_table_size;
+  const uint32_t* dequant_offsets; size_t num_dequant_offsets;
+
+  /* Dot dictionary placements, applied in order. */
+  const pikgpu_dot* dots; size_t num_dots;
+} pikgpu_frame;
+
+typedef struct pikgpu_config {
+  uint32_t abi_version; /* must be PIKGPU_ABI_VERSION */
+  uint32_t flags;       /* reserved, 0 */
+} pikgpu_config;
+
+typedef struct pikgpu_ctx pikgpu_ctx;
+
+/* Intermediate images that pikgpu_stage_dump can return after pikgpu_run.
+ * Sizes: DC stages are xsize_blocks x ysize_blocks; COEFFS is the
+ * (xsize_blocks*64) x ysize_blocks coefficient image after dequantisation
+ * and predictions; pixel stages up to DOTS are 8-padded
+ * (xsize_blocks*8 x ysize_blocks*8); LINEAR is xsize x ysize. */
+enum {
+  PIKGPU_STAGE_DC_DEQUANT = 0, /* DequantDC output */
+  PIKGPU_STAGE_DC = 1,         /* after AdaptiveDCReconstruction */
+  PIKGPU_STAGE_COEFFS = 2,     /* dequantised AC + LLF/LF/HF predictions */
+  PIKGPU_STAGE_IDCT = 3,       /* ReconOpsinImage output */
+  PIKGPU_STAGE_GUIDE = 4,      /* ConvolveGaborish(idct), the EPF guide */
+  PIKGPU_STAGE_EPF = 5,        /* AdaptiveFilter output before AR clamp */
+  PIKGPU_STAGE_AR = 6,         /* AdaptiveReconstruction output */
+  PIKGPU_STAGE_DOTS = 7,       /* after BlockDictionary::AddTo */
+  PIKGPU_STAGE_GABORISH = 8,   /* final ConvolveGaborish */
+  PIKGPU_STAGE_LINEAR = 9,     /* OpsinToLinear output = final Image3F */
+  PIKGPU_NUM_STAGES = 10
+};
+
+/* Replaces the ownership of FrameDecCache/GroupDecCache: creates a context on
+ * HIP device `device`. Returns NULL on failure. */
+PIKGPU_API pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config);
+PIKGPU_API void pikgpu_destroy(pikgpu_ctx* ctx);
+
+/* Message of the last failure on this context (never NULL). */
+PIKGPU_API const char* pikgpu_last_error(const pikgpu_ctx* ctx);
+
+/* Validates the frame and copies it to device memory (host -> HBM), growing
+ * the context's buffers if needed. Replaces the caller-side filling of
+ * FrameDecCache + frame-wide Image3S that precedes DequantImageAC. */
+PIKGPU_API int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* frame);
+
+/* Runs the whole reconstruction on the uploaded frame, device-resident,
+ * asynchronously on the context stream: DequantDC, DC EPF, dequant + CfL,
+ * predictions, variable IDCT, Gaborish guide, EPF, AR clamp, dots, Gaborish,
+ * OpsinToLinear. Equivalent to the decode-side sequence DecodeDCGroups ->
+ * per group {InitializeDecCache, DequantImageAC, ReconOpsinImage} ->
+ * FinalizeFrameDecoding. `keep_stages` != 0 retains intermediates for
+ * pikgpu_stage_dump (costs extra HBM traffic). */
+PIKGPU_API int pikgpu_run(pikgpu_ctx* ctx, int keep_stages);
+
+/* Waits for the context stream. */
+PIKGPU_API int pikgpu_sync(pikgpu_ctx* ctx);
+
+/* Copies the final linear Image3F (xsize x ysize, 3 planes) to host memory
+ * with the given row stride in bytes. */
+PIKGPU_API int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
+                    size_t out_stride_bytes);
+
+/* Convenience: upload + run + download (the drop-in for the decode-side
+ * sequence listed above; the output matches what FinalizeFrameDecoding
+ * leaves in *idct). */
+PIKGPU_API int pikgpu_reconstruct(pikgpu_ctx* ctx, const pikgpu_frame* frame,
+                       float* out_planes[3], size_t out_stride_bytes);
+
+/* Size (width, height in elements) of a stage image for the uploaded frame. */
+PIKGPU_API int pikgpu_stage_size(pikgpu_ctx* ctx, int stage, uint32_t* xsize,
+                      uint32_t* ysize);
+
+/* Copies stage `stage` (PIKGPU_STAGE_*) of the last pikgpu_run(ctx, 1) to
+ * host planes of the size given by pikgpu_stage_size. */
+PIKGPU_API int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
+                      size_t stride_bytes);
+
+/* Device-side timing of the last pikgpu_run, per kernel family, in
+ * milliseconds, measured with HIP events on the context stream.
+ * names[i] points to static strings. Returns the number of entries written
+ * (<= max_entries). */
+PIKGPU_API int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* ms,
+                        int max_entries);
+
+/* The HIP stream the context launches on (hipStream_t as void*). */
+PIKGPU_API void* pikgpu_stream(pikgpu_ctx* ctx);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* PIKGPU_H_ */

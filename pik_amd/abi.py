@@ -1,0 +1,139 @@
+"""ctypes mirror of include/pikgpu.h and loader of the HIP library.
+
+The product path is the C ABI in libpikgpu.so (built from pik_amd/csrc);
+this module only describes its structs for Python callers. Loading fails
+loudly when the library has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpikgpu.so")
+
+ABI_VERSION = 1
+
+OK = 0
+EINVAL = -1
+EHIP = -2
+ENOMEM = -3
+ESTATE = -4
+EUNSUPPORTED = -5
+
+# AC strategy raw types (pik/ac_strategy.h:55-75).
+ACS_DCT = 0
+ACS_IDENTITY = 1
+ACS_DCT2X2 = 2
+ACS_DCT4X4 = 3
+ACS_DCT16X16 = 4
+ACS_DCT32X32 = 5
+ACS_LINES = 6
+ACS_DCT_NOHF = 7
+ACS_DCT4X4_NOHF = 8
+
+GABORISH_OFF, GABORISH_500, GABORISH_750, GABORISH_875, GABORISH_1000 = range(5)
+
+STAGE_DC_DEQUANT = 0
+STAGE_DC = 1
+STAGE_COEFFS = 2
+STAGE_IDCT = 3
+STAGE_GUIDE = 4
+STAGE_EPF = 5
+STAGE_AR = 6
+STAGE_DOTS = 7
+STAGE_GABORISH = 8
+STAGE_LINEAR = 9
+NUM_STAGES = 10
+STAGE_NAMES = ["dc_dequant", "dc", "coeffs", "idct", "guide", "epf", "ar",
+               "dots", "gaborish", "linear"]
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class Dot(ctypes.Structure):
+    _fields_ = [
+        ("x", ctypes.c_uint32), ("y", ctypes.c_uint32),
+        ("xsize", ctypes.c_uint32), ("ysize", ctypes.c_uint32),
+        ("transform", ctypes.c_int32),
+        ("dx", ctypes.c_int32), ("dy", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("pixels", (ctypes.c_int8 * 64) * 3),
+    ]
+
+
+class FrameC(ctypes.Structure):
+    _fields_ = [
+        ("xsize", ctypes.c_uint32), ("ysize", ctypes.c_uint32),
+        ("gaborish", ctypes.c_int32),
+        ("predict_lf", ctypes.c_int32),
+        ("predict_hf", ctypes.c_int32),
+        ("adaptive_reconstruction", ctypes.c_int32),
+        ("epf_enable_adaptive", ctypes.c_int32),
+        ("epf_use_sharpened", ctypes.c_int32),
+        ("global_scale", ctypes.c_int32),
+        ("quant_dc", ctypes.c_int32),
+        ("ac", ctypes.c_void_p * 3), ("ac_stride", ctypes.c_size_t),
+        ("dc", ctypes.c_void_p * 3), ("dc_stride", ctypes.c_size_t),
+        ("quant_field", ctypes.c_void_p), ("quant_field_stride", ctypes.c_size_t),
+        ("ac_strategy", ctypes.c_void_p), ("ac_strategy_stride", ctypes.c_size_t),
+        ("sigma_lut_ids", ctypes.c_void_p), ("sigma_lut_stride", ctypes.c_size_t),
+        ("dequant_cf", ctypes.c_void_p), ("dequant_cf_stride", ctypes.c_size_t),
+        ("ytox_map", ctypes.c_void_p), ("ytox_stride", ctypes.c_size_t),
+        ("ytob_map", ctypes.c_void_p), ("ytob_stride", ctypes.c_size_t),
+        ("ytox_dc", ctypes.c_int32), ("ytob_dc", ctypes.c_int32),
+        ("dequant_map", ctypes.c_void_p),
+        ("dequant_table", ctypes.c_void_p), ("dequant_table_size", ctypes.c_size_t),
+        ("dequant_offsets", ctypes.c_void_p),
+        ("num_dequant_offsets", ctypes.c_size_t),
+        ("dots", ctypes.c_void_p), ("num_dots", ctypes.c_size_t),
+    ]
+
+
+class ConfigC(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def load():
+    """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libpikgpu.so is not built (%s); run __graft_entry__.build()"
+            % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.pikgpu_create.restype = vp
+    lib.pikgpu_create.argtypes = [ctypes.c_int, ctypes.POINTER(ConfigC)]
+    lib.pikgpu_destroy.argtypes = [vp]
+    lib.pikgpu_destroy.restype = None
+    lib.pikgpu_last_error.argtypes = [vp]
+    lib.pikgpu_last_error.restype = ctypes.c_char_p
+    lib.pikgpu_frame_upload.argtypes = [vp, ctypes.POINTER(FrameC)]
+    lib.pikgpu_run.argtypes = [vp, ctypes.c_int]
+    lib.pikgpu_sync.argtypes = [vp]
+    lib.pikgpu_download.argtypes = [vp, c_float_p * 3, ctypes.c_size_t]
+    lib.pikgpu_reconstruct.argtypes = [vp, ctypes.POINTER(FrameC),
+                                       c_float_p * 3, ctypes.c_size_t]
+    lib.pikgpu_stage_size.argtypes = [vp, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32)]
+    lib.pikgpu_stage_dump.argtypes = [vp, ctypes.c_int, c_float_p * 3,
+                                      ctypes.c_size_t]
+    lib.pikgpu_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p),
+                                        c_float_p, ctypes.c_int]
+    lib.pikgpu_stream.argtypes = [vp]
+    lib.pikgpu_stream.restype = vp
+    _lib = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = [
+    "pikgpu_create", "pikgpu_destroy", "pikgpu_last_error",
+    "pikgpu_frame_upload", "pikgpu_run", "pikgpu_sync", "pikgpu_download",
+    "pikgpu_reconstruct", "pikgpu_stage_size", "pikgpu_stage_dump",
+    "pikgpu_last_timings", "pikgpu_stream",
+]

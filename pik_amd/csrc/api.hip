@@ -1,0 +1,728 @@
+// C ABI of the gfx950 reconstruction (include/pikgpu.h): context, frame
+// upload with validation, launch sequence, downloads and stage dumps.
+//
+// The launch sequence mirrors the reference decoder's order: DequantDC and
+// AdaptiveDCReconstruction (DecodeDCGroups, pik/compressed_dc.cc:436-528),
+// the per-group prediction set-up of ReconOpsinImage
+// (pik/compressed_image.cc:936-1084), the per-block dequant/prediction/IDCT
+// (pik/compressed_image.cc:1086-1097), then FinalizeFrameDecoding
+// (pik/compressed_image.cc:1144-1181).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace pikgpu;
+
+namespace {
+
+enum Timer { kTDc, kTLf, kTBlocks, kTMinMax, kTEpf, kTDots, kTFinal, kNumTimers };
+const char* kTimerNames[kNumTimers] = {"dc",   "lf_predict", "blocks_idct",
+                                       "minmax", "epf_ar",   "dots",
+                                       "final_opsin"};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct pikgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  DeviceConsts* consts = nullptr;
+  HostTables host;
+  // device buffers (grown on demand)
+  DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots;
+  DevBuf dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
+      dots_img, gab, out, mm_partial, mm, ar_keep;
+  bool uploaded = false;
+  bool ran = false;
+  bool kept = false;
+  FrameParams P;
+  FrameBuffers B;
+  int num_tables = 0;
+  hipEvent_t ev[kNumTimers + 1] = {};
+  float timings[kNumTimers] = {};
+};
+
+namespace {
+
+int Fail(pikgpu_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                  \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      return Fail(ctx, PIKGPU_EHIP,                                         \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    }                                                                       \
+  } while (0)
+
+int Grow(pikgpu_ctx* ctx, DevBuf* b, size_t bytes) {
+  if (bytes == 0) bytes = 4;
+  if (b->cap >= bytes) return PIKGPU_OK;
+  if (b->p) (void)hipFree(b->p);
+  b->p = nullptr;
+  b->cap = 0;
+  hipError_t e = hipMalloc(&b->p, bytes);
+  if (e != hipSuccess) {
+    return Fail(ctx, PIKGPU_ENOMEM,
+                std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  b->cap = bytes;
+  return PIKGPU_OK;
+}
+
+void FreeAll(pikgpu_ctx* ctx) {
+  DevBuf* all[] = {&ctx->ac,      &ctx->dc16,   &ctx->qf,      &ctx->acs,
+                   &ctx->lut,     &ctx->dcf,    &ctx->ytox,    &ctx->ytob,
+                   &ctx->dqmap,   &ctx->table,  &ctx->offsets, &ctx->dots,
+                   &ctx->dcq,     &ctx->dcv,    &ctx->llf,     &ctx->tmp2x2,
+                   &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
+                   &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
+                   &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
+                   &ctx->ar_keep};
+  for (DevBuf* b : all) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->cap = 0;
+  }
+}
+
+// Copies a strided host plane into a dense device region.
+int Upload2D(pikgpu_ctx* ctx, void* dst, const void* src, size_t src_stride_b,
+             size_t row_b, size_t rows) {
+  HIP_TRY(ctx, hipMemcpy2DAsync(dst, row_b, src, src_stride_b, row_b, rows,
+                                hipMemcpyHostToDevice, ctx->stream));
+  return PIKGPU_OK;
+}
+
+float ToF(double v) { return static_cast<float>(v); }
+
+// Inverse of the opsin absorbance matrix as Inv3x3Matrix<float> computes it
+// (pik/linalg.h:153-170, pik/opsin_params.cc:13-26).
+void OpsinInverse(float out[9]) {
+  const float kScale = 255.0f;
+  float m[9] = {
+      ToF(0.29956550340058319 / kScale), ToF(0.63373087833825936 / kScale),
+      ToF(0.077705617820981968 / kScale), ToF(0.22158691104574774 / kScale),
+      ToF(0.68491388044116142 / kScale), ToF(0.096254234043612538 / kScale),
+      ToF(0.20062661225219422 / kScale), ToF(0.070366199217588729 / kScale),
+      ToF(0.5571760754215358 / kScale)};
+  float t[9];
+  t[0] = m[4] * m[8] - m[5] * m[7];
+  t[1] = m[2] * m[7] - m[1] * m[8];
+  t[2] = m[1] * m[5] - m[2] * m[4];
+  t[3] = m[5] * m[6] - m[3] * m[8];
+  t[4] = m[0] * m[8] - m[2] * m[6];
+  t[5] = m[2] * m[3] - m[0] * m[5];
+  t[6] = m[3] * m[7] - m[4] * m[6];
+  t[7] = m[1] * m[6] - m[0] * m[7];
+  t[8] = m[0] * m[4] - m[1] * m[3];
+  const float idet = 1.0 / (m[0] * t[0] + m[1] * t[3] + m[2] * t[6]);
+  for (int i = 0; i < 9; ++i) out[i] = t[i] * idet;
+}
+
+size_t DivCeil(size_t a, size_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+extern "C" {
+
+pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
+  if (config != nullptr && config->abi_version != PIKGPU_ABI_VERSION) {
+    return nullptr;
+  }
+  pikgpu_ctx* ctx = new pikgpu_ctx;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+          hipSuccess) {
+    delete ctx;
+    return nullptr;
+  }
+  for (int i = 0; i <= kNumTimers; ++i) {
+    if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+      delete ctx;
+      return nullptr;
+    }
+  }
+  // Constant tables.
+  DeviceConsts* h = new DeviceConsts;
+  memset(h, 0, sizeof(*h));
+  ComputeBasis(h->basis);
+  FillScaleTables(&h->tscale[0][0][0], &h->ascale[0][0][0]);
+  ComputeHostTables(&ctx->host);
+  memcpy(h->mul_table, ctx->host.mul_table, sizeof(h->mul_table));
+  OpsinInverse(h->opsin_inv);
+  if (hipMalloc(&ctx->consts, sizeof(DeviceConsts)) != hipSuccess ||
+      hipMemcpy(ctx->consts, h, sizeof(DeviceConsts), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    delete h;
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return nullptr;
+  }
+  delete h;
+  return ctx;
+}
+
+void pikgpu_destroy(pikgpu_ctx* ctx) {
+  if (ctx == nullptr) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  FreeAll(ctx);
+  if (ctx->consts) (void)hipFree(ctx->consts);
+  for (int i = 0; i <= kNumTimers; ++i) {
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* pikgpu_last_error(const pikgpu_ctx* ctx) {
+  if (ctx == nullptr) return "null context";
+  return ctx->error.c_str();
+}
+
+void* pikgpu_stream(pikgpu_ctx* ctx) {
+  return ctx ? static_cast<void*>(ctx->stream) : nullptr;
+}
+
+int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (f == nullptr) return Fail(ctx, PIKGPU_EINVAL, "null frame");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->uploaded = false;
+  ctx->ran = false;
+  // --- validation (the reference treats these as bitstream errors) ---
+  if (f->xsize == 0 || f->ysize == 0 || f->xsize > (1u << 16) ||
+      f->ysize > (1u << 16)) {
+    return Fail(ctx, PIKGPU_EINVAL, "invalid image size");
+  }
+  if (f->global_scale < 1 || f->global_scale > 32768 || f->quant_dc < 1 ||
+      f->quant_dc > 256) {
+    return Fail(ctx, PIKGPU_EINVAL, "invalid quantizer scalars");
+  }
+  if (f->gaborish < 0 || f->gaborish > 4) {
+    return Fail(ctx, PIKGPU_EINVAL, "invalid gaborish strength");
+  }
+  if (f->adaptive_reconstruction && !f->epf_enable_adaptive) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "fixed-sigma EPF (enable_adaptive = false) is not supported");
+  }
+  const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
+  const size_t tx = DivCeil(f->xsize, 64), ty = DivCeil(f->ysize, 64);
+  if (!f->ac[0] || !f->ac[1] || !f->ac[2] || !f->dc[0] || !f->dc[1] ||
+      !f->dc[2] || !f->quant_field || !f->ac_strategy || !f->sigma_lut_ids ||
+      !f->dequant_cf || !f->ytox_map || !f->ytob_map || !f->dequant_map ||
+      !f->dequant_table || !f->dequant_offsets) {
+    return Fail(ctx, PIKGPU_EINVAL, "null array in frame");
+  }
+  if (f->ac_stride < bx * 64 || f->dc_stride < bx ||
+      f->quant_field_stride < bx || f->ac_strategy_stride < bx ||
+      f->sigma_lut_stride < bx || f->dequant_cf_stride < tx ||
+      f->ytox_stride < tx || f->ytob_stride < tx) {
+    return Fail(ctx, PIKGPU_EINVAL, "stride smaller than row");
+  }
+  if (f->num_dequant_offsets == 0 ||
+      f->num_dequant_offsets % (kNumQuantKinds * 3) != 0) {
+    return Fail(ctx, PIKGPU_EINVAL, "dequant offsets not a multiple of 21");
+  }
+  const int num_tables = int(f->num_dequant_offsets / (kNumQuantKinds * 3));
+  // Every table entry a kind can address must lie in the table.
+  const int kind_size[kNumQuantKinds] = {64, 64, 64, 64, 256, 1024, 64};
+  for (int t = 0; t < num_tables; ++t) {
+    for (int k = 0; k < kNumQuantKinds; ++k) {
+      for (int c = 0; c < 3; ++c) {
+        const size_t o = f->dequant_offsets[(t * kNumQuantKinds + k) * 3 + c];
+        if (o + kind_size[k] > f->dequant_table_size) {
+          return Fail(ctx, PIKGPU_EINVAL, "dequant offset out of table");
+        }
+      }
+    }
+  }
+  for (int i = 0; i < 16 * 256; ++i) {
+    if (f->dequant_map[i] >= num_tables) {
+      return Fail(ctx, PIKGPU_EINVAL, "dequant_map refers to missing table");
+    }
+  }
+  for (size_t y = 0; y < ty; ++y) {
+    for (size_t x = 0; x < tx; ++x) {
+      if (f->dequant_cf[y * f->dequant_cf_stride + x] >= 16) {
+        return Fail(ctx, PIKGPU_EINVAL, "dequant control field >= 16");
+      }
+    }
+  }
+  // AC strategy: valid raw types, multi-block transforms complete, aligned
+  // and inside the image (as AcStrategyImage::SetFromRaw establishes).
+  for (size_t y = 0; y < by; ++y) {
+    for (size_t x = 0; x < bx; ++x) {
+      const uint8_t a = f->ac_strategy[y * f->ac_strategy_stride + x];
+      const int raw = a >> 4, idx = a & 15;
+      const int qfv = f->quant_field[y * f->quant_field_stride + x];
+      if (qfv < 1 || qfv > 256) {
+        return Fail(ctx, PIKGPU_EINVAL, "quant field outside [1, 256]");
+      }
+      if (f->sigma_lut_ids[y * f->sigma_lut_stride + x] > 1) {
+        return Fail(ctx, PIKGPU_EINVAL, "sigma lut id > 1");
+      }
+      if (raw > 8) return Fail(ctx, PIKGPU_EINVAL, "invalid AC strategy");
+      if (raw == kLines) {
+        return Fail(ctx, PIKGPU_EUNSUPPORTED, "LINES AC strategy");
+      }
+      const int s = AcsCovered(raw);
+      if (idx >= s * s) return Fail(ctx, PIKGPU_EINVAL, "invalid block index");
+      const size_t ox = x - idx % s, oy = y - idx / s;
+      if (idx % s > int(x) || idx / s > int(y) || ox % s || oy % s ||
+          ox + s > bx || oy + s > by) {
+        return Fail(ctx, PIKGPU_EINVAL, "misaligned multi-block transform");
+      }
+      for (int k = 0; k < s * s; ++k) {
+        const uint8_t e =
+            f->ac_strategy[(oy + k / s) * f->ac_strategy_stride + ox + k % s];
+        if (e != ((raw << 4) | k)) {
+          return Fail(ctx, PIKGPU_EINVAL, "inconsistent multi-block transform");
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < f->num_dots; ++i) {
+    const pikgpu_dot& d = f->dots[i];
+    if (d.xsize < 1 || d.xsize > 8 || d.ysize < 1 || d.ysize > 8) {
+      return Fail(ctx, PIKGPU_EINVAL, "dot size outside [1, 8]");
+    }
+    if (!d.transform && (d.x + d.xsize > bx * 8 || d.y + d.ysize > by * 8)) {
+      return Fail(ctx, PIKGPU_EINVAL, "dot outside the image");
+    }
+    if (d.transform && (d.width <= 0 || (d.dx == 0 && d.dy == 0))) {
+      return Fail(ctx, PIKGPU_EINVAL, "degenerate dot transform");
+    }
+  }
+
+  // --- geometry and derived scalars ---
+  FrameParams& P = ctx->P;
+  memset(&P, 0, sizeof(P));
+  P.xsize = f->xsize;
+  P.ysize = f->ysize;
+  P.bx = int(bx);
+  P.by = int(by);
+  P.px = int(bx * 8);
+  P.py = int(by * 8);
+  P.tx = int(tx);
+  P.ty = int(ty);
+  P.gx = int(DivCeil(bx, kGroupDimBlocks));
+  P.gy = int(DivCeil(by, kGroupDimBlocks));
+  P.predict_lf = f->predict_lf != 0;
+  P.predict_hf = f->predict_hf != 0;
+  P.gaborish = f->gaborish != 0;
+  P.ar = f->adaptive_reconstruction != 0;
+  P.use_sharpened = f->epf_use_sharpened != 0;
+  P.num_dots = int(f->num_dots);
+  // Quantizer::Decode (pik/quantizer.cc:63-71).
+  P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
+  P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
+  for (int c = 0; c < 3; ++c) {
+    const float m0 = f->dequant_table[f->dequant_offsets[c]];
+    P.mul_dc[c] = m0 * P.inv_quant_dc;
+    P.dc_half_step[c] = P.inv_quant_dc * m0 * 0.5f;
+  }
+  P.ytox_dc = float(f->ytox_dc - 128) * (1.0f / 256);
+  P.ytob_dc = float(f->ytob_dc) * (1.0f / 128);
+  P.dc_qf = f->quant_dc / 2;
+  GaborishWeights(f->gaborish, P.gab_w);
+  LfBlurWeights(P.lfblur_w);
+  Ub4Kernel(P.ub4);
+  const float kScale = 255.0f;
+  P.neg_bias[0] = -ToF(0.26786006338144885 / kScale);
+  P.neg_bias[1] = -ToF(0.24494032763907073 / kScale);
+  P.neg_bias[2] = -ToF(0.14255999980363571 / kScale);
+
+  // --- buffers ---
+  const size_t nblk = bx * by;
+  const size_t ngroups = size_t(P.gx) * P.gy;
+  const size_t ipitch = DivCeil(bx * 8, 32) * 32;
+  const size_t opitch = DivCeil(f->xsize, 32) * 32;
+  int rc;
+#define GROW(buf, bytes) \
+  if ((rc = Grow(ctx, &ctx->buf, (bytes))) != PIKGPU_OK) return rc
+  GROW(ac, 3 * nblk * 64 * sizeof(int16_t));
+  GROW(dc16, 3 * nblk * sizeof(int16_t));
+  GROW(qf, nblk * 4);
+  GROW(acs, nblk);
+  GROW(lut, nblk);
+  GROW(dcf, tx * ty);
+  GROW(ytox, tx * ty * 4);
+  GROW(ytob, tx * ty * 4);
+  GROW(dqmap, 16 * 256);
+  GROW(table, f->dequant_table_size * 4);
+  GROW(offsets, f->num_dequant_offsets * 4);
+  GROW(dots, f->num_dots * sizeof(pikgpu_dot));
+  GROW(dcq, 3 * nblk * 4);
+  GROW(dcv, 3 * nblk * 4);
+  GROW(llf, ngroups * 3 * kLlfPlane * 4);
+  GROW(tmp2x2, ngroups * 3 * kG2Plane * 4);
+  GROW(lf2x2, ngroups * 3 * kG2Plane * 4);
+  GROW(pred2x2, ngroups * 3 * kG2Plane * 4);
+  GROW(idct, 3 * ipitch * by * 8 * 4);
+  GROW(ar, 3 * ipitch * by * 8 * 4);
+  GROW(out, 3 * opitch * f->ysize * 4);
+  GROW(mm_partial, 1024 * 6 * 4);
+  GROW(mm, 8 * 4);
+#undef GROW
+
+  // --- copies ---
+  const size_t nb = nblk;
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = Upload2D(ctx, static_cast<int16_t*>(ctx->ac.p) + c * nb * 64,
+                       f->ac[c], f->ac_stride * 2, bx * 64 * 2, by)))
+      return rc;
+    if ((rc = Upload2D(ctx, static_cast<int16_t*>(ctx->dc16.p) + c * nb,
+                       f->dc[c], f->dc_stride * 2, bx * 2, by)))
+      return rc;
+  }
+  if ((rc = Upload2D(ctx, ctx->qf.p, f->quant_field, f->quant_field_stride * 4,
+                     bx * 4, by)))
+    return rc;
+  if ((rc = Upload2D(ctx, ctx->acs.p, f->ac_strategy, f->ac_strategy_stride,
+                     bx, by)))
+    return rc;
+  if ((rc = Upload2D(ctx, ctx->lut.p, f->sigma_lut_ids, f->sigma_lut_stride,
+                     bx, by)))
+    return rc;
+  if ((rc = Upload2D(ctx, ctx->dcf.p, f->dequant_cf, f->dequant_cf_stride, tx,
+                     ty)))
+    return rc;
+  if ((rc = Upload2D(ctx, ctx->ytox.p, f->ytox_map, f->ytox_stride * 4, tx * 4,
+                     ty)))
+    return rc;
+  if ((rc = Upload2D(ctx, ctx->ytob.p, f->ytob_map, f->ytob_stride * 4, tx * 4,
+                     ty)))
+    return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->dqmap.p, f->dequant_map, 16 * 256,
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->table.p, f->dequant_table,
+                              f->dequant_table_size * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->offsets.p, f->dequant_offsets,
+                              f->num_dequant_offsets * 4,
+                              hipMemcpyHostToDevice, ctx->stream));
+  if (f->num_dots) {
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->dots.p, f->dots,
+                                f->num_dots * sizeof(pikgpu_dot),
+                                hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+
+  FrameBuffers& B = ctx->B;
+  memset(&B, 0, sizeof(B));
+  B.ac = static_cast<const int16_t*>(ctx->ac.p);
+  B.dc16 = static_cast<const int16_t*>(ctx->dc16.p);
+  B.qf = static_cast<const int32_t*>(ctx->qf.p);
+  B.acs = static_cast<const uint8_t*>(ctx->acs.p);
+  B.lut = static_cast<const uint8_t*>(ctx->lut.p);
+  B.dcf = static_cast<const uint8_t*>(ctx->dcf.p);
+  B.ytox = static_cast<const int32_t*>(ctx->ytox.p);
+  B.ytob = static_cast<const int32_t*>(ctx->ytob.p);
+  B.dqmap = static_cast<const uint8_t*>(ctx->dqmap.p);
+  B.table = static_cast<const float*>(ctx->table.p);
+  B.offsets = static_cast<const uint32_t*>(ctx->offsets.p);
+  B.num_tables = num_tables;
+  B.dots = static_cast<const pikgpu_dot*>(ctx->dots.p);
+  B.dcq = static_cast<float*>(ctx->dcq.p);
+  B.dc = static_cast<float*>(ctx->dcv.p);
+  B.llf = static_cast<float*>(ctx->llf.p);
+  B.tmp2x2 = static_cast<float*>(ctx->tmp2x2.p);
+  B.lf2x2 = static_cast<float*>(ctx->lf2x2.p);
+  B.pred2x2 = static_cast<float*>(ctx->pred2x2.p);
+  B.idct = static_cast<float*>(ctx->idct.p);
+  B.ar = static_cast<float*>(ctx->ar.p);
+  B.out = static_cast<float*>(ctx->out.p);
+  B.mm_partial = static_cast<float*>(ctx->mm_partial.p);
+  B.mm = static_cast<float*>(ctx->mm.p);
+  B.ipitch = int(ipitch);
+  B.opitch = int(opitch);
+  ctx->num_tables = num_tables;
+  ctx->uploaded = true;
+  return PIKGPU_OK;
+}
+
+static int EnsureStageBuffers(pikgpu_ctx* ctx) {
+  const FrameParams& P = ctx->P;
+  const size_t nblk = size_t(P.bx) * P.by;
+  const size_t img = 3 * size_t(ctx->B.ipitch) * P.py * 4;
+  int rc;
+  if ((rc = Grow(ctx, &ctx->coeffs, 3 * nblk * 64 * 4))) return rc;
+  if ((rc = Grow(ctx, &ctx->guide, img))) return rc;
+  if ((rc = Grow(ctx, &ctx->epf, img))) return rc;
+  if ((rc = Grow(ctx, &ctx->ar_keep, img))) return rc;
+  if ((rc = Grow(ctx, &ctx->gab, img))) return rc;
+  ctx->B.coeffs = static_cast<float*>(ctx->coeffs.p);
+  ctx->B.guide = static_cast<float*>(ctx->guide.p);
+  ctx->B.epf = static_cast<float*>(ctx->epf.p);
+  ctx->B.dots_img = static_cast<float*>(ctx->ar_keep.p);
+  ctx->B.gab = static_cast<float*>(ctx->gab.p);
+  return PIKGPU_OK;
+}
+
+int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if (keep_stages && (rc = EnsureStageBuffers(ctx))) return rc;
+  FrameParams P = ctx->P;
+  FrameBuffers B = ctx->B;
+  B.keep = keep_stages != 0;
+  const DeviceConsts* K = ctx->consts;
+  hipStream_t st = ctx->stream;
+  const int nblk = P.bx * P.by;
+  const size_t ngroups = size_t(P.gx) * P.gy;
+  const size_t iplane = size_t(B.ipitch) * P.py;
+
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
+  // DC: dequant, min/max of the 8-padded DC image, EPF + clamp.
+  KDcDequant<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B);
+  {
+    Img dci;
+    dci.p = B.dcq;
+    dci.plane = size_t(nblk);
+    dci.pitch = P.bx;
+    dci.W = int(DivCeil(P.bx, 8) * 8);
+    dci.H = int(DivCeil(P.by, 8) * 8);
+    dci.vw = P.bx;
+    dci.vh = P.by;
+    const int nwg = int(std::min<size_t>(1024, DivCeil(size_t(dci.W) * dci.H, 256)));
+    KMinMaxPartial<<<nwg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
+    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, nwg, B.mm);
+    dim3 grid(DivCeil(dci.W, 32), DivCeil(dci.H, 32));
+    KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
+                               size_t(nblk));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
+  // LF-domain predictions per group.
+  {
+    const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
+    const int g2_pos = 4 * cache_pos;
+    dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
+    dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
+    KComputeLlf<<<gc, 256, 0, st>>>(P, B, K);
+    if (P.predict_lf) {
+      KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 0);
+      KLfBlur<<<g2, 256, 0, st>>>(P, B);
+      KLfFromDc2x2<<<gc, 256, 0, st>>>(P, B, K);
+    }
+    if (P.predict_hf) {
+      KZeroPred<<<DivCeil(ngroups * 3 * kG2Plane, 256), 256, 0, st>>>(P, B);
+      if (P.predict_lf) {
+        KPred2x2Border<<<gc, 256, 0, st>>>(P, B, K);
+      } else {
+        KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
+      }
+      KPred2x2Interior<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B, K);
+    }
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
+  {
+    dim3 grid(DivCeil(P.bx, 4), DivCeil(P.by, 4));
+    KBlocks<<<grid, 256, 0, st>>>(P, B, K);
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
+  if (P.ar) {
+    Img ii;
+    ii.p = B.idct;
+    ii.plane = iplane;
+    ii.pitch = B.ipitch;
+    ii.W = P.px;
+    ii.H = P.py;
+    ii.vw = P.px;
+    ii.vh = P.py;
+    const int gab = P.gaborish && !P.use_sharpened;
+    const int nwg = int(std::min<size_t>(1024, DivCeil(size_t(P.px) * P.py, 256)));
+    KMinMaxPartial<<<nwg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
+    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, nwg, B.mm);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+    dim3 grid(DivCeil(P.px, 32), DivCeil(P.py, 32));
+    KEpf<<<grid, 256, 0, st>>>(ii, gab, 0, P, B, K, B.mm, B.ar, B.ipitch,
+                               iplane);
+  } else {
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+    KCopyPlanes<<<DivCeil(3 * iplane, 256), 256, 0, st>>>(B.idct, B.ar,
+                                                          3 * iplane);
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
+  if (B.keep) {
+    if (!P.ar) {
+      HIP_TRY(ctx, hipMemcpyAsync(B.guide, B.idct, 3 * iplane * 4,
+                                  hipMemcpyDeviceToDevice, st));
+      HIP_TRY(ctx, hipMemcpyAsync(B.epf, B.idct, 3 * iplane * 4,
+                                  hipMemcpyDeviceToDevice, st));
+    }
+  }
+  if (B.keep) {
+    // B.dots_img keeps the AR output before the dots modify B.ar in place.
+    HIP_TRY(ctx, hipMemcpyAsync(B.dots_img, B.ar, 3 * iplane * 4,
+                                hipMemcpyDeviceToDevice, st));
+  }
+  if (P.num_dots) KDots<<<1, 256, 0, st>>>(P, B);
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
+  {
+    dim3 grid(DivCeil(P.px, 64), DivCeil(P.py, 4));
+    KFinal<<<grid, 256, 0, st>>>(P, B, K);
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
+  HIP_TRY(ctx, hipGetLastError());
+  ctx->ran = true;
+  ctx->kept = B.keep;
+  return PIKGPU_OK;
+}
+
+int pikgpu_sync(pikgpu_ctx* ctx) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->ran) {
+    for (int i = 0; i < kNumTimers; ++i) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) {
+        ctx->timings[i] = ms;
+      }
+    }
+  }
+  return PIKGPU_OK;
+}
+
+int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
+                    size_t out_stride_bytes) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->ran) return Fail(ctx, PIKGPU_ESTATE, "pikgpu_run has not run");
+  if (out_planes == nullptr || !out_planes[0] || !out_planes[1] ||
+      !out_planes[2]) {
+    return Fail(ctx, PIKGPU_EINVAL, "null output plane");
+  }
+  const FrameParams& P = ctx->P;
+  if (out_stride_bytes < size_t(P.xsize) * 4) {
+    return Fail(ctx, PIKGPU_EINVAL, "output stride smaller than a row");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t oplane = size_t(ctx->B.opitch) * P.ysize;
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(ctx, hipMemcpy2DAsync(out_planes[c], out_stride_bytes,
+                                  ctx->B.out + c * oplane,
+                                  size_t(ctx->B.opitch) * 4,
+                                  size_t(P.xsize) * 4, P.ysize,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  }
+  return pikgpu_sync(ctx);
+}
+
+int pikgpu_reconstruct(pikgpu_ctx* ctx, const pikgpu_frame* frame,
+                       float* out_planes[3], size_t out_stride_bytes) {
+  int rc = pikgpu_frame_upload(ctx, frame);
+  if (rc) return rc;
+  if ((rc = pikgpu_run(ctx, 0))) return rc;
+  return pikgpu_download(ctx, out_planes, out_stride_bytes);
+}
+
+int pikgpu_stage_size(pikgpu_ctx* ctx, int stage, uint32_t* xsize,
+                      uint32_t* ysize) {
+  if (ctx == nullptr || xsize == nullptr || ysize == nullptr) {
+    return PIKGPU_EINVAL;
+  }
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  const FrameParams& P = ctx->P;
+  switch (stage) {
+    case PIKGPU_STAGE_DC_DEQUANT:
+    case PIKGPU_STAGE_DC:
+      *xsize = P.bx;
+      *ysize = P.by;
+      return PIKGPU_OK;
+    case PIKGPU_STAGE_COEFFS:
+      *xsize = P.bx * 64;
+      *ysize = P.by;
+      return PIKGPU_OK;
+    case PIKGPU_STAGE_LINEAR:
+      *xsize = P.xsize;
+      *ysize = P.ysize;
+      return PIKGPU_OK;
+    default:
+      if (stage < 0 || stage >= PIKGPU_NUM_STAGES) {
+        return Fail(ctx, PIKGPU_EINVAL, "invalid stage");
+      }
+      *xsize = P.px;
+      *ysize = P.py;
+      return PIKGPU_OK;
+  }
+}
+
+int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
+                      size_t stride_bytes) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  uint32_t xs, ys;
+  int rc = pikgpu_stage_size(ctx, stage, &xs, &ys);
+  if (rc) return rc;
+  if (!ctx->ran || (!ctx->kept && stage != PIKGPU_STAGE_DC_DEQUANT &&
+                    stage != PIKGPU_STAGE_DC && stage != PIKGPU_STAGE_IDCT &&
+                    stage != PIKGPU_STAGE_AR && stage != PIKGPU_STAGE_LINEAR)) {
+    return Fail(ctx, PIKGPU_ESTATE, "stage not kept (run with keep_stages)");
+  }
+  if (planes == nullptr || !planes[0] || !planes[1] || !planes[2] ||
+      stride_bytes < size_t(xs) * 4) {
+    return Fail(ctx, PIKGPU_EINVAL, "bad output planes");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const FrameParams& P = ctx->P;
+  const FrameBuffers& B = ctx->B;
+  const float* src = nullptr;
+  size_t pitch = 0, plane = 0;
+  const size_t nblk = size_t(P.bx) * P.by;
+  const size_t iplane = size_t(B.ipitch) * P.py;
+  switch (stage) {
+    case PIKGPU_STAGE_DC_DEQUANT: src = B.dcq; pitch = P.bx; plane = nblk; break;
+    case PIKGPU_STAGE_DC: src = B.dc; pitch = P.bx; plane = nblk; break;
+    case PIKGPU_STAGE_COEFFS:
+      src = B.coeffs; pitch = P.bx * 64; plane = nblk * 64; break;
+    case PIKGPU_STAGE_IDCT: src = B.idct; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_GUIDE: src = B.guide; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_EPF: src = B.epf; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_AR:
+      // B.ar is modified in place by the dots; the copy taken before them
+      // is the AR stage when stages are kept.
+      src = ctx->kept ? B.dots_img : B.ar; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_DOTS: src = B.ar; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_GABORISH: src = B.gab; pitch = B.ipitch; plane = iplane; break;
+    case PIKGPU_STAGE_LINEAR:
+      src = B.out; pitch = B.opitch; plane = size_t(B.opitch) * P.ysize; break;
+  }
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(ctx, hipMemcpy2DAsync(planes[c], stride_bytes, src + c * plane,
+                                  pitch * 4, size_t(xs) * 4, ys,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return PIKGPU_OK;
+}
+
+int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* ms,
+                        int max_entries) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  int n = 0;
+  for (int i = 0; i < kNumTimers && n < max_entries; ++i, ++n) {
+    if (names) names[n] = kTimerNames[i];
+    if (ms) ms[n] = ctx->timings[i];
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Device-side data layout shared by the reconstruction kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "../../include/pikgpu.h"
+
+namespace pikgpu {
+
+// Per-context constant tables in device memory (set up once).
+struct DeviceConsts {
+  float basis[kBasisFloats];     // see ComputeBasis
+  float tscale[5][8][8];         // DCTTotalScale<2^(i+1)>(x, y) at [i][y][x]
+  float ascale[5][8][8];         // DCTInvTotalScale<2^(i+1)>(x, y)
+  int mul_table[kMaxSigma + 1];  // EPF WeightFast multipliers
+  float opsin_inv[9];            // inverse opsin absorbance matrix
+};
+
+// LF-domain per-group caches: (gw+2) x (gh+2) values per plane for llf and
+// 2x that in each direction for the 2x grids, at fixed pitches.
+constexpr int kLlfPitch = kGroupDimBlocks + 2;       // 66
+constexpr int kG2Pitch = 2 * (kGroupDimBlocks + 2);  // 132
+constexpr int kLlfPlane = kLlfPitch * kLlfPitch;
+constexpr int kG2Plane = kG2Pitch * kG2Pitch;
+
+// Device pointers and sizes of one frame.
+struct FrameBuffers {
+  // inputs
+  const int16_t* ac;      // 3 planes, each by x (bx*64)
+  const int16_t* dc16;    // 3 planes, each by x bx
+  const int32_t* qf;      // by x bx
+  const uint8_t* acs;     // by x bx
+  const uint8_t* lut;     // by x bx
+  const uint8_t* dcf;     // ty x tx
+  const int32_t* ytox;    // ty x tx
+  const int32_t* ytob;    // ty x tx
+  const uint8_t* dqmap;   // 16 x 256
+  const float* table;
+  const uint32_t* offsets;
+  int num_tables;
+  const pikgpu_dot* dots;
+  // intermediates
+  float* dcq;             // 3 planes by x bx (DequantDC)
+  float* dc;              // 3 planes by x bx (after DC AR)
+  float* llf;             // [group][3][kLlfPlane]
+  float* tmp2x2;          // [group][3][kG2Plane]
+  float* lf2x2;           // [group][3][kG2Plane]
+  float* pred2x2;         // [group][3][kG2Plane]
+  float* coeffs;          // 3 planes by x (bx*64) (stage dump only)
+  float* idct;            // 3 planes py x ipitch
+  float* guide;           // stage dump only, same layout as idct
+  float* epf;             // stage dump only
+  float* ar;              // 3 planes py x ipitch
+  float* dots_img;        // stage dump only
+  float* gab;             // stage dump only
+  float* out;             // 3 planes ysize x opitch
+  float* mm_partial;      // [num_wg][6]
+  float* mm;              // [8]: min[3], max[3], stretch/mul, all_min
+  int ipitch;             // floats per row of idct/ar
+  int opitch;             // floats per row of out
+  int keep;               // write stage dumps
+};
+
+}  // namespace pikgpu

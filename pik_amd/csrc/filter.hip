@@ -1,0 +1,649 @@
+// Frame filters (gfx950): min/max of the EPF guide, the edge-preserving
+// filter fused with adaptive reconstruction, the dot dictionary, and the
+// final Gaborish + XYB->linear conversion.
+//
+//   MinMax / MakeGuide           pik/epf_target.cc:834-1070
+//   SigmaFromQuant               pik/epf_target.cc:1072-1111
+//   AdaptiveFilter (scalar form) pik/epf_target.cc:1113-1277, 53-80, 304-376,
+//                                669-690
+//   AdaptiveReconstruction       pik/adaptive_reconstruction.cc:73-244,
+//                                334-471
+//   AdaptiveDCReconstruction     pik/adaptive_reconstruction.cc:300-332
+//   BlockDictionary::Apply<true> pik/block_dictionary.cc:346-482
+//   ConvolveGaborish             pik/gaborish.cc:119-159
+//   OpsinToLinear / XybToRgb     pik/opsin_inverse.cc:39-123
+#include <float.h>
+
+#include "kernels.h"
+
+namespace pikgpu {
+
+namespace {
+
+__device__ inline float ImgAt(const Img& in, int c, int x, int y) {
+  const int mx = min(Mirror(x, in.W), in.vw - 1);
+  const int my = min(Mirror(y, in.H), in.vh - 1);
+  return in.p[c * in.plane + size_t(my) * in.pitch + mx];
+}
+
+// Guide value (Gaborish of `in` or `in` itself) at an in-range position.
+__device__ inline float GuideAt(const Img& in, int gaborish, const float* gw,
+                                int c, int x, int y) {
+  if (!gaborish) return ImgAt(in, c, x, y);
+  const int xm = Mirror(x - 1, in.W), xp = Mirror(x + 1, in.W);
+  const int ym = Mirror(y - 1, in.H), yp = Mirror(y + 1, in.H);
+  const float tc = ImgAt(in, c, x, ym), bc = ImgAt(in, c, x, yp);
+  const float ml = ImgAt(in, c, xm, y), mr = ImgAt(in, c, xp, y);
+  const float mc = ImgAt(in, c, x, y);
+  const float tl = ImgAt(in, c, xm, ym), tr = ImgAt(in, c, xp, ym);
+  const float bl = ImgAt(in, c, xm, yp), br = ImgAt(in, c, xp, yp);
+  const float sum_tb = tc + bc;
+  const float mul0 = mc * gw[0];
+  const float sum_lr = ml + mr;
+  const float x1 = sum_tb + sum_lr;
+  const float mul1 = x1 * gw[1] + mul0;
+  const float sum_t2 = tl + tr;
+  const float sum_b2 = bl + br;
+  const float x2 = sum_t2 + sum_b2;
+  return x2 * gw[2] + mul1;
+}
+
+// LUTs of SigmaFromQuant (pik/epf_target.cc:1161-1196).
+__constant__ float kSigmaLut[32] = {
+    1.9815775622811198f, 1.9715084740908622f, 1.6819963065873933f,
+    1.2146133632942862f, 1.0395364091521881f, 0.93552327583169714f,
+    0.68568655651684773f, 0.51174440217871964f, 0.36397262821018583f,
+    0.31621830414136975f, 0.30262954326557712f, 0.246314237855494f,
+    0.21617524864418683f, 0.10f, 0.05f, 0.0f,
+    float(1.9815775622811198 / 2.0), float(1.9715084740908622 / 2.0),
+    float(1.6819963065873933 / 2.0), float(1.2146133632942862 / 2.0),
+    float(1.0395364091521881 / 2.0), float(0.93552327583169714 / 2.0),
+    float(0.68568655651684773 / 2.0), float(0.51174440217871964 / 2.0),
+    float(0.36397262821018583 / 2.0), float(0.31621830414136975 / 2.0),
+    float(0.30262954326557712 / 2.0), float(0.246314237855494 / 2.0),
+    float(0.21617524864418683 / 2.0), float(0.10 / 2.0), float(0.05 / 2.0),
+    float(0.0 / 2.0)};
+
+__device__ inline int SigmaFromQuant(float signal, float stretch, int lut_id) {
+  const float* lut = kSigmaLut + 16 * lut_id;
+  const float min_signal = 0.022156f;
+  const float max_signal = 0.531738f;
+  const float mul_signal = 29.435892f;
+  float unscaled;
+  if (signal <= min_signal) {
+    unscaled = lut[0];
+  } else if (signal >= max_signal) {
+    unscaled = lut[15];
+  } else {
+    const float pos = (signal - min_signal) * mul_signal;
+    const int trunc = int(pos);
+    const float frac = pos - float(trunc);
+    unscaled = frac * lut[trunc + 1] + (1.0f - frac) * lut[trunc];
+  }
+  const int sigma = int(unscaled * stretch + 0.5182760822018414f);
+  return min(sigma, kMaxSigma);
+}
+
+// AcStrategy::ARQuantScale (pik/ac_strategy.h:122-130).
+__device__ inline float ARQuantScale(int raw) {
+  if (raw == kDCT32) return float(1.2282996852328099);
+  if (raw == kDCT16) return float(1.1423171621463439);
+  if (raw == kDCT4X4 || raw == kDCT4X4NoHF) return 1.2f;
+  if (raw == kDCT2X2) return float(1.0098134203870499);
+  return 1.0f;
+}
+
+// WeightFast (pik/epf_target.cc:304-376): exponent-field approximation of
+// exp(-s * sad) from the per-sigma integer multiplier.
+__device__ inline float WeightOf(int sad, int mul) {
+  const int clamped = min(sad, kClampedSAD);
+  const int unbiased = ((clamped << 6) * mul) >> 16;
+  const int biased = unbiased + (127 << 7);
+  return __int_as_float(biased << 16);
+}
+
+__device__ inline uint32_t LdsDword(const uint8_t* p) {
+  return *reinterpret_cast<const uint32_t*>(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Partial per-channel min/max of the guide over a W x H image; one entry of
+// 6 floats per workgroup.
+__global__ void __launch_bounds__(256)
+    KMinMaxPartial(Img in, int gaborish, FrameParams P, float* partial) {
+  __shared__ float s_mm[6][256];
+  const int tid = threadIdx.x;
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  const size_t n = size_t(in.W) * in.H;
+  for (size_t i = size_t(blockIdx.x) * 256 + tid; i < n;
+       i += size_t(gridDim.x) * 256) {
+    const int x = int(i % in.W), y = int(i / in.W);
+    for (int c = 0; c < 3; ++c) {
+      const float v = GuideAt(in, gaborish, P.gab_w, c, x, y);
+      mn[c] = fminf(mn[c], v);
+      mx[c] = fmaxf(mx[c], v);
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    s_mm[c][tid] = mn[c];
+    s_mm[3 + c][tid] = mx[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      for (int c = 0; c < 3; ++c) {
+        s_mm[c][tid] = fminf(s_mm[c][tid], s_mm[c][tid + s]);
+        s_mm[3 + c][tid] = fmaxf(s_mm[3 + c][tid], s_mm[3 + c][tid + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < 6) partial[blockIdx.x * 6 + tid] = s_mm[tid][0];
+}
+
+// Reduces the partials to mm[0..5] = min[3], max[3]; mm[6] = 255 / range
+// (the guide multiplier and the sigma stretch, both 1 for a flat image),
+// mm[7] = the minimum over channels.
+__global__ void __launch_bounds__(256)
+    KMinMaxFinal(const float* partial, int n, float* mm) {
+  __shared__ float s_mm[6][256];
+  const int tid = threadIdx.x;
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int i = tid; i < n; i += 256) {
+    for (int c = 0; c < 3; ++c) {
+      v[c] = fminf(v[c], partial[i * 6 + c]);
+      v[3 + c] = fmaxf(v[3 + c], partial[i * 6 + 3 + c]);
+    }
+  }
+  for (int k = 0; k < 6; ++k) s_mm[k][tid] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      for (int c = 0; c < 3; ++c) {
+        s_mm[c][tid] = fminf(s_mm[c][tid], s_mm[c][tid + s]);
+        s_mm[3 + c][tid] = fmaxf(s_mm[3 + c][tid], s_mm[3 + c][tid + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    float all_min = s_mm[0][0], all_max = s_mm[3][0];
+    for (int c = 0; c < 3; ++c) {
+      mm[c] = s_mm[c][0];
+      mm[3 + c] = s_mm[3 + c][0];
+      all_min = fminf(all_min, s_mm[c][0]);
+      all_max = fmaxf(all_max, s_mm[3 + c][0]);
+    }
+    const float range = all_max - all_min;
+    mm[6] = range == 0.0f ? 1.0f : 255.0f / range;
+    mm[7] = all_min;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// EPF (+ AR clamp) on one 32x32 tile. dc_mode: the image is the 8-padded DC
+// image (constant quant field QuantDC/2, lut 0, DCT8 everywhere), no AR, and
+// the result is clamped to +-half a DC step and written to B.dc.
+__global__ void __launch_bounds__(256)
+    KEpf(Img in, int gaborish, int dc_mode, FrameParams P, FrameBuffers B,
+         const DeviceConsts* K, const float* mm, float* out, int out_pitch,
+         size_t out_plane) {
+  constexpr int kInP = 48;  // pitch of s_in rows (46 used)
+  constexpr int kGP = 48;   // pitch of s_g rows (44 used)
+  __shared__ float s_in[3][46 * kInP];
+  __shared__ __attribute__((aligned(16))) uint8_t s_g[3][44 * kGP];
+  __shared__ float s_out[3][1024];
+  __shared__ float s_coef[3][1024];
+  __shared__ float s_ratio[1024];
+  __shared__ float s_tmp[1024];
+  __shared__ uint8_t s_acs[16];
+  __shared__ int s_sigma[16];
+
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const float all_min = mm[7];
+  const float gmul = mm[6];
+
+  // 1. input tile with a 7-pixel mirrored border
+  for (int i = tid; i < 3 * 46 * 46; i += 256) {
+    const int c = i / (46 * 46), rem = i % (46 * 46);
+    const int ly = rem / 46, lx = rem % 46;
+    s_in[c][ly * kInP + lx] = ImgAt(in, c, x0 - 7 + lx, y0 - 7 + ly);
+  }
+  // 2. per-block sigma and strategy
+  if (tid < 16) {
+    const int bxl = tid & 3, byl = tid >> 2;
+    const int bxa = x0 / 8 + bxl, bya = y0 / 8 + byl;
+    int sigma = 0;
+    uint8_t a = 0xFF;
+    if (bxa * 8 < in.W && bya * 8 < in.H) {
+      float ac_q, scale;
+      int lut_id;
+      if (dc_mode) {
+        ac_q = float(P.dc_qf);
+        scale = 1.0f;
+        lut_id = 0;
+        a = 0;
+      } else {
+        const int blk = bya * P.bx + bxa;
+        a = B.acs[blk];
+        ac_q = float(B.qf[blk]);
+        scale = ARQuantScale(a >> 4);
+        lut_id = B.lut[blk];
+      }
+      const float quant = ac_q * scale * 0.039324273f;
+      sigma = SigmaFromQuant(quant, gmul, lut_id);
+    }
+    s_sigma[tid] = sigma;
+    s_acs[tid] = a;
+  }
+  __syncthreads();
+  // 3. u8 guide over the tile plus a 6-pixel border (MakeGuide)
+  for (int i = tid; i < 3 * 44 * 44; i += 256) {
+    const int c = i / (44 * 44), rem = i % (44 * 44);
+    const int ly = rem / 44, lx = rem % 44;
+    float g;
+    const float* si = s_in[c] + (ly + 1) * kInP + (lx + 1);
+    if (gaborish) {
+      const float tc = si[-kInP], bc = si[kInP], ml = si[-1], mr = si[1];
+      const float tl = si[-kInP - 1], tr = si[-kInP + 1];
+      const float bl = si[kInP - 1], br = si[kInP + 1];
+      const float sum_tb = tc + bc;
+      const float mul0 = si[0] * P.gab_w[0];
+      const float sum_lr = ml + mr;
+      const float x1 = sum_tb + sum_lr;
+      const float mul1 = x1 * P.gab_w[1] + mul0;
+      const float sum_t2 = tl + tr;
+      const float sum_b2 = bl + br;
+      const float x2 = sum_t2 + sum_b2;
+      g = x2 * P.gab_w[2] + mul1;
+    } else {
+      g = si[0];
+    }
+    const float scaled = (g - all_min) * gmul;
+    s_g[c][ly * kGP + lx] = static_cast<uint8_t>(static_cast<int>(scaled));
+    if (B.keep && !dc_mode && ly >= 6 && lx >= 6 && ly < 38 && lx < 38) {
+      const int x = x0 + lx - 6, y = y0 + ly - 6;
+      if (x < in.W && y < in.H) {
+        B.guide[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] = g;
+      }
+    }
+  }
+  __syncthreads();
+
+  // 4. EPF: each thread filters 4 horizontally adjacent pixels, two at a
+  //    time (SAD over 3x4 patches for 7x8 neighbours, channel maximum,
+  //    WeightFast, flush below 0.05, weighted mean of the input).
+  const int r = tid >> 3, c0 = (tid & 7) << 2;
+  {
+    const int sigma = s_sigma[(r >> 3) * 4 + (c0 >> 3)];
+    const bool inside = x0 + c0 < in.W && y0 + r < in.H;
+    if (inside && sigma < kMinSigma) {
+      for (int c = 0; c < 3; ++c) {
+        for (int j = 0; j < 4; ++j) {
+          s_out[c][r * 32 + c0 + j] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+        }
+      }
+    } else if (inside) {
+      const int mul = K->mul_table[sigma];
+      for (int jp = 0; jp < 4; jp += 2) {
+        int msad[2][56];
+        for (int cy = -3; cy <= 3; ++cy) {
+          int mx[2][8];
+          for (int j = 0; j < 2; ++j) {
+            for (int k = 0; k < 8; ++k) mx[j][k] = 0;
+          }
+          for (int ch = 0; ch < 3; ++ch) {
+            uint32_t acc[2][8];
+            for (int j = 0; j < 2; ++j) {
+              for (int k = 0; k < 8; ++k) acc[j][k] = 0;
+            }
+            for (int iy = -1; iy <= 1; ++iy) {
+              // reference patch rows: bytes c0+jp+5 .. +3 (+j)
+              const uint8_t* rr = s_g[ch] + (r + 6 + iy) * kGP + c0;
+              const uint32_t rd0 = LdsDword(rr + 4), rd1 = LdsDword(rr + 8);
+              const uint8_t* wr = s_g[ch] + (r + 6 + cy + iy) * kGP + c0;
+              const uint32_t wd[5] = {LdsDword(wr), LdsDword(wr + 4),
+                                      LdsDword(wr + 8), LdsDword(wr + 12),
+                                      LdsDword(wr + 16)};
+              for (int j = 0; j < 2; ++j) {
+                const int ro = jp + j + 1;  // ref start = c0 + 4 + ro
+                const uint32_t ref = __builtin_amdgcn_alignbyte(
+                    ro >= 4 ? LdsDword(rr + 12) : rd1, ro >= 4 ? rd1 : rd0,
+                    ro & 3);
+                for (int cx = -3; cx <= 4; ++cx) {
+                  const int o = jp + j + 5 + cx;  // window start - c0
+                  const uint32_t wnd = __builtin_amdgcn_alignbyte(
+                      wd[(o >> 2) + 1], wd[o >> 2], o & 3);
+                  acc[j][cx + 3] = __builtin_amdgcn_sad_u8(ref, wnd, acc[j][cx + 3]);
+                }
+              }
+            }
+            for (int j = 0; j < 2; ++j) {
+              for (int k = 0; k < 8; ++k) mx[j][k] = max(mx[j][k], int(acc[j][k]));
+            }
+          }
+          for (int j = 0; j < 2; ++j) {
+            for (int k = 0; k < 8; ++k) msad[j][(cy + 3) * 8 + k] = mx[j][k];
+          }
+        }
+        // weighted sums in the reference's sequential order
+        float ws[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        float sw[2] = {0.f, 0.f};
+        for (int cy = -3; cy <= 3; ++cy) {
+          for (int cx = -3; cx <= 4; ++cx) {
+            for (int j = 0; j < 2; ++j) {
+              float w = WeightOf(msad[j][(cy + 3) * 8 + cx + 3], mul);
+              w = w >= 0.05f ? w : 0.0f;
+              const int col = c0 + jp + j + cx + 7;
+              const int row = r + cy + 7;
+              for (int c = 0; c < 3; ++c) {
+                ws[j][c] = ws[j][c] + s_in[c][row * kInP + col] * w;
+              }
+              sw[j] = sw[j] + w;
+            }
+          }
+        }
+        for (int j = 0; j < 2; ++j) {
+          for (int c = 0; c < 3; ++c) {
+            s_out[c][r * 32 + c0 + jp + j] = ws[j][c] / sw[j];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  if (dc_mode) {
+    // Clamp to +-half a quantization step around the dequantised DC
+    // (pik/adaptive_reconstruction.cc:320-331); only the unpadded region.
+    const int y = y0 + r;
+    for (int j = 0; j < 4; ++j) {
+      const int x = x0 + c0 + j;
+      if (x < P.bx && y < P.by) {
+        for (int c = 0; c < 3; ++c) {
+          const float dc = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+          const float hs = P.dc_half_step[c];
+          const float lo = dc - hs, hi = dc + hs;
+          const float d = s_out[c][r * 32 + c0 + j];
+          const float mn = hi < d ? hi : d;
+          out[c * out_plane + size_t(y) * out_pitch + x] = lo < mn ? mn : lo;
+        }
+      }
+    }
+    return;
+  }
+  if (B.keep) {
+    const int y = y0 + r;
+    for (int j = 0; j < 4; ++j) {
+      const int x = x0 + c0 + j;
+      if (x < in.W && y < in.H) {
+        for (int c = 0; c < 3; ++c) {
+          B.epf[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] =
+              s_out[c][r * 32 + c0 + j];
+        }
+      }
+    }
+  }
+
+  // 5. AR: clamp the DCT of (filtered - original) to half the quantization
+  //    interval with one ratio per coefficient shared by X, Y, B.
+  const TileXform X = MakeXform(s_acs, tid);
+  for (int c = 0; c < 3; ++c) {
+    for (int j = 0; j < 4; ++j) {
+      s_out[c][r * 32 + c0 + j] =
+          s_out[c][r * 32 + c0 + j] - s_in[c][(r + 7) * kInP + c0 + 7 + j];
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < 3; ++c) TileForward(K, X, s_out[c], s_coef[c], s_tmp, false);
+  {
+    // coefficient entries owned by this thread
+    const int crow = tid >> 6, ccol = (tid & 63) << 2;
+    const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
+    if (ca != 0xFF) {
+      const int raw = ca >> 4, idx = ca & 15;
+      const int s = AcsCovered(raw);
+      const int N = 8 * s;
+      const int coby = crow - idx / s, cobx = (ccol >> 6) - idx % s;
+      const int cflat0 = (crow - coby) * 64 * s + (ccol - cobx * 64);
+      // origin (first) block of the transform: its quant value and table
+      const int obxa = x0 / 8 + cobx, obya = y0 / 8 + coby;
+      const int oblk = obya * P.bx + obxa;
+      const int qac = B.qf[oblk];
+      const float inv_quant_ac = P.inv_global_scale / float(qac);
+      const int tile = (obya >> 3) * P.tx + (obxa >> 3);
+      const int qt = B.dqmap[B.dcf[tile] * 256 + qac - 1];
+      const int kind = AcsQuantKind(raw);
+      for (int j = 0; j < 4; ++j) {
+        const int flat = cflat0 + j;
+        const int fi = flat / N, fj = flat % N;
+        float ratio = 1.0f;
+        for (int c = 0; c < 3; ++c) {
+          float interval;
+          if (fi < s && fj < s) {
+            const float dc_mul =
+                P.inv_quant_dc * B.table[B.offsets[c]];
+            const float lscale = s == 1 ? 1.0f : K->ascale[s == 2 ? 3 : 4][fi][fj];
+            interval = lscale * dc_mul;
+          } else {
+            const float m = B.table[B.offsets[(qt * kNumQuantKinds + kind) * 3 + c] + flat];
+            interval = m * inv_quant_ac;
+          }
+          const float upper = 0.5f * interval;
+          const float lower = -0.5f * interval;
+          const float corr = s_coef[c][crow * 256 + ccol + j];
+          const float m1 = lower < corr ? corr : lower;
+          const float clamped = m1 < upper ? m1 : upper;
+          const float divisor = __float_as_uint(corr) == 0u ? 1E-7f : corr;
+          const float cr = clamped / divisor;
+          ratio = cr < ratio ? cr : ratio;
+        }
+        s_ratio[crow * 256 + ccol + j] = ratio;
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int crow = tid >> 6, ccol = (tid & 63) << 2;
+    for (int c = 0; c < 3; ++c) {
+      for (int j = 0; j < 4; ++j) {
+        s_coef[c][crow * 256 + ccol + j] =
+            s_coef[c][crow * 256 + ccol + j] * s_ratio[crow * 256 + ccol + j];
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < 3; ++c) {
+    TileInverse(K, X, s_coef[c], s_out[c], s_tmp);
+  }
+  const int y = y0 + r;
+  if (y < in.H) {
+    for (int c = 0; c < 3; ++c) {
+      float* dst = out + c * out_plane + size_t(y) * out_pitch;
+      for (int j = 0; j < 4; ++j) {
+        const int x = x0 + c0 + j;
+        if (x < in.W) {
+          dst[x] = s_out[c][r * 32 + c0 + j] + s_in[c][(r + 7) * kInP + c0 + 7 + j];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dot dictionary: placements applied in order (copied ones first, then the
+// scaled/rotated ones), one workgroup, in place on B.ar.
+__global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
+  __shared__ float s_blk[3][21 * 21];
+  const int tid = threadIdx.x;
+  const float kBlockInvScale = 1.0f / 3.5f;
+  const float kXybRadius[3] = {0.0301006f, 0.4512295f, 0.4716444f};
+  const size_t plane = size_t(P.py) * B.ipitch;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int di = 0; di < P.num_dots; ++di) {
+      const pikgpu_dot d = B.dots[di];
+      if ((d.transform != 0) != (pass == 1)) continue;
+      const int xs = d.xsize, ys = d.ysize;
+      if (pass == 0) {
+        for (int i = tid; i < 3 * xs * ys; i += 256) {
+          const int c = i / (xs * ys), rem = i % (xs * ys);
+          const int iy = rem / xs, ix = rem % xs;
+          const int x = d.x + ix, y = d.y + iy;
+          if (x < P.px && y < P.py) {
+            const float val =
+                kBlockInvScale * kXybRadius[c] * float(d.pixels[c][iy * xs + ix]);
+            float* p = B.ar + c * plane + size_t(y) * B.ipitch + x;
+            *p = *p + val;
+          }
+        }
+        __syncthreads();
+        continue;
+      }
+      // 2x nearest-neighbour upsampling with a 2x2 box (block_dictionary.cc:
+      // 382-401), then a bilinear splat through the inverse affine map.
+      const int bw = 2 * xs + 5;
+      for (int i = tid; i < 3 * 21 * 21; i += 256) (&s_blk[0][0])[i] = 0.0f;
+      __syncthreads();
+      for (int i = tid; i < 3 * (2 * ys + 1) * (2 * xs + 1); i += 256) {
+        const int c = i / ((2 * ys + 1) * (2 * xs + 1));
+        const int rem = i % ((2 * ys + 1) * (2 * xs + 1));
+        const int iy = rem / (2 * xs + 1), ix = rem % (2 * xs + 1);
+        float val = 0.0f;
+        for (int dy = -1; dy < 1; dy++) {
+          for (int dx = -1; dx < 1; dx++) {
+            int sy = (iy + dy) / 2;
+            if (sy < 0) sy = 0;
+            if (sy >= ys) sy = ys - 1;
+            int sx = (ix + dx) / 2;
+            if (sx < 0) sx = 0;
+            if (sx >= xs) sx = xs - 1;
+            val += float(d.pixels[c][sy * xs + sx]);
+          }
+        }
+        s_blk[c][(iy + 2) * bw + (ix + 2)] = val * 0.25f;
+      }
+      __syncthreads();
+      const float by00 = float(d.y);
+      const float bx00 = float(d.x);
+      const float bx01 = bx00 + float(d.dx);
+      const float by01 = by00 + float(d.dy);
+      const float xnorm =
+          float(sqrt(double(int64_t(d.dx) * d.dx + int64_t(d.dy) * d.dy)));
+      const float ynorm = float(d.width) * 0.5f;
+      const float invxnorm = 1.0f / xnorm;
+      const float invynorm = 1.0f / ynorm;
+      const float deltax = float(d.dy) * invxnorm * ynorm;
+      const float deltay = float(-d.dx) * invxnorm * ynorm;
+      const float bx11 = bx01 + deltax;
+      const float by11 = by01 + deltay;
+      const float bx10 = bx00 + deltax;
+      const float by10 = by00 + deltay;
+      const float inv_det = 1.0f / (float(d.dx) * deltay - float(d.dy) * deltax);
+      const float it0 = float(xs) * deltay * inv_det;
+      const float it1 = -float(xs) * deltax * inv_det;
+      const float it2 = -float(ys) * float(d.dy) * inv_det;
+      const float it3 =
+          float(uint64_t(ys) * uint64_t(int64_t(d.dx))) * inv_det;
+      int64_t min_x = int64_t(fminf(fminf(bx00, bx01), fminf(bx10, bx11)));
+      int64_t min_y = int64_t(fminf(fminf(by00, by01), fminf(by10, by11)));
+      int64_t max_x = int64_t(fmaxf(fmaxf(bx00, bx01), fmaxf(bx10, bx11)) + 1);
+      int64_t max_y = int64_t(fmaxf(fmaxf(by00, by01), fmaxf(by10, by11)) + 1);
+      if (min_x < 0) min_x = 0;
+      if (min_y < 0) min_y = 0;
+      if (max_x > P.px) max_x = P.px;
+      if (max_y > P.py) max_y = P.py;
+      const float margin_x = 0.3f * invxnorm;
+      const float margin_y = 0.3f * invynorm;
+      const int w = int(max(int64_t(0), max_x - min_x));
+      const int h = int(max(int64_t(0), max_y - min_y));
+      for (int i = tid; i < w * h; i += 256) {
+        const int iy = int(min_y) + i / w, ix = int(min_x) + i % w;
+        const float x = float(ix) - bx00 + 0.5f;
+        const float y = float(iy) - by00 + 0.5f;
+        float ox = it0 * x + it1 * y;
+        float oy = it2 * x + it3 * y;
+        if (ox >= -margin_x && ox < float(xs) + margin_x && oy >= -margin_y &&
+            oy < float(ys) + margin_y) {
+          ox = 2 * ox + 2;
+          oy = 2 * oy + 2;
+          const int floorx = int(ox);
+          const int ceilx = floorx + 1;
+          const float fracx = ox - float(floorx);
+          const int floory = int(oy);
+          const int ceily = floory + 1;
+          const float fracy = oy - float(floory);
+          for (int c = 0; c < 3; ++c) {
+            const float* blk = s_blk[c];
+            const float val =
+                (fracx * fracy * blk[ceily * bw + ceilx] +
+                 fracx * (1.f - fracy) * blk[floory * bw + ceilx] +
+                 (1.f - fracx) * fracy * blk[ceily * bw + floorx] +
+                 (1.f - fracx) * (1.f - fracy) * blk[floory * bw + floorx]) *
+                kBlockInvScale * kXybRadius[c];
+            float* p = B.ar + c * plane + size_t(iy) * B.ipitch + ix;
+            *p = *p + val;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final Gaborish (on the 8-padded image) and XYB -> linear RGB of the
+// xsize x ysize output (pik/compressed_image.cc:1164-1179).
+__global__ void __launch_bounds__(256)
+    KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= P.px || y >= P.py) return;
+  const size_t plane = size_t(P.py) * B.ipitch;
+  float v[3];
+  for (int c = 0; c < 3; ++c) {
+    const float* src = B.ar + c * plane;
+    v[c] = P.gaborish ? Symmetric3(src, B.ipitch, P.px, P.py, x, y, P.gab_w[0],
+                                   P.gab_w[1], P.gab_w[2])
+                      : src[size_t(y) * B.ipitch + x];
+    if (B.keep) B.gab[c * plane + size_t(y) * B.ipitch + x] = v[c];
+  }
+  if (x >= P.xsize || y >= P.ysize) return;
+  const float* inv = K->opsin_inv;
+  const float gamma_r = 1.0f * (v[1] + v[0]);
+  const float gamma_g = 1.0f * (v[1] - v[0]);
+  const float gamma_b = v[2];
+  const float gamma_r2 = gamma_r * gamma_r;
+  const float gamma_g2 = gamma_g * gamma_g;
+  const float gamma_b2 = gamma_b * gamma_b;
+  const float mixed_r = gamma_r2 * gamma_r + P.neg_bias[0];
+  const float mixed_g = gamma_g2 * gamma_g + P.neg_bias[1];
+  const float mixed_b = gamma_b2 * gamma_b + P.neg_bias[2];
+  float lr = inv[0] * mixed_r;
+  float lg = inv[3] * mixed_r;
+  float lb = inv[6] * mixed_r;
+  const float tr = inv[1] * mixed_g;
+  const float tg = inv[4] * mixed_g;
+  const float tb = inv[7] * mixed_g;
+  lr = inv[2] * mixed_b + lr;
+  lg = inv[5] * mixed_b + lg;
+  lb = inv[8] * mixed_b + lb;
+  lr = lr + tr;
+  lg = lg + tg;
+  lb = lb + tb;
+  const size_t oplane = size_t(P.ysize) * B.opitch;
+  B.out[size_t(y) * B.opitch + x] = lr;
+  B.out[oplane + size_t(y) * B.opitch + x] = lg;
+  B.out[2 * oplane + size_t(y) * B.opitch + x] = lb;
+}
+
+__global__ void __launch_bounds__(256)
+    KCopyPlanes(const float* src, float* dst, size_t n) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+}  // namespace pikgpu

@@ -1,0 +1,246 @@
+// Host-side constant tables of the reconstruction, computed once per context.
+//
+// Each routine restates how the reference derives the same numbers so that
+// the device kernels see bit-identical constants:
+//  - EPF weight multipliers: MulTable::Init (pik/epf_target.cc:379-421) with
+//    the WeightFast bit trick (pik/epf_target.cc:304-376);
+//  - Gaborish 3x3 weights (pik/gaborish.cc:9-90);
+//  - LF prediction blur weights (pik/ac_predictions.cc:121-132);
+//  - the 4x upsampling blur of the HF prediction (ComputeUb4Kernel,
+//    pik/ac_predictions.cc:234-254, GaussianKernel, pik/gauss_blur.h:18-32);
+//  - the scale factors of the reinterpreting DCTs (DCTTotalScale /
+//    DCTInvTotalScale, pik/ac_strategy.cc:258-270) and of AR's lowest
+//    frequencies (pik/ac_strategy.h:33-38);
+//  - basis matrices of the transposed-scaled transforms.
+#include <math.h>
+#include <string.h>
+
+#include "common.h"
+
+namespace pikgpu {
+
+namespace {
+
+// First eight entries of the DCT normalisation tables (pik/dct.h:44-130,
+// 142-245), N = 2, 4, 8, 16, 32. These define the reference's coefficient
+// scaling and must be the same numbers for the reinterpreted LF coefficients
+// to match. Only indices < 8 are ever used (LF regions are at most 8x8).
+const float kDctScales[5][8] = {
+    {0.707106781186547524f, 0.707106781186547524f},
+    {0.5f, 0.653281482438188264f, 0.5f, 0.270598050073098492f},
+    {0.353553390593273762f, 0.254897789552079584f, 0.270598050073098492f,
+     0.30067244346752264f, 0.353553390593273762f, 0.449988111568207852f,
+     0.653281482438188264f, 1.28145772387075309f},
+    {0.25f, 0.351850934381595615f, 0.346759961330536865f,
+     0.33832950029358817f, 0.326640741219094132f, 0.311806253246667808f,
+     0.293968900604839679f, 0.273300466750439372f},
+    {0.176776695296636881f, 0.249698864051293098f, 0.248796181668049222f,
+     0.247294127491195243f, 0.245196320100807612f, 0.242507813298635998f,
+     0.239235083933052216f, 0.235386016295755195f}};
+const float kIdctScales[5][8] = {
+    {0.707106781186547524f, 0.707106781186547524f},
+    {0.5f, 0.382683432365089772f, 0.5f, 0.923879532511286756f},
+    {0.353553390593273762f, 0.490392640201615225f, 0.461939766255643378f,
+     0.415734806151272619f, 0.353553390593273762f, 0.277785116509801112f,
+     0.191341716182544886f, 0.0975451610080641339f},
+    {0.25f, 0.177632042131274808f, 0.180239955501736978f,
+     0.184731156892216368f, 0.191341716182544886f, 0.200444985785954314f,
+     0.212607523691814112f, 0.228686034616512494f},
+    {0.176776695296636881f, 0.125150749558799075f, 0.125604821547038926f,
+     0.126367739974385915f, 0.127448894776039792f, 0.128861827480656137f,
+     0.13062465373492222f, 0.132760647772446044f}};
+
+int Log2Index(int n) {  // 2->0, 4->1, 8->2, 16->3, 32->4
+  int i = 0;
+  while ((2 << i) < n) ++i;
+  return i;
+}
+
+int TrailingZeros(int k) {
+  int v = 0;
+  while ((k & 1) == 0) {
+    k >>= 1;
+    ++v;
+  }
+  return v;
+}
+
+// Relative L1 norm of a single-coefficient IDCT (pik/dct.h:140-245): the
+// product cos(pi/8) cos(pi/16) ... with log2(N) - 1 - v2(k) factors.
+double L1Norm(int n, int k) {
+  if (k == 0) return 1.0;
+  const int factors = Log2Index(n) - TrailingZeros(k);
+  double p = 1.0;
+  for (int i = 0; i < factors; ++i) p *= cos(M_PI / double(8 << i));
+  return p;
+}
+
+// The reference stores the L1 norms and their inverses as float literals of
+// the exact values; rounding the double value reproduces them, except the
+// 32-point inverses whose literals carry a little extra error in the last
+// digits, which still round to the same floats.
+float L1NormF(int n, int k) { return static_cast<float>(L1Norm(n, k)); }
+float L1NormInvF(int n, int k) { return static_cast<float>(1.0 / L1Norm(n, k)); }
+
+// DCTTotalScale<N>(x, y) and DCTInvTotalScale<N>(x, y), evaluated in float
+// left to right like the reference's constexpr functions.
+float TotalScale(int n, int x, int y) {
+  const int i = Log2Index(n);
+  float v = static_cast<float>(n);
+  v = v * kDctScales[i][x];
+  v = v * kDctScales[i][y];
+  v = v * L1NormInvF(n, x);
+  v = v * L1NormInvF(n, y);
+  return v;
+}
+float InvTotalScale(int n, int x, int y) {
+  const int i = Log2Index(n);
+  float v = static_cast<float>(n);
+  v = v * kIdctScales[i][x];
+  v = v * kIdctScales[i][y];
+  v = v * L1NormF(n, x);
+  v = v * L1NormF(n, y);
+  return v;
+}
+
+float WeightFromBits(int sad, int mul) {
+  const int clamped = sad < kClampedSAD ? sad : kClampedSAD;
+  const int16_t prescaled = static_cast<int16_t>(clamped << 6);
+  const int16_t unbiased =
+      static_cast<int16_t>((int32_t(prescaled) * int32_t(mul)) >> 16);
+  const int16_t biased = static_cast<int16_t>(unbiased + (127 << 7));
+  const uint32_t bits = static_cast<uint32_t>(static_cast<uint16_t>(biased))
+                        << 16;
+  float f;
+  memcpy(&f, &bits, 4);
+  return f;
+}
+
+}  // namespace
+
+void ComputeHostTables(HostTables* t) {
+  memset(t->mul_table, 0, sizeof(t->mul_table));
+  const int gap = 1 << kSigmaShift;
+  int mul = -32768;
+  for (int sigma = kMinSigma; sigma <= kMaxSigma; sigma += gap) {
+    for (; mul < 0; ++mul) {
+      if (WeightFromBits(sigma >> kSigmaShift, mul) > 0.5f) break;
+    }
+    t->mul_table[sigma] = mul;
+  }
+  for (int sigma = kMinSigma; sigma < kMaxSigma; sigma += gap) {
+    const float mul_step =
+        (t->mul_table[sigma + gap] - t->mul_table[sigma]) / float(gap);
+    for (int i = 1; i < gap; ++i) {
+      t->mul_table[sigma + i] =
+          static_cast<int>(t->mul_table[sigma] + i * mul_step);
+    }
+  }
+}
+
+void GaborishWeights(int strength, float w[3]) {
+  const float wu0 = 1.0f;
+  float wu1, wu2;
+  switch (strength) {
+    case 1:  // k500
+      wu1 = static_cast<float>(0.056007960760453189);
+      wu2 = static_cast<float>(0.045899074552453879);
+      break;
+    case 2: {  // k750
+      const float x = 0.75;
+      wu1 = static_cast<float>(x * 0.13959942428275746);
+      wu2 = static_cast<float>(x * 0.074240717189152386);
+      break;
+    }
+    case 3: {  // k875
+      const float x = 0.875;
+      wu1 = static_cast<float>(x * 0.11501538179658321);
+      wu2 = static_cast<float>(x * 0.089979079587015454);
+      break;
+    }
+    default:  // k1000
+      wu1 = static_cast<float>(0.11501538179658321);
+      wu2 = static_cast<float>(0.089979079587015454);
+      break;
+  }
+  const float mul = 1.0 / (wu0 + 4 * (wu1 + wu2));
+  w[0] = wu0 * mul;
+  w[1] = wu1 * mul;
+  w[2] = wu2 * mul;
+}
+
+void LfBlurWeights(float w[3]) {
+  w[0] = 0.41459272584128337f;
+  w[1] = 0.25489157325704559f;
+  w[2] = 0.046449679523692139f;
+}
+
+void Ub4Kernel(float k[3][4]) {
+  const float sigma = 2.0007879236394901f;
+  const int radius = 4;
+  float kernel[2 * radius + 1];
+  const float scaler = -1.0 / (2 * sigma * sigma);
+  double sum = 0.0;
+  for (int i = -radius; i <= radius; ++i) {
+    const float val = expf(scaler * i * i);
+    kernel[i + radius] = val;
+    sum += val;
+  }
+  for (int i = 0; i < 2 * radius + 1; ++i) kernel[i] /= sum;
+  for (int j = 0; j < 3; ++j) {
+    for (int q = 0; q < 4; ++q) k[j][q] = 0.0f;
+  }
+  for (int q = 0; q < 4; ++q) {
+    const int split0 = 4 - q;
+    const int split1 = 8 - q;
+    for (int j = 0; j < split0; ++j) k[0][q] += kernel[j];
+    for (int j = split0; j < split1; ++j) k[1][q] += kernel[j];
+    for (int j = split1; j < 2 * radius + 1; ++j) k[2][q] += kernel[j];
+  }
+}
+
+int BasisOffset(int n) {
+  int off = 0;
+  for (int m = 2; m < n; m *= 2) off += 2 * m * m;
+  return off;
+}
+
+void ComputeBasis(float* out) {
+  for (int n = 2; n <= 32; n *= 2) {
+    float* b = out + BasisOffset(n);
+    float* binv = b + n * n;
+    for (int k = 0; k < n; ++k) {
+      double w;
+      if (k == 0) {
+        w = 1.0;
+      } else if (n == 8) {
+        w = 1.0 / cos(k * M_PI / 16.0);
+      } else {
+        w = 2.0 * cos(k * M_PI / (2.0 * n));
+      }
+      const double alpha = (k == 0 ? 1.0 : 2.0) / n;
+      for (int x = 0; x < n; ++x) {
+        const double c = cos(M_PI * (2 * x + 1) * k / (2.0 * n));
+        b[k * n + x] = static_cast<float>(w * c);        // B[k][x]
+        binv[x * n + k] = static_cast<float>(alpha * c / w);  // Binv[x][k]
+      }
+    }
+  }
+}
+
+// Transform scale tables, laid out as consts->tscale[log2idx][y][x] and
+// ascale likewise; consumed through FillScaleTables.
+void FillScaleTables(float* tscale, float* ascale) {
+  for (int i = 0; i < 5; ++i) {
+    const int n = 2 << i;
+    for (int y = 0; y < 8; ++y) {
+      for (int x = 0; x < 8; ++x) {
+        const bool in = x < n && y < n;
+        tscale[(i * 8 + y) * 8 + x] = in ? TotalScale(n, x, y) : 0.0f;
+        ascale[(i * 8 + y) * 8 + x] = in ? InvTotalScale(n, x, y) : 0.0f;
+      }
+    }
+  }
+}
+
+}  // namespace pikgpu

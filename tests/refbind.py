@@ -1,0 +1,127 @@
+"""ctypes bindings to the reference harness libraries under oracle/_ref
+(built from the google/pik sources by oracle/Makefile). Test-only: the
+reference is the checker here, never the thing measured."""
+import ctypes
+import os
+
+import numpy as np
+
+from pik_amd import abi
+from pik_amd.frame import div_ceil
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+SCALAR = os.path.join(REF_DIR, "libpikref_scalar.so")
+AVX2 = os.path.join(REF_DIR, "libpikref_avx2.so")
+
+_libs = {}
+
+
+def available(kind="scalar"):
+    return os.path.exists(SCALAR if kind == "scalar" else AVX2)
+
+
+def lib(kind="scalar"):
+    if kind in _libs:
+        return _libs[kind]
+    path = SCALAR if kind == "scalar" else AVX2
+    if not os.path.exists(path):
+        raise RuntimeError("reference harness not built: %s" % path)
+    L = ctypes.CDLL(path)
+    L.pikref_last_error.restype = ctypes.c_char_p
+    L.pikref_recon.argtypes = [ctypes.POINTER(abi.FrameC), ctypes.c_int,
+                               ctypes.c_void_p, ctypes.c_void_p]
+    L.pikref_default_tables.argtypes = [
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    L.pikref_epf_weight.restype = ctypes.c_float
+    L.pikref_epf_weight.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.pikref_transform.argtypes = [ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    L.pikref_gaborish.argtypes = [ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_size_t, ctypes.c_size_t,
+                                  ctypes.c_void_p]
+    L.pikref_opsin_to_linear.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.c_size_t]
+    L.pikref_linear_to_xyb.argtypes = [ctypes.c_float] * 3 + [ctypes.c_void_p]
+    _libs[kind] = L
+    return L
+
+
+def default_tables(kind="scalar"):
+    L = lib(kind)
+    ts, no = ctypes.c_size_t(), ctypes.c_size_t()
+    L.pikref_default_tables(None, 0, None, 0, ctypes.byref(ts), ctypes.byref(no))
+    t = np.zeros(ts.value, np.float32)
+    o = np.zeros(no.value, np.uint32)
+    L.pikref_default_tables(t.ctypes.data, t.size, o.ctypes.data, o.size,
+                            ctypes.byref(ts), ctypes.byref(no))
+    return t, o
+
+
+def stage_shape(frame, stage):
+    bx, by = frame.xsize_blocks, frame.ysize_blocks
+    if stage in (abi.STAGE_DC_DEQUANT, abi.STAGE_DC):
+        return (by, bx)
+    if stage == abi.STAGE_COEFFS:
+        return (by, bx * 64)
+    if stage == abi.STAGE_LINEAR:
+        return (frame.ysize, frame.xsize)
+    return (by * 8, bx * 8)
+
+
+def recon(frame, threads=0, stages=True, kind="scalar"):
+    """Runs the reference reconstruction. Returns (linear (3,H,W) float32,
+    dict stage_name -> (3, h, w) float32)."""
+    L = lib(kind)
+    fc, keep = frame.to_c()
+    out = np.zeros((3, frame.ysize, frame.xsize), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    st = {}
+    st_ptrs = None
+    if stages:
+        st_ptrs = (ctypes.c_void_p * (3 * abi.NUM_STAGES))()
+        for s in range(abi.NUM_STAGES):
+            arr = np.zeros((3,) + stage_shape(frame, s), np.float32)
+            st[abi.STAGE_NAMES[s]] = arr
+            for c in range(3):
+                st_ptrs[3 * s + c] = arr[c].ctypes.data
+    rc = L.pikref_recon(ctypes.byref(fc), threads,
+                        ctypes.cast(st_ptrs, ctypes.c_void_p) if st_ptrs else None,
+                        ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikref_recon: %s" % L.pikref_last_error().decode())
+    return out, st
+
+
+def transform(raw, direction, block):
+    """Reference TransformToPixels (direction 0) / TransformFromPixels (1)."""
+    s = 4 if raw == abi.ACS_DCT32X32 else (2 if raw == abi.ACS_DCT16X16 else 1)
+    inp = np.ascontiguousarray(block, np.float32).reshape(-1)
+    assert inp.size == 64 * s * s
+    out = np.zeros(64 * s * s, np.float32)
+    lib().pikref_transform(raw, direction, inp.ctypes.data, out.ctypes.data)
+    return out.reshape(8 * s, 8 * s)
+
+
+def epf_weight(sigma, sad):
+    return lib().pikref_epf_weight(sigma, sad)
+
+
+def gaborish(strength, img):
+    img = np.ascontiguousarray(img, np.float32)
+    _, ys, xs = img.shape
+    out = np.zeros_like(img)
+    ip = (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
+    op = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    rc = lib().pikref_gaborish(strength, ip, xs, ys, op)
+    assert rc == 0
+    return out
+
+
+def opsin_to_linear(img):
+    img = np.array(img, np.float32, copy=True)
+    _, ys, xs = img.shape
+    p = (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
+    lib().pikref_opsin_to_linear(p, xs, ys)
+    return img
