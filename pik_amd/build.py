@@ -20,7 +20,10 @@ ARCH = os.environ.get("PIKGPU_ARCH", "gfx950")
 
 SOURCES = ["recon_lf.hip", "recon_blocks.hip", "filter.hip", "api.hip",
            "host_tables.cpp"]
-HEADERS = ["common.h", "device.h", "kernels.h"]
+HEADERS = ["common.h", "device.h", "kernels.h", "xform.h", "dct1d.h",
+           "dct_consts.h"]
+HOST_OUT = os.path.join(HERE, "libpikgpu_host.so")
+HOST_SOURCES = ["host_check.cpp", "host_tables.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
           "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
@@ -58,6 +61,12 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] +
          objs + ["-Wl,--no-undefined", "-Wl,-soname,libpikgpu.so"])
     os.replace(tmp, OUT)
+    # Host build of the transforms and tables for the CPU test suite.
+    tmp = HOST_OUT + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+          "-fvisibility=hidden", "-Wall", "-o", tmp] +
+         [os.path.join(CSRC, s) for s in HOST_SOURCES])
+    os.replace(tmp, HOST_OUT)
     return OUT
 
 

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "device.h"
+#include "xform.h"
 
 namespace pikgpu {
 
@@ -26,8 +27,10 @@ __device__ inline const float* BasisInv(const DeviceConsts* K, int n) {
   return K->basis + BasisOffsetD(n) + n * n;
 }
 
-// In-thread ts-IDCT of an n x n block (n <= 8). n == 2 uses the reference's
-// explicit 2-point form (pik/dct.h:491-504).
+// In-thread ts-IDCT / ts-DCT (with the 1/N^2 scaling of ScaleToBlock) of an
+// n x n block, n = 2, 4, 8, as the reinterpreting transforms use them
+// (pik/ac_strategy.cc:272-315); n == 2 is the reference's explicit 2-point
+// form (pik/dct.h:478-504).
 __device__ inline void SmallIdct(const DeviceConsts* K, const float* in,
                                  int in_stride, float* out, int out_stride,
                                  int n) {
@@ -38,28 +41,13 @@ __device__ inline void SmallIdct(const DeviceConsts* K, const float* in,
     out[1] = a00 + a01 - a10 - a11;
     out[out_stride] = a00 - a01 + a10 - a11;
     out[out_stride + 1] = a00 - a01 - a10 + a11;
-    return;
-  }
-  const float* b = BasisB(K, n);
-  float t[64];
-  // t[c][x] = sum_r F[r][c] B[r][x]
-  for (int c = 0; c < n; ++c) {
-    for (int x = 0; x < n; ++x) {
-      float acc = 0.0f;
-      for (int r = 0; r < n; ++r) acc = fmaf(in[r * in_stride + c], b[r * n + x], acc);
-      t[c * n + x] = acc;
-    }
-  }
-  for (int y = 0; y < n; ++y) {
-    for (int x = 0; x < n; ++x) {
-      float acc = 0.0f;
-      for (int c = 0; c < n; ++c) acc = fmaf(b[c * n + y], t[c * n + x], acc);
-      out[y * out_stride + x] = acc;
-    }
+  } else if (n == 4) {
+    dct1d::Idct2D<4>(in, in_stride, out, out_stride);
+  } else {
+    dct1d::Idct2D<8>(in, in_stride, out, out_stride);
   }
 }
 
-// In-thread ts-DCT (with the 1/N^2 scaling of ScaleToBlock) of n x n.
 __device__ inline void SmallDct(const DeviceConsts* K, const float* in,
                                 int in_stride, float* out, int out_stride,
                                 int n) {
@@ -70,25 +58,10 @@ __device__ inline void SmallDct(const DeviceConsts* K, const float* in,
     out[1] = (a00 + a01 - a10 - a11) * 0.25f;
     out[out_stride] = (a00 - a01 + a10 - a11) * 0.25f;
     out[out_stride + 1] = (a00 - a01 - a10 + a11) * 0.25f;
-    return;
-  }
-  const float* bi = BasisInv(K, n);
-  float u[64];
-  // u[y][r] = sum_x P[y][x] Binv[x][r]
-  for (int y = 0; y < n; ++y) {
-    for (int r = 0; r < n; ++r) {
-      float acc = 0.0f;
-      for (int x = 0; x < n; ++x) acc = fmaf(in[y * in_stride + x], bi[x * n + r], acc);
-      u[y * n + r] = acc;
-    }
-  }
-  // F[r][c] = sum_y Binv[y][c] u[y][r]
-  for (int r = 0; r < n; ++r) {
-    for (int c = 0; c < n; ++c) {
-      float acc = 0.0f;
-      for (int y = 0; y < n; ++y) acc = fmaf(bi[y * n + c], u[y * n + r], acc);
-      out[r * out_stride + c] = acc;
-    }
+  } else if (n == 4) {
+    dct1d::Dct2D<4>(in, in_stride, out, out_stride);
+  } else {
+    dct1d::Dct2D<8>(in, in_stride, out, out_stride);
   }
 }
 
@@ -155,251 +128,147 @@ __device__ inline float DequantCoef(const FrameParams& P, const FrameBuffers& B,
 }
 
 // ---------------------------------------------------------------------------
-// Cooperative transforms of one channel of a 32x32 tile (256 threads, each
-// owning a 4-pixel row segment). `acs_t` holds the 16 strategy bytes of the
-// tile's 8x8 blocks (0xFF outside the image). Coefficients live in the
-// reference's storage layout: 4 block rows x 256 floats.
+// Cooperative transforms of one channel of a 32x32 tile (256 threads).
+// `acs_t` holds the 16 strategy bytes of the tile's 8x8 blocks (0xFF outside
+// the image). Coefficients live in the reference's storage layout: 4 block
+// rows x 256 floats. The 2-D transforms run as the reference's two column
+// passes with a transpose in between (pik/dct.h:370-507): every line is a
+// 1-D butterfly of dct1d.h computed by one thread. Thread t works for the
+// 8x8 block t / 16 of the tile; only threads of the first block of a
+// transform have lines to do (16 per block: a DCT8 has 8 lines, a DCT4X4 4
+// lines in each of its 4 sub-blocks, DCT16 16 and DCT32 32, two per thread).
 
 struct TileXform {
-  int r, c0;       // thread's pixel row and first column in the tile
-  int kind;        // 0 none, 1 separable DCT N, 2 DCT4X4, 3 direct 8x8
   int raw, s, N;   // strategy, covered blocks, transform size
-  int oby, obx;    // origin block of the transform in the tile
-  int ly, lx;      // thread position relative to the origin (pixels)
-  bool owner;      // owns an 8x8 direct-kind block
+  int oby, obx;    // origin block of this thread's transform in the tile
+  int lane;        // thread index within its 8x8 block (0..15)
+  bool first;      // this thread's block is the transform's first block
+  bool valid;      // block inside the image
+  // pixel-quad ownership for elementwise work: row r, columns c0..c0+3
+  int r, c0;
 };
 
 __device__ inline TileXform MakeXform(const uint8_t* acs_t, int tid) {
   TileXform X;
   X.r = tid >> 3;
   X.c0 = (tid & 7) << 2;
-  const int lby = X.r >> 3, lbx = X.c0 >> 3;
-  const uint8_t a = acs_t[lby * 4 + lbx];
-  X.owner = false;
-  if (a == 0xFF) {
-    X.kind = 0;
-    X.raw = 0; X.s = 1; X.N = 8; X.oby = lby; X.obx = lbx; X.ly = 0; X.lx = 0;
-    return X;
-  }
-  X.raw = a >> 4;
-  const int idx = a & 15;
+  const int blk = tid >> 4;
+  X.lane = tid & 15;
+  const int lby = blk >> 2, lbx = blk & 3;
+  const uint8_t a = acs_t[blk];
+  X.valid = a != 0xFF;
+  X.raw = X.valid ? (a >> 4) : 0;
+  const int idx = X.valid ? (a & 15) : 0;
   X.s = AcsCovered(X.raw);
   X.N = 8 * X.s;
   X.oby = lby - idx / X.s;
   X.obx = lbx - idx % X.s;
-  X.ly = X.r - X.oby * 8;
-  X.lx = X.c0 - X.obx * 8;
-  if (AcsIsDCT(X.raw)) {
-    X.kind = 1;
-  } else if (AcsIsDCT4(X.raw)) {
-    X.kind = 2;
-  } else {
-    X.kind = 3;
-    X.owner = (X.r & 7) == 0 && (X.c0 & 7) == 0;
-  }
+  X.first = X.valid && idx == 0;
   return X;
 }
 
-__device__ inline float& CoefRef(float* coef, const TileXform& X, int flat) {
-  const int row = X.oby + flat / (64 * X.s);
-  const int col = X.obx * 64 + flat % (64 * X.s);
-  return coef[row * 256 + col];
+// Coefficient storage index of F[i][j] (row-major N x N, flat i*N + j) of a
+// transform with origin block (oby, obx) covering s x s blocks.
+__device__ inline int CoefIdx(int oby, int obx, int s, int flat) {
+  return (oby + flat / (64 * s)) * 256 + obx * 64 + flat % (64 * s);
 }
 
-// Direct 8x8 inverse transforms: DCT2X2 pyramid (pik/ac_strategy.cc:
-// 791-803, 330-356), IDENTITY (pik/ac_strategy.cc:725-761), and LINES is not
-// emitted by the encoder (pik/ac_strategy.cc:18) and decodes to zeros here.
-__device__ inline void DirectInverse8(int raw, const float* c, float* p,
-                                      int pstride) {
-  if (raw == kDCT2X2) {
-    float a[64];
-    for (int i = 0; i < 64; ++i) a[i] = c[i];
-    for (int S = 2; S <= 8; S *= 2) {
-      const int h = S / 2;
-      float t[64];
-      for (int y = 0; y < h; ++y) {
-        for (int x = 0; x < h; ++x) {
-          const float c00 = a[y * 8 + x];
-          const float c01 = a[y * 8 + h + x];
-          const float c10 = a[(y + h) * 8 + x];
-          const float c11 = a[(y + h) * 8 + h + x];
-          t[(2 * y) * 8 + 2 * x] = c00 + c01 + c10 + c11;
-          t[(2 * y) * 8 + 2 * x + 1] = c00 + c01 - c10 - c11;
-          t[(2 * y + 1) * 8 + 2 * x] = c00 - c01 + c10 - c11;
-          t[(2 * y + 1) * 8 + 2 * x + 1] = c00 - c01 - c10 + c11;
-        }
-      }
-      for (int y = 0; y < S; ++y) {
-        for (int x = 0; x < S; ++x) a[y * 8 + x] = t[y * 8 + x];
-      }
-    }
-    for (int y = 0; y < 8; ++y) {
-      for (int x = 0; x < 8; ++x) p[y * pstride + x] = a[y * 8 + x];
-    }
-    return;
-  }
-  if (raw == kIdentity) {
-    const float b00 = c[0], b01 = c[1], b10 = c[8], b11 = c[9];
-    float dcs[4];
-    dcs[0] = b00 + b01 + b10 + b11;
-    dcs[1] = b00 + b01 - b10 - b11;
-    dcs[2] = b00 - b01 + b10 - b11;
-    dcs[3] = b00 - b01 - b10 + b11;
-    for (int y = 0; y < 2; ++y) {
-      for (int x = 0; x < 2; ++x) {
-        const float block_dc = dcs[y * 2 + x];
-        float residual_sum = 0;
-        for (int iy = 0; iy < 4; ++iy) {
-          for (int ix = 0; ix < 4; ++ix) {
-            if (ix == 0 && iy == 0) continue;
-            residual_sum += c[(y + iy * 2) * 8 + x + ix * 2];
-          }
-        }
-        const float center = block_dc - residual_sum * (1.0f / 16);
-        p[(4 * y + 1) * pstride + 4 * x + 1] = center;
-        for (int iy = 0; iy < 4; ++iy) {
-          for (int ix = 0; ix < 4; ++ix) {
-            if (ix == 1 && iy == 1) continue;
-            p[(y * 4 + iy) * pstride + x * 4 + ix] =
-                c[(y + iy * 2) * 8 + x + ix * 2] + center;
-          }
-        }
-        p[y * 4 * pstride + x * 4] = c[(y + 2) * 8 + x + 2] + center;
-      }
-    }
-    return;
-  }
-  for (int y = 0; y < 8; ++y) {
-    for (int x = 0; x < 8; ++x) p[y * pstride + x] = 0.0f;
-  }
+template <int N>
+__device__ inline void InvLine1(const float* coef, float* tmp, int oby,
+                                int obx, int c) {
+  constexpr int s = N / 8;
+  float v[N], w[N];
+  for (int r = 0; r < N; ++r) v[r] = coef[CoefIdx(oby, obx, s, r * N + c)];
+  dct1d::Idct<N>(v, w);
+  for (int x = 0; x < N; ++x) tmp[(oby * 8 + c) * 32 + obx * 8 + x] = w[x];
 }
-
-// Direct 8x8 forward transforms (pik/ac_strategy.cc:569-589, 614-618).
-__device__ inline void DirectForward8(int raw, const float* p, int pstride,
-                                      float* c) {
-  if (raw == kDCT2X2) {
-    float a[64];
-    for (int y = 0; y < 8; ++y) {
-      for (int x = 0; x < 8; ++x) a[y * 8 + x] = p[y * pstride + x];
-    }
-    for (int S = 8; S >= 2; S /= 2) {
-      const int h = S / 2;
-      float t[64];
-      for (int y = 0; y < h; ++y) {
-        for (int x = 0; x < h; ++x) {
-          const float c00 = a[(2 * y) * 8 + 2 * x];
-          const float c01 = a[(2 * y) * 8 + 2 * x + 1];
-          const float c10 = a[(2 * y + 1) * 8 + 2 * x];
-          const float c11 = a[(2 * y + 1) * 8 + 2 * x + 1];
-          float r00 = c00 + c01 + c10 + c11;
-          float r01 = c00 + c01 - c10 - c11;
-          float r10 = c00 - c01 + c10 - c11;
-          float r11 = c00 - c01 - c10 + c11;
-          r00 *= 0.25f;
-          r01 *= 0.25f;
-          r10 *= 0.25f;
-          r11 *= 0.25f;
-          t[y * 8 + x] = r00;
-          t[y * 8 + h + x] = r01;
-          t[(y + h) * 8 + x] = r10;
-          t[(y + h) * 8 + h + x] = r11;
-        }
-      }
-      for (int y = 0; y < S; ++y) {
-        for (int x = 0; x < S; ++x) a[y * 8 + x] = t[y * 8 + x];
-      }
-    }
-    for (int i = 0; i < 64; ++i) c[i] = a[i];
-    return;
+template <int N>
+__device__ inline void InvLine2(const float* tmp, float* pix, int oby,
+                                int obx, int x) {
+  float v[N], w[N];
+  for (int c = 0; c < N; ++c) v[c] = tmp[(oby * 8 + c) * 32 + obx * 8 + x];
+  dct1d::Idct<N>(v, w);
+  for (int y = 0; y < N; ++y) pix[(oby * 8 + y) * 32 + obx * 8 + x] = w[y];
+}
+template <int N>
+__device__ inline void FwdLine1(const float* pix, float* tmp, int oby,
+                                int obx, int x) {
+  float v[N], w[N];
+  for (int y = 0; y < N; ++y) v[y] = pix[(oby * 8 + y) * 32 + obx * 8 + x];
+  dct1d::Dct<N>(v, w);
+  for (int k = 0; k < N; ++k) tmp[(oby * 8 + x) * 32 + obx * 8 + k] = w[k];
+}
+template <int N>
+__device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
+                                int obx, int k) {
+  constexpr int s = N / 8;
+  float v[N], w[N];
+  for (int x = 0; x < N; ++x) v[x] = tmp[(oby * 8 + x) * 32 + obx * 8 + k];
+  dct1d::Dct<N>(v, w);
+  const float scale = 1.0f / (N * N);
+  for (int j = 0; j < N; ++j) {
+    coef[CoefIdx(oby, obx, s, j * N + k)] = w[j] * scale;
   }
-  if (raw == kIdentity) {
-    for (int i = 0; i < 64; ++i) c[i] = 0.0f;
-    for (int y = 0; y < 2; ++y) {
-      for (int x = 0; x < 2; ++x) {
-        float block_dc = 0;
-        for (int iy = 0; iy < 4; ++iy) {
-          for (int ix = 0; ix < 4; ++ix) {
-            block_dc += p[(y * 4 + iy) * pstride + x * 4 + ix];
-          }
-        }
-        block_dc *= 1.0f / 16;
-        for (int iy = 0; iy < 4; ++iy) {
-          for (int ix = 0; ix < 4; ++ix) {
-            if (ix == 1 && iy == 1) continue;
-            c[(y + iy * 2) * 8 + x + ix * 2] =
-                p[(y * 4 + iy) * pstride + x * 4 + ix] -
-                p[(y * 4 + 1) * pstride + x * 4 + 1];
-          }
-        }
-        c[(y + 2) * 8 + x + 2] = c[y * 8 + x];
-        c[y * 8 + x] = block_dc;
-      }
-    }
-    const float b00 = c[0], b01 = c[1], b10 = c[8], b11 = c[9];
-    c[0] = (b00 + b01 + b10 + b11) * 0.25f;
-    c[1] = (b00 + b01 - b10 - b11) * 0.25f;
-    c[8] = (b00 - b01 + b10 - b11) * 0.25f;
-    c[9] = (b00 - b01 - b10 + b11) * 0.25f;
-    return;
-  }
-  for (int i = 0; i < 64; ++i) c[i] = 0.0f;
 }
 
 // coef (4 x 256) -> pix (32 x 32). All 256 threads must call.
 __device__ inline void TileInverse(const DeviceConsts* K, const TileXform& X,
                                    const float* coef, float* pix, float* tmp) {
-  if (X.kind == 1) {
-    const float* b = BasisB(K, X.N);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int q = 0; q < X.N; ++q) {
-      const int flat = q * X.N + X.ly;
-      const float f = coef[(X.oby + flat / (64 * X.s)) * 256 + X.obx * 64 +
-                           flat % (64 * X.s)];
-      const float* br = b + q * X.N + X.lx;
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(f, br[j], acc[j]);
+  if (X.first) {
+    const int l = X.lane;
+    if (X.raw == kDCT || X.raw == kDCTNoHF) {
+      if (l < 8) InvLine1<8>(coef, tmp, X.oby, X.obx, l);
+    } else if (X.raw == kDCT16) {
+      InvLine1<16>(coef, tmp, X.oby, X.obx, l);
+    } else if (X.raw == kDCT32) {
+      InvLine1<32>(coef, tmp, X.oby, X.obx, l);
+      InvLine1<32>(coef, tmp, X.oby, X.obx, l + 16);
+    } else if (AcsIsDCT4(X.raw)) {
+      // sub-block (sy, sx), column c of its 4x4 coefficient matrix G with
+      // G[iy][ix] = coef[(sy + 2 iy) * 8 + sx + 2 ix], G[0][0] from the
+      // Hadamard of the four DCs (pik/ac_strategy.cc:762-790)
+      const float* cb = coef + X.oby * 256 + X.obx * 64;
+      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, c = l & 3;
+      const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
+      float dc;
+      if (sub == 0) dc = b00 + b01 + b10 + b11;
+      else if (sub == 1) dc = b00 + b01 - b10 - b11;
+      else if (sub == 2) dc = b00 - b01 + b10 - b11;
+      else dc = b00 - b01 - b10 + b11;
+      float v[4], w[4];
+      for (int r = 0; r < 4; ++r) {
+        v[r] = (r == 0 && c == 0) ? dc : cb[(sy + r * 2) * 8 + sx + c * 2];
+      }
+      dct1d::Idct<4>(v, w);
+      for (int x = 0; x < 4; ++x) {
+        tmp[(X.oby * 8 + sy * 4 + c) * 32 + X.obx * 8 + sx * 4 + x] = w[x];
+      }
     }
-    for (int j = 0; j < 4; ++j) tmp[X.r * 32 + X.c0 + j] = acc[j];
-  } else if (X.kind == 2) {
-    const float* cb = coef + X.oby * 256 + X.obx * 64;
-    const int sy = X.ly >> 2, sx = X.lx >> 2, l = X.ly & 3;
-    const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
-    float dc;
-    const int d = sy * 2 + sx;
-    if (d == 0) dc = b00 + b01 + b10 + b11;
-    else if (d == 1) dc = b00 + b01 - b10 - b11;
-    else if (d == 2) dc = b00 - b01 + b10 - b11;
-    else dc = b00 - b01 - b10 + b11;
-    const float* b = BasisB(K, 4);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int q = 0; q < 4; ++q) {
-      const float g = (q == 0 && l == 0) ? dc : cb[(sy + q * 2) * 8 + sx + l * 2];
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(g, b[q * 4 + j], acc[j]);
-    }
-    for (int j = 0; j < 4; ++j) tmp[X.r * 32 + X.c0 + j] = acc[j];
   }
   __syncthreads();
-  if (X.kind == 1) {
-    const float* b = BasisB(K, X.N);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int q = 0; q < X.N; ++q) {
-      const float w = b[q * X.N + X.ly];
-      const float* tr = tmp + (X.oby * 8 + q) * 32 + X.c0;
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(w, tr[j], acc[j]);
+  if (X.first) {
+    const int l = X.lane;
+    if (X.raw == kDCT || X.raw == kDCTNoHF) {
+      if (l < 8) InvLine2<8>(tmp, pix, X.oby, X.obx, l);
+    } else if (X.raw == kDCT16) {
+      InvLine2<16>(tmp, pix, X.oby, X.obx, l);
+    } else if (X.raw == kDCT32) {
+      InvLine2<32>(tmp, pix, X.oby, X.obx, l);
+      InvLine2<32>(tmp, pix, X.oby, X.obx, l + 16);
+    } else if (AcsIsDCT4(X.raw)) {
+      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, x = l & 3;
+      float v[4], w[4];
+      for (int c = 0; c < 4; ++c) {
+        v[c] = tmp[(X.oby * 8 + sy * 4 + c) * 32 + X.obx * 8 + sx * 4 + x];
+      }
+      dct1d::Idct<4>(v, w);
+      for (int y = 0; y < 4; ++y) {
+        pix[(X.oby * 8 + sy * 4 + y) * 32 + X.obx * 8 + sx * 4 + x] = w[y];
+      }
+    } else if (l == 0) {
+      DirectInverse8(X.raw, coef + X.oby * 256 + X.obx * 64,
+                     pix + X.oby * 8 * 32 + X.obx * 8, 32);
     }
-    for (int j = 0; j < 4; ++j) pix[X.r * 32 + X.c0 + j] = acc[j];
-  } else if (X.kind == 2) {
-    const float* b = BasisB(K, 4);
-    const int sy = X.ly >> 2, l = X.ly & 3;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int q = 0; q < 4; ++q) {
-      const float w = b[q * 4 + l];
-      const float* tr = tmp + (X.oby * 8 + sy * 4 + q) * 32 + X.c0;
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(w, tr[j], acc[j]);
-    }
-    for (int j = 0; j < 4; ++j) pix[X.r * 32 + X.c0 + j] = acc[j];
-  } else if (X.kind == 3 && X.owner) {
-    DirectInverse8(X.raw, coef + X.oby * 256 + X.obx * 64,
-                   pix + X.r * 32 + X.c0, 32);
   }
   __syncthreads();
 }
@@ -409,58 +278,53 @@ __device__ inline void TileInverse(const DeviceConsts* K, const TileXform& X,
 __device__ inline void TileForward(const DeviceConsts* K, const TileXform& X,
                                    const float* pix, float* coef, float* tmp,
                                    bool hf_only) {
-  const bool active = X.kind != 0 && (!hf_only || AcsPredictHF(X.raw));
-  if (active && X.kind == 1) {
-    const float* bi = BasisInv(K, X.N);
-    const float* pr = pix + X.r * 32 + X.obx * 8;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int x = 0; x < X.N; ++x) {
-      const float v = pr[x];
-      const float* br = bi + x * X.N + X.lx;
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(v, br[j], acc[j]);
+  const bool active = X.first && (!hf_only || AcsPredictHF(X.raw));
+  const int l = X.lane;
+  if (active) {
+    if (X.raw == kDCT || X.raw == kDCTNoHF) {
+      if (l < 8) FwdLine1<8>(pix, tmp, X.oby, X.obx, l);
+    } else if (X.raw == kDCT16) {
+      FwdLine1<16>(pix, tmp, X.oby, X.obx, l);
+    } else if (X.raw == kDCT32) {
+      FwdLine1<32>(pix, tmp, X.oby, X.obx, l);
+      FwdLine1<32>(pix, tmp, X.oby, X.obx, l + 16);
+    } else if (AcsIsDCT4(X.raw)) {
+      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, x = l & 3;
+      float v[4], w[4];
+      for (int y = 0; y < 4; ++y) {
+        v[y] = pix[(X.oby * 8 + sy * 4 + y) * 32 + X.obx * 8 + sx * 4 + x];
+      }
+      dct1d::Dct<4>(v, w);
+      for (int k = 0; k < 4; ++k) {
+        tmp[(X.oby * 8 + sy * 4 + x) * 32 + X.obx * 8 + sx * 4 + k] = w[k];
+      }
     }
-    for (int j = 0; j < 4; ++j) tmp[X.r * 32 + X.c0 + j] = acc[j];
-  } else if (active && X.kind == 2) {
-    const float* bi = BasisInv(K, 4);
-    const float* pr = pix + X.r * 32 + X.c0;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int x = 0; x < 4; ++x) {
-      const float v = pr[x];
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(v, bi[x * 4 + j], acc[j]);
-    }
-    for (int j = 0; j < 4; ++j) tmp[X.r * 32 + X.c0 + j] = acc[j];
   }
   __syncthreads();
-  if (active && X.kind == 1) {
-    // F[i][j'] with i = ly, j' = lx + j: sum_y Binv[y][j'] U[y][i].
-    const float* bi = BasisInv(K, X.N);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int y = 0; y < X.N; ++y) {
-      const float u = tmp[(X.oby * 8 + y) * 32 + X.obx * 8 + X.ly];
-      const float* br = bi + y * X.N + X.lx;
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(br[j], u, acc[j]);
+  if (active) {
+    if (X.raw == kDCT || X.raw == kDCTNoHF) {
+      if (l < 8) FwdLine2<8>(tmp, coef, X.oby, X.obx, l);
+    } else if (X.raw == kDCT16) {
+      FwdLine2<16>(tmp, coef, X.oby, X.obx, l);
+    } else if (X.raw == kDCT32) {
+      FwdLine2<32>(tmp, coef, X.oby, X.obx, l);
+      FwdLine2<32>(tmp, coef, X.oby, X.obx, l + 16);
+    } else if (AcsIsDCT4(X.raw)) {
+      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, k = l & 3;
+      float v[4], w[4];
+      for (int x = 0; x < 4; ++x) {
+        v[x] = tmp[(X.oby * 8 + sy * 4 + x) * 32 + X.obx * 8 + sx * 4 + k];
+      }
+      dct1d::Dct<4>(v, w);
+      float* cb = coef + X.oby * 256 + X.obx * 64;
+      for (int j = 0; j < 4; ++j) cb[(sy + j * 2) * 8 + sx + k * 2] = w[j] * (1.0f / 16);
+    } else if (l == 0) {
+      DirectForward8(X.raw, pix + X.oby * 8 * 32 + X.obx * 8, 32,
+                     coef + X.oby * 256 + X.obx * 64);
     }
-    for (int j = 0; j < 4; ++j) {
-      const int flat = X.ly * X.N + X.lx + j;
-      coef[(X.oby + flat / (64 * X.s)) * 256 + X.obx * 64 + flat % (64 * X.s)] =
-          acc[j];
-    }
-  } else if (active && X.kind == 2) {
-    const float* bi = BasisInv(K, 4);
-    const int sy = X.ly >> 2, sx = X.lx >> 2, i = X.ly & 3;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int y = 0; y < 4; ++y) {
-      const float u = tmp[(X.oby * 8 + sy * 4 + y) * 32 + X.obx * 8 + sx * 4 + i];
-      for (int j = 0; j < 4; ++j) acc[j] = fmaf(bi[y * 4 + j], u, acc[j]);
-    }
-    float* cb = coef + X.oby * 256 + X.obx * 64;
-    for (int j = 0; j < 4; ++j) cb[(sy + i * 2) * 8 + sx + j * 2] = acc[j];
-  } else if (active && X.kind == 3 && X.owner) {
-    DirectForward8(X.raw, pix + X.r * 32 + X.c0, 32,
-                   coef + X.oby * 256 + X.obx * 64);
   }
   __syncthreads();
-  if (active && X.kind == 2 && X.ly == 0 && X.lx == 0) {
+  if (active && AcsIsDCT4(X.raw) && l == 0) {
     float* cb = coef + X.oby * 256 + X.obx * 64;
     const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
     cb[0] = (b00 + b01 + b10 + b11) * 0.25f;

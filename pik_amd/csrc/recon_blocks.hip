@@ -81,7 +81,9 @@ __global__ void __launch_bounds__(256)
     // 3. HF prediction: blurred 4x upsampling of pred2x2, forward
     //    transform, added except to the LF/LLF corner.
     if (P.predict_hf) {
-      if (X.kind != 0 && AcsPredictHF(X.raw)) {
+      // strategy of the block holding this thread's pixel quad
+      const uint8_t pa = s_acs[(X.r >> 3) * 4 + (X.c0 >> 3)];
+      if (pa != 0xFF && AcsPredictHF(pa >> 4)) {
         const float* p2 = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane;
         const int by_l = (tby - gby0) + (X.r >> 3);  // group-local block row
         const int R0 = 2 * by_l;
