@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the decoder-side reconstruction on one MI355X.
+
+Workload: one 4096x4096 (16.78 MP) synthetic frame at distance 1.0 with all
+AC strategies (DCT8/4X4/2X2/16/32, identity, no-HF variants), predictions,
+Gaborish k750, EPF + AR and dot patches. The frame (quantized coefficients
+and side information, what the host holds after entropy decoding) is
+uploaded once; each timed step is one full device-resident reconstruction
+(DC dequant + DC EPF, LF predictions, dequant/CfL/HF prediction/IDCT, EPF +
+AR, dots, Gaborish, OpsinToLinear) into the linear Image3F in HBM.
+
+Prints one JSON line. `roofline` covers the dominant kernel, timed with the
+library's HIP events on the stream it launches on; `cpu_baseline` times the
+reference's own AVX2 build (oracle/_ref, the reference's ThreadPool) on the
+same frame on this host's cores.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("recon MP/s @16MP d=1.0 (1 GPU vs host cores); batched MP/s; "
+          "fwd+adj MP/s")
+HBM_PEAK_GBPS = 8000.0
+
+# Algorithmic HBM bytes per pixel of the timed kernels (inputs read once,
+# outputs written once; per-block side data amortised over 64 pixels).
+#  epf_ar (KEpf on the image): idct in 3 x f32 + AR out 3 x f32
+#     + quant field i32 + strategy u8 + sigma LUT id u8 per block
+#  blocks_idct (KBlocks): int16 AC 3 x 2 B + idct out 3 x f32
+#     + quant field i32, strategy u8, colour-tile data per block
+#     + LLF, LF and pred2x2 f32 per block (1 + 4 + 4 values per channel)
+#  final_opsin (KFinal): AR+dots image in 3 x f32 + linear out 3 x f32
+BYTES_PER_PX = {
+    "epf_ar": 12.0 + 12.0 + 6.0 / 64,
+    "blocks_idct": 6.0 + 12.0 + 5.0 / 64 + 3 * 9 * 4.0 / 64,
+    "final_opsin": 12.0 + 12.0,
+}
+KERNEL_NAMES = {"epf_ar": "KEpf", "blocks_idct": "KBlocks",
+                "final_opsin": "KFinal"}
+# Whole-path algorithmic bytes per pixel: int16 AC (6) + f32 DC per block
+# (12/64) + i32 quant field per block (4/64) + strategy and sigma LUT id
+# bytes per block (2/64) + f32 linear out (12).
+PATH_BYTES_PER_PX = 6.0 + 12.0 / 64 + 4.0 / 64 + 2.0 / 64 + 12.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--dots", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="wall-time budget of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: host cores, "
+                    "at most 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def host_threads():
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` recorded by the rocprofv3 --pmc
+    summary under profiles/ (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_4096.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^\|\s*%s\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$" % kernel)
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line.strip())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(frame, seconds, threads):
+    """The reference's own reconstruction (AVX2 build, its ThreadPool) on
+    the same frame, repeated until `seconds` have passed (at least 2
+    frames)."""
+    from oracle import ref
+    if not ref.available("avx2"):
+        return None
+    ref.recon(frame, threads=threads, stages=False, kind="avx2")  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        ref.recon(frame, threads=threads, stages=False, kind="avx2")
+        reps += 1
+        el = time.perf_counter() - t0
+        if (reps >= 2 and el >= seconds) or el >= 2 * seconds:
+            break
+    mp = frame.xsize * frame.ysize / 1e6
+    return {"value": round(mp * reps / el, 3), "unit": "MP/s",
+            "cores": threads, "kind": "reference",
+            "sample": "%d full %dx%d frames, reference AVX2 build "
+                      "(DequantImageAC + ReconOpsinImage per group + "
+                      "FinalizeFrameDecoding), %d threads, %.1f s"
+                      % (reps, frame.xsize, frame.ysize, threads, el)}
+
+
+def main():
+    a = parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the reconstruction of a frame "
+              "runs on a single MI355X (multi-GPU sharding is not part of "
+              "this build); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from pik_amd import synth
+    from pik_amd.runtime import Reconstructor
+
+    t = time.perf_counter()
+    frame = synth.make_frame(a.size, a.size, seed=a.seed, distance=1.0,
+                             num_rects=200, line_art=True, num_dots=a.dots)
+    print("frame %dx%d synthesised in %.1f s" % (a.size, a.size,
+          time.perf_counter() - t), file=sys.stderr, flush=True)
+    mp = frame.xsize * frame.ysize / 1e6
+
+    R = Reconstructor(0)
+    R.upload(frame)
+    for _ in range(a.warmup):
+        R.run()
+        R.sync()
+    fams = {}
+    R.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        R.run()
+        R.sync()
+        for k, v in R.timings().items():
+            fams[k] = fams.get(k, 0.0) + v
+    el = time.perf_counter() - t0
+    ms_step = el * 1000.0 / a.steps
+    fam_ms = {k: v / a.steps for k, v in fams.items()}
+    value = mp * a.steps / el
+    print("per-family ms/step: " + json.dumps(
+        {k: round(v, 4) for k, v in fam_ms.items()}), file=sys.stderr,
+        flush=True)
+
+    # Dominant single-kernel family.
+    dom = max(BYTES_PER_PX, key=lambda k: fam_ms.get(k, 0.0))
+    pad_px = (frame.xsize_blocks * 8) * (frame.ysize_blocks * 8)
+    algo_bytes = BYTES_PER_PX[dom] * (pad_px if dom != "final_opsin"
+                                      else frame.xsize * frame.ysize)
+    achieved = algo_bytes / (fam_ms[dom] * 1e-3) / 1e9
+    roof = {"kernel": KERNEL_NAMES[dom], "bound": "hbm",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic_from_profiles(KERNEL_NAMES[dom]),
+            "algorithmic_bytes": int(algo_bytes),
+            "avg_ms": round(fam_ms[dom], 4)}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or host_threads()
+        cpu = cpu_baseline(frame, a.cpu_seconds, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "recon %dx%d d=1.0 single image, all AC "
+                               "strategies, gaborish k750 + EPF/AR + dots"
+                               % (a.size, a.size),
+                   "megapixels": round(mp, 3), "seed": a.seed,
+                   "path_GBps": round(PATH_BYTES_PER_PX * value / 1e3, 2),
+                   "family_ms": {k: round(v, 4) for k, v in fam_ms.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -179,6 +179,18 @@ PIKGPU_API void pikgpu_destroy(pikgpu_ctx* ctx);
 /* Message of the last failure on this context (never NULL). */
 PIKGPU_API const char* pikgpu_last_error(const pikgpu_ctx* ctx);
 
+/* Checks a frame without touching any device: sizes, strides, null arrays,
+ * quantizer scalars, quant field range, AC strategy consistency (complete,
+ * aligned multi-block transforms, as AcStrategyImage::SetFromRaw establishes,
+ * pik/ac_strategy.cc:1066-1099), dequant table/offset bounds and dot
+ * placements. The reference reports these as decoding failures (a false
+ * Status from DecodeFromBitstream / DecodeDCGroups,
+ * pik/compressed_image.cc:720-825). Returns PIKGPU_OK or the code
+ * pikgpu_frame_upload would return; `msg` (may be NULL) receives the reason,
+ * truncated to msg_size. */
+PIKGPU_API int pikgpu_validate_frame(const pikgpu_frame* frame, char* msg,
+                                     size_t msg_size);
+
 /* Validates the frame and copies it to device memory (host -> HBM), growing
  * the context's buffers if needed. Replaces the caller-side filling of
  * FrameDecCache + frame-wide Image3S that precedes DequantImageAC. */

@@ -1,6 +1,7 @@
 """ctypes bindings to the reference harness libraries under oracle/_ref
-(built from the google/pik sources by oracle/Makefile). Test-only: the
-reference is the checker here, never the thing measured."""
+(built from the google/pik sources by oracle/Makefile together with
+ref_harness.cc). Test infrastructure: the reference is the checker, or the
+CPU baseline timed beside the GPU, never part of the product."""
 import ctypes
 import os
 

@@ -200,26 +200,31 @@ void* pikgpu_stream(pikgpu_ctx* ctx) {
   return ctx ? static_cast<void*>(ctx->stream) : nullptr;
 }
 
-int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
-  if (ctx == nullptr) return PIKGPU_EINVAL;
-  if (f == nullptr) return Fail(ctx, PIKGPU_EINVAL, "null frame");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  ctx->uploaded = false;
-  ctx->ran = false;
-  // --- validation (the reference treats these as bitstream errors) ---
+namespace {
+
+int VFail(std::string* err, int code, const char* msg) {
+  if (err) *err = msg;
+  return code;
+}
+
+// Checks that a frame is complete and self-consistent (what the reference's
+// decoder establishes while reading the bitstream; a violation there is a
+// decoding error). Host-only; touches no device.
+int ValidateFrame(const pikgpu_frame* f, std::string* err) {
+  if (f == nullptr) return VFail(err, PIKGPU_EINVAL, "null frame");
   if (f->xsize == 0 || f->ysize == 0 || f->xsize > (1u << 16) ||
       f->ysize > (1u << 16)) {
-    return Fail(ctx, PIKGPU_EINVAL, "invalid image size");
+    return VFail(err, PIKGPU_EINVAL, "invalid image size");
   }
   if (f->global_scale < 1 || f->global_scale > 32768 || f->quant_dc < 1 ||
       f->quant_dc > 256) {
-    return Fail(ctx, PIKGPU_EINVAL, "invalid quantizer scalars");
+    return VFail(err, PIKGPU_EINVAL, "invalid quantizer scalars");
   }
   if (f->gaborish < 0 || f->gaborish > 4) {
-    return Fail(ctx, PIKGPU_EINVAL, "invalid gaborish strength");
+    return VFail(err, PIKGPU_EINVAL, "invalid gaborish strength");
   }
   if (f->adaptive_reconstruction && !f->epf_enable_adaptive) {
-    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+    return VFail(err, PIKGPU_EUNSUPPORTED,
                 "fixed-sigma EPF (enable_adaptive = false) is not supported");
   }
   const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
@@ -228,17 +233,17 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
       !f->dc[2] || !f->quant_field || !f->ac_strategy || !f->sigma_lut_ids ||
       !f->dequant_cf || !f->ytox_map || !f->ytob_map || !f->dequant_map ||
       !f->dequant_table || !f->dequant_offsets) {
-    return Fail(ctx, PIKGPU_EINVAL, "null array in frame");
+    return VFail(err, PIKGPU_EINVAL, "null array in frame");
   }
   if (f->ac_stride < bx * 64 || f->dc_stride < bx ||
       f->quant_field_stride < bx || f->ac_strategy_stride < bx ||
       f->sigma_lut_stride < bx || f->dequant_cf_stride < tx ||
       f->ytox_stride < tx || f->ytob_stride < tx) {
-    return Fail(ctx, PIKGPU_EINVAL, "stride smaller than row");
+    return VFail(err, PIKGPU_EINVAL, "stride smaller than row");
   }
   if (f->num_dequant_offsets == 0 ||
       f->num_dequant_offsets % (kNumQuantKinds * 3) != 0) {
-    return Fail(ctx, PIKGPU_EINVAL, "dequant offsets not a multiple of 21");
+    return VFail(err, PIKGPU_EINVAL, "dequant offsets not a multiple of 21");
   }
   const int num_tables = int(f->num_dequant_offsets / (kNumQuantKinds * 3));
   // Every table entry a kind can address must lie in the table.
@@ -248,20 +253,20 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
       for (int c = 0; c < 3; ++c) {
         const size_t o = f->dequant_offsets[(t * kNumQuantKinds + k) * 3 + c];
         if (o + kind_size[k] > f->dequant_table_size) {
-          return Fail(ctx, PIKGPU_EINVAL, "dequant offset out of table");
+          return VFail(err, PIKGPU_EINVAL, "dequant offset out of table");
         }
       }
     }
   }
   for (int i = 0; i < 16 * 256; ++i) {
     if (f->dequant_map[i] >= num_tables) {
-      return Fail(ctx, PIKGPU_EINVAL, "dequant_map refers to missing table");
+      return VFail(err, PIKGPU_EINVAL, "dequant_map refers to missing table");
     }
   }
   for (size_t y = 0; y < ty; ++y) {
     for (size_t x = 0; x < tx; ++x) {
       if (f->dequant_cf[y * f->dequant_cf_stride + x] >= 16) {
-        return Fail(ctx, PIKGPU_EINVAL, "dequant control field >= 16");
+        return VFail(err, PIKGPU_EINVAL, "dequant control field >= 16");
       }
     }
   }
@@ -273,27 +278,27 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
       const int raw = a >> 4, idx = a & 15;
       const int qfv = f->quant_field[y * f->quant_field_stride + x];
       if (qfv < 1 || qfv > 256) {
-        return Fail(ctx, PIKGPU_EINVAL, "quant field outside [1, 256]");
+        return VFail(err, PIKGPU_EINVAL, "quant field outside [1, 256]");
       }
       if (f->sigma_lut_ids[y * f->sigma_lut_stride + x] > 1) {
-        return Fail(ctx, PIKGPU_EINVAL, "sigma lut id > 1");
+        return VFail(err, PIKGPU_EINVAL, "sigma lut id > 1");
       }
-      if (raw > 8) return Fail(ctx, PIKGPU_EINVAL, "invalid AC strategy");
+      if (raw > 8) return VFail(err, PIKGPU_EINVAL, "invalid AC strategy");
       if (raw == kLines) {
-        return Fail(ctx, PIKGPU_EUNSUPPORTED, "LINES AC strategy");
+        return VFail(err, PIKGPU_EUNSUPPORTED, "LINES AC strategy");
       }
       const int s = AcsCovered(raw);
-      if (idx >= s * s) return Fail(ctx, PIKGPU_EINVAL, "invalid block index");
+      if (idx >= s * s) return VFail(err, PIKGPU_EINVAL, "invalid block index");
       const size_t ox = x - idx % s, oy = y - idx / s;
       if (idx % s > int(x) || idx / s > int(y) || ox % s || oy % s ||
           ox + s > bx || oy + s > by) {
-        return Fail(ctx, PIKGPU_EINVAL, "misaligned multi-block transform");
+        return VFail(err, PIKGPU_EINVAL, "misaligned multi-block transform");
       }
       for (int k = 0; k < s * s; ++k) {
         const uint8_t e =
             f->ac_strategy[(oy + k / s) * f->ac_strategy_stride + ox + k % s];
         if (e != ((raw << 4) | k)) {
-          return Fail(ctx, PIKGPU_EINVAL, "inconsistent multi-block transform");
+          return VFail(err, PIKGPU_EINVAL, "inconsistent multi-block transform");
         }
       }
     }
@@ -301,16 +306,44 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   for (size_t i = 0; i < f->num_dots; ++i) {
     const pikgpu_dot& d = f->dots[i];
     if (d.xsize < 1 || d.xsize > 8 || d.ysize < 1 || d.ysize > 8) {
-      return Fail(ctx, PIKGPU_EINVAL, "dot size outside [1, 8]");
+      return VFail(err, PIKGPU_EINVAL, "dot size outside [1, 8]");
     }
     if (!d.transform && (d.x + d.xsize > bx * 8 || d.y + d.ysize > by * 8)) {
-      return Fail(ctx, PIKGPU_EINVAL, "dot outside the image");
+      return VFail(err, PIKGPU_EINVAL, "dot outside the image");
     }
     if (d.transform && (d.width <= 0 || (d.dx == 0 && d.dy == 0))) {
-      return Fail(ctx, PIKGPU_EINVAL, "degenerate dot transform");
+      return VFail(err, PIKGPU_EINVAL, "degenerate dot transform");
     }
   }
 
+  return PIKGPU_OK;
+}
+
+}  // namespace
+
+int pikgpu_validate_frame(const pikgpu_frame* frame, char* msg,
+                          size_t msg_size) {
+  std::string err;
+  const int rc = ValidateFrame(frame, &err);
+  if (msg && msg_size) {
+    snprintf(msg, msg_size, "%s", rc == PIKGPU_OK ? "" : err.c_str());
+  }
+  return rc;
+}
+
+int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->uploaded = false;
+  ctx->ran = false;
+  {
+    std::string err;
+    const int rc = ValidateFrame(f, &err);
+    if (rc != PIKGPU_OK) return Fail(ctx, rc, err);
+  }
+  const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
+  const size_t tx = DivCeil(f->xsize, 64), ty = DivCeil(f->ysize, 64);
+  const int num_tables = int(f->num_dequant_offsets / (kNumQuantKinds * 3));
   // --- geometry and derived scalars ---
   FrameParams& P = ctx->P;
   memset(&P, 0, sizeof(P));

@@ -62,6 +62,14 @@ class Frame:
     def pixels(self):
         return self.xsize * self.ysize
 
+    def copy(self):
+        """Deep copy (arrays copied)."""
+        kw = {}
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            kw[f.name] = v.copy() if isinstance(v, np.ndarray) else v
+        return Frame(**kw)
+
     def contiguous(self):
         for f in dataclasses.fields(self):
             v = getattr(self, f.name)

@@ -20,6 +20,17 @@ class PikGpuError(RuntimeError):
         self.code = code
 
 
+def validate(frame):
+    """Host-only frame check (pikgpu_validate_frame). Raises PikGpuError
+    with the library's reason when the frame is not decodable."""
+    lib = abi.load()
+    fc, keep = frame.to_c()
+    msg = ctypes.create_string_buffer(256)
+    rc = lib.pikgpu_validate_frame(ctypes.byref(fc), msg, len(msg))
+    if rc != abi.OK:
+        raise PikGpuError(rc, msg.value.decode())
+
+
 class Reconstructor:
     def __init__(self, device=0):
         self._lib = abi.load()

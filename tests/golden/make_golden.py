@@ -1,0 +1,65 @@
+"""Writes the golden reconstruction fixtures tests/golden/recon_*.npz.
+
+Each fixture holds a synthetic frame (the decoder state after entropy
+decoding, as pik_amd.frame.Frame fields) and what the reference's own
+decode-side code computes from it: the DC after DequantDC and after
+AdaptiveDCReconstruction, the ReconOpsinImage output (IDCT stage), the
+AdaptiveReconstruction output and the final linear Image3F that
+FinalizeFrameDecoding leaves. The reference code is the scalar build under
+oracle/_ref (make -C oracle ref); run from the repository root:
+
+    python3 tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from pik_amd import abi, synth  # noqa: E402
+
+# name -> (xsize, ysize, make_frame keyword arguments, header overrides)
+CASES = {
+    "small_default": (64, 64, dict(seed=1, num_dots=2), {}),
+    "ragged_dots_lut": (136, 72, dict(seed=4, num_dots=5, lut_ids="rand"),
+                        {}),
+    "no_predict_g1000": (96, 88, dict(seed=7, predict=False),
+                         dict(gaborish=abi.GABORISH_1000)),
+    "no_ar_g500": (80, 48, dict(seed=9, num_dots=1),
+                   dict(adaptive_reconstruction=0,
+                        gaborish=abi.GABORISH_500)),
+    "sharpened_d3": (72, 72, dict(seed=12, distance=3.0),
+                     dict(epf_use_sharpened=1)),
+}
+STAGES = ("dc_dequant", "dc", "idct", "ar")
+
+
+def make(name):
+    xs, ys, kw, hdr = CASES[name]
+    fr = synth.make_frame(xs, ys, **kw)
+    for k, v in hdr.items():
+        setattr(fr, k, v)
+    out, st = ref.recon(fr, threads=1, stages=True, kind="scalar")
+    return fr, out, st
+
+
+def main():
+    for name in CASES:
+        fr, out, st = make(name)
+        path = os.path.join(HERE, "recon_%s.npz" % name)
+        d = {}
+        for k, v in vars(fr).items():
+            d["frame_" + k] = v if isinstance(v, np.ndarray) else np.array(v)
+        for s in STAGES:
+            d["stage_" + s] = st[s]
+        d["linear"] = out
+        np.savez_compressed(path, **d)
+        print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -9,7 +9,7 @@ import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(
     os.path.abspath(__file__)))))
-from tests import refbind  # noqa: E402
+from oracle import ref as refbind  # noqa: E402
 
 t, o = refbind.default_tables()
 out = os.path.join(refbind.ROOT, "pik_amd", "data", "dequant_default.npz")
