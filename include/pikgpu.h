@@ -162,7 +162,9 @@ enum {
   PIKGPU_STAGE_DC = 1,         /* after AdaptiveDCReconstruction */
   PIKGPU_STAGE_COEFFS = 2,     /* dequantised AC + LLF/LF/HF predictions */
   PIKGPU_STAGE_IDCT = 3,       /* ReconOpsinImage output */
-  PIKGPU_STAGE_GUIDE = 4,      /* ConvolveGaborish(idct), the EPF guide */
+  PIKGPU_STAGE_GUIDE = 4,      /* image the EPF guide bytes come from:
+                                  ConvolveGaborish(idct), or idct with
+                                  gaborish off or epf_use_sharpened */
   PIKGPU_STAGE_EPF = 5,        /* AdaptiveFilter output before AR clamp */
   PIKGPU_STAGE_AR = 6,         /* AdaptiveReconstruction output */
   PIKGPU_STAGE_DOTS = 7,       /* after BlockDictionary::AddTo */

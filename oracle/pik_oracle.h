@@ -1,0 +1,41 @@
+/*
+ * pik_oracle.h -- CPU restatement of the PIK decode-side reconstruction.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the oracle the test suite, the smoke
+ * check and the CPU-baseline leg of the benchmark compare the HIP library
+ * against; the product (pik_amd) never links or calls it.
+ *
+ * Single-threaded plain C, frame at a time, in the reference scalar build's
+ * floating-point operation order (compile with -ffp-contract=off), so that
+ * its output can be compared bit for bit with the reference built from its
+ * own sources (oracle/_ref) and with the GPU.
+ */
+#ifndef PIK_ORACLE_H_
+#define PIK_ORACLE_H_
+
+#include "../include/pikgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reconstructs `frame` into the linear Image3F `out` (3 planes of
+ * xsize x ysize floats, row-major, no padding). When `stages` is non-NULL,
+ * stages[3 * s + c] (s = PIKGPU_STAGE_*) receives plane c of stage s in the
+ * sizes pikgpu_stage_size reports, rows contiguous; NULL entries are
+ * skipped. Returns 0, or a negative PIKGPU_E* code with a message in
+ * pikoracle_last_error(). */
+int pikoracle_recon(const pikgpu_frame* frame, float* const* stages,
+                    float* const out[3]);
+
+const char* pikoracle_last_error(void);
+
+/* One block transform of raw strategy `raw`: dir 0 = TransformToPixels,
+ * 1 = TransformFromPixels; contiguous N x N in and out (N = 8, 16, 32). */
+int pikoracle_block_transform(int raw, int dir, const float* in, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIK_ORACLE_H_ */

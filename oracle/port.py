@@ -1,0 +1,82 @@
+"""ctypes binding of the oracle's C restatement (pik_oracle.c, built by
+`make -C oracle port` into oracle/_build/libpikoracle.so). Test
+infrastructure: a checker, or the CPU baseline timed beside the GPU, never
+part of the product."""
+import ctypes
+import os
+
+import numpy as np
+
+from pik_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "libpikoracle.so")
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError("oracle port not built: %s" % LIB)
+        L = ctypes.CDLL(LIB)
+        L.pikoracle_recon.argtypes = [ctypes.POINTER(abi.FrameC),
+                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.pikoracle_last_error.restype = ctypes.c_char_p
+        L.pikoracle_block_transform.argtypes = [ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_void_p,
+                                                ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def stage_shape(frame, stage):
+    bx, by = frame.xsize_blocks, frame.ysize_blocks
+    if stage in (abi.STAGE_DC_DEQUANT, abi.STAGE_DC):
+        return (by, bx)
+    if stage == abi.STAGE_COEFFS:
+        return (by, bx * 64)
+    if stage == abi.STAGE_LINEAR:
+        return (frame.ysize, frame.xsize)
+    return (by * 8, bx * 8)
+
+
+def recon(frame, stages=True):
+    """Returns (linear (3, H, W) float32, dict stage name -> array)."""
+    L = lib()
+    fc, keep = frame.to_c()
+    out = np.zeros((3, frame.ysize, frame.xsize), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    st = {}
+    st_ptrs = None
+    if stages:
+        st_ptrs = (ctypes.c_void_p * (3 * abi.NUM_STAGES))()
+        for s in range(abi.NUM_STAGES):
+            arr = np.zeros((3,) + stage_shape(frame, s), np.float32)
+            st[abi.STAGE_NAMES[s]] = arr
+            for c in range(3):
+                st_ptrs[3 * s + c] = arr[c].ctypes.data
+    rc = L.pikoracle_recon(ctypes.byref(fc),
+                           ctypes.cast(st_ptrs, ctypes.c_void_p)
+                           if st_ptrs else None,
+                           ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikoracle_recon: %s"
+                           % L.pikoracle_last_error().decode())
+    return out, st
+
+
+def transform(raw, direction, block):
+    """TransformToPixels (direction 0) / TransformFromPixels (1)."""
+    n = block.shape[0]
+    inp = np.ascontiguousarray(block, np.float32).reshape(-1)
+    out = np.zeros(n * n, np.float32)
+    rc = lib().pikoracle_block_transform(raw, direction, inp.ctypes.data,
+                                         out.ctypes.data)
+    assert rc == 0
+    return out.reshape(n, n)

@@ -407,7 +407,10 @@ API int pikref_recon(const pikgpu_frame* f, int num_threads,
       }
       guide = &smoothed;
     }
-    DumpImage3F(*guide, stage(PIKGPU_STAGE_GUIDE), bx * 8, by * 8);
+    // The EPF quantises its guide bytes from `idct` itself when
+    // use_sharpened is set (AdaptiveFilter, pik/epf_target.cc:1136-1137).
+    DumpImage3F(frame_header.epf_params.use_sharpened ? idct : *guide,
+                stage(PIKGPU_STAGE_GUIDE), bx * 8, by * 8);
     Image3F filtered;
     AdaptiveReconstructionAux aux;
     aux.filtered = &filtered;

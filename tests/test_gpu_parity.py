@@ -100,6 +100,9 @@ def test_stages_bit_exact(rec, name):
     rec.run(keep_stages=True)
     rec.sync()
     for s, sname in enumerate(abi.STAGE_NAMES):
+        if not frame.adaptive_reconstruction and s in (abi.STAGE_GUIDE,
+                                                       abi.STAGE_EPF):
+            continue  # no EPF guide or EPF output without AR
         got = rec.stage(s)
         w = want[sname]
         assert got.shape == w.shape, sname
