@@ -57,6 +57,11 @@ def parse_args():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--dots", type=int, default=64)
+    ap.add_argument("--adaptive-reconstruction", type=int, default=1,
+                    choices=(0, 1),
+                    help="frame header flag; 0 decodes without EPF/AR (the "
+                    "library then moves the IDCT image with its plain copy "
+                    "kernel, which calibrates the HBM byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="wall-time budget of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -75,18 +80,19 @@ def host_threads():
 
 
 def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` recorded by the rocprofv3 --pmc
-    summary under profiles/ (FETCH_SIZE + WRITE_SIZE, copy-kernel
-    calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_4096.md")
+    """HBM bytes per launch of `kernel` on this workload, as recorded in
+    profiles/hbm_traffic_recon_4096.md from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE) scaled by the copy-kernel calibration; None
+    when no such record exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_recon_4096.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(r"^\|\s*%s\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$" % kernel)
+    pat = re.compile(r"^\|\s*%s\s*\|(.*)\|\s*([0-9.eE+]+)\s*\|\s*$" % kernel)
     with open(path) as f:
         for line in f:
             m = pat.match(line.strip())
             if m:
-                return float(m.group(1))
+                return float(m.group(2))
     return None
 
 
@@ -127,6 +133,7 @@ def main():
     t = time.perf_counter()
     frame = synth.make_frame(a.size, a.size, seed=a.seed, distance=1.0,
                              num_rects=200, line_art=True, num_dots=a.dots)
+    frame.adaptive_reconstruction = a.adaptive_reconstruction
     print("frame %dx%d synthesised in %.1f s" % (a.size, a.size,
           time.perf_counter() - t), file=sys.stderr, flush=True)
     mp = frame.xsize * frame.ysize / 1e6
@@ -153,7 +160,8 @@ def main():
         flush=True)
 
     # Dominant single-kernel family.
-    dom = max(BYTES_PER_PX, key=lambda k: fam_ms.get(k, 0.0))
+    dom = max((k for k in BYTES_PER_PX if fam_ms.get(k, 0.0) > 0),
+              key=lambda k: fam_ms[k])
     pad_px = (frame.xsize_blocks * 8) * (frame.ysize_blocks * 8)
     algo_bytes = BYTES_PER_PX[dom] * (pad_px if dom != "final_opsin"
                                       else frame.xsize * frame.ysize)
@@ -187,6 +195,7 @@ def main():
                                "strategies, gaborish k750 + EPF/AR + dots"
                                % (a.size, a.size),
                    "megapixels": round(mp, 3), "seed": a.seed,
+                   "adaptive_reconstruction": a.adaptive_reconstruction,
                    "path_GBps": round(PATH_BYTES_PER_PX * value / 1e3, 2),
                    "family_ms": {k: round(v, 4) for k, v in fam_ms.items()}},
         "roofline": roof,
