@@ -97,9 +97,23 @@ __device__ inline float ARQuantScale(int raw) {
 // exp(-s * sad) from the per-sigma integer multiplier.
 __device__ inline float WeightOf(int sad, int mul) {
   const int clamped = min(sad, kClampedSAD);
-  const int unbiased = ((clamped << 6) * mul) >> 16;
+  const int unbiased = __mul24(clamped << 6, mul) >> 16;
   const int biased = unbiased + (127 << 7);
   return __int_as_float(biased << 16);
+}
+
+typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
+
+// Lane-wise maximum / minimum of two packed pairs of u16.
+__device__ inline uint32_t PkMaxU16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(
+      uint32_t, __builtin_elementwise_max(__builtin_bit_cast(U16x2, a),
+                                          __builtin_bit_cast(U16x2, b)));
+}
+__device__ inline uint32_t PkMinU16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(
+      uint32_t, __builtin_elementwise_min(__builtin_bit_cast(U16x2, a),
+                                          __builtin_bit_cast(U16x2, b)));
 }
 
 __device__ inline uint32_t LdsDword(const uint8_t* p) {
@@ -193,7 +207,7 @@ __global__ void __launch_bounds__(256)
          size_t out_plane) {
   constexpr int kInP = 48;  // pitch of s_in rows (46 used)
   constexpr int kGP = 48;   // pitch of s_g rows (44 used)
-  __shared__ float s_in[3][46 * kInP];
+  __shared__ __attribute__((aligned(16))) float s_in[3][46 * kInP];
   __shared__ __attribute__((aligned(16))) uint8_t s_g[3][44 * kGP];
   __shared__ float s_out[3][1024];
   __shared__ float s_coef[3][1024];
@@ -274,9 +288,15 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  // 4. EPF: each thread filters 4 horizontally adjacent pixels, two at a
-  //    time (SAD over 3x4 patches for 7x8 neighbours, channel maximum,
-  //    WeightFast, flush below 0.05, weighted mean of the input).
+  // 4. EPF (AdaptiveFilter, pik/epf_target.cc:1113-1277): each thread
+  //    filters 4 horizontally adjacent pixels of one block row. For every
+  //    neighbour row offset cy the SADs of the 3x4 patches against the 8
+  //    neighbours cx = -3..4 come from two quad-SAD instructions per patch
+  //    row and channel (v_qsad_pk_u16_u8: four byte-shifted 4-byte windows
+  //    of an 8-byte source against the 4-byte reference patch row, 16-bit
+  //    accumulators); channels are combined by their maximum (packed u16),
+  //    then WeightFast, the flush below 0.05 and the weighted mean of the
+  //    input follow in the reference's sequential neighbour order.
   const int r = tid >> 3, c0 = (tid & 7) << 2;
   {
     const int sigma = s_sigma[(r >> 3) * 4 + (c0 >> 3)];
@@ -289,68 +309,93 @@ __global__ void __launch_bounds__(256)
       }
     } else if (inside) {
       const int mul = K->mul_table[sigma];
-      for (int jp = 0; jp < 4; jp += 2) {
-        int msad[2][56];
-        for (int cy = -3; cy <= 3; ++cy) {
-          int mx[2][8];
-          for (int j = 0; j < 2; ++j) {
-            for (int k = 0; k < 8; ++k) mx[j][k] = 0;
-          }
-          for (int ch = 0; ch < 3; ++ch) {
-            uint32_t acc[2][8];
-            for (int j = 0; j < 2; ++j) {
-              for (int k = 0; k < 8; ++k) acc[j][k] = 0;
-            }
-            for (int iy = -1; iy <= 1; ++iy) {
-              // reference patch rows: bytes c0+jp+5 .. +3 (+j)
-              const uint8_t* rr = s_g[ch] + (r + 6 + iy) * kGP + c0;
-              const uint32_t rd0 = LdsDword(rr + 4), rd1 = LdsDword(rr + 8);
-              const uint8_t* wr = s_g[ch] + (r + 6 + cy + iy) * kGP + c0;
-              const uint32_t wd[5] = {LdsDword(wr), LdsDword(wr + 4),
-                                      LdsDword(wr + 8), LdsDword(wr + 12),
-                                      LdsDword(wr + 16)};
-              for (int j = 0; j < 2; ++j) {
-                const int ro = jp + j + 1;  // ref start = c0 + 4 + ro
-                const uint32_t ref = __builtin_amdgcn_alignbyte(
-                    ro >= 4 ? LdsDword(rr + 12) : rd1, ro >= 4 ? rd1 : rd0,
-                    ro & 3);
-                for (int cx = -3; cx <= 4; ++cx) {
-                  const int o = jp + j + 5 + cx;  // window start - c0
-                  const uint32_t wnd = __builtin_amdgcn_alignbyte(
-                      wd[(o >> 2) + 1], wd[o >> 2], o & 3);
-                  acc[j][cx + 3] = __builtin_amdgcn_sad_u8(ref, wnd, acc[j][cx + 3]);
-                }
-              }
-            }
-            for (int j = 0; j < 2; ++j) {
-              for (int k = 0; k < 8; ++k) mx[j][k] = max(mx[j][k], int(acc[j][k]));
-            }
-          }
-          for (int j = 0; j < 2; ++j) {
-            for (int k = 0; k < 8; ++k) msad[j][(cy + 3) * 8 + k] = mx[j][k];
+      // Reference patch rows of pixel j: guide bytes c0+5+j .. c0+8+j of
+      // rows r+5 .. r+7 (the guide tile has a 6-pixel border).
+      uint32_t ref[3][3][4];
+      for (int ch = 0; ch < 3; ++ch) {
+        for (int iy = 0; iy < 3; ++iy) {
+          const uint8_t* row = s_g[ch] + (r + 5 + iy) * kGP + c0 + 4;
+          const uint32_t d0 = LdsDword(row), d1 = LdsDword(row + 4);
+          for (int j = 0; j < 4; ++j) {
+            ref[ch][iy][j] = j == 3 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, j + 1);
           }
         }
-        // weighted sums in the reference's sequential order
-        float ws[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-        float sw[2] = {0.f, 0.f};
-        for (int cy = -3; cy <= 3; ++cy) {
-          for (int cx = -3; cx <= 4; ++cx) {
-            for (int j = 0; j < 2; ++j) {
-              float w = WeightOf(msad[j][(cy + 3) * 8 + cx + 3], mul);
-              w = w >= 0.05f ? w : 0.0f;
-              const int col = c0 + jp + j + cx + 7;
-              const int row = r + cy + 7;
-              for (int c = 0; c < 3; ++c) {
-                ws[j][c] = ws[j][c] + s_in[c][row * kInP + col] * w;
-              }
-              sw[j] = sw[j] + w;
+      }
+      float ws[4][3], sw[4];
+      for (int j = 0; j < 4; ++j) {
+        ws[j][0] = ws[j][1] = ws[j][2] = 0.0f;
+        sw[j] = 0.0f;
+      }
+      for (int cy = -3; cy <= 3; ++cy) {
+        // msad[j][w]: packed u16 SADs of pixel j, words w = cx pairs
+        // (-3,-2), (-1,0), (1,2), (3,4).
+        uint32_t msad[4][4];
+        for (int ch = 0; ch < 3; ++ch) {
+          uint64_t acc[4][2];
+          for (int j = 0; j < 4; ++j) acc[j][0] = acc[j][1] = 0;
+          for (int iy = 0; iy < 3; ++iy) {
+            const uint8_t* row = s_g[ch] + (r + 5 + cy + iy) * kGP + c0;
+            const uint32_t d[5] = {LdsDword(row), LdsDword(row + 4),
+                                   LdsDword(row + 8), LdsDword(row + 12),
+                                   LdsDword(row + 16)};
+            for (int j = 0; j < 4; ++j) {
+              // windows of neighbours cx = -3..0 start at byte c0+2+j,
+              // those of cx = 1..4 at byte c0+6+j
+              const int o = 2 + j;
+              const uint32_t a0 = __builtin_amdgcn_alignbyte(
+                  d[(o >> 2) + 1], d[o >> 2], o & 3);
+              const uint32_t a1 = __builtin_amdgcn_alignbyte(
+                  d[(o >> 2) + 2], d[(o >> 2) + 1], o & 3);
+              const uint32_t a2 = __builtin_amdgcn_alignbyte(
+                  d[(o >> 2) + 3], d[(o >> 2) + 2], o & 3);
+              const uint64_t lo = uint64_t(a0) | (uint64_t(a1) << 32);
+              const uint64_t hi = uint64_t(a1) | (uint64_t(a2) << 32);
+              acc[j][0] = __builtin_amdgcn_qsad_pk_u16_u8(lo, ref[ch][iy][j],
+                                                          acc[j][0]);
+              acc[j][1] = __builtin_amdgcn_qsad_pk_u16_u8(hi, ref[ch][iy][j],
+                                                          acc[j][1]);
+            }
+          }
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t w4[4] = {uint32_t(acc[j][0]),
+                                    uint32_t(acc[j][0] >> 32),
+                                    uint32_t(acc[j][1]),
+                                    uint32_t(acc[j][1] >> 32)};
+            for (int w = 0; w < 4; ++w) {
+              msad[j][w] = ch == 0 ? w4[w] : PkMaxU16(msad[j][w], w4[w]);
             }
           }
         }
-        for (int j = 0; j < 2; ++j) {
-          for (int c = 0; c < 3; ++c) {
-            s_out[c][r * 32 + c0 + jp + j] = ws[j][c] / sw[j];
+        // the input row of this cy, columns c0-3 .. c0+7 (+1 pad)
+        float v[3][12];
+        for (int c = 0; c < 3; ++c) {
+          const float4* src = reinterpret_cast<const float4*>(
+              s_in[c] + (r + 7 + cy) * kInP + c0 + 4);
+          for (int q = 0; q < 3; ++q) {
+            const float4 t = src[q];
+            v[c][4 * q] = t.x;
+            v[c][4 * q + 1] = t.y;
+            v[c][4 * q + 2] = t.z;
+            v[c][4 * q + 3] = t.w;
           }
+        }
+        for (int cx = -3; cx <= 4; ++cx) {
+          const int w = (cx + 3) >> 1, hi = (cx + 3) & 1;
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t word = PkMinU16(msad[j][w], 0x01FB01FBu);
+            const uint32_t sad = hi ? (word >> 16) : (word & 0xFFFFu);
+            float wt = WeightOf(int(sad), mul);
+            wt = wt >= 0.05f ? wt : 0.0f;
+            for (int c = 0; c < 3; ++c) {
+              ws[j][c] = ws[j][c] + v[c][j + cx + 3] * wt;
+            }
+            sw[j] = sw[j] + wt;
+          }
+        }
+      }
+      for (int j = 0; j < 4; ++j) {
+        for (int c = 0; c < 3; ++c) {
+          s_out[c][r * 32 + c0 + j] = ws[j][c] / sw[j];
         }
       }
     }
