@@ -173,6 +173,18 @@ enum {
   PIKGPU_NUM_STAGES = 10
 };
 
+/* The library dequantisation tables, i.e. what DequantMatrices::Compute
+ * (pik/quant_weights.cc:607-926) produces when the bitstream signals the
+ * predefined tables, in the layout of pikgpu_frame's dequant_table /
+ * dequant_offsets (one table set: 21 offsets, 4800 floats). Host-only.
+ * Writes at most table_cap floats and offsets_cap offsets (either pointer
+ * may be NULL to query the sizes) and returns PIKGPU_OK. */
+PIKGPU_API int pikgpu_default_dequant_tables(float* table, size_t table_cap,
+                                             uint32_t* offsets,
+                                             size_t offsets_cap,
+                                             size_t* table_size,
+                                             size_t* num_offsets);
+
 /* Replaces the ownership of FrameDecCache/GroupDecCache: creates a context on
  * HIP device `device`. Returns NULL on failure. */
 PIKGPU_API pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config);

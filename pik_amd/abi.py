@@ -127,6 +127,9 @@ def load():
                                         c_float_p, ctypes.c_int]
     lib.pikgpu_stream.argtypes = [vp]
     lib.pikgpu_stream.restype = vp
+    lib.pikgpu_default_dequant_tables.argtypes = [
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     lib.pikgpu_validate_frame.argtypes = [ctypes.POINTER(FrameC),
                                           ctypes.c_char_p, ctypes.c_size_t]
     _lib = lib
@@ -138,4 +141,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_frame_upload", "pikgpu_run", "pikgpu_sync", "pikgpu_download",
     "pikgpu_reconstruct", "pikgpu_stage_size", "pikgpu_stage_dump",
     "pikgpu_last_timings", "pikgpu_stream", "pikgpu_validate_frame",
+    "pikgpu_default_dequant_tables",
 ]

@@ -12,6 +12,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -40,7 +42,8 @@ struct pikgpu_ctx {
   DeviceConsts* consts = nullptr;
   HostTables host;
   // device buffers (grown on demand)
-  DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots;
+  DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
+      dotbins;
   DevBuf dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
       dots_img, gab, out, mm_partial, mm, ar_keep;
   bool uploaded = false;
@@ -88,6 +91,7 @@ void FreeAll(pikgpu_ctx* ctx) {
   DevBuf* all[] = {&ctx->ac,      &ctx->dc16,   &ctx->qf,      &ctx->acs,
                    &ctx->lut,     &ctx->dcf,    &ctx->ytox,    &ctx->ytob,
                    &ctx->dqmap,   &ctx->table,  &ctx->offsets, &ctx->dots,
+                   &ctx->dotbins,
                    &ctx->dcq,     &ctx->dcv,    &ctx->llf,     &ctx->tmp2x2,
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
@@ -319,7 +323,84 @@ int ValidateFrame(const pikgpu_frame* f, std::string* err) {
   return PIKGPU_OK;
 }
 
+// Bins the dots into 32x32 tiles of the padded image. Every dot goes to
+// each tile its footprint may touch (transformed dots: the bounding box of
+// the splat's four corners, widened by two pixels), and each tile's list
+// keeps the order BlockDictionary::Apply uses (pik/block_dictionary.cc:
+// 346-482): placements copied as-is first, then transformed ones, each in
+// list order. Output: tile ids, CSR starts, dot indices.
+void BinDots(const pikgpu_frame* f, int px, int py, std::vector<int32_t>* ids,
+             std::vector<int32_t>* starts, std::vector<int32_t>* list) {
+  const int tiles_x = int(DivCeil(px, 32)), tiles_y = int(DivCeil(py, 32));
+  std::vector<std::pair<int64_t, int32_t>> entries;  // (tile<<32 | rank, dot)
+  int64_t rank = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (size_t i = 0; i < f->num_dots; ++i) {
+      const pikgpu_dot& d = f->dots[i];
+      if ((d.transform != 0) != (pass == 1)) continue;
+      double x0, x1, y0, y1;
+      if (!d.transform) {
+        x0 = d.x;
+        y0 = d.y;
+        x1 = double(d.x) + d.xsize;
+        y1 = double(d.y) + d.ysize;
+      } else {
+        const double norm = std::sqrt(double(d.dx) * d.dx + double(d.dy) * d.dy);
+        const double half = d.width * 0.5;
+        const double ex = d.dy / norm * half, ey = -d.dx / norm * half;
+        const double cx[4] = {0.0, double(d.dx), ex, d.dx + ex};
+        const double cy[4] = {0.0, double(d.dy), ey, d.dy + ey};
+        x0 = x1 = d.x;
+        y0 = y1 = d.y;
+        for (int k = 0; k < 4; ++k) {
+          x0 = std::min(x0, d.x + cx[k]);
+          x1 = std::max(x1, d.x + cx[k]);
+          y0 = std::min(y0, d.y + cy[k]);
+          y1 = std::max(y1, d.y + cy[k]);
+        }
+        x0 -= 2;
+        y0 -= 2;
+        x1 += 3;
+        y1 += 3;
+      }
+      const int tx0 = int(std::max(0.0, std::floor(x0 / 32)));
+      const int ty0 = int(std::max(0.0, std::floor(y0 / 32)));
+      const int tx1 = int(std::min(double(tiles_x - 1), std::floor((x1 - 1) / 32)));
+      const int ty1 = int(std::min(double(tiles_y - 1), std::floor((y1 - 1) / 32)));
+      for (int ty = ty0; ty <= ty1; ++ty) {
+        for (int tx = tx0; tx <= tx1; ++tx) {
+          entries.emplace_back((int64_t(ty * tiles_x + tx) << 32) | rank,
+                               int32_t(i));
+        }
+      }
+      ++rank;
+    }
+  }
+  std::sort(entries.begin(), entries.end());
+  ids->clear();
+  starts->clear();
+  list->clear();
+  for (size_t k = 0; k < entries.size(); ++k) {
+    const int32_t tile = int32_t(entries[k].first >> 32);
+    if (ids->empty() || ids->back() != tile) {
+      ids->push_back(tile);
+      starts->push_back(int32_t(k));
+    }
+    list->push_back(entries[k].second);
+  }
+  starts->push_back(int32_t(entries.size()));
+}
+
 }  // namespace
+
+int pikgpu_default_dequant_tables(float* table, size_t table_cap,
+                                  uint32_t* offsets, size_t offsets_cap,
+                                  size_t* table_size, size_t* num_offsets) {
+  const size_t n =
+      DefaultDequantTables(table, table_cap, offsets, offsets_cap, num_offsets);
+  if (table_size) *table_size = n;
+  return PIKGPU_OK;
+}
 
 int pikgpu_validate_frame(const pikgpu_frame* frame, char* msg,
                           size_t msg_size) {
@@ -451,9 +532,22 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->offsets.p, f->dequant_offsets,
                               f->num_dequant_offsets * 4,
                               hipMemcpyHostToDevice, ctx->stream));
+  std::vector<int32_t> dot_ids, dot_starts, dot_list;
   if (f->num_dots) {
     HIP_TRY(ctx, hipMemcpyAsync(ctx->dots.p, f->dots,
                                 f->num_dots * sizeof(pikgpu_dot),
+                                hipMemcpyHostToDevice, ctx->stream));
+    BinDots(f, int(bx * 8), int(by * 8), &dot_ids, &dot_starts, &dot_list);
+    const size_t nbin = dot_ids.size() + dot_starts.size() + dot_list.size();
+    if ((rc = Grow(ctx, &ctx->dotbins, nbin * 4))) return rc;
+    int32_t* dst = static_cast<int32_t*>(ctx->dotbins.p);
+    HIP_TRY(ctx, hipMemcpyAsync(dst, dot_ids.data(), dot_ids.size() * 4,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dst + dot_ids.size(), dot_starts.data(),
+                                dot_starts.size() * 4, hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dst + dot_ids.size() + dot_starts.size(),
+                                dot_list.data(), dot_list.size() * 4,
                                 hipMemcpyHostToDevice, ctx->stream));
   }
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -473,6 +567,14 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   B.offsets = static_cast<const uint32_t*>(ctx->offsets.p);
   B.num_tables = num_tables;
   B.dots = static_cast<const pikgpu_dot*>(ctx->dots.p);
+  P.dot_tiles = int(dot_ids.size());
+  P.dot_tiles_x = int(DivCeil(bx * 8, 32));
+  if (P.dot_tiles) {
+    const int32_t* bins = static_cast<const int32_t*>(ctx->dotbins.p);
+    B.dot_tile_ids = bins;
+    B.dot_tile_start = bins + dot_ids.size();
+    B.dot_list = bins + dot_ids.size() + dot_starts.size();
+  }
   B.dcq = static_cast<float*>(ctx->dcq.p);
   B.dc = static_cast<float*>(ctx->dcv.p);
   B.llf = static_cast<float*>(ctx->llf.p);
@@ -489,6 +591,14 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   ctx->num_tables = num_tables;
   ctx->uploaded = true;
   return PIKGPU_OK;
+}
+
+// Grid of KMinMaxPartial: 256-column strips, row ranges of >= 8 rows, at most
+// 1024 workgroups (the size of the partials buffer).
+static dim3 MinMaxGrid(int w, int h) {
+  const int nx = int(DivCeil(w, 256));
+  const int ny = std::max(1, std::min(int(DivCeil(h, 8)), 1024 / nx));
+  return dim3(nx, ny);
 }
 
 static int EnsureStageBuffers(pikgpu_ctx* ctx) {
@@ -536,10 +646,10 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
     dci.H = int(DivCeil(P.by, 8) * 8);
     dci.vw = P.bx;
     dci.vh = P.by;
-    const int nwg = int(std::min<size_t>(1024, DivCeil(size_t(dci.W) * dci.H, 256)));
-    KMinMaxPartial<<<nwg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
-    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, nwg, B.mm);
-    dim3 grid(DivCeil(dci.W, 32), DivCeil(dci.H, 32));
+    const dim3 mmg = MinMaxGrid(dci.W, dci.H);
+    KMinMaxPartial<<<mmg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
+    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
+    const int grid = XcdGrid(int(DivCeil(dci.W, 32) * DivCeil(dci.H, 32)));
     KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
                                size_t(nblk));
   }
@@ -563,12 +673,12 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
       } else {
         KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
       }
-      KPred2x2Interior<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B, K);
+      KPred2x2Interior<<<dim3(DivCeil(nblk, 256), 3), 256, 0, st>>>(P, B, K);
     }
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   {
-    dim3 grid(DivCeil(P.bx, 4), DivCeil(P.by, 4));
+    const int grid = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
     KBlocks<<<grid, 256, 0, st>>>(P, B, K);
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
@@ -582,11 +692,11 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
     ii.vw = P.px;
     ii.vh = P.py;
     const int gab = P.gaborish && !P.use_sharpened;
-    const int nwg = int(std::min<size_t>(1024, DivCeil(size_t(P.px) * P.py, 256)));
-    KMinMaxPartial<<<nwg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
-    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, nwg, B.mm);
+    const dim3 mmg = MinMaxGrid(ii.W, ii.H);
+    KMinMaxPartial<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
+    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
     HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
-    dim3 grid(DivCeil(P.px, 32), DivCeil(P.py, 32));
+    const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
     KEpf<<<grid, 256, 0, st>>>(ii, gab, 0, P, B, K, B.mm, B.ar, B.ipitch,
                                iplane);
   } else {
@@ -608,10 +718,10 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
     HIP_TRY(ctx, hipMemcpyAsync(B.dots_img, B.ar, 3 * iplane * 4,
                                 hipMemcpyDeviceToDevice, st));
   }
-  if (P.num_dots) KDots<<<1, 256, 0, st>>>(P, B);
+  if (P.dot_tiles) KDots<<<P.dot_tiles, 256, 0, st>>>(P, B);
   HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
   {
-    dim3 grid(DivCeil(P.px, 64), DivCeil(P.py, 4));
+    const int grid = XcdGrid(int(DivCeil(P.px, 64) * DivCeil(P.py, 4)));
     KFinal<<<grid, 256, 0, st>>>(P, B, K);
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));

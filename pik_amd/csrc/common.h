@@ -78,6 +78,8 @@ struct FrameParams {
   int ar;                    // adaptive reconstruction on
   int use_sharpened;
   int num_dots;
+  int dot_tiles;             // 32x32 tiles touched by dots
+  int dot_tiles_x;           // tiles per row (ceil(px / 32))
   float inv_global_scale;    // Quantizer::InvGlobalScale()
   float inv_quant_dc;        // Quantizer::inv_quant_dc() (decoder form)
   float mul_dc[3];           // DC dequant multipliers
@@ -96,6 +98,10 @@ struct HostTables {
 };
 
 void ComputeHostTables(HostTables* t);
+// The library dequantisation tables (quant_tables.cpp); returns the number
+// of floats, writes at most table_cap floats and offsets_cap offsets.
+size_t DefaultDequantTables(float* table, size_t table_cap, uint32_t* offsets,
+                            size_t offsets_cap, size_t* num_offsets);
 void FillScaleTables(float* tscale, float* ascale);
 void GaborishWeights(int strength, float w[3]);
 void LfBlurWeights(float w[3]);

@@ -248,16 +248,22 @@ template <int n>
 PIKGPU_HD void Idct2D(const float* coef, int cstride, float* pix,
                       int pstride) {
   float t[n * n];
+#pragma unroll
   for (int c = 0; c < n; ++c) {
     float v[n], w[n];
+#pragma unroll
     for (int r = 0; r < n; ++r) v[r] = coef[r * cstride + c];
     Idct<n>(v, w);
+#pragma unroll
     for (int x = 0; x < n; ++x) t[c * n + x] = w[x];
   }
+#pragma unroll
   for (int x = 0; x < n; ++x) {
     float v[n], w[n];
+#pragma unroll
     for (int c = 0; c < n; ++c) v[c] = t[c * n + x];
     Idct<n>(v, w);
+#pragma unroll
     for (int y = 0; y < n; ++y) pix[y * pstride + x] = w[y];
   }
 }
@@ -266,17 +272,23 @@ template <int n>
 PIKGPU_HD void Dct2D(const float* pix, int pstride, float* coef,
                      int cstride) {
   float t[n * n];
+#pragma unroll
   for (int x = 0; x < n; ++x) {
     float v[n], w[n];
+#pragma unroll
     for (int y = 0; y < n; ++y) v[y] = pix[y * pstride + x];
     Dct<n>(v, w);
+#pragma unroll
     for (int k = 0; k < n; ++k) t[x * n + k] = w[k];
   }
   const float scale = 1.0f / (n * n);
+#pragma unroll
   for (int k = 0; k < n; ++k) {
     float v[n], w[n];
+#pragma unroll
     for (int x = 0; x < n; ++x) v[x] = t[x * n + k];
     Dct<n>(v, w);
+#pragma unroll
     for (int j = 0; j < n; ++j) coef[j * cstride + k] = w[j] * scale;
   }
 }

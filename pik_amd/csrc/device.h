@@ -40,6 +40,11 @@ struct FrameBuffers {
   const uint32_t* offsets;
   int num_tables;
   const pikgpu_dot* dots;
+  // dots binned by 32x32 tile: tile ids, CSR starts (dot_tiles + 1) and the
+  // dot indices of each tile in application order
+  const int32_t* dot_tile_ids;
+  const int32_t* dot_tile_start;
+  const int32_t* dot_list;
   // intermediates
   float* dcq;             // 3 planes by x bx (DequantDC)
   float* dc;              // 3 planes by x bx (after DC AR)

@@ -123,22 +123,53 @@ __device__ inline uint32_t LdsDword(const uint8_t* p) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Partial per-channel min/max of the guide over a W x H image; one entry of
-// 6 floats per workgroup.
+// Partial per-channel min/max of the guide over a W x H image (MinMax,
+// pik/epf_target.cc:975-1004); one entry of 6 floats per workgroup. Thread
+// x walks column x of its row range with the 3x3 window of the Gaborish in
+// registers (Symmetric3's operation order, pik/convolve.h:605-621).
 __global__ void __launch_bounds__(256)
     KMinMaxPartial(Img in, int gaborish, FrameParams P, float* partial) {
   __shared__ float s_mm[6][256];
   const int tid = threadIdx.x;
+  const int x = blockIdx.x * 256 + tid;
+  const int rows = (in.H + int(gridDim.y) - 1) / int(gridDim.y);
+  const int ya = int(blockIdx.y) * rows, yb = min(in.H, ya + rows);
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
   float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-  const size_t n = size_t(in.W) * in.H;
-  for (size_t i = size_t(blockIdx.x) * 256 + tid; i < n;
-       i += size_t(gridDim.x) * 256) {
-    const int x = int(i % in.W), y = int(i / in.W);
+  if (x < in.W) {
+    const int xm = Mirror(x - 1, in.W), xp = Mirror(x + 1, in.W);
     for (int c = 0; c < 3; ++c) {
-      const float v = GuideAt(in, gaborish, P.gab_w, c, x, y);
-      mn[c] = fminf(mn[c], v);
-      mx[c] = fmaxf(mx[c], v);
+      if (!gaborish) {
+        for (int y = ya; y < yb; ++y) {
+          const float v = ImgAt(in, c, x, y);
+          mn[c] = fminf(mn[c], v);
+          mx[c] = fmaxf(mx[c], v);
+        }
+        continue;
+      }
+      const int y0 = Mirror(ya - 1, in.H);
+      float tl = ImgAt(in, c, xm, y0), tc = ImgAt(in, c, x, y0),
+            tr = ImgAt(in, c, xp, y0);
+      float ml = ImgAt(in, c, xm, ya), mc = ImgAt(in, c, x, ya),
+            mr = ImgAt(in, c, xp, ya);
+      for (int y = ya; y < yb; ++y) {
+        const int yn = Mirror(y + 1, in.H);
+        const float bl = ImgAt(in, c, xm, yn), bc = ImgAt(in, c, x, yn),
+                    br = ImgAt(in, c, xp, yn);
+        const float sum_tb = tc + bc;
+        const float mul0 = mc * P.gab_w[0];
+        const float sum_lr = ml + mr;
+        const float x1 = sum_tb + sum_lr;
+        const float mul1 = x1 * P.gab_w[1] + mul0;
+        const float sum_t2 = tl + tr;
+        const float sum_b2 = bl + br;
+        const float x2 = sum_t2 + sum_b2;
+        const float v = x2 * P.gab_w[2] + mul1;
+        mn[c] = fminf(mn[c], v);
+        mx[c] = fmaxf(mx[c], v);
+        tl = ml; tc = mc; tr = mr;
+        ml = bl; mc = bc; mr = br;
+      }
     }
   }
   for (int c = 0; c < 3; ++c) {
@@ -155,7 +186,9 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  if (tid < 6) partial[blockIdx.x * 6 + tid] = s_mm[tid][0];
+  if (tid < 6) {
+    partial[(blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = s_mm[tid][0];
+  }
 }
 
 // Reduces the partials to mm[0..5] = min[3], max[3]; mm[6] = 255 / range
@@ -212,12 +245,16 @@ __global__ void __launch_bounds__(256)
   __shared__ float s_out[3][1024];
   __shared__ float s_coef[3][1024];
   __shared__ float s_ratio[1024];
-  __shared__ float s_tmp[1024];
+  __shared__ float s_tmp[3][1024];
   __shared__ uint8_t s_acs[16];
   __shared__ int s_sigma[16];
+  __shared__ TileTasks s_tasks;
 
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int tiles_x = (in.W + 31) / 32;
+  const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
+  if (tile < 0) return;
+  const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
   const float all_min = mm[7];
   const float gmul = mm[6];
 
@@ -436,7 +473,7 @@ __global__ void __launch_bounds__(256)
 
   // 5. AR: clamp the DCT of (filtered - original) to half the quantization
   //    interval with one ratio per coefficient shared by X, Y, B.
-  const TileXform X = MakeXform(s_acs, tid);
+  BuildTasks(s_acs, false, &s_tasks);
   for (int c = 0; c < 3; ++c) {
     for (int j = 0; j < 4; ++j) {
       s_out[c][r * 32 + c0 + j] =
@@ -444,7 +481,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   __syncthreads();
-  for (int c = 0; c < 3; ++c) TileForward(K, X, s_out[c], s_coef[c], s_tmp, false);
+  TileForward3(s_tasks, &s_out[0][0], &s_coef[0][0], &s_tmp[0][0]);
   {
     // coefficient entries owned by this thread
     const int crow = tid >> 6, ccol = (tid & 63) << 2;
@@ -502,9 +539,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   __syncthreads();
-  for (int c = 0; c < 3; ++c) {
-    TileInverse(K, X, s_coef[c], s_out[c], s_tmp);
-  }
+  TileInverse3(s_tasks, &s_coef[0][0], &s_out[0][0], &s_tmp[0][0]);
   const int y = y0 + r;
   if (y < in.H) {
     for (int c = 0; c < 3; ++c) {
@@ -520,121 +555,129 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// Dot dictionary: placements applied in order (copied ones first, then the
-// scaled/rotated ones), one workgroup, in place on B.ar.
+// Dot patches (BlockDictionary::Apply<true>, pik/block_dictionary.cc:
+// 346-482) on the AR output, in place. Workgroup t owns the 32x32 tile
+// dot_tile_ids[t]; each thread owns four pixels of it and adds the tile's
+// dots (listed in the reference's application order) to them one dot after
+// the other, so every pixel receives its additions in the reference's order.
 __global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
   __shared__ float s_blk[3][21 * 21];
   const int tid = threadIdx.x;
+  const int tile = B.dot_tile_ids[blockIdx.x];
+  const int tx0 = (tile % P.dot_tiles_x) * 32, ty0 = (tile / P.dot_tiles_x) * 32;
+  const int px = tx0 + (tid & 31);
+  const int begin = B.dot_tile_start[blockIdx.x];
+  const int end = B.dot_tile_start[blockIdx.x + 1];
   const float kBlockInvScale = 1.0f / 3.5f;
   const float kXybRadius[3] = {0.0301006f, 0.4512295f, 0.4716444f};
   const size_t plane = size_t(P.py) * B.ipitch;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int di = 0; di < P.num_dots; ++di) {
-      const pikgpu_dot d = B.dots[di];
-      if ((d.transform != 0) != (pass == 1)) continue;
-      const int xs = d.xsize, ys = d.ysize;
-      if (pass == 0) {
-        for (int i = tid; i < 3 * xs * ys; i += 256) {
-          const int c = i / (xs * ys), rem = i % (xs * ys);
-          const int iy = rem / xs, ix = rem % xs;
-          const int x = d.x + ix, y = d.y + iy;
-          if (x < P.px && y < P.py) {
-            const float val =
-                kBlockInvScale * kXybRadius[c] * float(d.pixels[c][iy * xs + ix]);
-            float* p = B.ar + c * plane + size_t(y) * B.ipitch + x;
-            *p = *p + val;
-          }
+  for (int li = begin; li < end; ++li) {
+    const pikgpu_dot& d = B.dots[B.dot_list[li]];  // read in place
+    const int xs = d.xsize, ys = d.ysize;
+    if (!d.transform) {
+      for (int k = 0; k < 4; ++k) {
+        const int py = ty0 + (tid >> 5) + 8 * k;
+        const int ix = px - int(d.x), iy = py - int(d.y);
+        if (ix < 0 || iy < 0 || ix >= xs || iy >= ys || px >= P.px ||
+            py >= P.py) {
+          continue;
         }
-        __syncthreads();
-        continue;
-      }
-      // 2x nearest-neighbour upsampling with a 2x2 box (block_dictionary.cc:
-      // 382-401), then a bilinear splat through the inverse affine map.
-      const int bw = 2 * xs + 5;
-      for (int i = tid; i < 3 * 21 * 21; i += 256) (&s_blk[0][0])[i] = 0.0f;
-      __syncthreads();
-      for (int i = tid; i < 3 * (2 * ys + 1) * (2 * xs + 1); i += 256) {
-        const int c = i / ((2 * ys + 1) * (2 * xs + 1));
-        const int rem = i % ((2 * ys + 1) * (2 * xs + 1));
-        const int iy = rem / (2 * xs + 1), ix = rem % (2 * xs + 1);
-        float val = 0.0f;
-        for (int dy = -1; dy < 1; dy++) {
-          for (int dx = -1; dx < 1; dx++) {
-            int sy = (iy + dy) / 2;
-            if (sy < 0) sy = 0;
-            if (sy >= ys) sy = ys - 1;
-            int sx = (ix + dx) / 2;
-            if (sx < 0) sx = 0;
-            if (sx >= xs) sx = xs - 1;
-            val += float(d.pixels[c][sy * xs + sx]);
-          }
-        }
-        s_blk[c][(iy + 2) * bw + (ix + 2)] = val * 0.25f;
-      }
-      __syncthreads();
-      const float by00 = float(d.y);
-      const float bx00 = float(d.x);
-      const float bx01 = bx00 + float(d.dx);
-      const float by01 = by00 + float(d.dy);
-      const float xnorm =
-          float(sqrt(double(int64_t(d.dx) * d.dx + int64_t(d.dy) * d.dy)));
-      const float ynorm = float(d.width) * 0.5f;
-      const float invxnorm = 1.0f / xnorm;
-      const float invynorm = 1.0f / ynorm;
-      const float deltax = float(d.dy) * invxnorm * ynorm;
-      const float deltay = float(-d.dx) * invxnorm * ynorm;
-      const float bx11 = bx01 + deltax;
-      const float by11 = by01 + deltay;
-      const float bx10 = bx00 + deltax;
-      const float by10 = by00 + deltay;
-      const float inv_det = 1.0f / (float(d.dx) * deltay - float(d.dy) * deltax);
-      const float it0 = float(xs) * deltay * inv_det;
-      const float it1 = -float(xs) * deltax * inv_det;
-      const float it2 = -float(ys) * float(d.dy) * inv_det;
-      const float it3 =
-          float(uint64_t(ys) * uint64_t(int64_t(d.dx))) * inv_det;
-      int64_t min_x = int64_t(fminf(fminf(bx00, bx01), fminf(bx10, bx11)));
-      int64_t min_y = int64_t(fminf(fminf(by00, by01), fminf(by10, by11)));
-      int64_t max_x = int64_t(fmaxf(fmaxf(bx00, bx01), fmaxf(bx10, bx11)) + 1);
-      int64_t max_y = int64_t(fmaxf(fmaxf(by00, by01), fmaxf(by10, by11)) + 1);
-      if (min_x < 0) min_x = 0;
-      if (min_y < 0) min_y = 0;
-      if (max_x > P.px) max_x = P.px;
-      if (max_y > P.py) max_y = P.py;
-      const float margin_x = 0.3f * invxnorm;
-      const float margin_y = 0.3f * invynorm;
-      const int w = int(max(int64_t(0), max_x - min_x));
-      const int h = int(max(int64_t(0), max_y - min_y));
-      for (int i = tid; i < w * h; i += 256) {
-        const int iy = int(min_y) + i / w, ix = int(min_x) + i % w;
-        const float x = float(ix) - bx00 + 0.5f;
-        const float y = float(iy) - by00 + 0.5f;
-        float ox = it0 * x + it1 * y;
-        float oy = it2 * x + it3 * y;
-        if (ox >= -margin_x && ox < float(xs) + margin_x && oy >= -margin_y &&
-            oy < float(ys) + margin_y) {
-          ox = 2 * ox + 2;
-          oy = 2 * oy + 2;
-          const int floorx = int(ox);
-          const int ceilx = floorx + 1;
-          const float fracx = ox - float(floorx);
-          const int floory = int(oy);
-          const int ceily = floory + 1;
-          const float fracy = oy - float(floory);
-          for (int c = 0; c < 3; ++c) {
-            const float* blk = s_blk[c];
-            const float val =
-                (fracx * fracy * blk[ceily * bw + ceilx] +
-                 fracx * (1.f - fracy) * blk[floory * bw + ceilx] +
-                 (1.f - fracx) * fracy * blk[ceily * bw + floorx] +
-                 (1.f - fracx) * (1.f - fracy) * blk[floory * bw + floorx]) *
-                kBlockInvScale * kXybRadius[c];
-            float* p = B.ar + c * plane + size_t(iy) * B.ipitch + ix;
-            *p = *p + val;
-          }
+        for (int c = 0; c < 3; ++c) {
+          const float val =
+              kBlockInvScale * kXybRadius[c] * float(d.pixels[c][iy * xs + ix]);
+          float* p = B.ar + c * plane + size_t(py) * B.ipitch + px;
+          *p = *p + val;
         }
       }
-      __syncthreads();
+      continue;
+    }
+    // 2x nearest-neighbour upsampling with a 2x2 box
+    // (pik/block_dictionary.cc:382-401), then a bilinear splat through the
+    // inverse affine map.
+    const int bw = 2 * xs + 5;
+    __syncthreads();
+    for (int i = tid; i < 3 * 21 * 21; i += 256) (&s_blk[0][0])[i] = 0.0f;
+    __syncthreads();
+    for (int i = tid; i < 3 * (2 * ys + 1) * (2 * xs + 1); i += 256) {
+      const int c = i / ((2 * ys + 1) * (2 * xs + 1));
+      const int rem = i % ((2 * ys + 1) * (2 * xs + 1));
+      const int iy = rem / (2 * xs + 1), ix = rem % (2 * xs + 1);
+      float val = 0.0f;
+      for (int dy = -1; dy < 1; dy++) {
+        for (int dx = -1; dx < 1; dx++) {
+          int sy = (iy + dy) / 2;
+          if (sy < 0) sy = 0;
+          if (sy >= ys) sy = ys - 1;
+          int sx = (ix + dx) / 2;
+          if (sx < 0) sx = 0;
+          if (sx >= xs) sx = xs - 1;
+          val += float(d.pixels[c][sy * xs + sx]);
+        }
+      }
+      s_blk[c][(iy + 2) * bw + (ix + 2)] = val * 0.25f;
+    }
+    __syncthreads();
+    // BlockPosition keeps dx, dy, width as int64 and the patch size as
+    // size_t; the expressions below keep those conversions.
+    const float by00 = float(d.y);
+    const float bx00 = float(d.x);
+    const float bx01 = bx00 + float(d.dx);
+    const float by01 = by00 + float(d.dy);
+    const float xnorm =
+        float(sqrt(double(int64_t(d.dx) * d.dx + int64_t(d.dy) * d.dy)));
+    const float ynorm = float(d.width) * 0.5f;
+    const float invxnorm = 1.0f / xnorm;
+    const float invynorm = 1.0f / ynorm;
+    const float deltax = float(d.dy) * invxnorm * ynorm;
+    const float deltay = float(-d.dx) * invxnorm * ynorm;
+    const float bx11 = bx01 + deltax;
+    const float by11 = by01 + deltay;
+    const float bx10 = bx00 + deltax;
+    const float by10 = by00 + deltay;
+    const float inv_det = 1.0f / (float(d.dx) * deltay - float(d.dy) * deltax);
+    const float it0 = float(xs) * deltay * inv_det;
+    const float it1 = -float(xs) * deltax * inv_det;
+    const float it2 = -float(ys) * float(d.dy) * inv_det;
+    const float it3 = float(uint64_t(ys) * uint64_t(int64_t(d.dx))) * inv_det;
+    int64_t min_x = int64_t(fminf(fminf(bx00, bx01), fminf(bx10, bx11)));
+    int64_t min_y = int64_t(fminf(fminf(by00, by01), fminf(by10, by11)));
+    int64_t max_x = int64_t(fmaxf(fmaxf(bx00, bx01), fmaxf(bx10, bx11)) + 1);
+    int64_t max_y = int64_t(fmaxf(fmaxf(by00, by01), fmaxf(by10, by11)) + 1);
+    if (min_x < 0) min_x = 0;
+    if (min_y < 0) min_y = 0;
+    if (max_x > P.px) max_x = P.px;
+    if (max_y > P.py) max_y = P.py;
+    const float margin_x = 0.3f * invxnorm;
+    const float margin_y = 0.3f * invynorm;
+    for (int k = 0; k < 4; ++k) {
+      const int py = ty0 + (tid >> 5) + 8 * k;
+      if (px < min_x || px >= max_x || py < min_y || py >= max_y) continue;
+      const float x = float(px) - bx00 + 0.5f;
+      const float y = float(py) - by00 + 0.5f;
+      float ox = it0 * x + it1 * y;
+      float oy = it2 * x + it3 * y;
+      if (ox >= -margin_x && ox < float(xs) + margin_x && oy >= -margin_y &&
+          oy < float(ys) + margin_y) {
+        ox = 2 * ox + 2;
+        oy = 2 * oy + 2;
+        const int floorx = int(ox);
+        const int ceilx = floorx + 1;
+        const float fracx = ox - float(floorx);
+        const int floory = int(oy);
+        const int ceily = floory + 1;
+        const float fracy = oy - float(floory);
+        for (int c = 0; c < 3; ++c) {
+          const float* blk = s_blk[c];
+          const float val =
+              (fracx * fracy * blk[ceily * bw + ceilx] +
+               fracx * (1.f - fracy) * blk[floory * bw + ceilx] +
+               (1.f - fracx) * fracy * blk[ceily * bw + floorx] +
+               (1.f - fracx) * (1.f - fracy) * blk[floory * bw + floorx]) *
+              kBlockInvScale * kXybRadius[c];
+          float* p = B.ar + c * plane + size_t(py) * B.ipitch + px;
+          *p = *p + val;
+        }
+      }
     }
   }
 }
@@ -644,8 +687,11 @@ __global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
 // xsize x ysize output (pik/compressed_image.cc:1164-1179).
 __global__ void __launch_bounds__(256)
     KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int tiles_x = (P.px + 63) / 64;
+  const int tile = XcdTile(blockIdx.x, tiles_x * ((P.py + 3) / 4));
+  if (tile < 0) return;
+  const int x = (tile % tiles_x) * 64 + (threadIdx.x & 63);
+  const int y = (tile / tiles_x) * 4 + (threadIdx.x >> 6);
   if (x >= P.px || y >= P.py) return;
   const size_t plane = size_t(P.py) * B.ipitch;
   float v[3];

@@ -17,16 +17,6 @@ namespace pikgpu {
 // P = B^T F^T B with the float basis B of ComputeBasis: the coefficient row
 // index is the horizontal frequency (pik/dct.h:351-507).
 
-__device__ inline int BasisOffsetD(int n) {
-  return n == 2 ? 0 : (n == 4 ? 8 : (n == 8 ? 40 : (n == 16 ? 168 : 680)));
-}
-__device__ inline const float* BasisB(const DeviceConsts* K, int n) {
-  return K->basis + BasisOffsetD(n);
-}
-__device__ inline const float* BasisInv(const DeviceConsts* K, int n) {
-  return K->basis + BasisOffsetD(n) + n * n;
-}
-
 // In-thread ts-IDCT / ts-DCT (with the 1/N^2 scaling of ScaleToBlock) of an
 // n x n block, n = 2, 4, 8, as the reinterpreting transforms use them
 // (pik/ac_strategy.cc:272-315); n == 2 is the reference's explicit 2-point
@@ -127,44 +117,96 @@ __device__ inline float DequantCoef(const FrameParams& P, const FrameBuffers& B,
   return f * yval + d;
 }
 
-// ---------------------------------------------------------------------------
-// Cooperative transforms of one channel of a 32x32 tile (256 threads).
-// `acs_t` holds the 16 strategy bytes of the tile's 8x8 blocks (0xFF outside
-// the image). Coefficients live in the reference's storage layout: 4 block
-// rows x 256 floats. The 2-D transforms run as the reference's two column
-// passes with a transpose in between (pik/dct.h:370-507): every line is a
-// 1-D butterfly of dct1d.h computed by one thread. Thread t works for the
-// 8x8 block t / 16 of the tile; only threads of the first block of a
-// transform have lines to do (16 per block: a DCT8 has 8 lines, a DCT4X4 4
-// lines in each of its 4 sub-blocks, DCT16 16 and DCT32 32, two per thread).
-
-struct TileXform {
-  int raw, s, N;   // strategy, covered blocks, transform size
-  int oby, obx;    // origin block of this thread's transform in the tile
-  int lane;        // thread index within its 8x8 block (0..15)
-  bool first;      // this thread's block is the transform's first block
-  bool valid;      // block inside the image
-  // pixel-quad ownership for elementwise work: row r, columns c0..c0+3
-  int r, c0;
+// Per-storage-block dequantisation parameters of plane c: the dequant
+// matrix row of the block's strategy, position and table, the scaled
+// 1 / quant field and the colour-correlation factor
+// (Dequant::DoAC, pik/compressed_image.cc:636-666).
+struct BlockDq {
+  const float* m;
+  float scaled;
+  float cc;
 };
 
-__device__ inline TileXform MakeXform(const uint8_t* acs_t, int tid) {
-  TileXform X;
-  X.r = tid >> 3;
-  X.c0 = (tid & 7) << 2;
-  const int blk = tid >> 4;
-  X.lane = tid & 15;
-  const int lby = blk >> 2, lbx = blk & 3;
-  const uint8_t a = acs_t[blk];
-  X.valid = a != 0xFF;
-  X.raw = X.valid ? (a >> 4) : 0;
-  const int idx = X.valid ? (a & 15) : 0;
-  X.s = AcsCovered(X.raw);
-  X.N = 8 * X.s;
-  X.oby = lby - idx / X.s;
-  X.obx = lbx - idx % X.s;
-  X.first = X.valid && idx == 0;
-  return X;
+__device__ inline BlockDq MakeBlockDq(const FrameParams& P,
+                                      const FrameBuffers& B, int c, int row,
+                                      int bxa) {
+  const int blk = row * P.bx + bxa;
+  const uint8_t a = B.acs[blk];
+  const int raw = a >> 4, idx = a & 15;
+  const int qf = B.qf[blk];
+  const int tile = (row >> 3) * P.tx + (bxa >> 3);
+  const int qt = B.dqmap[B.dcf[tile] * 256 + qf - 1];
+  BlockDq d;
+  d.m = B.table + B.offsets[(qt * kNumQuantKinds + AcsQuantKind(raw)) * 3 + c] +
+        idx * 64;
+  d.scaled = qf == 0 ? 1E10f : P.inv_global_scale / float(qf);
+  d.cc = c == 0 ? float(B.ytox[tile] - 128) * 0.00390625f
+                : (c == 2 ? float(B.ytob[tile]) * 0.0078125f : 0.0f);
+  return d;
+}
+
+// Coefficient k of a block from its quantized value q, in DequantCoef's
+// operation order.
+__device__ inline float DequantWith(const BlockDq& d, int c, int16_t q, int k,
+                                    float yval) {
+  const float mul = d.m[k] * d.scaled;
+  const float v = AdjustQuantBias(c, float(q)) * mul;
+  return c == 1 ? v : d.cc * yval + v;
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative transforms of the three channels of a 32x32 tile (256
+// threads). Coefficients live in the reference's storage layout, 4 block
+// rows x 256 floats per channel; pixels and the transpose buffer are 32x32
+// per channel. The 2-D transforms are the reference's two column passes
+// with a transpose in between (pik/dct.h:370-507); each line is a 1-D
+// butterfly of dct1d.h computed by one thread. Every line of every
+// transform of every channel is a task, and the tasks are spread over all
+// 256 threads: a DCT8 block has 8 line tasks, DCT16 16, DCT32 32, DCT4X4 16
+// (4 lines in each 4x4 sub-block), DCT2X2 and IDENTITY one direct task.
+
+struct TileTasks {
+  int off[17];      // first task of each block; off[16] = tasks per channel
+  uint8_t raw[16];  // strategy of the transform starting at the block, or
+                    // 0xFF (not a first block, outside, or masked out)
+};
+
+__device__ inline int TaskCount(int raw) {
+  if (raw == kDCT || raw == kDCTNoHF) return 8;
+  if (raw == kDCT16) return 16;
+  if (raw == kDCT32) return 32;
+  if (AcsIsDCT4(raw)) return 16;
+  return 1;
+}
+
+// All threads call; `acs_t` holds the tile's 16 strategy bytes (0xFF
+// outside the image). hf_only keeps only strategies with HF prediction.
+__device__ inline void BuildTasks(const uint8_t* acs_t, bool hf_only,
+                                  TileTasks* T) {
+  if (threadIdx.x == 0) {
+    int o = 0;
+    for (int b = 0; b < 16; ++b) {
+      T->off[b] = o;
+      const uint8_t a = acs_t[b];
+      uint8_t raw = 0xFF;
+      if (a != 0xFF && (a & 15) == 0 && (!hf_only || AcsPredictHF(a >> 4))) {
+        raw = a >> 4;
+      }
+      T->raw[b] = raw;
+      o += raw == 0xFF ? 0 : TaskCount(raw);
+    }
+    T->off[16] = o;
+  }
+  __syncthreads();
+}
+
+// Block of task t (binary search over the offsets).
+__device__ inline int TaskBlock(const TileTasks& T, int t) {
+  int lo = 0;
+  for (int step = 8; step > 0; step >>= 1) {
+    if (T.off[lo + step] <= t) lo += step;
+  }
+  return lo;
 }
 
 // Coefficient storage index of F[i][j] (row-major N x N, flat i*N + j) of a
@@ -211,128 +253,152 @@ __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
   }
 }
 
-// coef (4 x 256) -> pix (32 x 32). All 256 threads must call.
-__device__ inline void TileInverse(const DeviceConsts* K, const TileXform& X,
+// One task of the inverse transform: pass 0 columns of the coefficient
+// matrix into tmp, pass 1 columns of tmp into pixels (DCT4X4 per 4x4
+// sub-block with the Hadamard of the four DCs, pik/ac_strategy.cc:762-790;
+// DCT2X2 and IDENTITY directly in pass 1).
+__device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
                                    const float* coef, float* pix, float* tmp) {
-  if (X.first) {
-    const int l = X.lane;
-    if (X.raw == kDCT || X.raw == kDCTNoHF) {
-      if (l < 8) InvLine1<8>(coef, tmp, X.oby, X.obx, l);
-    } else if (X.raw == kDCT16) {
-      InvLine1<16>(coef, tmp, X.oby, X.obx, l);
-    } else if (X.raw == kDCT32) {
-      InvLine1<32>(coef, tmp, X.oby, X.obx, l);
-      InvLine1<32>(coef, tmp, X.oby, X.obx, l + 16);
-    } else if (AcsIsDCT4(X.raw)) {
-      // sub-block (sy, sx), column c of its 4x4 coefficient matrix G with
-      // G[iy][ix] = coef[(sy + 2 iy) * 8 + sx + 2 ix], G[0][0] from the
-      // Hadamard of the four DCs (pik/ac_strategy.cc:762-790)
-      const float* cb = coef + X.oby * 256 + X.obx * 64;
-      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, c = l & 3;
+  if (raw == kDCT || raw == kDCTNoHF) {
+    if (pass == 0) InvLine1<8>(coef, tmp, oby, obx, l);
+    else InvLine2<8>(tmp, pix, oby, obx, l);
+  } else if (raw == kDCT16) {
+    if (pass == 0) InvLine1<16>(coef, tmp, oby, obx, l);
+    else InvLine2<16>(tmp, pix, oby, obx, l);
+  } else if (raw == kDCT32) {
+    if (pass == 0) InvLine1<32>(coef, tmp, oby, obx, l);
+    else InvLine2<32>(tmp, pix, oby, obx, l);
+  } else if (AcsIsDCT4(raw)) {
+    const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, q = l & 3;
+    float v[4], w[4];
+    if (pass == 0) {
+      // column q of the sub-block's 4x4 matrix G, G[iy][ix] =
+      // coef[(sy + 2 iy) * 8 + sx + 2 ix], G[0][0] from the DC Hadamard
+      const float* cb = coef + oby * 256 + obx * 64;
       const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
       float dc;
       if (sub == 0) dc = b00 + b01 + b10 + b11;
       else if (sub == 1) dc = b00 + b01 - b10 - b11;
       else if (sub == 2) dc = b00 - b01 + b10 - b11;
       else dc = b00 - b01 - b10 + b11;
-      float v[4], w[4];
       for (int r = 0; r < 4; ++r) {
-        v[r] = (r == 0 && c == 0) ? dc : cb[(sy + r * 2) * 8 + sx + c * 2];
+        v[r] = (r == 0 && q == 0) ? dc : cb[(sy + r * 2) * 8 + sx + q * 2];
       }
       dct1d::Idct<4>(v, w);
       for (int x = 0; x < 4; ++x) {
-        tmp[(X.oby * 8 + sy * 4 + c) * 32 + X.obx * 8 + sx * 4 + x] = w[x];
+        tmp[(oby * 8 + sy * 4 + q) * 32 + obx * 8 + sx * 4 + x] = w[x];
       }
-    }
-  }
-  __syncthreads();
-  if (X.first) {
-    const int l = X.lane;
-    if (X.raw == kDCT || X.raw == kDCTNoHF) {
-      if (l < 8) InvLine2<8>(tmp, pix, X.oby, X.obx, l);
-    } else if (X.raw == kDCT16) {
-      InvLine2<16>(tmp, pix, X.oby, X.obx, l);
-    } else if (X.raw == kDCT32) {
-      InvLine2<32>(tmp, pix, X.oby, X.obx, l);
-      InvLine2<32>(tmp, pix, X.oby, X.obx, l + 16);
-    } else if (AcsIsDCT4(X.raw)) {
-      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, x = l & 3;
-      float v[4], w[4];
+    } else {
       for (int c = 0; c < 4; ++c) {
-        v[c] = tmp[(X.oby * 8 + sy * 4 + c) * 32 + X.obx * 8 + sx * 4 + x];
+        v[c] = tmp[(oby * 8 + sy * 4 + c) * 32 + obx * 8 + sx * 4 + q];
       }
       dct1d::Idct<4>(v, w);
       for (int y = 0; y < 4; ++y) {
-        pix[(X.oby * 8 + sy * 4 + y) * 32 + X.obx * 8 + sx * 4 + x] = w[y];
+        pix[(oby * 8 + sy * 4 + y) * 32 + obx * 8 + sx * 4 + q] = w[y];
       }
-    } else if (l == 0) {
-      DirectInverse8(X.raw, coef + X.oby * 256 + X.obx * 64,
-                     pix + X.oby * 8 * 32 + X.obx * 8, 32);
     }
+  } else if (pass == 1) {
+    DirectInverse8(raw, coef + oby * 256 + obx * 64,
+                   pix + oby * 8 * 32 + obx * 8, 32);
   }
-  __syncthreads();
 }
 
-// pix (32 x 32) -> coef (4 x 256). `hf_only` restricts to strategies with
-// HF prediction (others keep their coef entries untouched).
-__device__ inline void TileForward(const DeviceConsts* K, const TileXform& X,
-                                   const float* pix, float* coef, float* tmp,
-                                   bool hf_only) {
-  const bool active = X.first && (!hf_only || AcsPredictHF(X.raw));
-  const int l = X.lane;
-  if (active) {
-    if (X.raw == kDCT || X.raw == kDCTNoHF) {
-      if (l < 8) FwdLine1<8>(pix, tmp, X.oby, X.obx, l);
-    } else if (X.raw == kDCT16) {
-      FwdLine1<16>(pix, tmp, X.oby, X.obx, l);
-    } else if (X.raw == kDCT32) {
-      FwdLine1<32>(pix, tmp, X.oby, X.obx, l);
-      FwdLine1<32>(pix, tmp, X.oby, X.obx, l + 16);
-    } else if (AcsIsDCT4(X.raw)) {
-      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, x = l & 3;
-      float v[4], w[4];
+// One task of the forward transform: pass 0 columns of pixels into tmp,
+// pass 1 columns of tmp into coefficients (x 1/N^2), pass 2 the DC Hadamard
+// of DCT4X4 (pik/ac_strategy.cc:590-613); DCT2X2 and IDENTITY in pass 0.
+__device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
+                                   const float* pix, float* coef, float* tmp) {
+  if (raw == kDCT || raw == kDCTNoHF) {
+    if (pass == 0) FwdLine1<8>(pix, tmp, oby, obx, l);
+    else if (pass == 1) FwdLine2<8>(tmp, coef, oby, obx, l);
+  } else if (raw == kDCT16) {
+    if (pass == 0) FwdLine1<16>(pix, tmp, oby, obx, l);
+    else if (pass == 1) FwdLine2<16>(tmp, coef, oby, obx, l);
+  } else if (raw == kDCT32) {
+    if (pass == 0) FwdLine1<32>(pix, tmp, oby, obx, l);
+    else if (pass == 1) FwdLine2<32>(tmp, coef, oby, obx, l);
+  } else if (AcsIsDCT4(raw)) {
+    const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, q = l & 3;
+    float v[4], w[4];
+    float* cb = coef + oby * 256 + obx * 64;
+    if (pass == 0) {
       for (int y = 0; y < 4; ++y) {
-        v[y] = pix[(X.oby * 8 + sy * 4 + y) * 32 + X.obx * 8 + sx * 4 + x];
+        v[y] = pix[(oby * 8 + sy * 4 + y) * 32 + obx * 8 + sx * 4 + q];
       }
       dct1d::Dct<4>(v, w);
       for (int k = 0; k < 4; ++k) {
-        tmp[(X.oby * 8 + sy * 4 + x) * 32 + X.obx * 8 + sx * 4 + k] = w[k];
+        tmp[(oby * 8 + sy * 4 + q) * 32 + obx * 8 + sx * 4 + k] = w[k];
       }
-    }
-  }
-  __syncthreads();
-  if (active) {
-    if (X.raw == kDCT || X.raw == kDCTNoHF) {
-      if (l < 8) FwdLine2<8>(tmp, coef, X.oby, X.obx, l);
-    } else if (X.raw == kDCT16) {
-      FwdLine2<16>(tmp, coef, X.oby, X.obx, l);
-    } else if (X.raw == kDCT32) {
-      FwdLine2<32>(tmp, coef, X.oby, X.obx, l);
-      FwdLine2<32>(tmp, coef, X.oby, X.obx, l + 16);
-    } else if (AcsIsDCT4(X.raw)) {
-      const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, k = l & 3;
-      float v[4], w[4];
+    } else if (pass == 1) {
       for (int x = 0; x < 4; ++x) {
-        v[x] = tmp[(X.oby * 8 + sy * 4 + x) * 32 + X.obx * 8 + sx * 4 + k];
+        v[x] = tmp[(oby * 8 + sy * 4 + x) * 32 + obx * 8 + sx * 4 + q];
       }
       dct1d::Dct<4>(v, w);
-      float* cb = coef + X.oby * 256 + X.obx * 64;
-      for (int j = 0; j < 4; ++j) cb[(sy + j * 2) * 8 + sx + k * 2] = w[j] * (1.0f / 16);
+      for (int j = 0; j < 4; ++j) {
+        cb[(sy + j * 2) * 8 + sx + q * 2] = w[j] * (1.0f / 16);
+      }
     } else if (l == 0) {
-      DirectForward8(X.raw, pix + X.oby * 8 * 32 + X.obx * 8, 32,
-                     coef + X.oby * 256 + X.obx * 64);
+      const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
+      cb[0] = (b00 + b01 + b10 + b11) * 0.25f;
+      cb[1] = (b00 + b01 - b10 - b11) * 0.25f;
+      cb[8] = (b00 - b01 + b10 - b11) * 0.25f;
+      cb[9] = (b00 - b01 - b10 + b11) * 0.25f;
     }
+  } else if (pass == 0) {
+    DirectForward8(raw, pix + oby * 8 * 32 + obx * 8, 32,
+                   coef + oby * 256 + obx * 64);
   }
-  __syncthreads();
-  if (active && AcsIsDCT4(X.raw) && l == 0) {
-    float* cb = coef + X.oby * 256 + X.obx * 64;
-    const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
-    cb[0] = (b00 + b01 + b10 + b11) * 0.25f;
-    cb[1] = (b00 + b01 - b10 - b11) * 0.25f;
-    cb[8] = (b00 - b01 + b10 - b11) * 0.25f;
-    cb[9] = (b00 - b01 - b10 + b11) * 0.25f;
+}
+
+// coef[3][1024] -> pix[3][1024]. All 256 threads must call.
+__device__ inline void TileInverse3(const TileTasks& T, const float* coef,
+                                    float* pix, float* tmp) {
+  const int total = T.off[16];
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int k = threadIdx.x; k < 3 * total; k += 256) {
+      const int c = k / total, t = k - c * total;
+      const int b = TaskBlock(T, t);
+      InverseTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
+                  coef + c * 1024, pix + c * 1024, tmp + c * 1024);
+    }
+    __syncthreads();
   }
-  __syncthreads();
+}
+
+// pix[3][1024] -> coef[3][1024] for the transforms in T (others keep their
+// coefficient entries). All 256 threads must call.
+__device__ inline void TileForward3(const TileTasks& T, const float* pix,
+                                    float* coef, float* tmp) {
+  const int total = T.off[16];
+  for (int pass = 0; pass < 3; ++pass) {
+    for (int k = threadIdx.x; k < 3 * total; k += 256) {
+      const int c = k / total, t = k - c * total;
+      const int b = TaskBlock(T, t);
+      ForwardTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
+                  pix + c * 1024, coef + c * 1024, tmp + c * 1024);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup-to-tile order for the eight XCDs of an MI355X. Consecutive
+// workgroup ids are dealt round-robin to the XCDs, each with its own L2, so
+// in plain row-major order neighbouring tiles -- which read each other's
+// halo rows and columns -- sit in different L2s. Here XCD k walks the
+// contiguous row-major tile range [k * per, (k + 1) * per) in order.
+// Launch 8 * per workgroups (per = ceil(num_tiles / 8)); ids past the last
+// tile get -1 and return at once.
+constexpr int kNumXcds = 8;
+
+__host__ __device__ inline int XcdGrid(int num_tiles) {
+  return kNumXcds * ((num_tiles + kNumXcds - 1) / kNumXcds);
+}
+
+__device__ inline int XcdTile(int wg, int num_tiles) {
+  const int per = (num_tiles + kNumXcds - 1) / kNumXcds;
+  const int tile = (wg % kNumXcds) * per + wg / kNumXcds;
+  return tile < num_tiles ? tile : -1;
 }
 
 // ---------------------------------------------------------------------------

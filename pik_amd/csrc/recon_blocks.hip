@@ -16,27 +16,32 @@ namespace pikgpu {
 
 __global__ void __launch_bounds__(256)
     KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  __shared__ float s_coef[1024];
-  __shared__ float s_y[1024];
-  __shared__ float s_pix[1024];
-  __shared__ float s_pred[1024];
-  __shared__ float s_tmp[1024];
+  __shared__ float s_coef[3][1024];
+  __shared__ float s_pix[3][1024];
+  __shared__ float s_pred[3][1024];
+  __shared__ float s_tmp[3][1024];
   __shared__ uint8_t s_acs[16];
+  __shared__ TileTasks s_inv, s_fwd;
 
   const int tid = threadIdx.x;
-  const int tbx = blockIdx.x * 4, tby = blockIdx.y * 4;  // tile origin, blocks
+  const int tiles_x = (P.bx + 3) / 4;
+  const int tile = XcdTile(blockIdx.x, tiles_x * ((P.by + 3) / 4));
+  if (tile < 0) return;
+  const int tbx = (tile % tiles_x) * 4, tby = (tile / tiles_x) * 4;  // blocks
   if (tid < 16) {
     const int bxa = tbx + (tid & 3), bya = tby + (tid >> 2);
     s_acs[tid] = (bxa < P.bx && bya < P.by) ? B.acs[bya * P.bx + bxa] : 0xFF;
   }
   __syncthreads();
-  const TileXform X = MakeXform(s_acs, tid);
+  BuildTasks(s_acs, false, &s_inv);
+  if (P.predict_hf) BuildTasks(s_acs, true, &s_fwd);
 
   const int gxi = tbx / kGroupDimBlocks, gyi = tby / kGroupDimBlocks;
   const int g = gyi * P.gx + gxi;
   const int gbx0 = gxi * kGroupDimBlocks, gby0 = gyi * kGroupDimBlocks;
 
-  // Coefficient entries owned by this thread: row crow, cols ccol..ccol+3.
+  // Coefficient entries owned by this thread: storage row crow of the tile,
+  // columns ccol..ccol+3 (one 8x8 storage block per 16 threads).
   const int crow = tid >> 6, ccol = (tid & 63) << 2;
   const int cbx = tbx + (ccol >> 6), cby = tby + crow;  // frame block
   const bool cvalid = cbx < P.bx && cby < P.by;
@@ -50,48 +55,49 @@ __global__ void __launch_bounds__(256)
   // group-local origin block of the transform
   const int obxg = tbx + cobx - gbx0, obyg = tby + coby - gby0;
 
-  for (int ci = 0; ci < 3; ++ci) {
-    const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
-    // 1. dequantise (+ CfL)
-    if (cvalid) {
-      for (int j = 0; j < 4; ++j) {
-        const int col = cbx * 64 + ((ccol + j) & 63);
-        const float yv = c == 1 ? 0.0f : s_y[crow * 256 + ccol + j];
-        const float v = DequantCoef(P, B, c, cby, col, yv);
-        s_coef[crow * 256 + ccol + j] = v;
-        if (c == 1) s_y[crow * 256 + ccol + j] = v;
-      }
-    }
-    // 2. LLF and LF predictions
-    if (cvalid) {
+  // 1. dequantise Y, then X and B with their colour correlation
+  //    (Dequant::DoAC, pik/compressed_image.cc:586-693), and 2. the LLF and
+  //    LF predictions (UpdateLfForDecoder, pik/ac_predictions.cc:560-620).
+  if (cvalid) {
+    const size_t plane = size_t(P.by) * P.bx * 64;
+    const size_t at = size_t(cby) * P.bx * 64 + size_t(cbx) * 64 + (ccol & 63);
+    float yv[4];
+    for (int ci = 0; ci < 3; ++ci) {
+      const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+      const BlockDq dq = MakeBlockDq(P, B, c, cby, cbx);
+      const int16_t* q = B.ac + c * plane + at;
       const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
       const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
       for (int j = 0; j < 4; ++j) {
+        float v = DequantWith(dq, c, q[j], (ccol & 63) + j, yv[j]);
+        if (c == 1) yv[j] = v;
         const int flat = cflat0 + j;
         const int fi = flat / cN, fj = flat % cN;
-        float& v = s_coef[crow * 256 + ccol + j];
         if (fi < cs && fj < cs) {
           v = llf[(obyg + 1 + fi) * kLlfPitch + obxg + 1 + fj];
         } else if (P.predict_lf && fi < 2 * cs && fj < 2 * cs) {
           v += lf[(2 * (obyg + 1) + fi) * kG2Pitch + 2 * (obxg + 1) + fj];
         }
+        s_coef[c][crow * 256 + ccol + j] = v;
       }
     }
-    __syncthreads();
-    // 3. HF prediction: blurred 4x upsampling of pred2x2, forward
-    //    transform, added except to the LF/LLF corner.
-    if (P.predict_hf) {
-      // strategy of the block holding this thread's pixel quad
-      const uint8_t pa = s_acs[(X.r >> 3) * 4 + (X.c0 >> 3)];
-      if (pa != 0xFF && AcsPredictHF(pa >> 4)) {
+  }
+  // Pixel quad of this thread: row pr, columns pc0..pc0+3.
+  const int pr = tid >> 3, pc0 = (tid & 7) << 2;
+  // 3. HF prediction (AddPredictions / UpSample4x4BlurDCT<true>,
+  //    pik/ac_predictions.cc:263-374): 4x nearest-neighbour upsampling of
+  //    pred2x2 with the separable 3-tap blur, forward transform, added to
+  //    all but the 2s x 2s low-frequency corner.
+  if (P.predict_hf) {
+    const uint8_t pa = s_acs[(pr >> 3) * 4 + (pc0 >> 3)];
+    if (pa != 0xFF && AcsPredictHF(pa >> 4)) {
+      const int by_l = (tby - gby0) + (pr >> 3);  // group-local block row
+      const int py = pr & 7, iy = py & 3;
+      const int rbase = py < 4 ? 2 * by_l : 2 * by_l + 1;
+      for (int c = 0; c < 3; ++c) {
         const float* p2 = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane;
-        const int by_l = (tby - gby0) + (X.r >> 3);  // group-local block row
-        const int R0 = 2 * by_l;
-        const int py = X.r & 7;
-        const int iy = py & 3;
-        const int rbase = py < 4 ? R0 : R0 + 1;
         for (int j = 0; j < 4; ++j) {
-          const int Xc = (tbx - gbx0) * 8 + X.c0 + j;  // group-local column
+          const int Xc = (tbx - gbx0) * 8 + pc0 + j;  // group-local column
           const int xq = Xc >> 2, q = Xc & 3;
           float v[3];
           for (int t = 0; t < 3; ++t) {
@@ -99,38 +105,43 @@ __global__ void __launch_bounds__(256)
             v[t] = row[xq + 1] * P.ub4[0][q] + row[xq + 2] * P.ub4[1][q] +
                    row[xq + 3] * P.ub4[2][q];
           }
-          s_pred[X.r * 32 + X.c0 + j] =
+          s_pred[c][pr * 32 + pc0 + j] =
               v[0] * P.ub4[0][iy] + v[1] * P.ub4[1][iy] + v[2] * P.ub4[2][iy];
         }
       }
-      __syncthreads();
-      TileForward(K, X, s_pred, s_pix, s_tmp, true);
-      // s_pix now holds the prediction's coefficients for PredictHF blocks.
-      if (cvalid && ca != 0xFF && AcsPredictHF(craw)) {
-        for (int j = 0; j < 4; ++j) {
-          const int flat = cflat0 + j;
-          const int fi = flat / cN, fj = flat % cN;
-          if (!(fi < 2 * cs && fj < 2 * cs)) {
-            s_coef[crow * 256 + ccol + j] += s_pix[crow * 256 + ccol + j];
+    }
+    __syncthreads();
+    TileForward3(s_fwd, &s_pred[0][0], &s_pix[0][0], &s_tmp[0][0]);
+    // s_pix now holds the prediction's coefficients for PredictHF blocks.
+    if (cvalid && ca != 0xFF && AcsPredictHF(craw)) {
+      for (int j = 0; j < 4; ++j) {
+        const int flat = cflat0 + j;
+        const int fi = flat / cN, fj = flat % cN;
+        if (!(fi < 2 * cs && fj < 2 * cs)) {
+          for (int c = 0; c < 3; ++c) {
+            s_coef[c][crow * 256 + ccol + j] += s_pix[c][crow * 256 + ccol + j];
           }
         }
       }
-      __syncthreads();
     }
-    if (B.keep && cvalid) {
+  }
+  __syncthreads();
+  if (B.keep && cvalid) {
+    for (int c = 0; c < 3; ++c) {
       float* dst = B.coeffs + (size_t(c) * P.by + cby) * (P.bx * 64) + cbx * 64;
-      for (int j = 0; j < 4; ++j) dst[(ccol + j) & 63] = s_coef[crow * 256 + ccol + j];
+      for (int j = 0; j < 4; ++j) dst[(ccol + j) & 63] = s_coef[c][crow * 256 + ccol + j];
     }
-    // 4. IDCT
-    TileInverse(K, X, s_coef, s_pix, s_tmp);
-    const int py = tby * 8 + X.r;
-    if (py < P.py) {
-      float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + X.c0;
+  }
+  // 4. IDCT (InverseIntegralTransform, pik/compressed_image.cc:869-895)
+  TileInverse3(s_inv, &s_coef[0][0], &s_pix[0][0], &s_tmp[0][0]);
+  const int py = tby * 8 + pr;
+  if (py < P.py) {
+    for (int c = 0; c < 3; ++c) {
+      float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + pc0;
       for (int j = 0; j < 4; ++j) {
-        if (tbx * 8 + X.c0 + j < P.px) dst[j] = s_pix[X.r * 32 + X.c0 + j];
+        if (tbx * 8 + pc0 + j < P.px) dst[j] = s_pix[c][pr * 32 + pc0 + j];
       }
     }
-    __syncthreads();
   }
 }
 

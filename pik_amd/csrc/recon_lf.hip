@@ -65,7 +65,73 @@ __device__ inline uint8_t CacheAcs(const FrameParams& P, const FrameBuffers& B,
 }
 
 // ---------------------------------------------------------------------------
-// ComputeLlf: DC -> lowest-frequency coefficients per cache position.
+// Small reinterpreting transforms with compile-time sizes: every array index
+// is static after unrolling, so the blocks stay in registers.
+
+// 2-point transforms in the reference's explicit form (pik/dct.h:478-504).
+__device__ inline void Idct2x2(float a00, float a01, float a10, float a11,
+                               float* o, int os) {
+  o[0] = a00 + a01 + a10 + a11;
+  o[1] = a00 + a01 - a10 - a11;
+  o[os] = a00 - a01 + a10 - a11;
+  o[os + 1] = a00 - a01 - a10 + a11;
+}
+
+// Index of the scale tables for size n: log2(n) - 1.
+template <int N>
+constexpr int ScaleIdx() {
+  return N == 2 ? 0 : (N == 4 ? 1 : (N == 8 ? 2 : (N == 16 ? 3 : 4)));
+}
+
+// ReinterpretingDCT<DCTN, N, N> (pik/ac_strategy.cc:272-293), N >= 4:
+// N x N input at `in` (pitch ip) -> N x N output at `out` (pitch op); in and
+// out may alias.
+template <int DCTN, int N>
+__device__ inline void ReinterpDct(const DeviceConsts* K, const float* in,
+                                   int ip, float* out, int op) {
+  float blk[N * N], coef[N * N];
+  for (int y = 0; y < N; ++y) {
+    for (int x = 0; x < N; ++x) blk[y * N + x] = in[y * ip + x];
+  }
+  dct1d::Dct2D<N>(blk, N, coef, N);
+  constexpr int ti = ScaleIdx<N>(), ai = ScaleIdx<DCTN>();
+  for (int y = 0; y < N; ++y) {
+    for (int x = 0; x < N; ++x) {
+      const float v = coef[y * N + x] * K->tscale[ti][y][x];
+      out[y * op + x] = v * K->ascale[ai][y][x];
+    }
+  }
+}
+
+// ReinterpretingIDCT<DCTN, LF, N> (pik/ac_strategy.cc:296-315), N >= 4:
+// the LF x LF input (pitch ip) zero-padded to N x N.
+template <int DCTN, int LF, int N>
+__device__ inline void ReinterpIdct(const DeviceConsts* K, const float* in,
+                                    int ip, float* out, int op) {
+  float blk[N * N], pix[N * N];
+  constexpr int ai = ScaleIdx<N>(), ti = ScaleIdx<DCTN>();
+#pragma unroll
+  for (int y = 0; y < N; ++y) {
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+      float v = 0.0f;
+      if (y < LF && x < LF) {
+        v = (in[y * ip + x] * K->ascale[ai][y][x]) * K->tscale[ti][y][x];
+      }
+      blk[y * N + x] = v;
+    }
+  }
+  dct1d::Idct2D<N>(blk, N, pix, N);
+#pragma unroll
+  for (int y = 0; y < N; ++y) {
+#pragma unroll
+    for (int x = 0; x < N; ++x) out[y * op + x] = pix[y * N + x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ComputeLlf: DC -> lowest-frequency coefficients per cache position
+// (LowestFrequenciesFromDC, pik/ac_strategy.cc:850-887).
 __global__ void __launch_bounds__(256)
     KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
   const int pos = blockIdx.x * blockDim.x + threadIdx.x;
@@ -78,30 +144,39 @@ __global__ void __launch_bounds__(256)
   if ((a & 15) != 0) return;  // written by the first block
   const int raw = a >> 4;
   const float* dcp = B.dc + c * P.bx * P.by;
-  float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
-  const int s = AcsCovered(raw);
-  if (s == 1) {
-    llf[y * kLlfPitch + x] = GroupDc(P, dcp, G, x, y);
+  float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
+  if (raw == kDCT16 || raw == kDCT32) {
+    // ReinterpretingDCT<8s, s, s> of the s x s DC values.
+    float dc[16];
+    const int s = raw == kDCT16 ? 2 : 4;
+    for (int iy = 0; iy < 4; ++iy) {
+      for (int ix = 0; ix < 4; ++ix) {
+        dc[iy * 4 + ix] =
+            (iy < s && ix < s) ? GroupDc(P, dcp, G, x + ix, y + iy) : 0.0f;
+      }
+    }
+    if (raw == kDCT16) {
+      const float a00 = dc[0], a01 = dc[1], a10 = dc[4], a11 = dc[5];
+      const float d[4] = {(a00 + a01 + a10 + a11) * 0.25f,
+                          (a00 + a01 - a10 - a11) * 0.25f,
+                          (a00 - a01 + a10 - a11) * 0.25f,
+                          (a00 - a01 - a10 + a11) * 0.25f};
+      for (int iy = 0; iy < 2; ++iy) {
+        for (int ix = 0; ix < 2; ++ix) {
+          const float v = d[iy * 2 + ix] * K->tscale[0][iy][ix];
+          llf[iy * kLlfPitch + ix] = v * K->ascale[3][iy][ix];
+        }
+      }
+    } else {
+      ReinterpDct<32, 4>(K, dc, 4, llf, kLlfPitch);
+    }
     return;
   }
-  // ReinterpretingDCT<8s, s, s> of the s x s DC values
-  // (pik/ac_strategy.cc:272-293).
-  float in[16], out[16];
-  for (int iy = 0; iy < s; ++iy) {
-    for (int ix = 0; ix < s; ++ix) in[iy * s + ix] = GroupDc(P, dcp, G, x + ix, y + iy);
-  }
-  SmallDct(K, in, s, out, s, s);
-  const int ni = s == 2 ? 0 : 1;  // tscale index of N = s
-  const int di = s == 2 ? 3 : 4;  // ascale index of DCTN = 8s
-  for (int iy = 0; iy < s; ++iy) {
-    for (int ix = 0; ix < s; ++ix) {
-      const float v = out[iy * s + ix] * K->tscale[ni][iy][ix];
-      llf[(y + iy) * kLlfPitch + x + ix] = v * K->ascale[di][iy][ix];
-    }
-  }
+  llf[0] = GroupDc(P, dcp, G, x, y);
 }
 
-// DC2x2FromLowestFrequencies into tmp2x2 (pik/ac_strategy.cc:928-969).
+// DC2x2FromLowestFrequencies into tmp2x2 or the border of pred2x2
+// (pik/ac_strategy.cc:928-969).
 __global__ void __launch_bounds__(256)
     KDc2x2FromLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                   int to_pred2x2) {
@@ -118,40 +193,25 @@ __global__ void __launch_bounds__(256)
   if (to_pred2x2 && !border) return;
   if ((a & 15) != 0) return;
   const int raw = a >> 4;
-  const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
-  float* dst = (to_pred2x2 ? B.pred2x2 : B.tmp2x2) + (size_t(g) * 3 + c) * kG2Plane;
-  const int s = AcsCovered(raw);
-  float* o = dst + 2 * y * kG2Pitch + 2 * x;
-  const float l0 = llf[y * kLlfPitch + x];
-  if (s == 1) {
+  const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
+  float* o = (to_pred2x2 ? B.pred2x2 : B.tmp2x2) + (size_t(g) * 3 + c) * kG2Plane +
+             2 * y * kG2Pitch + 2 * x;
+  if (raw == kDCT16) {
+    ReinterpIdct<16, 2, 4>(K, llf, kLlfPitch, o, kG2Pitch);
+  } else if (raw == kDCT32) {
+    ReinterpIdct<32, 4, 8>(K, llf, kLlfPitch, o, kG2Pitch);
+  } else {
     float v;
     if (raw == kDCT || raw == kDCTNoHF || raw == kLines) {
       // ReinterpretingIDCT<8, 1, 2>: one coefficient, 2-point IDCT of it.
-      v = (l0 * K->ascale[0][0][0]) * K->tscale[2][0][0];
+      v = (llf[0] * K->ascale[0][0][0]) * K->tscale[2][0][0];
     } else {
-      v = l0;
+      v = llf[0];
     }
     o[0] = v;
     o[1] = v;
     o[kG2Pitch] = v;
     o[kG2Pitch + 1] = v;
-    return;
-  }
-  // ReinterpretingIDCT<8s, s, 2s>.
-  const int n = 2 * s;
-  float blk[64], out[64];
-  for (int i = 0; i < n * n; ++i) blk[i] = 0.0f;
-  const int ai = s == 2 ? 1 : 2;  // ascale index of N = 2s
-  const int ti = s == 2 ? 3 : 4;  // tscale index of DCTN = 8s
-  for (int iy = 0; iy < s; ++iy) {
-    for (int ix = 0; ix < s; ++ix) {
-      const float v = llf[(y + iy) * kLlfPitch + x + ix] * K->ascale[ai][iy][ix];
-      blk[iy * n + ix] = v * K->tscale[ti][iy][ix];
-    }
-  }
-  SmallIdct(K, blk, n, out, n, n);
-  for (int iy = 0; iy < n; ++iy) {
-    for (int ix = 0; ix < n; ++ix) o[iy * kG2Pitch + ix] = out[iy * n + ix];
   }
 }
 
@@ -185,29 +245,32 @@ __global__ void __launch_bounds__(256)
   if ((a & 15) != 0) return;
   const int raw = a >> 4;
   float* p = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane + 2 * y * kG2Pitch + 2 * x;
-  const int s = AcsCovered(raw);
-  if (s == 1 && !(raw == kDCT || raw == kDCTNoHF || raw == kLines)) {
-    const float b00 = p[0], b01 = p[1], b10 = p[kG2Pitch], b11 = p[kG2Pitch + 1];
-    p[0] = (b00 + b01 + b10 + b11) * 0.25f;
-    p[1] = (b00 + b01 - b10 - b11) * 0.25f;
-    p[kG2Pitch] = (b00 - b01 + b10 - b11) * 0.25f;
-    p[kG2Pitch + 1] = (b00 - b01 - b10 + b11) * 0.25f;
+  if (raw == kDCT16) {
+    ReinterpDct<16, 4>(K, p, kG2Pitch, p, kG2Pitch);
     return;
   }
-  // ReinterpretingDCT<8s, 2s, 2s>.
-  const int n = 2 * s;
-  float in[64], out[64];
-  for (int iy = 0; iy < n; ++iy) {
-    for (int ix = 0; ix < n; ++ix) in[iy * n + ix] = p[iy * kG2Pitch + ix];
+  if (raw == kDCT32) {
+    ReinterpDct<32, 8>(K, p, kG2Pitch, p, kG2Pitch);
+    return;
   }
-  SmallDct(K, in, n, out, n, n);
-  const int ni = s == 1 ? 0 : (s == 2 ? 1 : 2);  // tscale index of N = 2s
-  const int di = s == 1 ? 2 : (s == 2 ? 3 : 4);  // ascale index of 8s
-  for (int iy = 0; iy < n; ++iy) {
-    for (int ix = 0; ix < n; ++ix) {
-      const float v = out[iy * n + ix] * K->tscale[ni][iy][ix];
-      p[iy * kG2Pitch + ix] = v * K->ascale[di][iy][ix];
+  const float b00 = p[0], b01 = p[1], b10 = p[kG2Pitch], b11 = p[kG2Pitch + 1];
+  const float d[4] = {(b00 + b01 + b10 + b11) * 0.25f,
+                      (b00 + b01 - b10 - b11) * 0.25f,
+                      (b00 - b01 + b10 - b11) * 0.25f,
+                      (b00 - b01 - b10 + b11) * 0.25f};
+  if (raw == kDCT || raw == kDCTNoHF || raw == kLines) {
+    // ReinterpretingDCT<8, 2, 2>
+    for (int iy = 0; iy < 2; ++iy) {
+      for (int ix = 0; ix < 2; ++ix) {
+        const float v = d[iy * 2 + ix] * K->tscale[0][iy][ix];
+        p[iy * kG2Pitch + ix] = v * K->ascale[2][iy][ix];
+      }
     }
+  } else {
+    p[0] = d[0];
+    p[1] = d[1];
+    p[kG2Pitch] = d[2];
+    p[kG2Pitch + 1] = d[3];
   }
 }
 
@@ -244,74 +307,115 @@ __global__ void __launch_bounds__(256)
   o[kG2Pitch + 1] = a00 - a01 - a10 + a11;
 }
 
-// Interior of pred2x2: per first block, the decoder-side LF coefficients
-// (dequantised + LLF + LF prediction) reinterpreted as a 2x upsampled DC
-// block (DC2x2FromLowFrequencies, pik/ac_strategy.cc:971-1017, as called by
-// UpdateLfForDecoder, pik/ac_predictions.cc:621-633).
+// Interior of pred2x2: per first block and plane, the decoder-side LF
+// coefficients (dequantised + LLF + LF prediction) reinterpreted as a 2x
+// upsampled DC block (DC2x2FromLowFrequencies, pik/ac_strategy.cc:971-1017,
+// as called by UpdateLfForDecoder, pik/ac_predictions.cc:621-633). X and B
+// recompute the dequantised Y values their colour correlation needs.
+template <int S>
+__device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
+                                    const DeviceConsts* K, int c, int bxa,
+                                    int bya, int bxg, int byg, int g) {
+  constexpr int N = 2 * S;  // LF corner size
+  const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
+  const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
+  // Dequantisation parameters of the storage blocks the corner touches:
+  // coefficient F[i][j] (flat i*8S + j) lies in storage block
+  // (bxa + flat / 64, bya) for these sizes.
+  constexpr int kStore = S == 4 ? 4 : 1;
+  BlockDq dq[kStore], dqy[kStore];
+#pragma unroll
+  for (int k = 0; k < kStore; ++k) {
+    dq[k] = MakeBlockDq(P, B, c, bya, bxa + k);
+    if (c != 1) dqy[k] = MakeBlockDq(P, B, 1, bya, bxa + k);
+  }
+  const size_t plane = size_t(P.by) * P.bx * 64;
+  const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  float F[N * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      float v;
+      if (i < S && j < S) {
+        v = llf[(byg + 1 + i) * kLlfPitch + bxg + 1 + j];
+      } else {
+        const int flat = i * 8 * S + j;
+        const int sb = S == 4 ? flat >> 6 : 0, k = flat & 63;
+        float yv = 0.0f;
+        if (c != 1) yv = DequantWith(dqy[sb], 1, acy[flat], k, 0.0f);
+        v = DequantWith(dq[sb], c, acrow[flat], k, yv);
+        if (P.predict_lf) {
+          v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
+        }
+      }
+      F[i * N + j] = v;
+    }
+  }
+  float* o = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane +
+             2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
+  // ReinterpretingIDCT<8S, 2S, 2S>
+  if (S == 1) {
+    const float a00 = (F[0] * K->ascale[0][0][0]) * K->tscale[2][0][0];
+    const float a01 = (F[1] * K->ascale[0][0][1]) * K->tscale[2][0][1];
+    const float a10 = (F[2] * K->ascale[0][1][0]) * K->tscale[2][1][0];
+    const float a11 = (F[3] * K->ascale[0][1][1]) * K->tscale[2][1][1];
+    Idct2x2(a00, a01, a10, a11, o, kG2Pitch);
+  } else {
+    ReinterpIdct<8 * S, N, N>(K, F, N, o, kG2Pitch);
+  }
+}
+
 __global__ void __launch_bounds__(256)
     KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
   const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
   if (blk >= P.bx * P.by) return;
   const int bxa = blk % P.bx, bya = blk / P.bx;
   const uint8_t a = B.acs[blk];
   if ((a & 15) != 0) return;
   const int raw = a >> 4;
-  const int s = AcsCovered(raw);
-  const int n = 2 * s;
   const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
   const int g = gyi * P.gx + gxi;
   const int bxg = bxa - gxi * kGroupDimBlocks, byg = bya - gyi * kGroupDimBlocks;
-  float yv[64];
-  for (int ci = 0; ci < 3; ++ci) {
-    const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);  // Y first for CfL
+  if (raw == kDCT16) {
+    Pred2x2Block<2>(P, B, K, c, bxa, bya, bxg, byg, g);
+  } else if (raw == kDCT32) {
+    Pred2x2Block<4>(P, B, K, c, bxa, bya, bxg, byg, g);
+  } else if (raw == kDCT || raw == kDCTNoHF || raw == kLines) {
+    Pred2x2Block<1>(P, B, K, c, bxa, bya, bxg, byg, g);
+  } else {
+    // DCT2X2, DCT4X4(_NOHF), IDENTITY: 2x2 Hadamard of the LF corner.
+    const BlockDq dq = MakeBlockDq(P, B, c, bya, bxa);
+    BlockDq dqy;
+    if (c != 1) dqy = MakeBlockDq(P, B, 1, bya, bxa);
+    const size_t plane = size_t(P.by) * P.bx * 64;
+    const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
+    const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
     const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
     const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
-    float F[64];
-    for (int i = 0; i < n; ++i) {
-      for (int j = 0; j < n; ++j) {
+    float F[4];
+    for (int i = 0; i < 2; ++i) {
+      for (int j = 0; j < 2; ++j) {
         float v;
-        if (i < s && j < s) {
-          v = llf[(byg + 1 + i) * kLlfPitch + bxg + 1 + j];
+        if (i == 0 && j == 0) {
+          v = llf[(byg + 1) * kLlfPitch + bxg + 1];
         } else {
-          const int flat = i * 8 * s + j;
-          const int row = bya + flat / (64 * s);
-          const int col = bxa * 64 + flat % (64 * s);
-          const float y = c == 1 ? 0.0f : yv[i * n + j];
-          v = DequantCoef(P, B, c, row, col, y);
-          if (P.predict_lf) v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
+          const int flat = i * 8 + j;
+          float yv = 0.0f;
+          if (c != 1) yv = DequantWith(dqy, 1, acy[flat], flat, 0.0f);
+          v = DequantWith(dq, c, acrow[flat], flat, yv);
+          if (P.predict_lf) {
+            v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
+          }
         }
-        F[i * n + j] = v;
-        if (c == 1 && !(i < s && j < s)) {
-          const int flat = i * 8 * s + j;
-          yv[i * n + j] = DequantCoef(P, B, 1, bya + flat / (64 * s),
-                                      bxa * 64 + flat % (64 * s), 0.0f);
-        }
+        F[i * 2 + j] = v;
       }
     }
     float* o = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane +
                2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
-    if (s == 1 && !(raw == kDCT || raw == kDCTNoHF || raw == kLines)) {
-      const float b00 = F[0], b01 = F[1], b10 = F[2], b11 = F[3];
-      o[0] = b00 + b01 + b10 + b11;
-      o[1] = b00 + b01 - b10 - b11;
-      o[kG2Pitch] = b00 - b01 + b10 - b11;
-      o[kG2Pitch + 1] = b00 - b01 - b10 + b11;
-      continue;
-    }
-    // ReinterpretingIDCT<8s, 2s, 2s>.
-    const int ai = s == 1 ? 0 : (s == 2 ? 1 : 2);
-    const int ti = s == 1 ? 2 : (s == 2 ? 3 : 4);
-    float blkv[64], out[64];
-    for (int i = 0; i < n; ++i) {
-      for (int j = 0; j < n; ++j) {
-        const float v = F[i * n + j] * K->ascale[ai][i][j];
-        blkv[i * n + j] = v * K->tscale[ti][i][j];
-      }
-    }
-    SmallIdct(K, blkv, n, out, n, n);
-    for (int i = 0; i < n; ++i) {
-      for (int j = 0; j < n; ++j) o[i * kG2Pitch + j] = out[i * n + j];
-    }
+    Idct2x2(F[0], F[1], F[2], F[3], o, kG2Pitch);
   }
 }
 

@@ -1,16 +1,12 @@
-"""Dequantisation tables of the reference's library encodings
-(DequantMatrices::Compute, pik/quant_weights.cc:607-926).
-
-The table (4800 floats) and its offsets are data of the format: a decoder
-uses them to turn quantized coefficients into values. They are loaded from
-pik_amd/data/dequant_default.npz.
-"""
-import os
+"""The library dequantisation tables (DequantMatrices::Compute for the
+predefined table set, pik/quant_weights.cc:607-926), computed by the host
+side of libpikgpu.so (pikgpu_default_dequant_tables)."""
+import ctypes
 
 import numpy as np
 
-_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
-                     "dequant_default.npz")
+from . import abi
+
 _cache = None
 
 
@@ -18,7 +14,14 @@ def default_tables():
     """Returns (table float32[4800], offsets uint32[21])."""
     global _cache
     if _cache is None:
-        z = np.load(_PATH, allow_pickle=False)
-        _cache = (np.ascontiguousarray(z["table"], np.float32),
-                  np.ascontiguousarray(z["offsets"], np.uint32))
+        lib = abi.load()
+        ts, no = ctypes.c_size_t(), ctypes.c_size_t()
+        lib.pikgpu_default_dequant_tables(None, 0, None, 0, ctypes.byref(ts),
+                                          ctypes.byref(no))
+        table = np.zeros(ts.value, np.float32)
+        offsets = np.zeros(no.value, np.uint32)
+        lib.pikgpu_default_dequant_tables(table.ctypes.data, table.size,
+                                          offsets.ctypes.data, offsets.size,
+                                          ctypes.byref(ts), ctypes.byref(no))
+        _cache = (table, offsets)
     return _cache
