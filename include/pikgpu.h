@@ -219,6 +219,27 @@ PIKGPU_API int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* frame);
  * pikgpu_stage_dump (costs extra HBM traffic). */
 PIKGPU_API int pikgpu_run(pikgpu_ctx* ctx, int keep_stages);
 
+/* Outputs pikgpu_run produces (bit mask, default PIKGPU_OUTPUT_LINEAR). */
+enum {
+  PIKGPU_OUTPUT_LINEAR = 1, /* linear Image3F, 3 float planes in [0, 255] */
+  PIKGPU_OUTPUT_SRGB8 = 2   /* 8-bit sRGB, interleaved R, G, B */
+};
+
+/* Selects the outputs of the following pikgpu_run calls. The 8-bit output
+ * replaces the reference's conversion of the decoded image for an 8-bit
+ * sRGB file: the colour transform of the linear image to the sRGB transfer
+ * curve (ColorSpaceTransform, pik/color_management.cc, through lcms2), then
+ * clamping to [0, 1], scaling to 255 and rounding (CastClip01 and
+ * ToExternal, pik/external_image.cc:190-197, 666-696); it is computed in the
+ * final kernel, so a run that asks only for it writes 3 bytes per pixel
+ * instead of 12. */
+PIKGPU_API int pikgpu_set_output(pikgpu_ctx* ctx, int outputs);
+
+/* Copies the 8-bit sRGB output of the last run (xsize x ysize pixels, rows
+ * of 3 * xsize bytes) to host memory with the given row stride. */
+PIKGPU_API int pikgpu_download_srgb8(pikgpu_ctx* ctx, uint8_t* rgb,
+                                     size_t stride_bytes);
+
 /* Waits for the context stream. */
 PIKGPU_API int pikgpu_sync(pikgpu_ctx* ctx);
 
@@ -251,6 +272,39 @@ PIKGPU_API int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* m
 
 /* The HIP stream the context launches on (hipStream_t as void*). */
 PIKGPU_API void* pikgpu_stream(pikgpu_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Batches: many frames of uneven sizes reconstructed by one launch. The
+ * reference decodes one frame at a time (PikPassToPixels per frame,
+ * pik/pik_frame.cc:1106-1181); a batch replaces a loop of those calls. Each
+ * frame keeps its own device buffers; pikgpu_batch_run launches one hipGraph
+ * holding every frame's kernels in parallel branches. */
+typedef struct pikgpu_batch pikgpu_batch;
+
+PIKGPU_API pikgpu_batch* pikgpu_batch_create(int device,
+                                             const pikgpu_config* config);
+PIKGPU_API void pikgpu_batch_destroy(pikgpu_batch* batch);
+PIKGPU_API const char* pikgpu_batch_last_error(const pikgpu_batch* batch);
+/* Validates and uploads n frames (host -> HBM); frame i's failure is
+ * reported as "frame i: <reason>". */
+PIKGPU_API int pikgpu_batch_upload(pikgpu_batch* batch,
+                                   const pikgpu_frame* frames, int n);
+/* Launches the reconstruction of every uploaded frame (asynchronous). The
+ * graph is built on the first run after an upload. */
+PIKGPU_API int pikgpu_batch_run(pikgpu_batch* batch);
+PIKGPU_API int pikgpu_batch_sync(pikgpu_batch* batch);
+/* Device time of the last run in milliseconds (valid after a sync). */
+PIKGPU_API float pikgpu_batch_last_ms(const pikgpu_batch* batch);
+/* Linear Image3F of frame i of the last run (waits for the run). */
+PIKGPU_API int pikgpu_batch_download(pikgpu_batch* batch, int i,
+                                     float* out_planes[3],
+                                     size_t out_stride_bytes);
+/* Upload + run + download: out_planes holds 3 plane pointers per frame,
+ * out_stride_bytes one row stride per frame. */
+PIKGPU_API int pikgpu_reconstruct_batch(pikgpu_batch* batch,
+                                        const pikgpu_frame* frames, int n,
+                                        float* const* out_planes,
+                                        const size_t* out_stride_bytes);
 
 #ifdef __cplusplus
 }  /* extern "C" */

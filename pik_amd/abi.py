@@ -30,6 +30,9 @@ ACS_LINES = 6
 ACS_DCT_NOHF = 7
 ACS_DCT4X4_NOHF = 8
 
+OUTPUT_LINEAR = 1
+OUTPUT_SRGB8 = 2
+
 GABORISH_OFF, GABORISH_500, GABORISH_750, GABORISH_875, GABORISH_1000 = range(5)
 
 STAGE_DC_DEQUANT = 0
@@ -127,6 +130,23 @@ def load():
                                         c_float_p, ctypes.c_int]
     lib.pikgpu_stream.argtypes = [vp]
     lib.pikgpu_stream.restype = vp
+    lib.pikgpu_batch_create.restype = vp
+    lib.pikgpu_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ConfigC)]
+    lib.pikgpu_batch_destroy.argtypes = [vp]
+    lib.pikgpu_batch_destroy.restype = None
+    lib.pikgpu_batch_last_error.argtypes = [vp]
+    lib.pikgpu_batch_last_error.restype = ctypes.c_char_p
+    lib.pikgpu_batch_upload.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+    lib.pikgpu_batch_run.argtypes = [vp]
+    lib.pikgpu_batch_sync.argtypes = [vp]
+    lib.pikgpu_batch_last_ms.argtypes = [vp]
+    lib.pikgpu_batch_last_ms.restype = ctypes.c_float
+    lib.pikgpu_batch_download.argtypes = [vp, ctypes.c_int, c_float_p * 3,
+                                          ctypes.c_size_t]
+    lib.pikgpu_reconstruct_batch.argtypes = [vp, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+    lib.pikgpu_set_output.argtypes = [vp, ctypes.c_int]
+    lib.pikgpu_download_srgb8.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
     lib.pikgpu_default_dequant_tables.argtypes = [
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
@@ -141,5 +161,9 @@ EXPORTED_SYMBOLS = [
     "pikgpu_frame_upload", "pikgpu_run", "pikgpu_sync", "pikgpu_download",
     "pikgpu_reconstruct", "pikgpu_stage_size", "pikgpu_stage_dump",
     "pikgpu_last_timings", "pikgpu_stream", "pikgpu_validate_frame",
-    "pikgpu_default_dequant_tables",
+    "pikgpu_default_dequant_tables", "pikgpu_set_output",
+    "pikgpu_download_srgb8", "pikgpu_batch_create", "pikgpu_batch_destroy",
+    "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
+    "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
+    "pikgpu_reconstruct_batch",
 ]

@@ -45,10 +45,14 @@ struct pikgpu_ctx {
   DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
       dotbins;
   DevBuf dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
-      dots_img, gab, out, mm_partial, mm, ar_keep;
+      dots_img, gab, out, mm_partial, mm, ar_keep, srgb8;
   bool uploaded = false;
   bool ran = false;
   bool kept = false;
+  int out_flags = kOutLinear;
+  int ran_flags = 0;
+  int spitch = 0;
+  bool timed = false;   // the last run recorded timing events
   FrameParams P;
   FrameBuffers B;
   int num_tables = 0;
@@ -91,7 +95,7 @@ void FreeAll(pikgpu_ctx* ctx) {
   DevBuf* all[] = {&ctx->ac,      &ctx->dc16,   &ctx->qf,      &ctx->acs,
                    &ctx->lut,     &ctx->dcf,    &ctx->ytox,    &ctx->ytob,
                    &ctx->dqmap,   &ctx->table,  &ctx->offsets, &ctx->dots,
-                   &ctx->dotbins,
+                   &ctx->dotbins, &ctx->srgb8,
                    &ctx->dcq,     &ctx->dcv,    &ctx->llf,     &ctx->tmp2x2,
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
@@ -619,22 +623,30 @@ static int EnsureStageBuffers(pikgpu_ctx* ctx) {
   return PIKGPU_OK;
 }
 
-int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
-  if (ctx == nullptr) return PIKGPU_EINVAL;
-  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Enqueues the whole reconstruction of the uploaded frame on stream `st`.
+// With `timing`, HIP events between the kernel families record their
+// durations (pikgpu_last_timings). Allocates nothing unless stage dumps or
+// the 8-bit output need buffers that do not exist yet, so a batch can
+// capture it into a graph.
+static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
+                   bool timing) {
   int rc;
   if (keep_stages && (rc = EnsureStageBuffers(ctx))) return rc;
   FrameParams P = ctx->P;
   FrameBuffers B = ctx->B;
   B.keep = keep_stages != 0;
+  P.out_flags = ctx->out_flags;
+  if (P.out_flags & kOutSrgb8) {
+    B.spitch = int(DivCeil(size_t(P.xsize) * 3, 64) * 64);
+    if ((rc = Grow(ctx, &ctx->srgb8, size_t(B.spitch) * P.ysize))) return rc;
+    B.srgb8 = static_cast<uint8_t*>(ctx->srgb8.p);
+  }
   const DeviceConsts* K = ctx->consts;
-  hipStream_t st = ctx->stream;
   const int nblk = P.bx * P.by;
   const size_t ngroups = size_t(P.gx) * P.gy;
   const size_t iplane = size_t(B.ipitch) * P.py;
 
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
   // DC: dequant, min/max of the 8-padded DC image, EPF + clamp.
   KDcDequant<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B);
   {
@@ -653,7 +665,7 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
     KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
                                size_t(nblk));
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
   // LF-domain predictions per group.
   {
     const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
@@ -676,12 +688,12 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
       KPred2x2Interior<<<dim3(DivCeil(nblk, 256), 3), 256, 0, st>>>(P, B, K);
     }
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   {
     const int grid = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
     KBlocks<<<grid, 256, 0, st>>>(P, B, K);
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   if (P.ar) {
     Img ii;
     ii.p = B.idct;
@@ -695,16 +707,16 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
     const dim3 mmg = MinMaxGrid(ii.W, ii.H);
     KMinMaxPartial<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
     KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
     const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
     KEpf<<<grid, 256, 0, st>>>(ii, gab, 0, P, B, K, B.mm, B.ar, B.ipitch,
                                iplane);
   } else {
-    HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
     KCopyPlanes<<<DivCeil(3 * iplane, 256), 256, 0, st>>>(B.idct, B.ar,
                                                           3 * iplane);
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
   if (B.keep) {
     if (!P.ar) {
       HIP_TRY(ctx, hipMemcpyAsync(B.guide, B.idct, 3 * iplane * 4,
@@ -719,23 +731,33 @@ int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
                                 hipMemcpyDeviceToDevice, st));
   }
   if (P.dot_tiles) KDots<<<P.dot_tiles, 256, 0, st>>>(P, B);
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
   {
     const int grid = XcdGrid(int(DivCeil(P.px, 64) * DivCeil(P.py, 4)));
     KFinal<<<grid, 256, 0, st>>>(P, B, K);
   }
-  HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
+  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
   HIP_TRY(ctx, hipGetLastError());
   ctx->ran = true;
   ctx->kept = B.keep;
+  ctx->ran_flags = P.out_flags;
+  ctx->spitch = B.spitch;
   return PIKGPU_OK;
+}
+
+int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->timed = true;
+  return Enqueue(ctx, ctx->stream, keep_stages != 0, true);
 }
 
 int pikgpu_sync(pikgpu_ctx* ctx) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (ctx->ran) {
+  if (ctx->ran && ctx->timed) {
     for (int i = 0; i < kNumTimers; ++i) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) {
@@ -750,6 +772,9 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
                     size_t out_stride_bytes) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
   if (!ctx->ran) return Fail(ctx, PIKGPU_ESTATE, "pikgpu_run has not run");
+  if (!(ctx->ran_flags & kOutLinear)) {
+    return Fail(ctx, PIKGPU_ESTATE, "the last run produced no linear output");
+  }
   if (out_planes == nullptr || !out_planes[0] || !out_planes[1] ||
       !out_planes[2]) {
     return Fail(ctx, PIKGPU_EINVAL, "null output plane");
@@ -767,6 +792,32 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
                                   size_t(P.xsize) * 4, P.ysize,
                                   hipMemcpyDeviceToHost, ctx->stream));
   }
+  return pikgpu_sync(ctx);
+}
+
+int pikgpu_set_output(pikgpu_ctx* ctx, int outputs) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (outputs <= 0 || (outputs & ~(kOutLinear | kOutSrgb8)) != 0) {
+    return Fail(ctx, PIKGPU_EINVAL, "invalid output selection");
+  }
+  ctx->out_flags = outputs;
+  return PIKGPU_OK;
+}
+
+int pikgpu_download_srgb8(pikgpu_ctx* ctx, uint8_t* rgb, size_t stride_bytes) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->ran) return Fail(ctx, PIKGPU_ESTATE, "pikgpu_run has not run");
+  if (!(ctx->ran_flags & kOutSrgb8)) {
+    return Fail(ctx, PIKGPU_ESTATE, "the last run produced no 8-bit output");
+  }
+  const FrameParams& P = ctx->P;
+  if (rgb == nullptr || stride_bytes < size_t(P.xsize) * 3) {
+    return Fail(ctx, PIKGPU_EINVAL, "bad output buffer or stride");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpy2DAsync(rgb, stride_bytes, ctx->srgb8.p,
+                                size_t(ctx->spitch), size_t(P.xsize) * 3,
+                                P.ysize, hipMemcpyDeviceToHost, ctx->stream));
   return pikgpu_sync(ctx);
 }
 
@@ -846,6 +897,9 @@ int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
     case PIKGPU_STAGE_DOTS: src = B.ar; pitch = B.ipitch; plane = iplane; break;
     case PIKGPU_STAGE_GABORISH: src = B.gab; pitch = B.ipitch; plane = iplane; break;
     case PIKGPU_STAGE_LINEAR:
+      if (!(ctx->ran_flags & kOutLinear)) {
+        return Fail(ctx, PIKGPU_ESTATE, "the last run produced no linear output");
+      }
       src = B.out; pitch = B.opitch; plane = size_t(B.opitch) * P.ysize; break;
   }
   for (int c = 0; c < 3; ++c) {
@@ -869,3 +923,213 @@ int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* ms,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: several frames of any sizes reconstructed by one graph launch.
+// Each frame keeps its own buffers (a context per frame); the launch
+// sequences of all frames are captured once into a hipGraph whose kBranches
+// parallel branches each run a share of the frames one after the other, so
+// kernels of different frames overlap and the whole batch costs one launch.
+
+namespace {
+constexpr int kBranches = 8;
+}  // namespace
+
+struct pikgpu_batch {
+  int device = 0;
+  pikgpu_config config = {PIKGPU_ABI_VERSION, 0};
+  std::string error;
+  std::vector<pikgpu_ctx*> items;
+  int n = 0;
+  hipStream_t main = nullptr;
+  hipStream_t branch[kBranches] = {};
+  hipEvent_t fork = nullptr, join[kBranches] = {}, t0 = nullptr, t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ran = false;
+  float last_ms = 0.0f;
+};
+
+namespace {
+
+int BFail(pikgpu_batch* b, int code, const std::string& msg) {
+  if (b) b->error = msg;
+  return code;
+}
+
+#define BHIP_TRY(b, expr)                                                   \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      return BFail(b, PIKGPU_EHIP,                                          \
+                   std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    }                                                                       \
+  } while (0)
+
+void DropGraph(pikgpu_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int BuildGraph(pikgpu_batch* b) {
+  DropGraph(b);
+  BHIP_TRY(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+  int rc = PIKGPU_OK;
+  std::string err;
+  hipError_t e = hipEventRecord(b->fork, b->main);
+  for (int k = 0; k < kBranches && e == hipSuccess; ++k) {
+    e = hipStreamWaitEvent(b->branch[k], b->fork, 0);
+  }
+  for (int i = 0; i < b->n && e == hipSuccess && rc == PIKGPU_OK; ++i) {
+    rc = Enqueue(b->items[i], b->branch[i % kBranches], false, false);
+    if (rc != PIKGPU_OK) {
+      err = "frame " + std::to_string(i) + ": " + b->items[i]->error;
+    }
+  }
+  for (int k = 0; k < kBranches && e == hipSuccess; ++k) {
+    e = hipEventRecord(b->join[k], b->branch[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->join[k], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(b->main, &g);
+  if (rc != PIKGPU_OK || e != hipSuccess || e_end != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    if (rc != PIKGPU_OK) return BFail(b, rc, err);
+    return BFail(b, PIKGPU_EHIP,
+                 std::string("graph capture: ") +
+                     hipGetErrorString(e != hipSuccess ? e : e_end));
+  }
+  b->graph = g;
+  BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return PIKGPU_OK;
+}
+
+}  // namespace
+
+pikgpu_batch* pikgpu_batch_create(int device, const pikgpu_config* config) {
+  if (config && config->abi_version != PIKGPU_ABI_VERSION) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  pikgpu_batch* b = new pikgpu_batch;
+  b->device = device;
+  if (config) b->config = *config;
+  bool ok = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreate(&b->t0) == hipSuccess &&
+            hipEventCreate(&b->t1) == hipSuccess;
+  for (int k = 0; k < kBranches && ok; ++k) {
+    ok = hipStreamCreateWithFlags(&b->branch[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+  }
+  if (!ok) {
+    pikgpu_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void pikgpu_batch_destroy(pikgpu_batch* b) {
+  if (b == nullptr) return;
+  hipSetDevice(b->device);
+  if (b->main) hipStreamSynchronize(b->main);
+  DropGraph(b);
+  for (pikgpu_ctx* c : b->items) pikgpu_destroy(c);
+  for (int k = 0; k < kBranches; ++k) {
+    if (b->branch[k]) hipStreamDestroy(b->branch[k]);
+    if (b->join[k]) hipEventDestroy(b->join[k]);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->t0) hipEventDestroy(b->t0);
+  if (b->t1) hipEventDestroy(b->t1);
+  if (b->main) hipStreamDestroy(b->main);
+  delete b;
+}
+
+const char* pikgpu_batch_last_error(const pikgpu_batch* b) {
+  return b ? b->error.c_str() : "null batch";
+}
+
+int pikgpu_batch_upload(pikgpu_batch* b, const pikgpu_frame* frames, int n) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  if (n < 1 || frames == nullptr) return BFail(b, PIKGPU_EINVAL, "empty batch");
+  BHIP_TRY(b, hipSetDevice(b->device));
+  BHIP_TRY(b, hipStreamSynchronize(b->main));
+  DropGraph(b);
+  b->ran = false;
+  b->n = 0;
+  while (int(b->items.size()) < n) {
+    pikgpu_ctx* c = pikgpu_create(b->device, &b->config);
+    if (c == nullptr) return BFail(b, PIKGPU_EHIP, "context creation failed");
+    b->items.push_back(c);
+  }
+  for (int i = 0; i < n; ++i) {
+    const int rc = pikgpu_frame_upload(b->items[i], &frames[i]);
+    if (rc != PIKGPU_OK) {
+      return BFail(b, rc, "frame " + std::to_string(i) + ": " +
+                              b->items[i]->error);
+    }
+  }
+  b->n = n;
+  return PIKGPU_OK;
+}
+
+int pikgpu_batch_run(pikgpu_batch* b) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  if (b->n == 0) return BFail(b, PIKGPU_ESTATE, "no frames uploaded");
+  BHIP_TRY(b, hipSetDevice(b->device));
+  int rc;
+  if (b->exec == nullptr && (rc = BuildGraph(b)) != PIKGPU_OK) return rc;
+  BHIP_TRY(b, hipEventRecord(b->t0, b->main));
+  BHIP_TRY(b, hipGraphLaunch(b->exec, b->main));
+  BHIP_TRY(b, hipEventRecord(b->t1, b->main));
+  b->ran = true;
+  return PIKGPU_OK;
+}
+
+int pikgpu_batch_sync(pikgpu_batch* b) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  BHIP_TRY(b, hipSetDevice(b->device));
+  BHIP_TRY(b, hipStreamSynchronize(b->main));
+  if (b->ran) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->t0, b->t1) == hipSuccess) b->last_ms = ms;
+  }
+  return PIKGPU_OK;
+}
+
+float pikgpu_batch_last_ms(const pikgpu_batch* b) {
+  return b ? b->last_ms : 0.0f;
+}
+
+int pikgpu_batch_download(pikgpu_batch* b, int i, float* out_planes[3],
+                          size_t out_stride_bytes) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  if (!b->ran) return BFail(b, PIKGPU_ESTATE, "pikgpu_batch_run has not run");
+  if (i < 0 || i >= b->n) return BFail(b, PIKGPU_EINVAL, "frame index");
+  int rc = pikgpu_batch_sync(b);
+  if (rc != PIKGPU_OK) return rc;
+  rc = pikgpu_download(b->items[i], out_planes, out_stride_bytes);
+  if (rc != PIKGPU_OK) return BFail(b, rc, b->items[i]->error);
+  return PIKGPU_OK;
+}
+
+int pikgpu_reconstruct_batch(pikgpu_batch* b, const pikgpu_frame* frames,
+                             int n, float* const* out_planes,
+                             const size_t* out_stride_bytes) {
+  int rc = pikgpu_batch_upload(b, frames, n);
+  if (rc != PIKGPU_OK) return rc;
+  if ((rc = pikgpu_batch_run(b)) != PIKGPU_OK) return rc;
+  if (out_planes == nullptr || out_stride_bytes == nullptr) {
+    return BFail(b, PIKGPU_EINVAL, "null outputs");
+  }
+  for (int i = 0; i < n; ++i) {
+    float* planes[3] = {out_planes[3 * i], out_planes[3 * i + 1],
+                        out_planes[3 * i + 2]};
+    if ((rc = pikgpu_batch_download(b, i, planes, out_stride_bytes[i])) !=
+        PIKGPU_OK) {
+      return rc;
+    }
+  }
+  return PIKGPU_OK;
+}

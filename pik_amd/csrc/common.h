@@ -67,6 +67,9 @@ __host__ __device__ inline int Mirror(int x, int size) {
 
 // Per-frame constants derived on the host and passed to every kernel by
 // value (one struct, < 4 KiB, lives in the kernel argument segment).
+// Output formats of the final kernel.
+enum { kOutLinear = 1, kOutSrgb8 = 2 };
+
 struct FrameParams {
   int xsize, ysize;          // pixels
   int bx, by;                // blocks
@@ -78,6 +81,7 @@ struct FrameParams {
   int ar;                    // adaptive reconstruction on
   int use_sharpened;
   int num_dots;
+  int out_flags;             // kOutLinear | kOutSrgb8
   int dot_tiles;             // 32x32 tiles touched by dots
   int dot_tiles_x;           // tiles per row (ceil(px / 32))
   float inv_global_scale;    // Quantizer::InvGlobalScale()

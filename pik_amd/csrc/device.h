@@ -60,10 +60,12 @@ struct FrameBuffers {
   float* dots_img;        // stage dump only
   float* gab;             // stage dump only
   float* out;             // 3 planes ysize x opitch
+  uint8_t* srgb8;         // ysize rows of spitch bytes, interleaved RGB
   float* mm_partial;      // [num_wg][6]
   float* mm;              // [8]: min[3], max[3], stretch/mul, all_min
   int ipitch;             // floats per row of idct/ar
   int opitch;             // floats per row of out
+  int spitch;             // bytes per row of srgb8
   int keep;               // write stage dumps
 };
 

@@ -116,6 +116,18 @@ __device__ inline uint32_t PkMinU16(uint32_t a, uint32_t b) {
                                           __builtin_bit_cast(U16x2, b)));
 }
 
+// Linear sample in [0, 255] -> 8-bit sRGB: the sRGB transfer function
+// (IEC 61966-2-1) of the [0, 1]-scaled value, clamped to [0, 1], times 255
+// and rounded as the reference's external-image conversion does
+// (CastClip01 and ToExternal, pik/external_image.cc:190-197, 666-696).
+__device__ inline uint8_t Srgb8(float linear255) {
+  const float v = linear255 * (1.0f / 255);
+  float e = v <= 0.0031308f ? 12.92f * v
+                            : 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+  e = fminf(fmaxf(e, 0.0f), 1.0f);
+  return static_cast<uint8_t>(static_cast<int>(e * 255.0f + 0.5f));
+}
+
 __device__ inline uint32_t LdsDword(const uint8_t* p) {
   return *reinterpret_cast<const uint32_t*>(p);
 }
@@ -725,10 +737,18 @@ __global__ void __launch_bounds__(256)
   lr = lr + tr;
   lg = lg + tg;
   lb = lb + tb;
-  const size_t oplane = size_t(P.ysize) * B.opitch;
-  B.out[size_t(y) * B.opitch + x] = lr;
-  B.out[oplane + size_t(y) * B.opitch + x] = lg;
-  B.out[2 * oplane + size_t(y) * B.opitch + x] = lb;
+  if (P.out_flags & kOutLinear) {
+    const size_t oplane = size_t(P.ysize) * B.opitch;
+    B.out[size_t(y) * B.opitch + x] = lr;
+    B.out[oplane + size_t(y) * B.opitch + x] = lg;
+    B.out[2 * oplane + size_t(y) * B.opitch + x] = lb;
+  }
+  if (P.out_flags & kOutSrgb8) {
+    uint8_t* o = B.srgb8 + size_t(y) * B.spitch + 3 * x;
+    o[0] = Srgb8(lr);
+    o[1] = Srgb8(lg);
+    o[2] = Srgb8(lb);
+  }
 }
 
 __global__ void __launch_bounds__(256)
