@@ -13,6 +13,15 @@ Prints one JSON line. `roofline` covers the dominant kernel, timed with the
 library's HIP events on the stream it launches on; `cpu_baseline` times the
 reference's own AVX2 build (oracle/_ref, the reference's ThreadPool) on the
 same frame on this host's cores.
+
+Two further figures ride on the same line:
+  batch    256 frames of seeded uneven sizes (W, H ~ U[256, 2048], distance
+           ~ U[0.5, 3.0]; 32 distinct synthetic frames cycled) reconstructed
+           by one graph launch per step (pikgpu_batch_run); MP/s.
+  fwd_adj  forward model + adjoint (pikgpu_fwd_adj, device pointers) of a
+           2048x2048 frame, loss sum (linear - target)^2, gradient with
+           respect to the 256x256 float quant field and the dequantised
+           coefficients; MP/s.
 """
 import argparse
 import json
@@ -68,6 +77,12 @@ def parse_args():
                     help="threads of the CPU baseline (0: host cores, "
                     "at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-frames", type=int, default=256,
+                    help="frames per batch (0: skip the batch figure)")
+    ap.add_argument("--batch-distinct", type=int, default=32,
+                    help="distinct synthetic frames cycled over the batch")
+    ap.add_argument("--adj-size", type=int, default=2048,
+                    help="frame size of the fwd_adj figure (0: skip)")
     return ap.parse_args()
 
 
@@ -120,6 +135,110 @@ def cpu_baseline(frame, seconds, threads):
                       % (reps, frame.xsize, frame.ysize, threads, el)}
 
 
+def _batch_frame(args):
+    """Synthetic frame i of the batch: size and distance drawn from the
+    seeded generator."""
+    i, seed = args
+    from pik_amd import synth
+    import numpy as np
+    rng = np.random.default_rng(seed * 1000003 + i)
+    w, h = (int(v) for v in rng.integers(256, 2049, size=2))
+    d = float(rng.uniform(0.5, 3.0))
+    return synth.make_frame(w, h, seed=seed * 1000 + i, distance=d,
+                            num_rects=40, num_dots=4)
+
+
+def synth_batch(n, distinct, seed):
+    """The batch's frames, synthesised by a process pool (run before the
+    GPU is touched: the workers are forked)."""
+    from multiprocessing import Pool
+    t = time.perf_counter()
+    distinct = max(1, min(distinct, n))
+    with Pool(min(distinct, host_threads())) as pool:
+        pool_frames = pool.map(_batch_frame,
+                               [(i, seed) for i in range(distinct)])
+    frames = [pool_frames[i % distinct] for i in range(n)]
+    print("batch: %d frames (%d distinct), %.1f MP, synthesised in %.1f s"
+          % (n, distinct, sum(f.xsize * f.ysize for f in frames) / 1e6,
+             time.perf_counter() - t), file=sys.stderr, flush=True)
+    return frames, distinct
+
+
+def bench_batch(frames, distinct, steps, warmup):
+    """MP/s of pikgpu_batch_run over the frames (device-resident, one graph
+    launch per step)."""
+    from pik_amd.runtime import BatchReconstructor
+    n = len(frames)
+    mp = sum(f.xsize * f.ysize for f in frames) / 1e6
+    with BatchReconstructor(0) as B:
+        B.upload(frames)
+        for _ in range(max(1, warmup)):
+            B.run()
+            B.sync()
+        dev_ms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            B.run()
+            B.sync()
+            dev_ms += B.last_ms
+        el = time.perf_counter() - t0
+    return {"value": round(mp * steps / el, 2), "unit": "MP/s",
+            "ms_per_step": round(el * 1000.0 / steps, 3),
+            "device_ms_per_step": round(dev_ms / steps, 3),
+            "frames": n, "distinct_frames": distinct,
+            "megapixels": round(mp, 2), "steps": steps,
+            "workload": "batch of %d frames, W,H ~ U[256,2048], d ~ "
+                        "U[0.5,3.0], one graph launch per step" % n}
+
+
+# Algorithmic HBM bytes per pixel of fwd_adj: int16 AC in (6), target in
+# (12), dL/dd out (12), plus per block the float and integer quant field,
+# strategy byte and dL/dqf (13 B / 64).
+ADJ_PATH_BYTES_PER_PX = 6.0 + 12.0 + 12.0 + 13.0 / 64
+
+
+def bench_fwd_adj(size, steps, warmup, seed):
+    """MP/s of pikgpu_fwd_adj on a size x size frame, device pointers."""
+    import numpy as np
+    from pik_amd import synth
+    from pik_amd.devmem import DeviceArray
+    from pik_amd.runtime import Reconstructor
+    frame = synth.make_frame(size, size, seed=seed + 17, distance=1.0,
+                             num_rects=100, num_dots=8)
+    by, bx = frame.quant_field.shape
+    with Reconstructor(0) as R:
+        target = R.reconstruct(frame)
+        R.upload(frame)
+        rng = np.random.default_rng(seed)
+        qf = DeviceArray.from_numpy((frame.quant_field * rng.uniform(
+            0.8, 1.25, (by, bx))).astype(np.float32))
+        tt = DeviceArray.from_numpy(target)
+        dqf = DeviceArray((by, bx), np.float32)
+        dc = DeviceArray((3, by, bx * 64), np.float32)
+        loss = DeviceArray.zeros((1,), np.float64)
+        for _ in range(max(1, warmup)):
+            R.fwd_adj_device(qf, dqf, target=tt, dl_dcoeffs=dc, loss=loss)
+            R.sync()
+        fams = {}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.fwd_adj_device(qf, dqf, target=tt, dl_dcoeffs=dc, loss=loss)
+            R.sync()
+            for k in ("adj_forward", "adj_backward"):
+                fams[k] = fams.get(k, 0.0) + R.timings()[k]
+        el = time.perf_counter() - t0
+    mp = frame.xsize * frame.ysize / 1e6
+    value = mp * steps / el
+    return {"value": round(value, 2), "unit": "MP/s",
+            "ms_per_step": round(el * 1000.0 / steps, 4),
+            "kernel_ms": {k: round(v / steps, 4) for k, v in fams.items()},
+            "path_GBps": round(ADJ_PATH_BYTES_PER_PX * value / 1e3, 2),
+            "steps": steps,
+            "workload": "fwd+adj %dx%d d=1.0, %dx%d float quant field, "
+                        "loss sum (linear - target)^2, dL/dqf + dL/dd"
+                        % (size, size, by, bx)}
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -127,6 +246,9 @@ def main():
               "runs on a single MI355X (multi-GPU sharding is not part of "
               "this build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    batch = None
+    if a.batch_frames > 0:
+        batch = synth_batch(a.batch_frames, a.batch_distinct, a.seed)
     from pik_amd import synth
     from pik_amd.runtime import Reconstructor
 
@@ -203,6 +325,16 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    R.close()
+    if batch:
+        out["batch"] = bench_batch(batch[0], batch[1], max(1, a.steps // 4),
+                                   a.warmup)
+        print("batch: " + json.dumps(out["batch"]), file=sys.stderr,
+              flush=True)
+    if a.adj_size > 0:
+        out["fwd_adj"] = bench_fwd_adj(a.adj_size, a.steps, a.warmup, a.seed)
+        print("fwd_adj: " + json.dumps(out["fwd_adj"]), file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -263,8 +263,9 @@ PIKGPU_API int pikgpu_stage_size(pikgpu_ctx* ctx, int stage, uint32_t* xsize,
 PIKGPU_API int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
                       size_t stride_bytes);
 
-/* Device-side timing of the last pikgpu_run, per kernel family, in
- * milliseconds, measured with HIP events on the context stream.
+/* Device-side timing of the last pikgpu_run (and, in the adj_* entries, of
+ * the last pikgpu_fwd_adj), per kernel family, in milliseconds, measured
+ * with HIP events on the context stream.
  * names[i] points to static strings. Returns the number of entries written
  * (<= max_entries). */
 PIKGPU_API int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* ms,
@@ -305,6 +306,50 @@ PIKGPU_API int pikgpu_reconstruct_batch(pikgpu_batch* batch,
                                         const pikgpu_frame* frames, int n,
                                         float* const* out_planes,
                                         const size_t* out_stride_bytes);
+
+/* ------------------------------------------------------------------------
+ * Forward model and adjoint for a quant-field search. The reference's
+ * encoder searches the quant field by repeated round trips, each a full
+ * quantise + decode + Butteraugli (FindBestQuantization,
+ * pik/adaptive_quantization.cc:713-852, decoding through RoundtripImage,
+ * :580-671); it has no gradient. This evaluates the decode of the uploaded
+ * frame as a function of a real-valued quant field qf (one value per 8x8
+ * block) and the gradient of a loss with respect to qf:
+ *   d      = AdjustQuantBias(q) * dequant matrix * (inv_global_scale / qf)
+ *            (the table stays the one the frame's integer field selects)
+ *   linear = OpsinToLinear(crop(Gaborish(IDCT(LLF/LF + CfL(d)))))
+ * i.e. the frame's decode with HF prediction, adaptive reconstruction and
+ * dots switched off; with qf equal to the integer field, `linear` is bit for
+ * bit pikgpu_run's output for a frame with those three flags off.
+ *
+ * The loss is sum (linear - target)^2 when target[0] is set (dL/dlinear =
+ * 2 (linear - target)); otherwise dl_dlinear supplies dL/dlinear. */
+enum { PIKGPU_ADJ_DEVICE_POINTERS = 1 };
+
+typedef struct pikgpu_adj_io {
+  uint32_t flags;               /* PIKGPU_ADJ_DEVICE_POINTERS: every pointer
+                                   below is device memory and the call is
+                                   asynchronous on pikgpu_stream(ctx) */
+  const float* qf;              /* ysize_blocks rows, finite and > 0 */
+  size_t qf_stride;             /* floats per row */
+  const float* target[3];       /* xsize x ysize linear planes, or NULL */
+  const float* dl_dlinear[3];   /* used when target[0] is NULL */
+  size_t in_stride;             /* floats per row of target / dl_dlinear */
+  float* linear[3];             /* optional forward output (all or none) */
+  size_t linear_stride;         /* floats per row */
+  float* dl_dqf;                /* ysize_blocks rows of xsize_blocks */
+  size_t dqf_stride;            /* floats per row */
+  float* dl_dcoeffs[3];         /* optional dL/dd per channel, before CfL
+                                   (zero at the LLF positions): ysize_blocks
+                                   rows of xsize_blocks * 64 (all or none) */
+  size_t dcoeffs_stride;        /* floats per row */
+  double* loss;                 /* optional, with target: the loss */
+} pikgpu_adj_io;
+
+/* One forward + adjoint evaluation on the uploaded frame. Host pointers:
+ * returns when the outputs are written. Device pointers: enqueued on the
+ * context stream (pikgpu_sync waits); qf values are not checked. */
+PIKGPU_API int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -1,26 +1,26 @@
 """Float64 reference of the differentiable reconstruction model and of its
 gradient. Test infrastructure only (the checker of pikgpu_fwd_adj).
 
-The model is the part of the decode that the quant field acts on directly,
-dequant -> CfL -> IDCT -> Gaborish -> OpsinInverse, with the frame's integer
-data held fixed and a real-valued quant field qf (one value per 8x8 block):
+The model is the decode of the frame with HF prediction, adaptive
+reconstruction and dots switched off, as a function of a real-valued quant
+field qf (one value per 8x8 block), the frame's integer data held fixed:
 
-  C[c][k]  = AdjustQuantBias_c(q) * M_c[k] * inv_global_scale / qf_block
-             at AC positions (pik/compressed_image.cc:621-693; the dequant
-             table is chosen by the frame's integer quant field), and the
-             lowest-frequency coefficients the decoder derives from the DC
-             at the LLF positions (pik/ac_predictions.cc:380-405, 580-590);
-  C'_X = C_X + ytox * C_Y, C'_B = C_B + ytob * C_Y at AC positions;
-  P    = TransformToPixels(C') per transform (pik/ac_strategy.cc:674-840);
-  G    = ConvolveGaborish(P) (mirrored borders), cropped to xsize x ysize;
-  L    = OpsinToLinear(G) (pik/opsin_inverse.cc:39-123).
+  d[c][k] = AdjustQuantBias_c(q) * M_c[k] * inv_global_scale / qf_block
+            (pik/compressed_image.cc:621-693; the dequant table is chosen
+            by the frame's integer quant field);
+  C'      = the LLF values at the LLF positions; elsewhere
+            C'_Y = d_Y, C'_X = d_X + ytox * d_Y, C'_B = d_B + ytob * d_Y,
+            plus the LF predictions at the LF positions when predict_lf
+            (pik/ac_predictions.cc:380-470, 560-620);
+  P       = TransformToPixels(C') per transform (pik/ac_strategy.cc:674-840);
+  G       = ConvolveGaborish(P) (mirrored borders), cropped to xsize x ysize;
+  L       = OpsinToLinear(G) (pik/opsin_inverse.cc:39-123).
 
-Predictions, EPF/AR and dots are not part of the model. Every linear step is
-built here as an explicit float64 operator, so the gradient is the exact
+The LLF values and LF predictions depend on the DC only; they are taken from
+the oracle's decode of the frame with all AC values zero. Every linear step
+is built here as an explicit float64 operator, so the gradient is the exact
 transpose (no finite differences, no autodiff).
 """
-import ctypes
-
 import numpy as np
 
 from pik_amd import abi
@@ -206,22 +206,36 @@ NEG_BIAS = -np.array([np.float32(0.26786006338144885 / np.float32(255.0)),
 
 # ------------------------------------------------------------ model
 
-def llf_coefficients(frame):
-    """LLF coefficient image from the oracle (3, by, bx*64) float64."""
+def model_frame(frame):
+    """The frame as the model decodes it: HF prediction, adaptive
+    reconstruction and dots switched off, everything else unchanged."""
+    f = frame.copy()
+    f.predict_hf = 0
+    f.adaptive_reconstruction = 0
+    f.dots = f.dots[:0]
+    return f
+
+
+def fixed_coefficients(frame):
+    """The coefficient image the decoder builds when every quantized AC
+    value is zero: the LLF values (from the DC) at the LLF positions and,
+    with predict_lf, the LF predictions at the LF positions
+    (pik/ac_predictions.cc:560-620). These do not depend on the quant field.
+    Taken from the oracle's own decode (coefficient stage); float64."""
     from oracle import port
-    L = port.lib()
-    L.pikoracle_llf.argtypes = [ctypes.POINTER(abi.FrameC), ctypes.c_void_p]
-    fc, keep = frame.to_c()
-    out = np.zeros((3, frame.ysize_blocks, frame.xsize_blocks * 64),
-                   np.float32)
-    ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
-    rc = L.pikoracle_llf(ctypes.byref(fc), ctypes.cast(ptrs, ctypes.c_void_p))
-    assert rc == 0
-    return out.astype(np.float64)
+    z = model_frame(frame)
+    z.ac = np.zeros_like(z.ac)
+    _, st = port.recon(z, stages=True)
+    return st["coeffs"].astype(np.float64)
 
 
 class Model:
-    """Float64 forward and adjoint for one frame."""
+    """Float64 forward and adjoint for one frame.
+
+    Coefficients: d = dequantised AC per channel (before CfL) for a real
+    quant field; C' = CfL(d) + fixed outside the LLF positions, fixed at
+    them. The gradient returned for the coefficients is dL/dd, zero at the
+    LLF positions (d has no effect there)."""
 
     def __init__(self, frame):
         self.f = frame
@@ -249,8 +263,7 @@ class Model:
                 np.float32(0.145)) / np.where(q[c] == 0, 1, q[c]),
                 np.where(aq[c] >= 0.5, np.sign(q[c]) * one, 0.0))
         self.inv_gs = float(np.float32(65536.0 / frame.global_scale))
-        # LLF positions and values
-        self.llf = llf_coefficients(frame)
+        self.fixed = fixed_coefficients(frame)
         self.llf_mask = np.zeros((by, bx * 64), bool)
         self.blocks = {}
         for y0, x0 in zip(*np.nonzero(idx == 0)):
@@ -278,22 +291,26 @@ class Model:
         return out
 
     def dequant(self, qf):
-        """Dequantised coefficients (3, by, bx*64): AC values for quant field
-        qf, LLF values at the LLF positions."""
+        """Dequantised AC coefficients d (3, by, bx*64) for quant field qf
+        (by, bx), before CfL."""
         scale = self.inv_gs / np.repeat(np.asarray(qf, np.float64), 64, axis=1)
-        d = self.adj * self.mult * scale[None]
-        return np.where(self.llf_mask[None], self.llf, d), d
+        return self.adj * self.mult * scale[None]
+
+    def coefficients(self, d):
+        """C' from d: CfL outside the LLF positions, plus the fixed part."""
+        Cp = d.copy()
+        Cp[0] = d[0] + self.cfx * d[1]
+        Cp[2] = d[2] + self.cfb * d[1]
+        return np.where(self.llf_mask[None], self.fixed, Cp + self.fixed)
 
     def forward(self, qf):
         """Returns (linear (3, ysize, xsize), cache)."""
-        C, d = self.dequant(qf)
-        lin, cache = self.forward_coeffs(C)
+        d = self.dequant(qf)
+        lin, cache = self.forward_d(d)
         return lin, (d,) + cache
 
-    def forward_coeffs(self, C):
-        Cp = C.copy()
-        Cp[0] = np.where(self.llf_mask, C[0], C[0] + self.cfx * C[1])
-        Cp[2] = np.where(self.llf_mask, C[2], C[2] + self.cfb * C[1])
+    def forward_d(self, d):
+        Cp = self.coefficients(d)
         P = np.zeros((3, self.by * 8, self.bx * 8))
         for c in range(3):
             for r in self.blocks:
@@ -310,7 +327,7 @@ class Model:
         return lin, ((gr, gg, gb),)
 
     def adjoint(self, qf, dlin, cache):
-        """Returns (dL/dqf (by, bx), dL/dC (3, by, bx*64))."""
+        """Returns (dL/dqf (by, bx), dL/dd (3, by, bx*64))."""
         d, (gr, gg, gb) = cache
         dmix = np.einsum("ij,iyx->jyx", self.minv, dlin)
         dgr, dgg, dgb = 3 * gr ** 2 * dmix[0], 3 * gg ** 2 * dmix[1], \
@@ -331,16 +348,17 @@ class Model:
                 for i, (y0, x0) in enumerate(self.blocks[r]):
                     dCp[c, y0:y0 + s, x0 * 64:x0 * 64 + 64 * s] = \
                         coef[i].reshape(s, 64 * s)
-        dC = dCp.copy()
-        ac = ~self.llf_mask
-        dC[1] = dCp[1] + np.where(ac, self.cfx * dCp[0] + self.cfb * dCp[2], 0)
+        ac = ~self.llf_mask[None]
+        dd = np.where(ac, dCp, 0.0)
+        dd[1] = dd[1] + np.where(ac[0], self.cfx * dCp[0] + self.cfb * dCp[2],
+                                 0.0)
         q = np.repeat(np.asarray(qf, np.float64), 64, axis=1)
-        per = np.where(ac[None], dC * (-d / q[None]), 0.0).sum(axis=0)
+        per = (dd * (-d / q[None])).sum(axis=0)
         dqf = per.reshape(self.by, self.bx, 64).sum(axis=2)
-        return dqf, dC
+        return dqf, dd
 
     def loss_and_grad(self, qf, target):
         lin, cache = self.forward(qf)
         diff = lin - target
-        dqf, dC = self.adjoint(qf, 2 * diff, cache)
-        return float((diff ** 2).sum()), lin, dqf, dC
+        dqf, dd = self.adjoint(qf, 2 * diff, cache)
+        return float((diff ** 2).sum()), lin, dqf, dd

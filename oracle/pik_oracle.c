@@ -1312,85 +1312,6 @@ static int reconstruct_group(const Ctx* k, int gx, int gy, float* const dc[3],
   return 0;
 }
 
-/* The lowest-frequency coefficients every first block gets from the
- * reconstructed DC (ComputeLlf per group, then the LLF write of
- * UpdateLfForDecoder): the coefficient image with LLF values at the LLF
- * positions and zeros elsewhere. */
-static void llf_coefficients(const Ctx* k, float* const dc[3],
-                             float* const ac[3]) {
-  const size_t row = (size_t)k->bx * 64;
-  for (int gy = 0; gy < (int)div_ceil(k->by, kGroupBlocks); ++gy) {
-    for (int gx = 0; gx < (int)div_ceil(k->bx, kGroupBlocks); ++gx) {
-      const int x0 = gx * kGroupBlocks, y0 = gy * kGroupBlocks;
-      const int gw = (k->bx - x0) < kGroupBlocks ? k->bx - x0 : kGroupBlocks;
-      const int gh = (k->by - y0) < kGroupBlocks ? k->by - y0 : kGroupBlocks;
-      const int cw = gw + 2, ch = gh + 2;
-      float* cdc = (float*)malloc(sizeof(float) * 2 * cw * ch);
-      if (!cdc) return;
-      float* llf = cdc + cw * ch;
-      for (int c = 0; c < 3; ++c) {
-        for (int y = 0; y < ch; ++y) {
-          const size_t sy = source_coord(y + y0, k->by);
-          for (int x = 0; x < cw; ++x) {
-            cdc[y * cw + x] = dc[c][sy * k->bx + source_coord(x + x0, k->bx)];
-          }
-        }
-        for (int y = 0; y < ch; ++y) {
-          for (int x = 0; x < cw; ++x) {
-            const int border = x == 0 || y == 0 || x == cw - 1 || y == ch - 1;
-            if (!border && !acs_first_at(k, x0 + x - 1, y0 + y - 1)) continue;
-            llf_from_dc(border ? kDCT : acs_raw_at(k, x0 + x - 1, y0 + y - 1),
-                        cdc + y * cw + x, cw, llf + y * cw + x, cw);
-          }
-        }
-        for (int by = 0; by < gh; ++by) {
-          for (int bx = 0; bx < gw; ++bx) {
-            if (!acs_first_at(k, x0 + bx, y0 + by)) continue;
-            const int s = covered(acs_raw_at(k, x0 + bx, y0 + by));
-            float* a = ac[c] + (size_t)(y0 + by) * row + (size_t)(x0 + bx) * 64;
-            for (int y = 0; y < s; ++y) {
-              for (int x = 0; x < s; ++x) {
-                a[s * 8 * y + x] = llf[(by + 1 + y) * cw + bx + 1 + x];
-              }
-            }
-          }
-        }
-      }
-      free(cdc);
-    }
-  }
-}
-
-int pikoracle_llf(const pikgpu_frame* f, float* const out[3]) {
-  init_tables();
-  int rc = check_frame(f);
-  if (rc) return rc;
-  Ctx k;
-  k.f = f;
-  k.bx = (int)div_ceil(f->xsize, 8);
-  k.by = (int)div_ceil(f->ysize, 8);
-  k.px = 8 * k.bx;
-  k.py = 8 * k.by;
-  k.tx = (int)div_ceil(f->xsize, 64);
-  k.inv_global_scale = (float)(1.0 * 65536 / f->global_scale);
-  k.inv_quant_dc = k.inv_global_scale / f->quant_dc;
-  k.table = f->dequant_table;
-  k.offsets = f->dequant_offsets;
-  const size_t nb = (size_t)k.bx * k.by;
-  float* mem = (float*)calloc(6 * nb, sizeof(float));
-  if (!mem) return fail(PIKGPU_ENOMEM, "out of memory");
-  float *dcq[3], *dc[3];
-  for (int c = 0; c < 3; ++c) {
-    dcq[c] = mem + c * nb;
-    dc[c] = mem + (3 + c) * nb;
-    memset(out[c], 0, sizeof(float) * nb * 64);
-  }
-  rc = reconstruct_dc(&k, dcq, dc);
-  if (!rc) llf_coefficients(&k, dc, out);
-  free(mem);
-  return rc;
-}
-
 /* ------------------------------------------------------------------------ */
 /* AdaptiveReconstruction's clamp, pik/adaptive_reconstruction.cc:73-244,   */
 /* 334-471: per transform, the DCT of (filtered - original) is scaled by    */

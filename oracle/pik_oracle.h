@@ -30,12 +30,6 @@ int pikoracle_recon(const pikgpu_frame* frame, float* const* stages,
 
 const char* pikoracle_last_error(void);
 
-/* The lowest-frequency coefficients the decoder derives from the DC
- * (DequantDC, AdaptiveDCReconstruction, ComputeLlf): 3 planes of
- * ysize_blocks x (xsize_blocks * 64) floats, LLF values at the LLF
- * positions of every first block, zeros elsewhere. */
-int pikoracle_llf(const pikgpu_frame* frame, float* const out[3]);
-
 /* One block transform of raw strategy `raw`: dir 0 = TransformToPixels,
  * 1 = TransformFromPixels; contiguous N x N in and out (N = 8, 16, 32). */
 int pikoracle_block_transform(int raw, int dir, const float* in, float* out);

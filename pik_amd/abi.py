@@ -98,6 +98,28 @@ class ConfigC(ctypes.Structure):
 _lib = None
 
 
+ADJ_DEVICE_POINTERS = 1
+
+
+class AdjIoC(ctypes.Structure):
+    """pikgpu_adj_io."""
+    _fields_ = [
+        ("flags", ctypes.c_uint32),
+        ("qf", ctypes.c_void_p),
+        ("qf_stride", ctypes.c_size_t),
+        ("target", ctypes.c_void_p * 3),
+        ("dl_dlinear", ctypes.c_void_p * 3),
+        ("in_stride", ctypes.c_size_t),
+        ("linear", ctypes.c_void_p * 3),
+        ("linear_stride", ctypes.c_size_t),
+        ("dl_dqf", ctypes.c_void_p),
+        ("dqf_stride", ctypes.c_size_t),
+        ("dl_dcoeffs", ctypes.c_void_p * 3),
+        ("dcoeffs_stride", ctypes.c_size_t),
+        ("loss", ctypes.c_void_p),
+    ]
+
+
 def load():
     """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
     global _lib
@@ -152,6 +174,7 @@ def load():
         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     lib.pikgpu_validate_frame.argtypes = [ctypes.POINTER(FrameC),
                                           ctypes.c_char_p, ctypes.c_size_t]
+    lib.pikgpu_fwd_adj.argtypes = [vp, ctypes.POINTER(AdjIoC)]
     _lib = lib
     return lib
 
@@ -165,5 +188,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_download_srgb8", "pikgpu_batch_create", "pikgpu_batch_destroy",
     "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
-    "pikgpu_reconstruct_batch",
+    "pikgpu_reconstruct_batch", "pikgpu_fwd_adj",
 ]

@@ -18,7 +18,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIKGPU_ARCH", "gfx950")
 
-SOURCES = ["recon_lf.hip", "recon_blocks.hip", "filter.hip", "api.hip",
+SOURCES = ["recon_lf.hip", "recon_blocks.hip", "filter.hip", "adjoint.hip",
+           "api.hip",
            "host_tables.cpp", "quant_tables.cpp"]
 HEADERS = ["common.h", "device.h", "kernels.h", "xform.h", "dct1d.h",
            "dct_consts.h"]
@@ -64,6 +65,7 @@ def build(verbose=False):
     # Host build of the transforms and tables for the CPU test suite.
     tmp = HOST_OUT + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+          "-Wno-unknown-pragmas",
           "-fvisibility=hidden", "-Wall", "-o", tmp] +
          [os.path.join(CSRC, s) for s in HOST_SOURCES])
     os.replace(tmp, HOST_OUT)

@@ -23,10 +23,16 @@ using namespace pikgpu;
 
 namespace {
 
-enum Timer { kTDc, kTLf, kTBlocks, kTMinMax, kTEpf, kTDots, kTFinal, kNumTimers };
-const char* kTimerNames[kNumTimers] = {"dc",   "lf_predict", "blocks_idct",
-                                       "minmax", "epf_ar",   "dots",
-                                       "final_opsin"};
+// Kernel families timed by pikgpu_run (events ev[0..kNumRunTimers]) and by
+// pikgpu_fwd_adj (adj_ev[0..2]).
+enum Timer {
+  kTDc, kTLf, kTBlocks, kTMinMax, kTEpf, kTDots, kTFinal, kNumRunTimers,
+  kTAdjForward = kNumRunTimers, kTAdjBackward, kNumTimers
+};
+const char* kTimerNames[kNumTimers] = {"dc",     "lf_predict", "blocks_idct",
+                                       "minmax", "epf_ar",     "dots",
+                                       "final_opsin", "adj_forward",
+                                       "adj_backward"};
 
 struct DevBuf {
   void* p = nullptr;
@@ -46,6 +52,10 @@ struct pikgpu_ctx {
       dotbins;
   DevBuf dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
       dots_img, gab, out, mm_partial, mm, ar_keep, srgb8;
+  // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
+  DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
+  bool adj_ready = false;  // LLF / LF caches hold the uploaded frame's
+  bool adj_timed = false;
   bool uploaded = false;
   bool ran = false;
   bool kept = false;
@@ -56,7 +66,8 @@ struct pikgpu_ctx {
   FrameParams P;
   FrameBuffers B;
   int num_tables = 0;
-  hipEvent_t ev[kNumTimers + 1] = {};
+  hipEvent_t ev[kNumRunTimers + 1] = {};
+  hipEvent_t adj_ev[3] = {};
   float timings[kNumTimers] = {};
 };
 
@@ -100,7 +111,8 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
                    &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
-                   &ctx->ar_keep};
+                   &ctx->ar_keep, &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
+                   &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss};
   for (DevBuf* b : all) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
@@ -160,8 +172,10 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
     delete ctx;
     return nullptr;
   }
-  for (int i = 0; i <= kNumTimers; ++i) {
-    if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+  for (int i = 0; i <= kNumRunTimers + 3; ++i) {
+    hipEvent_t* e = i <= kNumRunTimers ? &ctx->ev[i]
+                                       : &ctx->adj_ev[i - kNumRunTimers - 1];
+    if (hipEventCreate(e) != hipSuccess) {
       delete ctx;
       return nullptr;
     }
@@ -174,6 +188,7 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
   ComputeHostTables(&ctx->host);
   memcpy(h->mul_table, ctx->host.mul_table, sizeof(h->mul_table));
   OpsinInverse(h->opsin_inv);
+  AdjointNorms(h->adj_norm);
   if (hipMalloc(&ctx->consts, sizeof(DeviceConsts)) != hipSuccess ||
       hipMemcpy(ctx->consts, h, sizeof(DeviceConsts), hipMemcpyHostToDevice) !=
           hipSuccess) {
@@ -192,8 +207,11 @@ void pikgpu_destroy(pikgpu_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   FreeAll(ctx);
   if (ctx->consts) (void)hipFree(ctx->consts);
-  for (int i = 0; i <= kNumTimers; ++i) {
+  for (int i = 0; i <= kNumRunTimers; ++i) {
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  }
+  for (hipEvent_t e : ctx->adj_ev) {
+    if (e) (void)hipEventDestroy(e);
   }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -421,6 +439,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   ctx->uploaded = false;
   ctx->ran = false;
+  ctx->adj_ready = false;
   {
     std::string err;
     const int rc = ValidateFrame(f, &err);
@@ -623,6 +642,55 @@ static int EnsureStageBuffers(pikgpu_ctx* ctx) {
   return PIKGPU_OK;
 }
 
+// DC: dequant, min/max of the 8-padded DC image, EPF + clamp
+// (DequantDC + AdaptiveDCReconstruction).
+static void EnqueueDc(const FrameParams& P, const FrameBuffers& B,
+                      const DeviceConsts* K, hipStream_t st) {
+  const int nblk = P.bx * P.by;
+  KDcDequant<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B);
+  Img dci;
+  dci.p = B.dcq;
+  dci.plane = size_t(nblk);
+  dci.pitch = P.bx;
+  dci.W = int(DivCeil(P.bx, 8) * 8);
+  dci.H = int(DivCeil(P.by, 8) * 8);
+  dci.vw = P.bx;
+  dci.vh = P.by;
+  const dim3 mmg = MinMaxGrid(dci.W, dci.H);
+  KMinMaxPartial<<<mmg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
+  KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
+  const int grid = XcdGrid(int(DivCeil(dci.W, 32) * DivCeil(dci.H, 32)));
+  KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
+                             size_t(nblk));
+}
+
+// LF-domain predictions per group: LLF, LF predictions (predict_lf) and,
+// with `hf`, the pred2x2 image of the HF prediction.
+static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
+                      const DeviceConsts* K, hipStream_t st, bool hf) {
+  const int nblk = P.bx * P.by;
+  const size_t ngroups = size_t(P.gx) * P.gy;
+  const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
+  const int g2_pos = 4 * cache_pos;
+  dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
+  dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
+  KComputeLlf<<<gc, 256, 0, st>>>(P, B, K);
+  if (P.predict_lf) {
+    KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 0);
+    KLfBlur<<<g2, 256, 0, st>>>(P, B);
+    KLfFromDc2x2<<<gc, 256, 0, st>>>(P, B, K);
+  }
+  if (hf) {
+    KZeroPred<<<DivCeil(ngroups * 3 * kG2Plane, 256), 256, 0, st>>>(P, B);
+    if (P.predict_lf) {
+      KPred2x2Border<<<gc, 256, 0, st>>>(P, B, K);
+    } else {
+      KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
+    }
+    KPred2x2Interior<<<dim3(DivCeil(nblk, 256), 3), 256, 0, st>>>(P, B, K);
+  }
+}
+
 // Enqueues the whole reconstruction of the uploaded frame on stream `st`.
 // With `timing`, HIP events between the kernel families record their
 // durations (pikgpu_last_timings). Allocates nothing unless stage dumps or
@@ -642,52 +710,12 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     B.srgb8 = static_cast<uint8_t*>(ctx->srgb8.p);
   }
   const DeviceConsts* K = ctx->consts;
-  const int nblk = P.bx * P.by;
-  const size_t ngroups = size_t(P.gx) * P.gy;
   const size_t iplane = size_t(B.ipitch) * P.py;
 
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
-  // DC: dequant, min/max of the 8-padded DC image, EPF + clamp.
-  KDcDequant<<<DivCeil(nblk, 256), 256, 0, st>>>(P, B);
-  {
-    Img dci;
-    dci.p = B.dcq;
-    dci.plane = size_t(nblk);
-    dci.pitch = P.bx;
-    dci.W = int(DivCeil(P.bx, 8) * 8);
-    dci.H = int(DivCeil(P.by, 8) * 8);
-    dci.vw = P.bx;
-    dci.vh = P.by;
-    const dim3 mmg = MinMaxGrid(dci.W, dci.H);
-    KMinMaxPartial<<<mmg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
-    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
-    const int grid = XcdGrid(int(DivCeil(dci.W, 32) * DivCeil(dci.H, 32)));
-    KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
-                               size_t(nblk));
-  }
+  EnqueueDc(P, B, K, st);
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
-  // LF-domain predictions per group.
-  {
-    const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
-    const int g2_pos = 4 * cache_pos;
-    dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
-    dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
-    KComputeLlf<<<gc, 256, 0, st>>>(P, B, K);
-    if (P.predict_lf) {
-      KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 0);
-      KLfBlur<<<g2, 256, 0, st>>>(P, B);
-      KLfFromDc2x2<<<gc, 256, 0, st>>>(P, B, K);
-    }
-    if (P.predict_hf) {
-      KZeroPred<<<DivCeil(ngroups * 3 * kG2Plane, 256), 256, 0, st>>>(P, B);
-      if (P.predict_lf) {
-        KPred2x2Border<<<gc, 256, 0, st>>>(P, B, K);
-      } else {
-        KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
-      }
-      KPred2x2Interior<<<dim3(DivCeil(nblk, 256), 3), 256, 0, st>>>(P, B, K);
-    }
-  }
+  EnqueueLf(P, B, K, st, P.predict_hf != 0);
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   {
     const int grid = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
@@ -757,8 +785,18 @@ int pikgpu_sync(pikgpu_ctx* ctx) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->adj_timed) {
+    for (int i = 0; i < 2; ++i) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ctx->adj_ev[i], ctx->adj_ev[i + 1]) ==
+          hipSuccess) {
+        ctx->timings[kTAdjForward + i] = ms;
+      }
+    }
+    ctx->adj_timed = false;
+  }
   if (ctx->ran && ctx->timed) {
-    for (int i = 0; i < kNumTimers; ++i) {
+    for (int i = 0; i < kNumRunTimers; ++i) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) {
         ctx->timings[i] = ms;
@@ -922,6 +960,170 @@ int pikgpu_last_timings(pikgpu_ctx* ctx, const char** names, float* ms,
   return n;
 }
 
+// Forward model + adjoint (see pikgpu.h). The LLF / LF caches depend on the
+// DC only, so they are computed on the first call after an upload; each
+// call then runs KAdjForward and KAdjBackward (and KAdjLoss for the loss).
+int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (io == nullptr) return Fail(ctx, PIKGPU_EINVAL, "null io");
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  const FrameParams& P0 = ctx->P;
+  const size_t bx = size_t(P0.bx), by = size_t(P0.by);
+  const size_t xs = size_t(P0.xsize), ys = size_t(P0.ysize);
+  const bool dev = (io->flags & PIKGPU_ADJ_DEVICE_POINTERS) != 0;
+  if ((io->flags & ~uint32_t(PIKGPU_ADJ_DEVICE_POINTERS)) != 0) {
+    return Fail(ctx, PIKGPU_EINVAL, "unknown flags");
+  }
+  if (io->qf == nullptr || io->dl_dqf == nullptr) {
+    return Fail(ctx, PIKGPU_EINVAL, "qf and dl_dqf are required");
+  }
+  const bool use_target = io->target[0] != nullptr;
+  const float* const* in = use_target ? io->target : io->dl_dlinear;
+  if (!in[0] || !in[1] || !in[2]) {
+    return Fail(ctx, PIKGPU_EINVAL,
+                "need three target planes or three dl_dlinear planes");
+  }
+  const int nlin = (io->linear[0] != nullptr) + (io->linear[1] != nullptr) +
+                   (io->linear[2] != nullptr);
+  const int ndc = (io->dl_dcoeffs[0] != nullptr) +
+                  (io->dl_dcoeffs[1] != nullptr) +
+                  (io->dl_dcoeffs[2] != nullptr);
+  if ((nlin != 0 && nlin != 3) || (ndc != 0 && ndc != 3)) {
+    return Fail(ctx, PIKGPU_EINVAL,
+                "linear and dl_dcoeffs take all three planes or none");
+  }
+  if (io->loss != nullptr && !use_target) {
+    return Fail(ctx, PIKGPU_EINVAL, "the loss needs a target");
+  }
+  if (io->qf_stride < bx || io->dqf_stride < bx || io->in_stride < xs ||
+      (nlin && io->linear_stride < xs) ||
+      (ndc && io->dcoeffs_stride < bx * 64)) {
+    return Fail(ctx, PIKGPU_EINVAL, "stride smaller than a row");
+  }
+  if (io->qf_stride > (1u << 30) || io->dqf_stride > (1u << 30) ||
+      io->in_stride > (1u << 30) || io->linear_stride > (1u << 30)) {
+    return Fail(ctx, PIKGPU_EINVAL, "stride too large");
+  }
+  if (!dev) {
+    for (size_t y = 0; y < by; ++y) {
+      const float* r = io->qf + y * io->qf_stride;
+      for (size_t x = 0; x < bx; ++x) {
+        if (!(r[x] > 0.0f) || !std::isfinite(r[x])) {
+          return Fail(ctx, PIKGPU_EINVAL,
+                      "quant field values must be finite and positive");
+        }
+      }
+    }
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int tiles = int(DivCeil(bx, 4) * DivCeil(by, 4));
+  int rc;
+  if ((rc = Grow(ctx, &ctx->adj_loss, size_t(tiles) * 4 + 16))) return rc;
+
+  AdjArgs A;
+  memset(&A, 0, sizeof(A));
+  const size_t plane_in = xs * ys;
+  if (dev) {
+    A.qf = io->qf;
+    A.qf_pitch = int(io->qf_stride);
+    for (int c = 0; c < 3; ++c) {
+      (use_target ? A.target : A.upstream)[c] = in[c];
+      A.linear[c] = nlin ? io->linear[c] : nullptr;
+      A.dcoef[c] = ndc ? io->dl_dcoeffs[c] : nullptr;
+    }
+    A.in_pitch = int(io->in_stride);
+    A.lin_pitch = int(io->linear_stride);
+    A.dqf = io->dl_dqf;
+    A.dqf_pitch = int(io->dqf_stride);
+    A.dc_pitch = io->dcoeffs_stride;
+  } else {
+    if ((rc = Grow(ctx, &ctx->adj_qf, bx * by * 4)) ||
+        (rc = Grow(ctx, &ctx->adj_in, 3 * plane_in * 4)) ||
+        (rc = Grow(ctx, &ctx->adj_dqf, bx * by * 4)) ||
+        (nlin && (rc = Grow(ctx, &ctx->adj_lin, 3 * plane_in * 4))) ||
+        (ndc && (rc = Grow(ctx, &ctx->adj_dc, 3 * by * bx * 64 * 4)))) {
+      return rc;
+    }
+    float* qf = static_cast<float*>(ctx->adj_qf.p);
+    float* inb = static_cast<float*>(ctx->adj_in.p);
+    if ((rc = Upload2D(ctx, qf, io->qf, io->qf_stride * 4, bx * 4, by))) {
+      return rc;
+    }
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = Upload2D(ctx, inb + c * plane_in, in[c], io->in_stride * 4,
+                         xs * 4, ys))) {
+        return rc;
+      }
+    }
+    A.qf = qf;
+    A.qf_pitch = int(bx);
+    for (int c = 0; c < 3; ++c) {
+      (use_target ? A.target : A.upstream)[c] = inb + c * plane_in;
+      A.linear[c] =
+          nlin ? static_cast<float*>(ctx->adj_lin.p) + c * plane_in : nullptr;
+      A.dcoef[c] =
+          ndc ? static_cast<float*>(ctx->adj_dc.p) + c * by * bx * 64 : nullptr;
+    }
+    A.in_pitch = int(xs);
+    A.lin_pitch = int(xs);
+    A.dqf = static_cast<float*>(ctx->adj_dqf.p);
+    A.dqf_pitch = int(bx);
+    A.dc_pitch = bx * 64;
+  }
+  float* loss_part = static_cast<float*>(ctx->adj_loss.p);
+  double* loss_dev = reinterpret_cast<double*>(
+      static_cast<char*>(ctx->adj_loss.p) + DivCeil(size_t(tiles) * 4, 8) * 8);
+  A.loss_part = use_target ? loss_part : nullptr;
+
+  FrameParams P = ctx->P;
+  FrameBuffers B = ctx->B;
+  B.keep = 0;
+  const DeviceConsts* K = ctx->consts;
+  if (!ctx->adj_ready) {
+    EnqueueDc(P, B, K, st);
+    EnqueueLf(P, B, K, st, false);
+    ctx->adj_ready = true;
+  }
+  const int grid = XcdGrid(tiles);
+  HIP_TRY(ctx, hipEventRecord(ctx->adj_ev[0], st));
+  KAdjForward<<<grid, 256, 0, st>>>(P, B, K, A);
+  HIP_TRY(ctx, hipEventRecord(ctx->adj_ev[1], st));
+  KAdjBackward<<<grid, 256, 0, st>>>(P, B, K, A);
+  HIP_TRY(ctx, hipEventRecord(ctx->adj_ev[2], st));
+  if (io->loss) KAdjLoss<<<1, 256, 0, st>>>(loss_part, tiles, loss_dev);
+  HIP_TRY(ctx, hipGetLastError());
+  ctx->adj_timed = true;
+  // pikgpu_run's output is not what the device buffers hold any more.
+  ctx->ran = false;
+  if (dev) {
+    if (io->loss) {
+      HIP_TRY(ctx, hipMemcpyAsync(io->loss, loss_dev, 8,
+                                  hipMemcpyDeviceToDevice, st));
+    }
+    return PIKGPU_OK;
+  }
+  HIP_TRY(ctx, hipMemcpy2DAsync(io->dl_dqf, io->dqf_stride * 4, A.dqf, bx * 4,
+                                bx * 4, by, hipMemcpyDeviceToHost, st));
+  for (int c = 0; c < 3; ++c) {
+    if (nlin) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(io->linear[c], io->linear_stride * 4,
+                                    A.linear[c], xs * 4, xs * 4, ys,
+                                    hipMemcpyDeviceToHost, st));
+    }
+    if (ndc) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(io->dl_dcoeffs[c], io->dcoeffs_stride * 4,
+                                    A.dcoef[c], bx * 64 * 4, bx * 64 * 4, by,
+                                    hipMemcpyDeviceToHost, st));
+    }
+  }
+  if (io->loss) {
+    HIP_TRY(ctx, hipMemcpyAsync(io->loss, loss_dev, 8, hipMemcpyDeviceToHost,
+                                st));
+  }
+  return pikgpu_sync(ctx);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -967,8 +1169,8 @@ int BFail(pikgpu_batch* b, int code, const std::string& msg) {
   } while (0)
 
 void DropGraph(pikgpu_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
 }
@@ -995,7 +1197,7 @@ int BuildGraph(pikgpu_batch* b) {
   hipGraph_t g = nullptr;
   const hipError_t e_end = hipStreamEndCapture(b->main, &g);
   if (rc != PIKGPU_OK || e != hipSuccess || e_end != hipSuccess) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     if (rc != PIKGPU_OK) return BFail(b, rc, err);
     return BFail(b, PIKGPU_EHIP,
                  std::string("graph capture: ") +
@@ -1031,18 +1233,18 @@ pikgpu_batch* pikgpu_batch_create(int device, const pikgpu_config* config) {
 
 void pikgpu_batch_destroy(pikgpu_batch* b) {
   if (b == nullptr) return;
-  hipSetDevice(b->device);
-  if (b->main) hipStreamSynchronize(b->main);
+  (void)hipSetDevice(b->device);
+  if (b->main) (void)hipStreamSynchronize(b->main);
   DropGraph(b);
   for (pikgpu_ctx* c : b->items) pikgpu_destroy(c);
   for (int k = 0; k < kBranches; ++k) {
-    if (b->branch[k]) hipStreamDestroy(b->branch[k]);
-    if (b->join[k]) hipEventDestroy(b->join[k]);
+    if (b->branch[k]) (void)hipStreamDestroy(b->branch[k]);
+    if (b->join[k]) (void)hipEventDestroy(b->join[k]);
   }
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->t0) hipEventDestroy(b->t0);
-  if (b->t1) hipEventDestroy(b->t1);
-  if (b->main) hipStreamDestroy(b->main);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  if (b->main) (void)hipStreamDestroy(b->main);
   delete b;
 }
 

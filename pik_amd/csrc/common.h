@@ -107,6 +107,14 @@ void ComputeHostTables(HostTables* t);
 size_t DefaultDequantTables(float* table, size_t table_cap, uint32_t* offsets,
                             size_t offsets_cap, size_t* num_offsets);
 void FillScaleTables(float* tscale, float* ascale);
+// Squared norms D_k of the rows of the 1-D transposed-scaled IDCT basis for
+// N = 4, 8, 16, 32, packed at AdjNormOffset(N) (the transpose of an inverse
+// transform is its forward transform scaled by D_i D_j, see xform.h).
+constexpr int kAdjNormFloats = 4 + 8 + 16 + 32;
+__host__ __device__ inline int AdjNormOffset(int n) {
+  return n == 4 ? 0 : (n == 8 ? 4 : (n == 16 ? 12 : 28));
+}
+void AdjointNorms(float* d);
 void GaborishWeights(int strength, float w[3]);
 void LfBlurWeights(float w[3]);
 void Ub4Kernel(float k[3][4]);

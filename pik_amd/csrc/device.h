@@ -15,6 +15,7 @@ struct DeviceConsts {
   float ascale[5][8][8];         // DCTInvTotalScale<2^(i+1)>(x, y)
   int mul_table[kMaxSigma + 1];  // EPF WeightFast multipliers
   float opsin_inv[9];            // inverse opsin absorbance matrix
+  float adj_norm[kAdjNormFloats];  // AdjointNorms
 };
 
 // LF-domain per-group caches: (gw+2) x (gh+2) values per plane for llf and
@@ -67,6 +68,25 @@ struct FrameBuffers {
   int opitch;             // floats per row of out
   int spitch;             // bytes per row of srgb8
   int keep;               // write stage dumps
+};
+
+// Inputs and outputs of one forward + adjoint evaluation (device memory).
+// Image planes: ysize rows of the given pitch (floats); block fields:
+// ysize_blocks rows; coefficient planes: ysize_blocks rows of
+// xsize_blocks * 64 values.
+struct AdjArgs {
+  const float* qf;           // real-valued quant field, > 0
+  int qf_pitch;
+  const float* target[3];    // linear target, or null (then upstream)
+  const float* upstream[3];  // dL/dlinear when target is null
+  int in_pitch;
+  float* linear[3];          // optional forward output
+  int lin_pitch;
+  float* dqf;                // dL/dqf
+  int dqf_pitch;
+  float* dcoef[3];           // optional dL/dd (before CfL)
+  size_t dc_pitch;
+  float* loss_part;          // per-tile sums of squared error, or null
 };
 
 }  // namespace pikgpu

@@ -22,6 +22,19 @@ HAPI int pikgpu_host_block_transform(int raw, int dir, const float* in,
   return 0;
 }
 
+// Transpose of the inverse transform of one block (contiguous N x N).
+HAPI int pikgpu_host_block_adjoint(int raw, const float* in, float* out) {
+  float norms[pikgpu::kAdjNormFloats];
+  pikgpu::AdjointNorms(norms);
+  const int n = 8 * pikgpu::AcsCovered(raw);
+  pikgpu::BlockAdjoint(raw, norms, in, n, out);
+  return 0;
+}
+
+HAPI void pikgpu_host_adjoint_norms(float* out) {
+  pikgpu::AdjointNorms(out);
+}
+
 HAPI void pikgpu_host_mul_table(int* out) {
   pikgpu::HostTables t;
   pikgpu::ComputeHostTables(&t);

@@ -13,6 +13,8 @@
 //    frequencies (pik/ac_strategy.h:33-38);
 //  - basis matrices of the transposed-scaled transforms.
 #include <math.h>
+
+#include <cmath>
 #include <string.h>
 
 #include "common.h"
@@ -239,6 +241,21 @@ void FillScaleTables(float* tscale, float* ascale) {
         tscale[(i * 8 + y) * 8 + x] = in ? TotalScale(n, x, y) : 0.0f;
         ascale[(i * 8 + y) * 8 + x] = in ? InvTotalScale(n, x, y) : 0.0f;
       }
+    }
+  }
+}
+
+// D_k = sum_x b_k(x)^2 for b_k(x) = w_k cos((2x + 1) k pi / 2N), w_0 = 1,
+// w_k = 1 / cos(k pi / 16) for N = 8 (the AAN scaling of the 8-point IDCT)
+// and 2 cos(k pi / 2N) otherwise: N for k = 0, N/2 w_k^2 for k > 0.
+void AdjointNorms(float* d) {
+  for (int n = 4; n <= 32; n *= 2) {
+    float* o = d + AdjNormOffset(n);
+    o[0] = float(n);
+    for (int k = 1; k < n; ++k) {
+      const double w = n == 8 ? 1.0 / std::cos(k * M_PI / 16)
+                              : 2.0 * std::cos(k * M_PI / (2 * n));
+      o[k] = float(0.5 * n * w * w);
     }
   }
 }

@@ -306,6 +306,9 @@ __device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
 // One task of the forward transform: pass 0 columns of pixels into tmp,
 // pass 1 columns of tmp into coefficients (x 1/N^2), pass 2 the DC Hadamard
 // of DCT4X4 (pik/ac_strategy.cc:590-613); DCT2X2 and IDENTITY in pass 0.
+// kAdjoint replaces IDENTITY's forward transform by the transpose of its
+// inverse (the other strategies' transposes are scaled forward transforms).
+template <bool kAdjoint = false>
 __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
                                    const float* pix, float* coef, float* tmp) {
   if (raw == kDCT || raw == kDCTNoHF) {
@@ -345,8 +348,13 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
       cb[9] = (b00 - b01 - b10 + b11) * 0.25f;
     }
   } else if (pass == 0) {
-    DirectForward8(raw, pix + oby * 8 * 32 + obx * 8, 32,
-                   coef + oby * 256 + obx * 64);
+    if (kAdjoint && raw == kIdentity) {
+      IdentityAdjoint8(pix + oby * 8 * 32 + obx * 8, 32,
+                       coef + oby * 256 + obx * 64);
+    } else {
+      DirectForward8(raw, pix + oby * 8 * 32 + obx * 8, 32,
+                     coef + oby * 256 + obx * 64);
+    }
   }
 }
 
@@ -367,6 +375,7 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
 
 // pix[3][1024] -> coef[3][1024] for the transforms in T (others keep their
 // coefficient entries). All 256 threads must call.
+template <bool kAdjoint = false>
 __device__ inline void TileForward3(const TileTasks& T, const float* pix,
                                     float* coef, float* tmp) {
   const int total = T.off[16];
@@ -374,8 +383,8 @@ __device__ inline void TileForward3(const TileTasks& T, const float* pix,
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
       const int c = k / total, t = k - c * total;
       const int b = TaskBlock(T, t);
-      ForwardTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
-                  pix + c * 1024, coef + c * 1024, tmp + c * 1024);
+      ForwardTask<kAdjoint>(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
+                            pix + c * 1024, coef + c * 1024, tmp + c * 1024);
     }
     __syncthreads();
   }
@@ -413,6 +422,11 @@ __global__ void KZeroPred(FrameParams P, FrameBuffers B);
 __global__ void KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K);
+__global__ void KAdjForward(FrameParams P, FrameBuffers B,
+                            const DeviceConsts* K, AdjArgs A);
+__global__ void KAdjBackward(FrameParams P, FrameBuffers B,
+                             const DeviceConsts* K, AdjArgs A);
+__global__ void KAdjLoss(const float* part, int n, double* out);
 
 // Generic image description for the filters.
 struct Img {

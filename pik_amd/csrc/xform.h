@@ -216,4 +216,89 @@ PIKGPU_HD void BlockForward(int raw, const float* pix, int pstride,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoints (transposes) of the inverse transforms, for the gradient of the
+// reconstruction with respect to the coefficients.
+//
+// The 1-D transposed-scaled IDCT is pix(x) = sum_k F_k b_k(x) with mutually
+// orthogonal basis rows b_k (b_k(x) = w_k cos((2x+1) k pi / 2N); w_0 = 1,
+// w_k = 1/cos(k pi/16) for N = 8 and 2 cos(k pi/2N) otherwise). Its forward
+// transform is the exact inverse, F_k = (1/D_k) sum_x pix(x) b_k(x) with
+// D_k = sum_x b_k(x)^2, so the transpose is the forward transform with
+// coefficient k scaled by D_k: D_0 = N, D_k = 4/cos^2(k pi/16) (N = 8),
+// D_k = 2N cos^2(k pi/2N) (N = 4, 16, 32). In 2-D coefficient (i, j) is
+// scaled by D_i D_j. DCT4X4: the same per 4x4 sub-block, and the four DCs
+// pass through the DC Hadamard H (H H = 4 I), whose transpose is H = 4 H/4:
+// scale 4 D_0^2 = 64. DCT2X2: each of the three Hadamard levels S = 8, 4, 2
+// (region S x S) transposes to 4x its inverse level, so a coefficient gets
+// 4 per level that covers it: 64 in the 2x2 corner, 16 in the rest of the
+// 4x4 corner, 4 elsewhere. IDENTITY has its own adjoint (IdentityAdjoint8).
+// Scale of coefficient `flat` (i * N + j of the N x N coefficient matrix in
+// storage order; 0..63 for the 8x8 strategies) of a transform of raw
+// strategy `raw` applied after the forward transform. `norms` holds D_k for
+// N = 4, 8, 16, 32 at AdjNormOffset(N).
+PIKGPU_HD float AdjointScale(const float* norms, int raw, int flat) {
+  if (AcsIsDCT(raw)) {
+    const int n = 8 * AcsCovered(raw);
+    const float* d = norms + AdjNormOffset(n);
+    return d[flat / n] * d[flat % n];
+  }
+  const int ky = flat >> 3, kx = flat & 7;
+  if (AcsIsDCT4(raw)) {
+    if (ky < 2 && kx < 2) return 64.0f;
+    const float* d = norms + AdjNormOffset(4);
+    return d[ky >> 1] * d[kx >> 1];
+  }
+  if (raw == kDCT2X2) {
+    return (ky < 2 && kx < 2) ? 64.0f : ((ky < 4 && kx < 4) ? 16.0f : 4.0f);
+  }
+  return 1.0f;
+}
+
+// Transpose of DirectInverse8(kIdentity): every pixel of a 4x4 sub-block
+// holds the sub-block's centre value, which is its Hadamard DC minus 1/16
+// of the sum of its 15 non-DC coefficients; the other pixels add one
+// coefficient each (the one stored at (1, 1) goes to pixel (0, 0)).
+PIKGPU_HD void IdentityAdjoint8(const float* p, int pstride, float* c) {
+  float dcs[4];
+  for (int y = 0; y < 2; ++y) {
+    for (int x = 0; x < 2; ++x) {
+      float sum = 0.0f;
+      for (int iy = 0; iy < 4; ++iy) {
+        for (int ix = 0; ix < 4; ++ix) {
+          sum += p[(4 * y + iy) * pstride + 4 * x + ix];
+        }
+      }
+      dcs[y * 2 + x] = sum;
+      const float share = sum * (1.0f / 16);
+      for (int iy = 0; iy < 4; ++iy) {
+        for (int ix = 0; ix < 4; ++ix) {
+          if ((iy == 0 && ix == 0) || (iy == 1 && ix == 1)) continue;
+          c[(y + 2 * iy) * 8 + x + 2 * ix] =
+              p[(4 * y + iy) * pstride + 4 * x + ix] - share;
+        }
+      }
+      c[(y + 2) * 8 + x + 2] = p[4 * y * pstride + 4 * x] - share;
+    }
+  }
+  c[0] = dcs[0] + dcs[1] + dcs[2] + dcs[3];
+  c[1] = dcs[0] + dcs[1] - dcs[2] - dcs[3];
+  c[8] = dcs[0] - dcs[1] + dcs[2] - dcs[3];
+  c[9] = dcs[0] - dcs[1] - dcs[2] + dcs[3];
+}
+
+// Transpose of BlockInverse (contiguous N x N in and out).
+PIKGPU_HD void BlockAdjoint(int raw, const float* norms, const float* pix,
+                            int pstride, float* coef) {
+  if (raw == kIdentity) {
+    IdentityAdjoint8(pix, pstride, coef);
+    return;
+  }
+  BlockForward(raw, pix, pstride, coef);
+  const int n = 8 * AcsCovered(raw);
+  if (AcsIsDCT(raw) || AcsIsDCT4(raw) || raw == kDCT2X2) {
+    for (int i = 0; i < n * n; ++i) coef[i] *= AdjointScale(norms, raw, i);
+  }
+}
+
 }  // namespace pikgpu

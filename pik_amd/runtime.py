@@ -119,6 +119,89 @@ class Reconstructor:
                                                 xs.value * 4))
         return out
 
+    def fwd_adj(self, qf, target=None, dl_dlinear=None, want_linear=False,
+                want_dcoeffs=False):
+        """Forward model + adjoint of the uploaded frame for a real-valued
+        quant field (pikgpu_fwd_adj), host arrays. qf: (by, bx) float32 > 0;
+        target or dl_dlinear: (3, ysize, xsize) float32. Returns a dict with
+        dl_dqf (by, bx), and loss (with target), linear (3, ysize, xsize),
+        dl_dcoeffs (3, by, bx*64) when asked for."""
+        f = self._frame
+        bx, by = f.xsize_blocks, f.ysize_blocks
+        qf = np.ascontiguousarray(qf, np.float32)
+        if qf.shape != (by, bx):
+            raise ValueError("qf must be (%d, %d)" % (by, bx))
+        src = target if target is not None else dl_dlinear
+        if src is None:
+            raise ValueError("need target or dl_dlinear")
+        src = np.ascontiguousarray(src, np.float32)
+        if src.shape != (3, f.ysize, f.xsize):
+            raise ValueError("target / dl_dlinear must be (3, %d, %d)"
+                             % (f.ysize, f.xsize))
+        io = abi.AdjIoC()
+        io.flags = 0
+        io.qf = qf.ctypes.data
+        io.qf_stride = bx
+        planes = [src[c].ctypes.data for c in range(3)]
+        if target is not None:
+            io.target = (ctypes.c_void_p * 3)(*planes)
+        else:
+            io.dl_dlinear = (ctypes.c_void_p * 3)(*planes)
+        io.in_stride = f.xsize
+        out = {"dl_dqf": np.empty((by, bx), np.float32)}
+        io.dl_dqf = out["dl_dqf"].ctypes.data
+        io.dqf_stride = bx
+        if want_linear:
+            out["linear"] = np.empty((3, f.ysize, f.xsize), np.float32)
+            io.linear = (ctypes.c_void_p * 3)(
+                *[out["linear"][c].ctypes.data for c in range(3)])
+            io.linear_stride = f.xsize
+        if want_dcoeffs:
+            out["dl_dcoeffs"] = np.empty((3, by, bx * 64), np.float32)
+            io.dl_dcoeffs = (ctypes.c_void_p * 3)(
+                *[out["dl_dcoeffs"][c].ctypes.data for c in range(3)])
+            io.dcoeffs_stride = bx * 64
+        loss = ctypes.c_double(0.0)
+        if target is not None:
+            io.loss = ctypes.addressof(loss)
+        self._check(self._lib.pikgpu_fwd_adj(self._ctx, ctypes.byref(io)))
+        if target is not None:
+            out["loss"] = loss.value
+        return out
+
+    def fwd_adj_device(self, qf, dl_dqf, target=None, dl_dlinear=None,
+                       linear=None, dl_dcoeffs=None, loss=None):
+        """pikgpu_fwd_adj on device memory: every argument is a contiguous
+        device array of the shapes fwd_adj uses (loss: one float64), e.g. a
+        pik_amd.devmem.DeviceArray; anything with data_ptr() and, for the
+        planes, first-axis indexing works. Asynchronous on the context
+        stream (sync() waits)."""
+        f = self._frame
+        io = abi.AdjIoC()
+        io.flags = abi.ADJ_DEVICE_POINTERS
+        io.qf = qf.data_ptr()
+        io.qf_stride = f.xsize_blocks
+        src = target if target is not None else dl_dlinear
+        planes = [src[c].data_ptr() for c in range(3)]
+        if target is not None:
+            io.target = (ctypes.c_void_p * 3)(*planes)
+        else:
+            io.dl_dlinear = (ctypes.c_void_p * 3)(*planes)
+        io.in_stride = f.xsize
+        io.dl_dqf = dl_dqf.data_ptr()
+        io.dqf_stride = f.xsize_blocks
+        if linear is not None:
+            io.linear = (ctypes.c_void_p * 3)(
+                *[linear[c].data_ptr() for c in range(3)])
+            io.linear_stride = f.xsize
+        if dl_dcoeffs is not None:
+            io.dl_dcoeffs = (ctypes.c_void_p * 3)(
+                *[dl_dcoeffs[c].data_ptr() for c in range(3)])
+            io.dcoeffs_stride = f.xsize_blocks * 64
+        if loss is not None:
+            io.loss = loss.data_ptr()
+        self._check(self._lib.pikgpu_fwd_adj(self._ctx, ctypes.byref(io)))
+
     def timings(self):
         names = (ctypes.c_char_p * 16)()
         ms = (ctypes.c_float * 16)()
