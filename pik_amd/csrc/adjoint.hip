@@ -83,9 +83,9 @@ __device__ inline float DequantPlain(const float* m, float scaled, int c,
 __global__ void __launch_bounds__(256)
     KAdjForward(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                 AdjArgs A) {
-  __shared__ float s_coef[3][1024];
-  __shared__ float s_pix[3][1024];
-  __shared__ float s_tmp[3][1024];
+  __shared__ float s_coef[3][kCoefPlane];
+  __shared__ float s_pix[3][kTilePlane];
+  __shared__ float s_tmp[3][kTilePlane];
   __shared__ uint8_t s_acs[16];
   __shared__ TileTasks s_inv;
 
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256)
         } else if (P.predict_lf && fi < 2 * S.cs && fj < 2 * S.cs) {
           v += lf[(2 * (S.obyg + 1) + fi) * kG2Pitch + 2 * (S.obxg + 1) + fj];
         }
-        s_coef[c][S.crow * 256 + S.ccol + j] = v;
+        s_coef[c][CoefAt(S.crow, S.ccol + j)] = v;
       }
     }
   }
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256)
     for (int c = 0; c < 3; ++c) {
       float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + pc0;
       for (int j = 0; j < 4; ++j) {
-        if (tbx * 8 + pc0 + j < P.px) dst[j] = s_pix[c][pr * 32 + pc0 + j];
+        if (tbx * 8 + pc0 + j < P.px) dst[j] = s_pix[c][PixAt(pr, pc0 + j)];
       }
     }
   }
@@ -151,12 +151,14 @@ constexpr int kGW = 34;
 __global__ void __launch_bounds__(256)
     KAdjBackward(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                  AdjArgs A) {
-  // s_p (P ring) is dead once dG is formed; the forward transform's
-  // scratch reuses it. s_dg is dead once dP is formed; the coefficients
-  // reuse it.
+  // s_a (the P ring) is dead once dG is formed; the forward transform's
+  // scratch reuses it. s_b (the dG ring) is dead once dP is formed; the
+  // coefficients reuse it.
+  static_assert(kPW * kPW >= kTilePlane && kGW * kGW >= kCoefPlane,
+                "the rings must hold the transform buffers");
   __shared__ float s_a[3 * kPW * kPW];  // P ring, then transform scratch
   __shared__ float s_b[3 * kGW * kGW];  // dG ring, then coefficients
-  __shared__ float s_pix[3][1024];      // dP of the tile
+  __shared__ float s_pix[3][kTilePlane];  // dP of the tile
   __shared__ uint8_t s_acs[16];
   __shared__ TileTasks s_fwd;
   __shared__ float s_loss[4];
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256)
     const int i = k >> 5, j = k & 31;
     for (int c = 0; c < 3; ++c) {
       const float* g = s_b + c * kGW * kGW;
-      s_pix[c][k] = P.gaborish ? Symmetric3(g, kGW, kGW, kGW, j + 1, i + 1,
+      s_pix[c][PixAt(i, j)] = P.gaborish ? Symmetric3(g, kGW, kGW, kGW, j + 1, i + 1,
                                             P.gab_w[0], P.gab_w[1], P.gab_w[2])
                                : g[(i + 1) * kGW + j + 1];
     }
@@ -311,15 +313,15 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 4; ++j) {
       const int flat = S.cflat0 + j;
       const int fi = flat / S.cN, fj = flat % S.cN;
-      const int e = S.crow * 256 + S.ccol + j;
+      const int e = CoefAt(S.crow, S.ccol + j);
       if (fi < S.cs && fj < S.cs) {
         out[0][j] = out[1][j] = out[2][j] = 0.0f;
         continue;
       }
       const float sc = AdjointScale(K->adj_norm, S.raw, flat);
       const float gx = s_coef[e] * sc;
-      const float gy = s_coef[1024 + e] * sc;
-      const float gb = s_coef[2048 + e] * sc;
+      const float gy = s_coef[kCoefPlane + e] * sc;
+      const float gb = s_coef[2 * kCoefPlane + e] * sc;
       const float gyt = gy + dx.cc * gx + db.cc * gb;
       const int k = (S.ccol & 63) + j;
       const float vx = DequantPlain(dx.m, scaled, 0, B.ac[at + j], k);

@@ -183,7 +183,6 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
   // Constant tables.
   DeviceConsts* h = new DeviceConsts;
   memset(h, 0, sizeof(*h));
-  ComputeBasis(h->basis);
   FillScaleTables(&h->tscale[0][0][0], &h->ascale[0][0][0]);
   ComputeHostTables(&ctx->host);
   memcpy(h->mul_table, ctx->host.mul_table, sizeof(h->mul_table));
@@ -761,7 +760,8 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
   if (P.dot_tiles) KDots<<<P.dot_tiles, 256, 0, st>>>(P, B);
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
   {
-    const int grid = XcdGrid(int(DivCeil(P.px, 64) * DivCeil(P.py, 4)));
+    const int grid =
+        XcdGrid(int(DivCeil(P.px, 256) * DivCeil(P.py, kFinalRows)));
     KFinal<<<grid, 256, 0, st>>>(P, B, K);
   }
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));

@@ -118,11 +118,5 @@ void AdjointNorms(float* d);
 void GaborishWeights(int strength, float w[3]);
 void LfBlurWeights(float w[3]);
 void Ub4Kernel(float k[3][4]);
-// Float basis matrices of the transposed-scaled transforms:
-// inverse B_N[k][n] and forward Binv_N[n][k], N = 2,4,8,16,32, packed at
-// offsets BasisOffset(N).
-int BasisOffset(int n);
-constexpr int kBasisFloats = 2 * (4 + 16 + 64 + 256 + 1024);
-void ComputeBasis(float* out);
 
 }  // namespace pikgpu

@@ -10,7 +10,6 @@ namespace pikgpu {
 
 // Per-context constant tables in device memory (set up once).
 struct DeviceConsts {
-  float basis[kBasisFloats];     // see ComputeBasis
   float tscale[5][8][8];         // DCTTotalScale<2^(i+1)>(x, y) at [i][y][x]
   float ascale[5][8][8];         // DCTInvTotalScale<2^(i+1)>(x, y)
   int mul_table[kMaxSigma + 1];  // EPF WeightFast multipliers

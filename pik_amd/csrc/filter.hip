@@ -26,28 +26,6 @@ __device__ inline float ImgAt(const Img& in, int c, int x, int y) {
   return in.p[c * in.plane + size_t(my) * in.pitch + mx];
 }
 
-// Guide value (Gaborish of `in` or `in` itself) at an in-range position.
-__device__ inline float GuideAt(const Img& in, int gaborish, const float* gw,
-                                int c, int x, int y) {
-  if (!gaborish) return ImgAt(in, c, x, y);
-  const int xm = Mirror(x - 1, in.W), xp = Mirror(x + 1, in.W);
-  const int ym = Mirror(y - 1, in.H), yp = Mirror(y + 1, in.H);
-  const float tc = ImgAt(in, c, x, ym), bc = ImgAt(in, c, x, yp);
-  const float ml = ImgAt(in, c, xm, y), mr = ImgAt(in, c, xp, y);
-  const float mc = ImgAt(in, c, x, y);
-  const float tl = ImgAt(in, c, xm, ym), tr = ImgAt(in, c, xp, ym);
-  const float bl = ImgAt(in, c, xm, yp), br = ImgAt(in, c, xp, yp);
-  const float sum_tb = tc + bc;
-  const float mul0 = mc * gw[0];
-  const float sum_lr = ml + mr;
-  const float x1 = sum_tb + sum_lr;
-  const float mul1 = x1 * gw[1] + mul0;
-  const float sum_t2 = tl + tr;
-  const float sum_b2 = bl + br;
-  const float x2 = sum_t2 + sum_b2;
-  return x2 * gw[2] + mul1;
-}
-
 // LUTs of SigmaFromQuant (pik/epf_target.cc:1161-1196).
 __constant__ float kSigmaLut[32] = {
     1.9815775622811198f, 1.9715084740908622f, 1.6819963065873933f,
@@ -149,38 +127,58 @@ __global__ void __launch_bounds__(256)
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
   float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   if (x < in.W) {
-    const int xm = Mirror(x - 1, in.W), xp = Mirror(x + 1, in.W);
-    for (int c = 0; c < 3; ++c) {
-      if (!gaborish) {
-        for (int y = ya; y < yb; ++y) {
-          const float v = ImgAt(in, c, x, y);
+    // Source columns and rows: mirrored at W x H, then replicated beyond
+    // the valid region (ImgAt). The three channels advance together so
+    // each row step issues nine independent loads.
+    const int xc = min(x, in.vw - 1);
+    const int xm = min(Mirror(x - 1, in.W), in.vw - 1);
+    const int xp = min(Mirror(x + 1, in.W), in.vw - 1);
+    if (!gaborish) {
+      for (int y = ya; y < yb; ++y) {
+        const size_t r = size_t(min(y, in.vh - 1)) * in.pitch;
+        for (int c = 0; c < 3; ++c) {
+          const float v = in.p[c * in.plane + r + xc];
           mn[c] = fminf(mn[c], v);
           mx[c] = fmaxf(mx[c], v);
         }
-        continue;
       }
-      const int y0 = Mirror(ya - 1, in.H);
-      float tl = ImgAt(in, c, xm, y0), tc = ImgAt(in, c, x, y0),
-            tr = ImgAt(in, c, xp, y0);
-      float ml = ImgAt(in, c, xm, ya), mc = ImgAt(in, c, x, ya),
-            mr = ImgAt(in, c, xp, ya);
+    } else {
+      float t[3][3], m[3][3];  // [channel][left, centre, right]
+      const size_t r0 = size_t(min(Mirror(ya - 1, in.H), in.vh - 1)) * in.pitch;
+      const size_t r1 = size_t(min(ya, in.vh - 1)) * in.pitch;
+      for (int c = 0; c < 3; ++c) {
+        const float* pc = in.p + c * in.plane;
+        t[c][0] = pc[r0 + xm];
+        t[c][1] = pc[r0 + xc];
+        t[c][2] = pc[r0 + xp];
+        m[c][0] = pc[r1 + xm];
+        m[c][1] = pc[r1 + xc];
+        m[c][2] = pc[r1 + xp];
+      }
       for (int y = ya; y < yb; ++y) {
-        const int yn = Mirror(y + 1, in.H);
-        const float bl = ImgAt(in, c, xm, yn), bc = ImgAt(in, c, x, yn),
-                    br = ImgAt(in, c, xp, yn);
-        const float sum_tb = tc + bc;
-        const float mul0 = mc * P.gab_w[0];
-        const float sum_lr = ml + mr;
-        const float x1 = sum_tb + sum_lr;
-        const float mul1 = x1 * P.gab_w[1] + mul0;
-        const float sum_t2 = tl + tr;
-        const float sum_b2 = bl + br;
-        const float x2 = sum_t2 + sum_b2;
-        const float v = x2 * P.gab_w[2] + mul1;
-        mn[c] = fminf(mn[c], v);
-        mx[c] = fmaxf(mx[c], v);
-        tl = ml; tc = mc; tr = mr;
-        ml = bl; mc = bc; mr = br;
+        const int yn = y + 1 < in.H ? y + 1 : in.H - 1;
+        const size_t rb = size_t(min(yn, in.vh - 1)) * in.pitch;
+        for (int c = 0; c < 3; ++c) {
+          const float* pc = in.p + c * in.plane;
+          const float bl = pc[rb + xm], bc = pc[rb + xc], br = pc[rb + xp];
+          const float sum_tb = t[c][1] + bc;
+          const float mul0 = m[c][1] * P.gab_w[0];
+          const float sum_lr = m[c][0] + m[c][2];
+          const float x1 = sum_tb + sum_lr;
+          const float mul1 = x1 * P.gab_w[1] + mul0;
+          const float sum_t2 = t[c][0] + t[c][2];
+          const float sum_b2 = bl + br;
+          const float x2 = sum_t2 + sum_b2;
+          const float v = x2 * P.gab_w[2] + mul1;
+          mn[c] = fminf(mn[c], v);
+          mx[c] = fmaxf(mx[c], v);
+          t[c][0] = m[c][0];
+          t[c][1] = m[c][1];
+          t[c][2] = m[c][2];
+          m[c][0] = bl;
+          m[c][1] = bc;
+          m[c][2] = br;
+        }
       }
     }
   }
@@ -246,21 +244,27 @@ __global__ void __launch_bounds__(256)
 // EPF (+ AR clamp) on one 32x32 tile. dc_mode: the image is the 8-padded DC
 // image (constant quant field QuantDC/2, lut 0, DCT8 everywhere), no AR, and
 // the result is clamped to +-half a DC step and written to B.dc.
-__global__ void __launch_bounds__(256)
+//
+// LDS: the input window (46 x 46 per channel) is dead once the filter has
+// run -- each thread keeps the 12 original values it needs for AR in
+// registers -- so AR's coefficients and transform scratch reuse it; with
+// ~45 KB per workgroup three workgroups share a CU.
+__global__ void __launch_bounds__(256, 3)
     KEpf(Img in, int gaborish, int dc_mode, FrameParams P, FrameBuffers B,
          const DeviceConsts* K, const float* mm, float* out, int out_pitch,
          size_t out_plane) {
   constexpr int kInP = 48;  // pitch of s_in rows (46 used)
   constexpr int kGP = 48;   // pitch of s_g rows (44 used)
-  __shared__ __attribute__((aligned(16))) float s_in[3][46 * kInP];
+  constexpr int kInPlane = 46 * kInP;
+  static_assert(3 * kInPlane >= 3 * (kCoefPlane + kTilePlane),
+                "AR buffers must fit");
+  __shared__ __attribute__((aligned(16))) float s_big[3 * kInPlane];
   __shared__ __attribute__((aligned(16))) uint8_t s_g[3][44 * kGP];
-  __shared__ float s_out[3][1024];
-  __shared__ float s_coef[3][1024];
-  __shared__ float s_ratio[1024];
-  __shared__ float s_tmp[3][1024];
+  __shared__ float s_out[3][kTilePlane];
   __shared__ uint8_t s_acs[16];
   __shared__ int s_sigma[16];
   __shared__ TileTasks s_tasks;
+  float (*s_in)[kInPlane] = reinterpret_cast<float (*)[kInPlane]>(s_big);
 
   const int tid = threadIdx.x;
   const int tiles_x = (in.W + 31) / 32;
@@ -353,7 +357,7 @@ __global__ void __launch_bounds__(256)
     if (inside && sigma < kMinSigma) {
       for (int c = 0; c < 3; ++c) {
         for (int j = 0; j < 4; ++j) {
-          s_out[c][r * 32 + c0 + j] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+          s_out[c][PixAt(r, c0 + j)] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
         }
       }
     } else if (inside) {
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(256)
       }
       for (int j = 0; j < 4; ++j) {
         for (int c = 0; c < 3; ++c) {
-          s_out[c][r * 32 + c0 + j] = ws[j][c] / sw[j];
+          s_out[c][PixAt(r, c0 + j)] = ws[j][c] / sw[j];
         }
       }
     }
@@ -462,7 +466,7 @@ __global__ void __launch_bounds__(256)
           const float dc = s_in[c][(r + 7) * kInP + c0 + 7 + j];
           const float hs = P.dc_half_step[c];
           const float lo = dc - hs, hi = dc + hs;
-          const float d = s_out[c][r * 32 + c0 + j];
+          const float d = s_out[c][PixAt(r, c0 + j)];
           const float mn = hi < d ? hi : d;
           out[c * out_plane + size_t(y) * out_pitch + x] = lo < mn ? mn : lo;
         }
@@ -477,7 +481,7 @@ __global__ void __launch_bounds__(256)
       if (x < in.W && y < in.H) {
         for (int c = 0; c < 3; ++c) {
           B.epf[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] =
-              s_out[c][r * 32 + c0 + j];
+              s_out[c][PixAt(r, c0 + j)];
         }
       }
     }
@@ -485,17 +489,23 @@ __global__ void __launch_bounds__(256)
 
   // 5. AR: clamp the DCT of (filtered - original) to half the quantization
   //    interval with one ratio per coefficient shared by X, Y, B.
-  BuildTasks(s_acs, false, &s_tasks);
+  float orig[3][4];
   for (int c = 0; c < 3; ++c) {
     for (int j = 0; j < 4; ++j) {
-      s_out[c][r * 32 + c0 + j] =
-          s_out[c][r * 32 + c0 + j] - s_in[c][(r + 7) * kInP + c0 + 7 + j];
+      orig[c][j] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+      s_out[c][PixAt(r, c0 + j)] = s_out[c][PixAt(r, c0 + j)] - orig[c][j];
     }
   }
-  __syncthreads();
+  // (BuildTasks ends with a barrier: every thread has read s_in, so the
+  // coefficient and scratch buffers may overwrite it.)
+  BuildTasks(s_acs, false, &s_tasks);
+  float (*s_coef)[kCoefPlane] = reinterpret_cast<float (*)[kCoefPlane]>(s_big);
+  float (*s_tmp)[kTilePlane] =
+      reinterpret_cast<float (*)[kTilePlane]>(s_big + 3 * kCoefPlane);
   TileForward3(s_tasks, &s_out[0][0], &s_coef[0][0], &s_tmp[0][0]);
   {
-    // coefficient entries owned by this thread
+    // coefficient entries owned by this thread; the same thread scales all
+    // three channels of its entries
     const int crow = tid >> 6, ccol = (tid & 63) << 2;
     const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
     if (ca != 0xFF) {
@@ -529,24 +539,17 @@ __global__ void __launch_bounds__(256)
           }
           const float upper = 0.5f * interval;
           const float lower = -0.5f * interval;
-          const float corr = s_coef[c][crow * 256 + ccol + j];
+          const float corr = s_coef[c][CoefAt(crow, ccol + j)];
           const float m1 = lower < corr ? corr : lower;
           const float clamped = m1 < upper ? m1 : upper;
           const float divisor = __float_as_uint(corr) == 0u ? 1E-7f : corr;
           const float cr = clamped / divisor;
           ratio = cr < ratio ? cr : ratio;
         }
-        s_ratio[crow * 256 + ccol + j] = ratio;
-      }
-    }
-  }
-  __syncthreads();
-  {
-    const int crow = tid >> 6, ccol = (tid & 63) << 2;
-    for (int c = 0; c < 3; ++c) {
-      for (int j = 0; j < 4; ++j) {
-        s_coef[c][crow * 256 + ccol + j] =
-            s_coef[c][crow * 256 + ccol + j] * s_ratio[crow * 256 + ccol + j];
+        for (int c = 0; c < 3; ++c) {
+          s_coef[c][CoefAt(crow, ccol + j)] =
+              s_coef[c][CoefAt(crow, ccol + j)] * ratio;
+        }
       }
     }
   }
@@ -558,9 +561,7 @@ __global__ void __launch_bounds__(256)
       float* dst = out + c * out_plane + size_t(y) * out_pitch;
       for (int j = 0; j < 4; ++j) {
         const int x = x0 + c0 + j;
-        if (x < in.W) {
-          dst[x] = s_out[c][r * 32 + c0 + j] + s_in[c][(r + 7) * kInP + c0 + 7 + j];
-        }
+        if (x < in.W) dst[x] = s_out[c][PixAt(r, c0 + j)] + orig[c][j];
       }
     }
   }
@@ -696,58 +697,100 @@ __global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
 
 // ---------------------------------------------------------------------------
 // Final Gaborish (on the 8-padded image) and XYB -> linear RGB of the
-// xsize x ysize output (pik/compressed_image.cc:1164-1179).
+// xsize x ysize output (pik/compressed_image.cc:1164-1179). Thread x of a
+// workgroup walks kFinalRows rows of column x of its 256-column strip with
+// the 3x3 window of each channel in registers, so every input value is
+// loaded three times (left, centre, right) instead of nine.
 __global__ void __launch_bounds__(256)
     KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int tiles_x = (P.px + 63) / 64;
-  const int tile = XcdTile(blockIdx.x, tiles_x * ((P.py + 3) / 4));
+  const int strips = (P.px + 255) / 256;
+  const int tile =
+      XcdTile(blockIdx.x, strips * ((P.py + kFinalRows - 1) / kFinalRows));
   if (tile < 0) return;
-  const int x = (tile % tiles_x) * 64 + (threadIdx.x & 63);
-  const int y = (tile / tiles_x) * 4 + (threadIdx.x >> 6);
-  if (x >= P.px || y >= P.py) return;
+  const int x = (tile % strips) * 256 + threadIdx.x;
+  const int ya = (tile / strips) * kFinalRows;
+  const int yb = min(P.py, ya + kFinalRows);
+  if (x >= P.px) return;
   const size_t plane = size_t(P.py) * B.ipitch;
-  float v[3];
-  for (int c = 0; c < 3; ++c) {
-    const float* src = B.ar + c * plane;
-    v[c] = P.gaborish ? Symmetric3(src, B.ipitch, P.px, P.py, x, y, P.gab_w[0],
-                                   P.gab_w[1], P.gab_w[2])
-                      : src[size_t(y) * B.ipitch + x];
-    if (B.keep) B.gab[c * plane + size_t(y) * B.ipitch + x] = v[c];
-  }
-  if (x >= P.xsize || y >= P.ysize) return;
+  const int xm = Mirror(x - 1, P.px), xp = Mirror(x + 1, P.px);
   const float* inv = K->opsin_inv;
-  const float gamma_r = 1.0f * (v[1] + v[0]);
-  const float gamma_g = 1.0f * (v[1] - v[0]);
-  const float gamma_b = v[2];
-  const float gamma_r2 = gamma_r * gamma_r;
-  const float gamma_g2 = gamma_g * gamma_g;
-  const float gamma_b2 = gamma_b * gamma_b;
-  const float mixed_r = gamma_r2 * gamma_r + P.neg_bias[0];
-  const float mixed_g = gamma_g2 * gamma_g + P.neg_bias[1];
-  const float mixed_b = gamma_b2 * gamma_b + P.neg_bias[2];
-  float lr = inv[0] * mixed_r;
-  float lg = inv[3] * mixed_r;
-  float lb = inv[6] * mixed_r;
-  const float tr = inv[1] * mixed_g;
-  const float tg = inv[4] * mixed_g;
-  const float tb = inv[7] * mixed_g;
-  lr = inv[2] * mixed_b + lr;
-  lg = inv[5] * mixed_b + lg;
-  lb = inv[8] * mixed_b + lb;
-  lr = lr + tr;
-  lg = lg + tg;
-  lb = lb + tb;
-  if (P.out_flags & kOutLinear) {
-    const size_t oplane = size_t(P.ysize) * B.opitch;
-    B.out[size_t(y) * B.opitch + x] = lr;
-    B.out[oplane + size_t(y) * B.opitch + x] = lg;
-    B.out[2 * oplane + size_t(y) * B.opitch + x] = lb;
+  // rows above (t) and at (m) the current row: [channel][left, centre, right]
+  float t[3][3], m[3][3];
+  {
+    const size_t rt = size_t(Mirror(ya - 1, P.py)) * B.ipitch;
+    const size_t rm = size_t(ya) * B.ipitch;
+    for (int c = 0; c < 3; ++c) {
+      const float* src = B.ar + c * plane;
+      t[c][0] = src[rt + xm];
+      t[c][1] = src[rt + x];
+      t[c][2] = src[rt + xp];
+      m[c][0] = src[rm + xm];
+      m[c][1] = src[rm + x];
+      m[c][2] = src[rm + xp];
+    }
   }
-  if (P.out_flags & kOutSrgb8) {
-    uint8_t* o = B.srgb8 + size_t(y) * B.spitch + 3 * x;
-    o[0] = Srgb8(lr);
-    o[1] = Srgb8(lg);
-    o[2] = Srgb8(lb);
+  for (int y = ya; y < yb; ++y) {
+    const size_t rb = size_t(y + 1 < P.py ? y + 1 : P.py - 1) * B.ipitch;
+    float v[3];
+    for (int c = 0; c < 3; ++c) {
+      const float* src = B.ar + c * plane;
+      const float bl = src[rb + xm], bc = src[rb + x], br = src[rb + xp];
+      if (P.gaborish) {
+        // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
+        const float sum_tb = t[c][1] + bc;
+        const float mul0 = m[c][1] * P.gab_w[0];
+        const float sum_lr = m[c][0] + m[c][2];
+        const float x1 = sum_tb + sum_lr;
+        const float mul1 = x1 * P.gab_w[1] + mul0;
+        const float sum_t2 = t[c][0] + t[c][2];
+        const float sum_b2 = bl + br;
+        const float x2 = sum_t2 + sum_b2;
+        v[c] = x2 * P.gab_w[2] + mul1;
+      } else {
+        v[c] = m[c][1];
+      }
+      if (B.keep) B.gab[c * plane + size_t(y) * B.ipitch + x] = v[c];
+      t[c][0] = m[c][0];
+      t[c][1] = m[c][1];
+      t[c][2] = m[c][2];
+      m[c][0] = bl;
+      m[c][1] = bc;
+      m[c][2] = br;
+    }
+    if (x >= P.xsize || y >= P.ysize) continue;
+    const float gamma_r = 1.0f * (v[1] + v[0]);
+    const float gamma_g = 1.0f * (v[1] - v[0]);
+    const float gamma_b = v[2];
+    const float gamma_r2 = gamma_r * gamma_r;
+    const float gamma_g2 = gamma_g * gamma_g;
+    const float gamma_b2 = gamma_b * gamma_b;
+    const float mixed_r = gamma_r2 * gamma_r + P.neg_bias[0];
+    const float mixed_g = gamma_g2 * gamma_g + P.neg_bias[1];
+    const float mixed_b = gamma_b2 * gamma_b + P.neg_bias[2];
+    float lr = inv[0] * mixed_r;
+    float lg = inv[3] * mixed_r;
+    float lb = inv[6] * mixed_r;
+    const float tr = inv[1] * mixed_g;
+    const float tg = inv[4] * mixed_g;
+    const float tb = inv[7] * mixed_g;
+    lr = inv[2] * mixed_b + lr;
+    lg = inv[5] * mixed_b + lg;
+    lb = inv[8] * mixed_b + lb;
+    lr = lr + tr;
+    lg = lg + tg;
+    lb = lb + tb;
+    if (P.out_flags & kOutLinear) {
+      const size_t oplane = size_t(P.ysize) * B.opitch;
+      B.out[size_t(y) * B.opitch + x] = lr;
+      B.out[oplane + size_t(y) * B.opitch + x] = lg;
+      B.out[2 * oplane + size_t(y) * B.opitch + x] = lb;
+    }
+    if (P.out_flags & kOutSrgb8) {
+      uint8_t* o = B.srgb8 + size_t(y) * B.spitch + 3 * x;
+      o[0] = Srgb8(lr);
+      o[1] = Srgb8(lg);
+      o[2] = Srgb8(lb);
+    }
   }
 }
 

@@ -11,7 +11,6 @@
 //  - the scale factors of the reinterpreting DCTs (DCTTotalScale /
 //    DCTInvTotalScale, pik/ac_strategy.cc:258-270) and of AR's lowest
 //    frequencies (pik/ac_strategy.h:33-38);
-//  - basis matrices of the transposed-scaled transforms.
 #include <math.h>
 
 #include <cmath>
@@ -198,35 +197,6 @@ void Ub4Kernel(float k[3][4]) {
     for (int j = 0; j < split0; ++j) k[0][q] += kernel[j];
     for (int j = split0; j < split1; ++j) k[1][q] += kernel[j];
     for (int j = split1; j < 2 * radius + 1; ++j) k[2][q] += kernel[j];
-  }
-}
-
-int BasisOffset(int n) {
-  int off = 0;
-  for (int m = 2; m < n; m *= 2) off += 2 * m * m;
-  return off;
-}
-
-void ComputeBasis(float* out) {
-  for (int n = 2; n <= 32; n *= 2) {
-    float* b = out + BasisOffset(n);
-    float* binv = b + n * n;
-    for (int k = 0; k < n; ++k) {
-      double w;
-      if (k == 0) {
-        w = 1.0;
-      } else if (n == 8) {
-        w = 1.0 / cos(k * M_PI / 16.0);
-      } else {
-        w = 2.0 * cos(k * M_PI / (2.0 * n));
-      }
-      const double alpha = (k == 0 ? 1.0 : 2.0) / n;
-      for (int x = 0; x < n; ++x) {
-        const double c = cos(M_PI * (2 * x + 1) * k / (2.0 * n));
-        b[k * n + x] = static_cast<float>(w * c);        // B[k][x]
-        binv[x * n + k] = static_cast<float>(alpha * c / w);  // Binv[x][k]
-      }
-    }
   }
 }
 

@@ -10,51 +10,6 @@
 
 namespace pikgpu {
 
-// ---------------------------------------------------------------------------
-// Transposed-scaled transforms.
-//
-// Pixels P (N x N, [y][x]) and coefficients F (N x N, row-major) relate by
-// P = B^T F^T B with the float basis B of ComputeBasis: the coefficient row
-// index is the horizontal frequency (pik/dct.h:351-507).
-
-// In-thread ts-IDCT / ts-DCT (with the 1/N^2 scaling of ScaleToBlock) of an
-// n x n block, n = 2, 4, 8, as the reinterpreting transforms use them
-// (pik/ac_strategy.cc:272-315); n == 2 is the reference's explicit 2-point
-// form (pik/dct.h:478-504).
-__device__ inline void SmallIdct(const DeviceConsts* K, const float* in,
-                                 int in_stride, float* out, int out_stride,
-                                 int n) {
-  if (n == 2) {
-    const float a00 = in[0], a01 = in[1], a10 = in[in_stride],
-                a11 = in[in_stride + 1];
-    out[0] = a00 + a01 + a10 + a11;
-    out[1] = a00 + a01 - a10 - a11;
-    out[out_stride] = a00 - a01 + a10 - a11;
-    out[out_stride + 1] = a00 - a01 - a10 + a11;
-  } else if (n == 4) {
-    dct1d::Idct2D<4>(in, in_stride, out, out_stride);
-  } else {
-    dct1d::Idct2D<8>(in, in_stride, out, out_stride);
-  }
-}
-
-__device__ inline void SmallDct(const DeviceConsts* K, const float* in,
-                                int in_stride, float* out, int out_stride,
-                                int n) {
-  if (n == 2) {
-    const float a00 = in[0], a01 = in[1], a10 = in[in_stride],
-                a11 = in[in_stride + 1];
-    out[0] = (a00 + a01 + a10 + a11) * 0.25f;
-    out[1] = (a00 + a01 - a10 - a11) * 0.25f;
-    out[out_stride] = (a00 - a01 + a10 - a11) * 0.25f;
-    out[out_stride + 1] = (a00 - a01 - a10 + a11) * 0.25f;
-  } else if (n == 4) {
-    dct1d::Dct2D<4>(in, in_stride, out, out_stride);
-  } else {
-    dct1d::Dct2D<8>(in, in_stride, out, out_stride);
-  }
-}
-
 // 3x3 symmetric convolution of one plane at (x, y) with mirrored borders,
 // in the operation order of strategy::Symmetric3 (pik/convolve.h:605-621).
 __device__ inline float Symmetric3(const float* in, int pitch, int W, int H,
@@ -85,36 +40,6 @@ __device__ inline float AdjustQuantBias(int c, float q) {
   if (aq >= 1.5f) return q - 0.145f / q;
   if (aq >= 0.5f) return copysignf(1 - one_bias, q);
   return q;
-}
-
-// Dequantised (and for X/B colour-correlated) AC coefficient at frame-wide
-// storage position (row = block row, col = block x * 64 + k) of plane c;
-// `yval` is the dequantised Y value at the same position
-// (Dequant::DoAC, pik/compressed_image.cc:621-693).
-__device__ inline float DequantCoef(const FrameParams& P, const FrameBuffers& B,
-                                    int c, int row, int col, float yval) {
-  const int bxa = col >> 6, k = col & 63;
-  const int blk = row * P.bx + bxa;
-  const uint8_t a = B.acs[blk];
-  const int raw = a >> 4, idx = a & 15;
-  const int qf = B.qf[blk];
-  const int tile = (row >> 3) * P.tx + (bxa >> 3);
-  const int qt = B.dqmap[B.dcf[tile] * 256 + qf - 1];
-  const int kind = AcsQuantKind(raw);
-  const float m = B.table[B.offsets[(qt * kNumQuantKinds + kind) * 3 + c] +
-                          idx * 64 + k];
-  const float scaled = qf == 0 ? 1E10f : P.inv_global_scale / float(qf);
-  const float mul = m * scaled;
-  const float q = float(B.ac[(size_t(c) * P.by + row) * (P.bx * 64) + col]);
-  const float d = AdjustQuantBias(c, q) * mul;
-  if (c == 1) return d;
-  float f;
-  if (c == 0) {
-    f = float(B.ytox[tile] - 128) * 0.00390625f;
-  } else {
-    f = float(B.ytob[tile]) * 0.0078125f;
-  }
-  return f * yval + d;
 }
 
 // Per-storage-block dequantisation parameters of plane c: the dequant
@@ -156,14 +81,37 @@ __device__ inline float DequantWith(const BlockDq& d, int c, int16_t q, int k,
 
 // ---------------------------------------------------------------------------
 // Cooperative transforms of the three channels of a 32x32 tile (256
-// threads). Coefficients live in the reference's storage layout, 4 block
-// rows x 256 floats per channel; pixels and the transpose buffer are 32x32
-// per channel. The 2-D transforms are the reference's two column passes
-// with a transpose in between (pik/dct.h:370-507); each line is a 1-D
-// butterfly of dct1d.h computed by one thread. Every line of every
-// transform of every channel is a task, and the tasks are spread over all
-// 256 threads: a DCT8 block has 8 line tasks, DCT16 16, DCT32 32, DCT4X4 16
-// (4 lines in each 4x4 sub-block), DCT2X2 and IDENTITY one direct task.
+// threads). The 2-D transforms are the reference's two column passes with a
+// transpose in between (pik/dct.h:370-507); each line is a 1-D butterfly of
+// dct1d.h computed by one thread. Every line of every transform of every
+// channel is a task, and the tasks are spread over all 256 threads: a DCT8
+// block has 8 line tasks, DCT16 16, DCT32 32, DCT4X4 16 (4 lines in each
+// 4x4 sub-block), DCT2X2 and IDENTITY one direct task.
+//
+// LDS layouts (per channel):
+//  * pixels and the transpose scratch: 32 rows of kTilePitch = 33 floats;
+//  * coefficients: the reference's storage order (4 block rows; block bx
+//    of a row holds its 64 coefficients, a DCT16/32's F matrix runs across
+//    the s blocks of its s rows), with kCoefBlock = 72 floats per block.
+// With power-of-two pitches the lines that neighbouring lanes read or write
+// would all sit in the same LDS bank (e.g. a DCT8 column pass writing 8
+// transposed rows: an 8- to 32-way conflict); the odd pixel pitch and the
+// 8-float gap between blocks give the lanes of a wave distinct banks for
+// the DCT8/16/32 passes.
+constexpr int kTilePitch = 33;
+constexpr int kTilePlane = 32 * kTilePitch;  // 1056
+constexpr int kCoefBlock = 72;
+constexpr int kCoefRow = 4 * kCoefBlock;     // 288
+constexpr int kCoefPlane = 4 * kCoefRow;     // 1152
+
+// Pixel (y, x) of a tile channel.
+__host__ __device__ inline int PixAt(int y, int x) {
+  return y * kTilePitch + x;
+}
+// Coefficient at storage row `row` (0..3), column `col` (block * 64 + k).
+__host__ __device__ inline int CoefAt(int row, int col) {
+  return row * kCoefRow + (col >> 6) * kCoefBlock + (col & 63);
+}
 
 struct TileTasks {
   int off[17];      // first task of each block; off[16] = tasks per channel
@@ -209,10 +157,10 @@ __device__ inline int TaskBlock(const TileTasks& T, int t) {
   return lo;
 }
 
-// Coefficient storage index of F[i][j] (row-major N x N, flat i*N + j) of a
+// Coefficient index of F[i][j] (row-major N x N, flat i*N + j) of a
 // transform with origin block (oby, obx) covering s x s blocks.
 __device__ inline int CoefIdx(int oby, int obx, int s, int flat) {
-  return (oby + flat / (64 * s)) * 256 + obx * 64 + flat % (64 * s);
+  return CoefAt(oby + flat / (64 * s), obx * 64 + flat % (64 * s));
 }
 
 template <int N>
@@ -222,30 +170,30 @@ __device__ inline void InvLine1(const float* coef, float* tmp, int oby,
   float v[N], w[N];
   for (int r = 0; r < N; ++r) v[r] = coef[CoefIdx(oby, obx, s, r * N + c)];
   dct1d::Idct<N>(v, w);
-  for (int x = 0; x < N; ++x) tmp[(oby * 8 + c) * 32 + obx * 8 + x] = w[x];
+  for (int x = 0; x < N; ++x) tmp[PixAt(oby * 8 + c, obx * 8 + x)] = w[x];
 }
 template <int N>
 __device__ inline void InvLine2(const float* tmp, float* pix, int oby,
                                 int obx, int x) {
   float v[N], w[N];
-  for (int c = 0; c < N; ++c) v[c] = tmp[(oby * 8 + c) * 32 + obx * 8 + x];
+  for (int c = 0; c < N; ++c) v[c] = tmp[PixAt(oby * 8 + c, obx * 8 + x)];
   dct1d::Idct<N>(v, w);
-  for (int y = 0; y < N; ++y) pix[(oby * 8 + y) * 32 + obx * 8 + x] = w[y];
+  for (int y = 0; y < N; ++y) pix[PixAt(oby * 8 + y, obx * 8 + x)] = w[y];
 }
 template <int N>
 __device__ inline void FwdLine1(const float* pix, float* tmp, int oby,
                                 int obx, int x) {
   float v[N], w[N];
-  for (int y = 0; y < N; ++y) v[y] = pix[(oby * 8 + y) * 32 + obx * 8 + x];
+  for (int y = 0; y < N; ++y) v[y] = pix[PixAt(oby * 8 + y, obx * 8 + x)];
   dct1d::Dct<N>(v, w);
-  for (int k = 0; k < N; ++k) tmp[(oby * 8 + x) * 32 + obx * 8 + k] = w[k];
+  for (int k = 0; k < N; ++k) tmp[PixAt(oby * 8 + x, obx * 8 + k)] = w[k];
 }
 template <int N>
 __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
                                 int obx, int k) {
   constexpr int s = N / 8;
   float v[N], w[N];
-  for (int x = 0; x < N; ++x) v[x] = tmp[(oby * 8 + x) * 32 + obx * 8 + k];
+  for (int x = 0; x < N; ++x) v[x] = tmp[PixAt(oby * 8 + x, obx * 8 + k)];
   dct1d::Dct<N>(v, w);
   const float scale = 1.0f / (N * N);
   for (int j = 0; j < N; ++j) {
@@ -274,7 +222,7 @@ __device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
     if (pass == 0) {
       // column q of the sub-block's 4x4 matrix G, G[iy][ix] =
       // coef[(sy + 2 iy) * 8 + sx + 2 ix], G[0][0] from the DC Hadamard
-      const float* cb = coef + oby * 256 + obx * 64;
+      const float* cb = coef + CoefAt(oby, obx * 64);
       const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
       float dc;
       if (sub == 0) dc = b00 + b01 + b10 + b11;
@@ -286,20 +234,20 @@ __device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
       }
       dct1d::Idct<4>(v, w);
       for (int x = 0; x < 4; ++x) {
-        tmp[(oby * 8 + sy * 4 + q) * 32 + obx * 8 + sx * 4 + x] = w[x];
+        tmp[PixAt(oby * 8 + sy * 4 + q, obx * 8 + sx * 4 + x)] = w[x];
       }
     } else {
       for (int c = 0; c < 4; ++c) {
-        v[c] = tmp[(oby * 8 + sy * 4 + c) * 32 + obx * 8 + sx * 4 + q];
+        v[c] = tmp[PixAt(oby * 8 + sy * 4 + c, obx * 8 + sx * 4 + q)];
       }
       dct1d::Idct<4>(v, w);
       for (int y = 0; y < 4; ++y) {
-        pix[(oby * 8 + sy * 4 + y) * 32 + obx * 8 + sx * 4 + q] = w[y];
+        pix[PixAt(oby * 8 + sy * 4 + y, obx * 8 + sx * 4 + q)] = w[y];
       }
     }
   } else if (pass == 1) {
-    DirectInverse8(raw, coef + oby * 256 + obx * 64,
-                   pix + oby * 8 * 32 + obx * 8, 32);
+    DirectInverse8(raw, coef + CoefAt(oby, obx * 64),
+                   pix + PixAt(oby * 8, obx * 8), kTilePitch);
   }
 }
 
@@ -323,18 +271,18 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
   } else if (AcsIsDCT4(raw)) {
     const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, q = l & 3;
     float v[4], w[4];
-    float* cb = coef + oby * 256 + obx * 64;
+    float* cb = coef + CoefAt(oby, obx * 64);
     if (pass == 0) {
       for (int y = 0; y < 4; ++y) {
-        v[y] = pix[(oby * 8 + sy * 4 + y) * 32 + obx * 8 + sx * 4 + q];
+        v[y] = pix[PixAt(oby * 8 + sy * 4 + y, obx * 8 + sx * 4 + q)];
       }
       dct1d::Dct<4>(v, w);
       for (int k = 0; k < 4; ++k) {
-        tmp[(oby * 8 + sy * 4 + q) * 32 + obx * 8 + sx * 4 + k] = w[k];
+        tmp[PixAt(oby * 8 + sy * 4 + q, obx * 8 + sx * 4 + k)] = w[k];
       }
     } else if (pass == 1) {
       for (int x = 0; x < 4; ++x) {
-        v[x] = tmp[(oby * 8 + sy * 4 + x) * 32 + obx * 8 + sx * 4 + q];
+        v[x] = tmp[PixAt(oby * 8 + sy * 4 + x, obx * 8 + sx * 4 + q)];
       }
       dct1d::Dct<4>(v, w);
       for (int j = 0; j < 4; ++j) {
@@ -349,16 +297,17 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
     }
   } else if (pass == 0) {
     if (kAdjoint && raw == kIdentity) {
-      IdentityAdjoint8(pix + oby * 8 * 32 + obx * 8, 32,
-                       coef + oby * 256 + obx * 64);
+      IdentityAdjoint8(pix + PixAt(oby * 8, obx * 8), kTilePitch,
+                       coef + CoefAt(oby, obx * 64));
     } else {
-      DirectForward8(raw, pix + oby * 8 * 32 + obx * 8, 32,
-                     coef + oby * 256 + obx * 64);
+      DirectForward8(raw, pix + PixAt(oby * 8, obx * 8), kTilePitch,
+                     coef + CoefAt(oby, obx * 64));
     }
   }
 }
 
-// coef[3][1024] -> pix[3][1024]. All 256 threads must call.
+// coef (3 x kCoefPlane) -> pix (3 x kTilePlane), tmp 3 x kTilePlane. All
+// 256 threads must call.
 __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
                                     float* pix, float* tmp) {
   const int total = T.off[16];
@@ -367,14 +316,16 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
       const int c = k / total, t = k - c * total;
       const int b = TaskBlock(T, t);
       InverseTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
-                  coef + c * 1024, pix + c * 1024, tmp + c * 1024);
+                  coef + c * kCoefPlane, pix + c * kTilePlane,
+                  tmp + c * kTilePlane);
     }
     __syncthreads();
   }
 }
 
-// pix[3][1024] -> coef[3][1024] for the transforms in T (others keep their
-// coefficient entries). All 256 threads must call.
+// pix (3 x kTilePlane) -> coef (3 x kCoefPlane) for the transforms in T
+// (others keep their coefficient entries), tmp 3 x kTilePlane. All 256
+// threads must call.
 template <bool kAdjoint = false>
 __device__ inline void TileForward3(const TileTasks& T, const float* pix,
                                     float* coef, float* tmp) {
@@ -384,7 +335,8 @@ __device__ inline void TileForward3(const TileTasks& T, const float* pix,
       const int c = k / total, t = k - c * total;
       const int b = TaskBlock(T, t);
       ForwardTask<kAdjoint>(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
-                            pix + c * 1024, coef + c * 1024, tmp + c * 1024);
+                            pix + c * kTilePlane, coef + c * kCoefPlane,
+                            tmp + c * kTilePlane);
     }
     __syncthreads();
   }
@@ -444,6 +396,8 @@ __global__ void KEpf(Img in, int gaborish, int dc_mode, FrameParams P,
                      FrameBuffers B, const DeviceConsts* K, const float* mm,
                      float* out, int out_pitch, size_t out_plane);
 __global__ void KDots(FrameParams P, FrameBuffers B);
+// KFinal: rows per workgroup of its 256-column strips.
+constexpr int kFinalRows = 16;
 __global__ void KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KCopyPlanes(const float* src, float* dst, size_t n);
 

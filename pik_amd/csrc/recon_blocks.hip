@@ -16,10 +16,12 @@ namespace pikgpu {
 
 __global__ void __launch_bounds__(256)
     KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  __shared__ float s_coef[3][1024];
-  __shared__ float s_pix[3][1024];
-  __shared__ float s_pred[3][1024];
-  __shared__ float s_tmp[3][1024];
+  // s_pix holds pixels (kTilePlane per channel) except while it receives
+  // the HF prediction's coefficients, hence kCoefPlane.
+  __shared__ float s_coef[3][kCoefPlane];
+  __shared__ float s_pix[3 * kCoefPlane];
+  __shared__ float s_pred[3][kTilePlane];
+  __shared__ float s_tmp[3][kTilePlane];
   __shared__ uint8_t s_acs[16];
   __shared__ TileTasks s_inv, s_fwd;
 
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__(256)
         } else if (P.predict_lf && fi < 2 * cs && fj < 2 * cs) {
           v += lf[(2 * (obyg + 1) + fi) * kG2Pitch + 2 * (obxg + 1) + fj];
         }
-        s_coef[c][crow * 256 + ccol + j] = v;
+        s_coef[c][CoefAt(crow, ccol + j)] = v;
       }
     }
   }
@@ -105,21 +107,23 @@ __global__ void __launch_bounds__(256)
             v[t] = row[xq + 1] * P.ub4[0][q] + row[xq + 2] * P.ub4[1][q] +
                    row[xq + 3] * P.ub4[2][q];
           }
-          s_pred[c][pr * 32 + pc0 + j] =
+          s_pred[c][PixAt(pr, pc0 + j)] =
               v[0] * P.ub4[0][iy] + v[1] * P.ub4[1][iy] + v[2] * P.ub4[2][iy];
         }
       }
     }
     __syncthreads();
-    TileForward3(s_fwd, &s_pred[0][0], &s_pix[0][0], &s_tmp[0][0]);
-    // s_pix now holds the prediction's coefficients for PredictHF blocks.
+    TileForward3(s_fwd, &s_pred[0][0], s_pix, &s_tmp[0][0]);
+    // s_pix now holds the prediction's coefficients for PredictHF blocks
+    // (channel stride kCoefPlane).
     if (cvalid && ca != 0xFF && AcsPredictHF(craw)) {
       for (int j = 0; j < 4; ++j) {
         const int flat = cflat0 + j;
         const int fi = flat / cN, fj = flat % cN;
         if (!(fi < 2 * cs && fj < 2 * cs)) {
           for (int c = 0; c < 3; ++c) {
-            s_coef[c][crow * 256 + ccol + j] += s_pix[c][crow * 256 + ccol + j];
+            s_coef[c][CoefAt(crow, ccol + j)] +=
+                s_pix[c * kCoefPlane + CoefAt(crow, ccol + j)];
           }
         }
       }
@@ -129,17 +133,19 @@ __global__ void __launch_bounds__(256)
   if (B.keep && cvalid) {
     for (int c = 0; c < 3; ++c) {
       float* dst = B.coeffs + (size_t(c) * P.by + cby) * (P.bx * 64) + cbx * 64;
-      for (int j = 0; j < 4; ++j) dst[(ccol + j) & 63] = s_coef[c][crow * 256 + ccol + j];
+      for (int j = 0; j < 4; ++j) dst[(ccol + j) & 63] = s_coef[c][CoefAt(crow, ccol + j)];
     }
   }
   // 4. IDCT (InverseIntegralTransform, pik/compressed_image.cc:869-895)
-  TileInverse3(s_inv, &s_coef[0][0], &s_pix[0][0], &s_tmp[0][0]);
+  TileInverse3(s_inv, &s_coef[0][0], s_pix, &s_tmp[0][0]);
   const int py = tby * 8 + pr;
   if (py < P.py) {
     for (int c = 0; c < 3; ++c) {
       float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + pc0;
       for (int j = 0; j < 4; ++j) {
-        if (tbx * 8 + pc0 + j < P.px) dst[j] = s_pix[c][pr * 32 + pc0 + j];
+        if (tbx * 8 + pc0 + j < P.px) {
+          dst[j] = s_pix[c * kTilePlane + PixAt(pr, pc0 + j)];
+        }
       }
     }
   }
