@@ -30,17 +30,31 @@ __global__ void __launch_bounds__(256)
   const int tile = XcdTile(blockIdx.x, tiles_x * ((P.by + 3) / 4));
   if (tile < 0) return;
   const int tbx = (tile % tiles_x) * 4, tby = (tile / tiles_x) * 4;  // blocks
+  // pred2x2 window of the tile's HF prediction, 10 x 10 values per channel
+  // (the tile's 8 x 8 plus the blur's one-value ring). It lives in the
+  // transform scratch, which is first written after the last read of the
+  // window (three workgroups per CU need the LDS under 160 KB / 3).
+  float (*s_p2)[100] = reinterpret_cast<float (*)[100]>(&s_tmp[0][0]);
   if (tid < 16) {
     const int bxa = tbx + (tid & 3), bya = tby + (tid >> 2);
     s_acs[tid] = (bxa < P.bx && bya < P.by) ? B.acs[bya * P.bx + bxa] : 0xFF;
   }
   __syncthreads();
   BuildTasks(s_acs, false, &s_inv);
-  if (P.predict_hf) BuildTasks(s_acs, true, &s_fwd);
 
   const int gxi = tbx / kGroupDimBlocks, gyi = tby / kGroupDimBlocks;
   const int g = gyi * P.gx + gxi;
   const int gbx0 = gxi * kGroupDimBlocks, gby0 = gyi * kGroupDimBlocks;
+  // pred2x2 rows / columns 2 * (tile's group-local block) + 1 .. + 10
+  const int p2r0 = 2 * (tby - gby0) + 1, p2c0 = 2 * (tbx - gbx0) + 1;
+  if (P.predict_hf) {
+    for (int k = tid; k < 300; k += 256) {
+      const int c = k / 100, i = (k % 100) / 10, j = k % 10;
+      s_p2[c][i * 10 + j] = B.pred2x2[(size_t(g) * 3 + c) * kG2Plane +
+                                      (p2r0 + i) * kG2Pitch + p2c0 + j];
+    }
+    BuildTasks(s_acs, true, &s_fwd);  // (its barrier publishes s_p2)
+  }
 
   // Coefficient entries owned by this thread: storage row crow of the tile,
   // columns ccol..ccol+3 (one 8x8 storage block per 16 threads).
@@ -63,15 +77,39 @@ __global__ void __launch_bounds__(256)
   if (cvalid) {
     const size_t plane = size_t(P.by) * P.bx * 64;
     const size_t at = size_t(cby) * P.bx * 64 + size_t(cbx) * 64 + (ccol & 63);
+    // The block's dequantisation parameters (MakeBlockDq, shared by the
+    // three channels) and every load are issued before the arithmetic.
+    const int blk = cby * P.bx + cbx;
+    const int qfi = B.qf[blk];
+    const int ctile = (cby >> 3) * P.tx + (cbx >> 3);
+    const int qt = B.dqmap[B.dcf[ctile] * 256 + qfi - 1];
+    const int kind = AcsQuantKind(craw);
+    const float scaled = qfi == 0 ? 1E10f : P.inv_global_scale / float(qfi);
+    const float ccf[3] = {float(B.ytox[ctile] - 128) * 0.00390625f, 0.0f,
+                          float(B.ytob[ctile]) * 0.0078125f};
+    float mv[3][4];
+    int16_t qv[3][4];
+    for (int c = 0; c < 3; ++c) {
+      const float* m = B.table + B.offsets[(qt * kNumQuantKinds + kind) * 3 + c] +
+                       cidx * 64 + (ccol & 63);
+      // four int16 coefficients, 8-byte aligned (at is a multiple of 4)
+      const uint2 q4 = *reinterpret_cast<const uint2*>(B.ac + c * plane + at);
+      qv[c][0] = int16_t(q4.x & 0xFFFF);
+      qv[c][1] = int16_t(q4.x >> 16);
+      qv[c][2] = int16_t(q4.y & 0xFFFF);
+      qv[c][3] = int16_t(q4.y >> 16);
+      for (int j = 0; j < 4; ++j) mv[c][j] = m[j];
+    }
     float yv[4];
     for (int ci = 0; ci < 3; ++ci) {
       const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
-      const BlockDq dq = MakeBlockDq(P, B, c, cby, cbx);
-      const int16_t* q = B.ac + c * plane + at;
       const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
       const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
       for (int j = 0; j < 4; ++j) {
-        float v = DequantWith(dq, c, q[j], (ccol & 63) + j, yv[j]);
+        // DequantWith's operation order
+        const float mul = mv[c][j] * scaled;
+        const float d = AdjustQuantBias(c, float(qv[c][j])) * mul;
+        float v = c == 1 ? d : ccf[c] * yv[j] + d;
         if (c == 1) yv[j] = v;
         const int flat = cflat0 + j;
         const int fi = flat / cN, fj = flat % cN;
@@ -93,19 +131,24 @@ __global__ void __launch_bounds__(256)
   if (P.predict_hf) {
     const uint8_t pa = s_acs[(pr >> 3) * 4 + (pc0 >> 3)];
     if (pa != 0xFF && AcsPredictHF(pa >> 4)) {
-      const int by_l = (tby - gby0) + (pr >> 3);  // group-local block row
+      // Pixel row pr of the tile reads pred2x2 rows rbase + 1 .. + 3 and
+      // its four columns (one upsampled pred2x2 column, q = j) columns
+      // xq + 1 .. + 3, where rbase = 2 * block row + (pr & 7 >= 4) and
+      // xq = pc0 / 4, both tile-local; in window coordinates that is rows
+      // rbase - 1 + 1 + t and columns xq + t.
       const int py = pr & 7, iy = py & 3;
-      const int rbase = py < 4 ? 2 * by_l : 2 * by_l + 1;
+      const int wr = 2 * (pr >> 3) + (py < 4 ? 0 : 1);  // window row of t = 0
+      const int wc = pc0 >> 2;                         // window column
       for (int c = 0; c < 3; ++c) {
-        const float* p2 = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane;
+        float rw[3][3];
+        for (int t = 0; t < 3; ++t) {
+          for (int u = 0; u < 3; ++u) rw[t][u] = s_p2[c][(wr + t) * 10 + wc + u];
+        }
         for (int j = 0; j < 4; ++j) {
-          const int Xc = (tbx - gbx0) * 8 + pc0 + j;  // group-local column
-          const int xq = Xc >> 2, q = Xc & 3;
           float v[3];
           for (int t = 0; t < 3; ++t) {
-            const float* row = p2 + (rbase + t + 1) * kG2Pitch;
-            v[t] = row[xq + 1] * P.ub4[0][q] + row[xq + 2] * P.ub4[1][q] +
-                   row[xq + 3] * P.ub4[2][q];
+            v[t] = rw[t][0] * P.ub4[0][j] + rw[t][1] * P.ub4[1][j] +
+                   rw[t][2] * P.ub4[2][j];
           }
           s_pred[c][PixAt(pr, pc0 + j)] =
               v[0] * P.ub4[0][iy] + v[1] * P.ub4[1][iy] + v[2] * P.ub4[2][iy];
