@@ -50,7 +50,7 @@ struct pikgpu_ctx {
   // device buffers (grown on demand)
   DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
       dotbins;
-  DevBuf dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
+  DevBuf desc, dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
       dots_img, gab, out, mm_partial, mm, ar_keep, srgb8;
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
@@ -107,7 +107,7 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->lut,     &ctx->dcf,    &ctx->ytox,    &ctx->ytob,
                    &ctx->dqmap,   &ctx->table,  &ctx->offsets, &ctx->dots,
                    &ctx->dotbins, &ctx->srgb8,
-                   &ctx->dcq,     &ctx->dcv,    &ctx->llf,     &ctx->tmp2x2,
+                   &ctx->desc,    &ctx->dcq,     &ctx->dcv,    &ctx->llf,     &ctx->tmp2x2,
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
                    &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
@@ -505,6 +505,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   GROW(table, f->dequant_table_size * 4);
   GROW(offsets, f->num_dequant_offsets * 4);
   GROW(dots, f->num_dots * sizeof(pikgpu_dot));
+  GROW(desc, 2 * nblk * 16);
   GROW(dcq, 3 * nblk * 4);
   GROW(dcv, 3 * nblk * 4);
   GROW(llf, ngroups * 3 * kLlfPlane * 4);
@@ -597,6 +598,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
     B.dot_tile_start = bins + dot_ids.size();
     B.dot_list = bins + dot_ids.size() + dot_starts.size();
   }
+  B.desc = static_cast<uint4*>(ctx->desc.p);
   B.dcq = static_cast<float*>(ctx->dcq.p);
   B.dc = static_cast<float*>(ctx->dcv.p);
   B.llf = static_cast<float*>(ctx->llf.p);

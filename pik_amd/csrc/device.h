@@ -46,6 +46,12 @@ struct FrameBuffers {
   const int32_t* dot_tile_start;
   const int32_t* dot_list;
   // intermediates
+  // Per block two descriptors (KDcDequant): {table offset of the block's
+  // dequant matrix slot for channels 0..2 (offsets[(qt * 7 + kind) * 3 + c]
+  // + idx * 64), bits of inv_global_scale / qf} for the block's own quant
+  // value, then the same for the quant value of its transform's first
+  // block (AR's quantisation intervals).
+  uint4* desc;
   float* dcq;             // 3 planes by x bx (DequantDC)
   float* dc;              // 3 planes by x bx (after DC AR)
   float* llf;             // [group][3][kLlfPlane]

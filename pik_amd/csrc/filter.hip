@@ -274,11 +274,59 @@ __global__ void __launch_bounds__(256, 3)
   const float all_min = mm[7];
   const float gmul = mm[6];
 
-  // 1. input tile with a 7-pixel mirrored border
-  for (int i = tid; i < 3 * 46 * 46; i += 256) {
-    const int c = i / (46 * 46), rem = i % (46 * 46);
-    const int ly = rem / 46, lx = rem % 46;
-    s_in[c][ly * kInP + lx] = ImgAt(in, c, x0 - 7 + lx, y0 - 7 + ly);
+  // AR (step 5) needs the dequant matrix entries of the coefficients this
+  // thread owns: their loads are issued here, after the input window's,
+  // and complete while the guide and the filter run.
+  const int crow = tid >> 6, ccol = (tid & 63) << 2;
+  const int ar_blk =
+      (y0 / 8 + crow) * P.bx + x0 / 8 + (ccol >> 6);
+  const bool ar_valid =
+      !dc_mode && x0 / 8 + (ccol >> 6) < P.bx && y0 / 8 + crow < P.by;
+  uint4 ar_desc = make_uint4(0, 0, 0, 0);
+  if (ar_valid) ar_desc = B.desc[2 * ar_blk + 1];
+
+  // 1. input tile with a 7-pixel mirrored border. One reflection gives the
+  //    mirror for every position within 7 of the image (both images are at
+  //    least 8 wide and high); positions further out (a tile wider than a
+  //    small image reaches up to 38 past its origin) are only clamped into
+  //    the image, their values are never used. All loads are issued before
+  //    the LDS stores.
+  {
+    constexpr int kWin = 3 * 46 * 46;
+    constexpr int kPer = (kWin + 255) / 256;
+    float wv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      float v = 0.0f;
+      if (i < kWin) {
+        const int c = i / (46 * 46), rem = i - c * (46 * 46);
+        const int ly = rem / 46, lx = rem - ly * 46;
+        int x = x0 - 7 + lx, y = y0 - 7 + ly;
+        x = x < 0 ? -x - 1 : (x >= in.W ? 2 * in.W - 1 - x : x);
+        y = y < 0 ? -y - 1 : (y >= in.H ? 2 * in.H - 1 - y : y);
+        x = min(max(x, 0), in.vw - 1);
+        y = min(max(y, 0), in.vh - 1);
+        v = in.p[c * in.plane + size_t(y) * in.pitch + x];
+      }
+      wv[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      if (i < kWin) {
+        const int c = i / (46 * 46), rem = i - c * (46 * 46);
+        const int ly = rem / 46, lx = rem - ly * 46;
+        s_in[c][ly * kInP + lx] = wv[u];
+      }
+    }
+  }
+  float ar_m[3][4];
+  if (ar_valid) {
+    const uint32_t slot[3] = {ar_desc.x, ar_desc.y, ar_desc.z};
+    for (int c = 0; c < 3; ++c) {
+      for (int j = 0; j < 4; ++j) ar_m[c][j] = B.table[slot[c] + (ccol & 63) + j];
+    }
   }
   // 2. per-block sigma and strategy
   if (tid < 16) {
@@ -505,8 +553,10 @@ __global__ void __launch_bounds__(256, 3)
   TileForward3(s_tasks, &s_out[0][0], &s_coef[0][0], &s_tmp[0][0]);
   {
     // coefficient entries owned by this thread; the same thread scales all
-    // three channels of its entries
-    const int crow = tid >> 6, ccol = (tid & 63) << 2;
+    // three channels of its entries. The quantisation interval uses the
+    // quant value of the transform's first block (ar_desc: its
+    // inv_global_scale / qf and its matrix slots for this block, so entry
+    // k of this storage block is the matrix's flat index idx * 64 + k).
     const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
     if (ca != 0xFF) {
       const int raw = ca >> 4, idx = ca & 15;
@@ -514,14 +564,7 @@ __global__ void __launch_bounds__(256, 3)
       const int N = 8 * s;
       const int coby = crow - idx / s, cobx = (ccol >> 6) - idx % s;
       const int cflat0 = (crow - coby) * 64 * s + (ccol - cobx * 64);
-      // origin (first) block of the transform: its quant value and table
-      const int obxa = x0 / 8 + cobx, obya = y0 / 8 + coby;
-      const int oblk = obya * P.bx + obxa;
-      const int qac = B.qf[oblk];
-      const float inv_quant_ac = P.inv_global_scale / float(qac);
-      const int tile = (obya >> 3) * P.tx + (obxa >> 3);
-      const int qt = B.dqmap[B.dcf[tile] * 256 + qac - 1];
-      const int kind = AcsQuantKind(raw);
+      const float inv_quant_ac = __uint_as_float(ar_desc.w);
       for (int j = 0; j < 4; ++j) {
         const int flat = cflat0 + j;
         const int fi = flat / N, fj = flat % N;
@@ -529,13 +572,11 @@ __global__ void __launch_bounds__(256, 3)
         for (int c = 0; c < 3; ++c) {
           float interval;
           if (fi < s && fj < s) {
-            const float dc_mul =
-                P.inv_quant_dc * B.table[B.offsets[c]];
+            // P.mul_dc[c] = table[offsets[c]] * inv_quant_dc
             const float lscale = s == 1 ? 1.0f : K->ascale[s == 2 ? 3 : 4][fi][fj];
-            interval = lscale * dc_mul;
+            interval = lscale * P.mul_dc[c];
           } else {
-            const float m = B.table[B.offsets[(qt * kNumQuantKinds + kind) * 3 + c] + flat];
-            interval = m * inv_quant_ac;
+            interval = ar_m[c][j] * inv_quant_ac;
           }
           const float upper = 0.5f * interval;
           const float lower = -0.5f * interval;

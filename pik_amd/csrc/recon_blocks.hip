@@ -77,21 +77,20 @@ __global__ void __launch_bounds__(256)
   if (cvalid) {
     const size_t plane = size_t(P.by) * P.bx * 64;
     const size_t at = size_t(cby) * P.bx * 64 + size_t(cbx) * 64 + (ccol & 63);
-    // The block's dequantisation parameters (MakeBlockDq, shared by the
-    // three channels) and every load are issued before the arithmetic.
+    // The block's dequantisation descriptor (matrix slots and
+    // inv_global_scale / qf, MakeBlockDq's values) and every load are
+    // issued before the arithmetic.
     const int blk = cby * P.bx + cbx;
-    const int qfi = B.qf[blk];
+    const uint4 dsc = B.desc[2 * blk];
     const int ctile = (cby >> 3) * P.tx + (cbx >> 3);
-    const int qt = B.dqmap[B.dcf[ctile] * 256 + qfi - 1];
-    const int kind = AcsQuantKind(craw);
-    const float scaled = qfi == 0 ? 1E10f : P.inv_global_scale / float(qfi);
+    const float scaled = __uint_as_float(dsc.w);
+    const uint32_t slot[3] = {dsc.x, dsc.y, dsc.z};
     const float ccf[3] = {float(B.ytox[ctile] - 128) * 0.00390625f, 0.0f,
                           float(B.ytob[ctile]) * 0.0078125f};
     float mv[3][4];
     int16_t qv[3][4];
     for (int c = 0; c < 3; ++c) {
-      const float* m = B.table + B.offsets[(qt * kNumQuantKinds + kind) * 3 + c] +
-                       cidx * 64 + (ccol & 63);
+      const float* m = B.table + slot[c] + (ccol & 63);
       // four int16 coefficients, 8-byte aligned (at is a multiple of 4)
       const uint2 q4 = *reinterpret_cast<const uint2*>(B.ac + c * plane + at);
       qv[c][0] = int16_t(q4.x & 0xFFFF);

@@ -16,11 +16,36 @@
 
 namespace pikgpu {
 
+// Also writes each block's dequantisation descriptor (BlockDesc), so the
+// block kernels reach the dequant matrices with one dependent load instead
+// of the chain strategy -> quant field -> control field -> table index ->
+// table offset.
 __global__ void __launch_bounds__(256)
     KDcDequant(FrameParams P, FrameBuffers B) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = P.bx * P.by;
   if (i >= n) return;
+  {
+    const int bxa = i % P.bx, bya = i / P.bx;
+    const uint8_t a = B.acs[i];
+    const int raw = a >> 4, idx = a & 15;
+    const int s = AcsCovered(raw);
+    const int kind = AcsQuantKind(raw);
+    const int obxa = bxa - idx % s, obya = bya - idx / s;
+    const int oblk = obya * P.bx + obxa;
+    const int qf = B.qf[i], qfo = B.qf[oblk];
+    const int qt = B.dqmap[B.dcf[(bya >> 3) * P.tx + (bxa >> 3)] * 256 + qf - 1];
+    const int qto =
+        B.dqmap[B.dcf[(obya >> 3) * P.tx + (obxa >> 3)] * 256 + qfo - 1];
+    const uint32_t* off = B.offsets + (qt * kNumQuantKinds + kind) * 3;
+    const uint32_t* offo = B.offsets + (qto * kNumQuantKinds + kind) * 3;
+    B.desc[2 * i] = make_uint4(
+        off[0] + idx * 64, off[1] + idx * 64, off[2] + idx * 64,
+        __float_as_uint(qf == 0 ? 1E10f : P.inv_global_scale / float(qf)));
+    B.desc[2 * i + 1] = make_uint4(
+        offo[0] + idx * 64, offo[1] + idx * 64, offo[2] + idx * 64,
+        __float_as_uint(P.inv_global_scale / float(qfo)));
+  }
   // Y first, then X/B = ytox|ytob * Y + q * mul (the portable target's
   // mul_add is a separate multiply and add).
   const float y = float(B.dc16[n + i]) * P.mul_dc[1];
