@@ -30,7 +30,7 @@ enum Timer {
   kTAdjForward = kNumRunTimers, kTAdjBackward, kNumTimers
 };
 const char* kTimerNames[kNumTimers] = {"dc",     "lf_predict", "blocks_idct",
-                                       "minmax", "epf_ar",     "dots",
+                                       "guide",  "epf_ar",     "dots",
                                        "final_opsin", "adj_forward",
                                        "adj_backward"};
 
@@ -51,7 +51,7 @@ struct pikgpu_ctx {
   DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
       dotbins;
   DevBuf desc, dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
-      dots_img, gab, out, mm_partial, mm, ar_keep, srgb8;
+      dots_img, gab, out, mm_partial, mm, ar_keep, srgb8, gimg;
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
   bool adj_ready = false;  // LLF / LF caches hold the uploaded frame's
@@ -111,7 +111,7 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
                    &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
-                   &ctx->ar_keep, &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
+                   &ctx->ar_keep, &ctx->gimg,   &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
                    &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss};
   for (DevBuf* b : all) {
     if (b->p) (void)hipFree(b->p);
@@ -514,6 +514,11 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   GROW(pred2x2, ngroups * 3 * kG2Plane * 4);
   GROW(idct, 3 * ipitch * by * 8 * 4);
   GROW(ar, 3 * ipitch * by * 8 * 4);
+  // guide image: 6-pixel border, rows and pitch cover the 44-row, 44-byte
+  // windows of every (possibly partial) 32x32 tile
+  const size_t grows = DivCeil(by * 8, 32) * 32 + 16;
+  const size_t gpitch = DivCeil(bx * 8, 32) * 32 + 64;
+  GROW(gimg, 3 * grows * gpitch);
   GROW(out, 3 * opitch * f->ysize * 4);
   GROW(mm_partial, 1024 * 6 * 4);
   GROW(mm, 8 * 4);
@@ -610,6 +615,9 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   B.out = static_cast<float*>(ctx->out.p);
   B.mm_partial = static_cast<float*>(ctx->mm_partial.p);
   B.mm = static_cast<float*>(ctx->mm.p);
+  B.gimg = static_cast<uint8_t*>(ctx->gimg.p);
+  B.gpitch = int(gpitch);
+  B.grows = int(grows);
   B.ipitch = int(ipitch);
   B.opitch = int(opitch);
   ctx->num_tables = num_tables;
@@ -661,8 +669,8 @@ static void EnqueueDc(const FrameParams& P, const FrameBuffers& B,
   KMinMaxPartial<<<mmg, 256, 0, st>>>(dci, 0, P, B.mm_partial);
   KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
   const int grid = XcdGrid(int(DivCeil(dci.W, 32) * DivCeil(dci.H, 32)));
-  KEpf<<<grid, 256, 0, st>>>(dci, 0, 1, P, B, K, B.mm, B.dc, P.bx,
-                             size_t(nblk));
+  KEpf<true><<<grid, 256, 0, st>>>(dci, P, B, K, B.mm, B.dc, P.bx,
+                                   size_t(nblk));
 }
 
 // LF-domain predictions per group: LLF, LF predictions (predict_lf) and,
@@ -736,10 +744,13 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     const dim3 mmg = MinMaxGrid(ii.W, ii.H);
     KMinMaxPartial<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
     KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
-    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
     const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
-    KEpf<<<grid, 256, 0, st>>>(ii, gab, 0, P, B, K, B.mm, B.ar, B.ipitch,
-                               iplane);
+    KGuide<<<grid, 256, 0, st>>>(ii, gab, P, B, B.mm);
+    const int ring = 3 * (2 * 6 * (P.px + 12) + 2 * 6 * P.py);
+    KGuideRing<<<DivCeil(ring, 256), 256, 0, st>>>(P, B);
+    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+    KEpf<false><<<grid, 256, 0, st>>>(ii, P, B, K, B.mm, B.ar, B.ipitch,
+                                      iplane);
   } else {
     if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
     KCopyPlanes<<<DivCeil(3 * iplane, 256), 256, 0, st>>>(B.idct, B.ar,

@@ -63,6 +63,10 @@ struct FrameBuffers {
   float* guide;           // stage dump only, same layout as idct
   float* epf;             // stage dump only
   float* ar;              // 3 planes py x ipitch
+  // u8 EPF guide, 3 planes of grows rows x gpitch bytes; pixel (x, y) at
+  // (x + 6, y + 6), the 6-pixel border mirrored (KGuide, KGuideRing)
+  uint8_t* gimg;
+  int gpitch, grows;
   float* dots_img;        // stage dump only
   float* gab;             // stage dump only
   float* out;             // 3 planes ysize x opitch

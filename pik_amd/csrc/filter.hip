@@ -241,58 +241,26 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// EPF (+ AR clamp) on one 32x32 tile. dc_mode: the image is the 8-padded DC
-// image (constant quant field QuantDC/2, lut 0, DCT8 everywhere), no AR, and
-// the result is clamped to +-half a DC step and written to B.dc.
-//
-// LDS: the input window (46 x 46 per channel) is dead once the filter has
-// run -- each thread keeps the 12 original values it needs for AR in
-// registers -- so AR's coefficients and transform scratch reuse it; with
-// ~45 KB per workgroup three workgroups share a CU.
-__global__ void __launch_bounds__(256, 3)
-    KEpf(Img in, int gaborish, int dc_mode, FrameParams P, FrameBuffers B,
-         const DeviceConsts* K, const float* mm, float* out, int out_pitch,
-         size_t out_plane) {
-  constexpr int kInP = 48;  // pitch of s_in rows (46 used)
-  constexpr int kGP = 48;   // pitch of s_g rows (44 used)
-  constexpr int kInPlane = 46 * kInP;
-  static_assert(3 * kInPlane >= 3 * (kCoefPlane + kTilePlane),
-                "AR buffers must fit");
-  __shared__ __attribute__((aligned(16))) float s_big[3 * kInPlane];
-  __shared__ __attribute__((aligned(16))) uint8_t s_g[3][44 * kGP];
-  __shared__ float s_out[3][kTilePlane];
-  __shared__ uint8_t s_acs[16];
-  __shared__ int s_sigma[16];
-  __shared__ TileTasks s_tasks;
-  float (*s_in)[kInPlane] = reinterpret_cast<float (*)[kInPlane]>(s_big);
-
+// u8 EPF guide of the IDCT image (the guide AdaptiveFilter builds,
+// pik/epf_target.cc:1129-1148): the Gaborish-smoothed image (the image
+// itself for the sharpened guide), minus the minimum over channels, times
+// the guide stretch, truncated to bytes. One 32x32 tile per workgroup,
+// written at (x + 6, y + 6) of the guide image; KGuideRing fills its
+// 6-pixel border.
+__global__ void __launch_bounds__(256)
+    KGuide(Img in, int gaborish, FrameParams P, FrameBuffers B,
+           const float* mm) {
+  constexpr int kWP = 35;  // pitch of the 34 x 34 window rows
+  __shared__ float s_w[3][34 * kWP];
   const int tid = threadIdx.x;
   const int tiles_x = (in.W + 31) / 32;
   const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
   if (tile < 0) return;
   const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
-  const float all_min = mm[7];
-  const float gmul = mm[6];
-
-  // AR (step 5) needs the dequant matrix entries of the coefficients this
-  // thread owns: their loads are issued here, after the input window's,
-  // and complete while the guide and the filter run.
-  const int crow = tid >> 6, ccol = (tid & 63) << 2;
-  const int ar_blk =
-      (y0 / 8 + crow) * P.bx + x0 / 8 + (ccol >> 6);
-  const bool ar_valid =
-      !dc_mode && x0 / 8 + (ccol >> 6) < P.bx && y0 / 8 + crow < P.by;
-  uint4 ar_desc = make_uint4(0, 0, 0, 0);
-  if (ar_valid) ar_desc = B.desc[2 * ar_blk + 1];
-
-  // 1. input tile with a 7-pixel mirrored border. One reflection gives the
-  //    mirror for every position within 7 of the image (both images are at
-  //    least 8 wide and high); positions further out (a tile wider than a
-  //    small image reaches up to 38 past its origin) are only clamped into
-  //    the image, their values are never used. All loads are issued before
-  //    the LDS stores.
+  // The tile with a 1-pixel mirrored ring; positions past the ring (a tile
+  // wider than the image) are clamped into the image and never used.
   {
-    constexpr int kWin = 3 * 46 * 46;
+    constexpr int kWin = 3 * 34 * 34;
     constexpr int kPer = (kWin + 255) / 256;
     float wv[kPer];
 #pragma unroll
@@ -300,13 +268,13 @@ __global__ void __launch_bounds__(256, 3)
       const int i = tid + u * 256;
       float v = 0.0f;
       if (i < kWin) {
-        const int c = i / (46 * 46), rem = i - c * (46 * 46);
-        const int ly = rem / 46, lx = rem - ly * 46;
-        int x = x0 - 7 + lx, y = y0 - 7 + ly;
+        const int c = i / (34 * 34), rem = i - c * (34 * 34);
+        const int ly = rem / 34, lx = rem - ly * 34;
+        int x = x0 - 1 + lx, y = y0 - 1 + ly;
         x = x < 0 ? -x - 1 : (x >= in.W ? 2 * in.W - 1 - x : x);
         y = y < 0 ? -y - 1 : (y >= in.H ? 2 * in.H - 1 - y : y);
-        x = min(max(x, 0), in.vw - 1);
-        y = min(max(y, 0), in.vh - 1);
+        x = min(max(x, 0), in.W - 1);
+        y = min(max(y, 0), in.H - 1);
         v = in.p[c * in.plane + size_t(y) * in.pitch + x];
       }
       wv[u] = v;
@@ -315,57 +283,27 @@ __global__ void __launch_bounds__(256, 3)
     for (int u = 0; u < kPer; ++u) {
       const int i = tid + u * 256;
       if (i < kWin) {
-        const int c = i / (46 * 46), rem = i - c * (46 * 46);
-        const int ly = rem / 46, lx = rem - ly * 46;
-        s_in[c][ly * kInP + lx] = wv[u];
+        const int c = i / (34 * 34), rem = i - c * (34 * 34);
+        const int ly = rem / 34, lx = rem - ly * 34;
+        s_w[c][ly * kWP + lx] = wv[u];
       }
     }
-  }
-  float ar_m[3][4];
-  if (ar_valid) {
-    const uint32_t slot[3] = {ar_desc.x, ar_desc.y, ar_desc.z};
-    for (int c = 0; c < 3; ++c) {
-      for (int j = 0; j < 4; ++j) ar_m[c][j] = B.table[slot[c] + (ccol & 63) + j];
-    }
-  }
-  // 2. per-block sigma and strategy
-  if (tid < 16) {
-    const int bxl = tid & 3, byl = tid >> 2;
-    const int bxa = x0 / 8 + bxl, bya = y0 / 8 + byl;
-    int sigma = 0;
-    uint8_t a = 0xFF;
-    if (bxa * 8 < in.W && bya * 8 < in.H) {
-      float ac_q, scale;
-      int lut_id;
-      if (dc_mode) {
-        ac_q = float(P.dc_qf);
-        scale = 1.0f;
-        lut_id = 0;
-        a = 0;
-      } else {
-        const int blk = bya * P.bx + bxa;
-        a = B.acs[blk];
-        ac_q = float(B.qf[blk]);
-        scale = ARQuantScale(a >> 4);
-        lut_id = B.lut[blk];
-      }
-      const float quant = ac_q * scale * 0.039324273f;
-      sigma = SigmaFromQuant(quant, gmul, lut_id);
-    }
-    s_sigma[tid] = sigma;
-    s_acs[tid] = a;
   }
   __syncthreads();
-  // 3. u8 guide over the tile plus a 6-pixel border (MakeGuide)
-  for (int i = tid; i < 3 * 44 * 44; i += 256) {
-    const int c = i / (44 * 44), rem = i % (44 * 44);
-    const int ly = rem / 44, lx = rem % 44;
+  const float all_min = mm[7];
+  const float gmul = mm[6];
+  const size_t gplane = size_t(B.grows) * B.gpitch;
+  for (int k = tid; k < 3 * 1024; k += 256) {
+    const int c = k >> 10, i = (k >> 5) & 31, j = k & 31;
+    const int x = x0 + j, y = y0 + i;
+    if (x >= in.W || y >= in.H) continue;
+    const float* si = s_w[c] + (i + 1) * kWP + (j + 1);
     float g;
-    const float* si = s_in[c] + (ly + 1) * kInP + (lx + 1);
     if (gaborish) {
-      const float tc = si[-kInP], bc = si[kInP], ml = si[-1], mr = si[1];
-      const float tl = si[-kInP - 1], tr = si[-kInP + 1];
-      const float bl = si[kInP - 1], br = si[kInP + 1];
+      // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
+      const float tc = si[-kWP], bc = si[kWP], ml = si[-1], mr = si[1];
+      const float tl = si[-kWP - 1], tr = si[-kWP + 1];
+      const float bl = si[kWP - 1], br = si[kWP + 1];
       const float sum_tb = tc + bc;
       const float mul0 = si[0] * P.gab_w[0];
       const float sum_lr = ml + mr;
@@ -378,16 +316,207 @@ __global__ void __launch_bounds__(256, 3)
     } else {
       g = si[0];
     }
+    if (B.keep) {
+      B.guide[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] = g;
+    }
     const float scaled = (g - all_min) * gmul;
-    s_g[c][ly * kGP + lx] = static_cast<uint8_t>(static_cast<int>(scaled));
-    if (B.keep && !dc_mode && ly >= 6 && lx >= 6 && ly < 38 && lx < 38) {
-      const int x = x0 + lx - 6, y = y0 + ly - 6;
-      if (x < in.W && y < in.H) {
-        B.guide[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] = g;
+    B.gimg[c * gplane + size_t(y + 6) * B.gpitch + x + 6] =
+        static_cast<uint8_t>(static_cast<int>(scaled));
+  }
+}
+
+// The 6-pixel border of the guide image: each position takes the byte of
+// the pixel it mirrors. (The guide of a mirrored position equals its
+// source pixel's bit for bit: the 3x3 sums of the mirrored window only
+// exchange the operands of commutative additions.)
+__global__ void __launch_bounds__(256) KGuideRing(FrameParams P, FrameBuffers B) {
+  const int W = P.px, H = P.py;
+  const int pw = W + 12;
+  const int n_tb = 6 * pw, n_lr = 6 * H;
+  const int per_c = 2 * n_tb + 2 * n_lr;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= 3 * per_c) return;
+  const int c = k / per_c;
+  int r = k - c * per_c;
+  int X, Y;
+  if (r < n_tb) {
+    Y = r / pw;
+    X = r - Y * pw;
+  } else if ((r -= n_tb) < n_tb) {
+    Y = H + 6 + r / pw;
+    X = r - (r / pw) * pw;
+  } else if ((r -= n_tb) < n_lr) {
+    Y = 6 + r / 6;
+    X = r - (r / 6) * 6;
+  } else {
+    r -= n_lr;
+    Y = 6 + r / 6;
+    X = W + 6 + (r - (r / 6) * 6);
+  }
+  const int sx = Mirror(X - 6, W), sy = Mirror(Y - 6, H);
+  uint8_t* g = B.gimg + size_t(c) * B.grows * B.gpitch;
+  g[size_t(Y) * B.gpitch + X] = g[size_t(sy + 6) * B.gpitch + sx + 6];
+}
+
+// ---------------------------------------------------------------------------
+// EPF (+ AR clamp) on one 32x32 tile. dc_mode: the image is the 8-padded DC
+// image (constant quant field QuantDC/2, lut 0, DCT8 everywhere), no AR, and
+// the result is clamped to +-half a DC step and written to B.dc.
+//
+// LDS: the input window (46 x 46 per channel) is dead once the filter has
+// run -- each thread keeps the 12 original values it needs for AR in
+// registers -- so AR's coefficients and transform scratch reuse it; with
+// ~45 KB per workgroup three workgroups share a CU.
+//
+// Image mode (!kDc) reads the u8 guide from the guide image KGuide wrote;
+// the DC mode quantises its input itself (no Gaborish on the DC image).
+template <bool kDc>
+__global__ void __launch_bounds__(256, kDc ? 3 : 4)
+    KEpf(Img in, FrameParams P, FrameBuffers B, const DeviceConsts* K,
+         const float* mm, float* out, int out_pitch, size_t out_plane) {
+  constexpr int dc_mode = kDc;
+  // Input window: the DC mode keeps the tile with a 7-pixel border (its
+  // guide is computed from it); the image mode only the filter's window,
+  // pixels -3 .. +35 in x and -3 .. +34 in y. Window row / column 0 is
+  // pixel -kOrg of the tile.
+  constexpr int kOrg = kDc ? 7 : 3;
+  constexpr int kRows = kDc ? 46 : 38;
+  constexpr int kCols = kDc ? 46 : 39;
+  constexpr int kInP = kDc ? 48 : 40;  // pitch of s_in rows
+  constexpr int kGP = 48;              // pitch of s_g rows (44 used)
+  constexpr int kInPlane = kRows * kInP;
+  constexpr int kGuideFloats = 3 * 44 * kGP / 4;
+  constexpr int kBig = 3 * kInPlane + kGuideFloats > 3 * (kCoefPlane + kTilePlane)
+                           ? 3 * kInPlane + kGuideFloats
+                           : 3 * (kCoefPlane + kTilePlane);
+  // s_in and s_g, then (AR) s_coef and s_tmp; ~39 KB in the image mode:
+  // four workgroups per CU.
+  __shared__ __attribute__((aligned(16))) float s_big[kBig];
+  __shared__ float s_out[3][kTilePlane];
+  __shared__ uint8_t s_acs[16];
+  __shared__ int s_sigma[16];
+  __shared__ TileTasks s_tasks;
+  float (*s_in)[kInPlane] = reinterpret_cast<float (*)[kInPlane]>(s_big);
+  uint8_t (*s_g)[44 * kGP] =
+      reinterpret_cast<uint8_t (*)[44 * kGP]>(s_big + 3 * kInPlane);
+
+  const int tid = threadIdx.x;
+  const int tiles_x = (in.W + 31) / 32;
+  const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
+  if (tile < 0) return;
+  const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
+  const float all_min = mm[7];
+  const float gmul = mm[6];
+
+  // AR (step 5) needs the descriptor of the storage block whose
+  // coefficients this thread owns (the matrix slots and 1 / qf of the
+  // transform's first block); it loads with the input window.
+  const int crow = tid >> 6, ccol = (tid & 63) << 2;
+  const int ar_blk =
+      (y0 / 8 + crow) * P.bx + x0 / 8 + (ccol >> 6);
+  const bool ar_valid =
+      !dc_mode && x0 / 8 + (ccol >> 6) < P.bx && y0 / 8 + crow < P.by;
+  uint4 ar_desc = make_uint4(0, 0, 0, 0);
+  if (ar_valid) ar_desc = B.desc[2 * ar_blk + 1];
+  // inputs of the per-block sigma (step 2), loaded with the window
+  uint8_t sg_acs = 0;
+  int sg_qf = 0, sg_lut = 0;
+  const bool sg_valid = tid < 16 && (x0 / 8 + (tid & 3)) * 8 < in.W &&
+                        (y0 / 8 + (tid >> 2)) * 8 < in.H;
+  if (!kDc && sg_valid) {
+    const int blk = (y0 / 8 + (tid >> 2)) * P.bx + x0 / 8 + (tid & 3);
+    sg_acs = B.acs[blk];
+    sg_qf = B.qf[blk];
+    sg_lut = B.lut[blk];
+  }
+
+  // 1. input tile with a 7-pixel mirrored border (the DC mode computes its
+  //    guide from it); the image mode only needs the filter's window,
+  //    pixels -3 .. +35 of the tile in x and -3 .. +34 in y. One reflection
+  //    gives the mirror for every position within 7 of the image (both
+  //    images are at least 8 wide and high); positions further out (a tile
+  //    wider than a small image reaches up to 38 past its origin) are only
+  //    clamped into the image, their values are never used. All loads are
+  //    issued before the LDS stores.
+  {
+    constexpr int kWin = 3 * kRows * kCols;
+    constexpr int kPer = (kWin + 255) / 256;
+    float wv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      float v = 0.0f;
+      if (i < kWin) {
+        const int c = i / (kRows * kCols), rem = i - c * (kRows * kCols);
+        const int ly = rem / kCols, lx = rem - ly * kCols;
+        int x = x0 - kOrg + lx, y = y0 - kOrg + ly;
+        x = x < 0 ? -x - 1 : (x >= in.W ? 2 * in.W - 1 - x : x);
+        y = y < 0 ? -y - 1 : (y >= in.H ? 2 * in.H - 1 - y : y);
+        x = min(max(x, 0), in.vw - 1);
+        y = min(max(y, 0), in.vh - 1);
+        v = in.p[c * in.plane + size_t(y) * in.pitch + x];
+      }
+      wv[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      if (i < kWin) {
+        const int c = i / (kRows * kCols), rem = i - c * (kRows * kCols);
+        const int ly = rem / kCols, lx = rem - ly * kCols;
+        s_in[c][ly * kInP + lx] = wv[u];
       }
     }
   }
+  if (!kDc) {
+    // the guide window: 44 rows of 11 aligned dwords per channel (the guide
+    // image's 6-pixel border makes window column 0 the padded column x0)
+    const size_t gplane = size_t(B.grows) * B.gpitch;
+    for (int k = tid; k < 3 * 44 * 11; k += 256) {
+      const int c = k / (44 * 11), rem = k - c * (44 * 11);
+      const int row = rem / 11, d = rem - row * 11;
+      const uint32_t v = *reinterpret_cast<const uint32_t*>(
+          B.gimg + c * gplane + size_t(y0 + row) * B.gpitch + x0 + 4 * d);
+      *reinterpret_cast<uint32_t*>(&s_g[c][row * kGP + 4 * d]) = v;
+    }
+  }
+  // 2. per-block sigma and strategy
+  if (tid < 16) {
+    int sigma = 0;
+    uint8_t a = 0xFF;
+    if (sg_valid) {
+      float ac_q, scale;
+      int lut_id;
+      if (dc_mode) {
+        ac_q = float(P.dc_qf);
+        scale = 1.0f;
+        lut_id = 0;
+        a = 0;
+      } else {
+        a = sg_acs;
+        ac_q = float(sg_qf);
+        scale = ARQuantScale(a >> 4);
+        lut_id = sg_lut;
+      }
+      const float quant = ac_q * scale * 0.039324273f;
+      sigma = SigmaFromQuant(quant, gmul, lut_id);
+    }
+    s_sigma[tid] = sigma;
+    s_acs[tid] = a;
+  }
   __syncthreads();
+  // 3. DC mode: u8 guide over the tile plus a 6-pixel border (MakeGuide,
+  //    no Gaborish on the DC image).
+  if (kDc) {
+    for (int i = tid; i < 3 * 44 * 44; i += 256) {
+      const int c = i / (44 * 44), rem = i % (44 * 44);
+      const int ly = rem / 44, lx = rem % 44;
+      const float g = s_in[c][(ly + 1) * kInP + (lx + 1)];
+      const float scaled = (g - all_min) * gmul;
+      s_g[c][ly * kGP + lx] = static_cast<uint8_t>(static_cast<int>(scaled));
+    }
+    __syncthreads();
+  }
 
   // 4. EPF (AdaptiveFilter, pik/epf_target.cc:1113-1277): each thread
   //    filters 4 horizontally adjacent pixels of one block row. For every
@@ -405,7 +534,8 @@ __global__ void __launch_bounds__(256, 3)
     if (inside && sigma < kMinSigma) {
       for (int c = 0; c < 3; ++c) {
         for (int j = 0; j < 4; ++j) {
-          s_out[c][PixAt(r, c0 + j)] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+          s_out[c][PixAt(r, c0 + j)] =
+              s_in[c][(r + kOrg) * kInP + c0 + kOrg + j];
         }
       }
     } else if (inside) {
@@ -471,7 +601,7 @@ __global__ void __launch_bounds__(256, 3)
         float v[3][12];
         for (int c = 0; c < 3; ++c) {
           const float4* src = reinterpret_cast<const float4*>(
-              s_in[c] + (r + 7 + cy) * kInP + c0 + 4);
+              s_in[c] + (r + kOrg + cy) * kInP + c0 + kOrg - 3);
           for (int q = 0; q < 3; ++q) {
             const float4 t = src[q];
             v[c][4 * q] = t.x;
@@ -511,7 +641,7 @@ __global__ void __launch_bounds__(256, 3)
       const int x = x0 + c0 + j;
       if (x < P.bx && y < P.by) {
         for (int c = 0; c < 3; ++c) {
-          const float dc = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+          const float dc = s_in[c][(r + kOrg) * kInP + c0 + kOrg + j];
           const float hs = P.dc_half_step[c];
           const float lo = dc - hs, hi = dc + hs;
           const float d = s_out[c][PixAt(r, c0 + j)];
@@ -536,11 +666,20 @@ __global__ void __launch_bounds__(256, 3)
   }
 
   // 5. AR: clamp the DCT of (filtered - original) to half the quantization
-  //    interval with one ratio per coefficient shared by X, Y, B.
+  //    interval with one ratio per coefficient shared by X, Y, B. The
+  //    matrix entries of this thread's coefficients load during the forward
+  //    transform.
+  float ar_m[3][4];
+  if (ar_valid) {
+    const uint32_t slot[3] = {ar_desc.x, ar_desc.y, ar_desc.z};
+    for (int c = 0; c < 3; ++c) {
+      for (int j = 0; j < 4; ++j) ar_m[c][j] = B.table[slot[c] + (ccol & 63) + j];
+    }
+  }
   float orig[3][4];
   for (int c = 0; c < 3; ++c) {
     for (int j = 0; j < 4; ++j) {
-      orig[c][j] = s_in[c][(r + 7) * kInP + c0 + 7 + j];
+      orig[c][j] = s_in[c][(r + kOrg) * kInP + c0 + kOrg + j];
       s_out[c][PixAt(r, c0 + j)] = s_out[c][PixAt(r, c0 + j)] - orig[c][j];
     }
   }
@@ -840,5 +979,12 @@ __global__ void __launch_bounds__(256)
   const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
+
+template __global__ void KEpf<true>(Img, FrameParams, FrameBuffers,
+                                    const DeviceConsts*, const float*, float*,
+                                    int, size_t);
+template __global__ void KEpf<false>(Img, FrameParams, FrameBuffers,
+                                     const DeviceConsts*, const float*, float*,
+                                     int, size_t);
 
 }  // namespace pikgpu

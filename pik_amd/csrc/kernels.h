@@ -188,7 +188,10 @@ __device__ inline void FwdLine1(const float* pix, float* tmp, int oby,
   dct1d::Dct<N>(v, w);
   for (int k = 0; k < N; ++k) tmp[PixAt(oby * 8 + x, obx * 8 + k)] = w[k];
 }
-template <int N>
+// kAddHF: add the coefficients outside the 2s x 2s low-frequency corner
+// to coef instead of storing them (AddBlockExceptLFAndLLFTo,
+// pik/ac_predictions.cc:38-70).
+template <int N, bool kAddHF = false>
 __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
                                 int obx, int k) {
   constexpr int s = N / 8;
@@ -197,7 +200,12 @@ __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
   dct1d::Dct<N>(v, w);
   const float scale = 1.0f / (N * N);
   for (int j = 0; j < N; ++j) {
-    coef[CoefIdx(oby, obx, s, j * N + k)] = w[j] * scale;
+    float& dst = coef[CoefIdx(oby, obx, s, j * N + k)];
+    if (!kAddHF) {
+      dst = w[j] * scale;
+    } else if (!(j < 2 * s && k < 2 * s)) {
+      dst = dst + w[j] * scale;
+    }
   }
 }
 
@@ -256,18 +264,22 @@ __device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
 // of DCT4X4 (pik/ac_strategy.cc:590-613); DCT2X2 and IDENTITY in pass 0.
 // kAdjoint replaces IDENTITY's forward transform by the transpose of its
 // inverse (the other strategies' transposes are scaled forward transforms).
-template <bool kAdjoint = false>
+// kAddHF adds the result outside each transform's 2s x 2s low-frequency
+// corner to coef (the HF prediction; only strategies with PredictHF, so no
+// DCT2X2 or IDENTITY, and the DCT4X4 DC Hadamard, which only touches that
+// corner, is skipped).
+template <bool kAdjoint = false, bool kAddHF = false>
 __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
                                    const float* pix, float* coef, float* tmp) {
   if (raw == kDCT || raw == kDCTNoHF) {
     if (pass == 0) FwdLine1<8>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<8>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<8, kAddHF>(tmp, coef, oby, obx, l);
   } else if (raw == kDCT16) {
     if (pass == 0) FwdLine1<16>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<16>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<16, kAddHF>(tmp, coef, oby, obx, l);
   } else if (raw == kDCT32) {
     if (pass == 0) FwdLine1<32>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<32>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<32, kAddHF>(tmp, coef, oby, obx, l);
   } else if (AcsIsDCT4(raw)) {
     const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, q = l & 3;
     float v[4], w[4];
@@ -286,9 +298,14 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
       }
       dct1d::Dct<4>(v, w);
       for (int j = 0; j < 4; ++j) {
-        cb[(sy + j * 2) * 8 + sx + q * 2] = w[j] * (1.0f / 16);
+        float& dst = cb[(sy + j * 2) * 8 + sx + q * 2];
+        if (!kAddHF) {
+          dst = w[j] * (1.0f / 16);
+        } else if (!(j == 0 && q == 0)) {  // outside the 2x2 corner
+          dst = dst + w[j] * (1.0f / 16);
+        }
       }
-    } else if (l == 0) {
+    } else if (!kAddHF && l == 0) {
       const float b00 = cb[0], b01 = cb[1], b10 = cb[8], b11 = cb[9];
       cb[0] = (b00 + b01 + b10 + b11) * 0.25f;
       cb[1] = (b00 + b01 - b10 - b11) * 0.25f;
@@ -326,17 +343,18 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
 // pix (3 x kTilePlane) -> coef (3 x kCoefPlane) for the transforms in T
 // (others keep their coefficient entries), tmp 3 x kTilePlane. All 256
 // threads must call.
-template <bool kAdjoint = false>
+template <bool kAdjoint = false, bool kAddHF = false>
 __device__ inline void TileForward3(const TileTasks& T, const float* pix,
                                     float* coef, float* tmp) {
   const int total = T.off[16];
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 0; pass < (kAddHF ? 2 : 3); ++pass) {
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
       const int c = k / total, t = k - c * total;
       const int b = TaskBlock(T, t);
-      ForwardTask<kAdjoint>(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
-                            pix + c * kTilePlane, coef + c * kCoefPlane,
-                            tmp + c * kTilePlane);
+      ForwardTask<kAdjoint, kAddHF>(pass, T.raw[b], t - T.off[b], b >> 2,
+                                    b & 3, pix + c * kTilePlane,
+                                    coef + c * kCoefPlane,
+                                    tmp + c * kTilePlane);
     }
     __syncthreads();
   }
@@ -392,9 +410,13 @@ struct Img {
 __global__ void KMinMaxPartial(Img in, int gaborish, FrameParams P,
                                float* partial);
 __global__ void KMinMaxFinal(const float* partial, int n, float* mm);
-__global__ void KEpf(Img in, int gaborish, int dc_mode, FrameParams P,
-                     FrameBuffers B, const DeviceConsts* K, const float* mm,
-                     float* out, int out_pitch, size_t out_plane);
+template <bool kDc>
+__global__ void KEpf(Img in, FrameParams P, FrameBuffers B,
+                     const DeviceConsts* K, const float* mm, float* out,
+                     int out_pitch, size_t out_plane);
+__global__ void KGuide(Img in, int gaborish, FrameParams P, FrameBuffers B,
+                       const float* mm);
+__global__ void KGuideRing(FrameParams P, FrameBuffers B);
 __global__ void KDots(FrameParams P, FrameBuffers B);
 // KFinal: rows per workgroup of its 256-column strips.
 constexpr int kFinalRows = 16;

@@ -14,14 +14,14 @@
 
 namespace pikgpu {
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
     KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  // s_pix holds pixels (kTilePlane per channel) except while it receives
-  // the HF prediction's coefficients, hence kCoefPlane.
+  // ~39 KB: four workgroups per CU. The IDCT's pixels reuse the HF
+  // prediction's pixel buffer (dead once its transform is added).
   __shared__ float s_coef[3][kCoefPlane];
-  __shared__ float s_pix[3 * kCoefPlane];
   __shared__ float s_pred[3][kTilePlane];
   __shared__ float s_tmp[3][kTilePlane];
+  float* s_pix = &s_pred[0][0];
   __shared__ uint8_t s_acs[16];
   __shared__ TileTasks s_inv, s_fwd;
 
@@ -35,33 +35,73 @@ __global__ void __launch_bounds__(256)
   // transform scratch, which is first written after the last read of the
   // window (three workgroups per CU need the LDS under 160 KB / 3).
   float (*s_p2)[100] = reinterpret_cast<float (*)[100]>(&s_tmp[0][0]);
-  if (tid < 16) {
-    const int bxa = tbx + (tid & 3), bya = tby + (tid >> 2);
-    s_acs[tid] = (bxa < P.bx && bya < P.by) ? B.acs[bya * P.bx + bxa] : 0xFF;
-  }
-  __syncthreads();
-  BuildTasks(s_acs, false, &s_inv);
-
   const int gxi = tbx / kGroupDimBlocks, gyi = tby / kGroupDimBlocks;
   const int g = gyi * P.gx + gxi;
   const int gbx0 = gxi * kGroupDimBlocks, gby0 = gyi * kGroupDimBlocks;
   // pred2x2 rows / columns 2 * (tile's group-local block) + 1 .. + 10
   const int p2r0 = 2 * (tby - gby0) + 1, p2c0 = 2 * (tbx - gbx0) + 1;
-  if (P.predict_hf) {
-    for (int k = tid; k < 300; k += 256) {
-      const int c = k / 100, i = (k % 100) / 10, j = k % 10;
-      s_p2[c][i * 10 + j] = B.pred2x2[(size_t(g) * 3 + c) * kG2Plane +
-                                      (p2r0 + i) * kG2Pitch + p2c0 + j];
-    }
-    BuildTasks(s_acs, true, &s_fwd);  // (its barrier publishes s_p2)
-  }
-
   // Coefficient entries owned by this thread: storage row crow of the tile,
   // columns ccol..ccol+3 (one 8x8 storage block per 16 threads).
   const int crow = tid >> 6, ccol = (tid & 63) << 2;
   const int cbx = tbx + (ccol >> 6), cby = tby + crow;  // frame block
   const bool cvalid = cbx < P.bx && cby < P.by;
-  // F-matrix coordinates of those entries.
+  const size_t plane = size_t(P.by) * P.bx * 64;
+  const size_t at = size_t(cby) * P.bx * 64 + size_t(cbx) * 64 + (ccol & 63);
+  const int ctile = (cby >> 3) * P.tx + (cbx >> 3);
+
+  // Every global load that depends only on the tile's position is issued
+  // before the first wait: the strategy bytes, the pred2x2 window, the
+  // block's dequantisation descriptor (matrix slots and inv_global_scale /
+  // qf, MakeBlockDq's values), its quantized coefficients and CfL factors.
+  uint8_t a_own = 0xFF;
+  if (tid < 16) {
+    const int bxa = tbx + (tid & 3), bya = tby + (tid >> 2);
+    if (bxa < P.bx && bya < P.by) a_own = B.acs[bya * P.bx + bxa];
+  }
+  float p2v[2] = {0.0f, 0.0f};
+  if (P.predict_hf) {
+    for (int u = 0; u < 2; ++u) {
+      const int k = tid + 256 * u;
+      if (k < 300) {
+        const int c = k / 100, i = (k % 100) / 10, j = k % 10;
+        p2v[u] = B.pred2x2[(size_t(g) * 3 + c) * kG2Plane +
+                           (p2r0 + i) * kG2Pitch + p2c0 + j];
+      }
+    }
+  }
+  uint4 dsc = make_uint4(0, 0, 0, 0);
+  uint2 q4[3] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+  int32_t ytox = 128, ytob = 0;
+  if (cvalid) {
+    dsc = B.desc[2 * (cby * P.bx + cbx)];
+    // four int16 coefficients, 8-byte aligned (at is a multiple of 4)
+    for (int c = 0; c < 3; ++c) {
+      q4[c] = *reinterpret_cast<const uint2*>(B.ac + c * plane + at);
+    }
+    ytox = B.ytox[ctile];
+    ytob = B.ytob[ctile];
+  }
+  if (tid < 16) s_acs[tid] = a_own;
+  if (P.predict_hf) {
+    for (int u = 0; u < 2; ++u) {
+      const int k = tid + 256 * u;
+      if (k < 300) s_p2[k / 100][k % 100] = p2v[u];
+    }
+  }
+  __syncthreads();
+  // The dequant matrix entries need the descriptor: their loads go out
+  // before the task tables are built.
+  float mv[3][4];
+  if (cvalid) {
+    const uint32_t slot[3] = {dsc.x, dsc.y, dsc.z};
+    for (int c = 0; c < 3; ++c) {
+      for (int j = 0; j < 4; ++j) mv[c][j] = B.table[slot[c] + (ccol & 63) + j];
+    }
+  }
+  BuildTasks(s_acs, false, &s_inv);
+  if (P.predict_hf) BuildTasks(s_acs, true, &s_fwd);
+
+  // F-matrix coordinates of this thread's entries.
   const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
   const int craw = ca >> 4, cidx = ca & 15;
   const int cs = ca == 0xFF ? 1 : AcsCovered(craw);
@@ -75,29 +115,15 @@ __global__ void __launch_bounds__(256)
   //    (Dequant::DoAC, pik/compressed_image.cc:586-693), and 2. the LLF and
   //    LF predictions (UpdateLfForDecoder, pik/ac_predictions.cc:560-620).
   if (cvalid) {
-    const size_t plane = size_t(P.by) * P.bx * 64;
-    const size_t at = size_t(cby) * P.bx * 64 + size_t(cbx) * 64 + (ccol & 63);
-    // The block's dequantisation descriptor (matrix slots and
-    // inv_global_scale / qf, MakeBlockDq's values) and every load are
-    // issued before the arithmetic.
-    const int blk = cby * P.bx + cbx;
-    const uint4 dsc = B.desc[2 * blk];
-    const int ctile = (cby >> 3) * P.tx + (cbx >> 3);
     const float scaled = __uint_as_float(dsc.w);
-    const uint32_t slot[3] = {dsc.x, dsc.y, dsc.z};
-    const float ccf[3] = {float(B.ytox[ctile] - 128) * 0.00390625f, 0.0f,
-                          float(B.ytob[ctile]) * 0.0078125f};
-    float mv[3][4];
+    const float ccf[3] = {float(ytox - 128) * 0.00390625f, 0.0f,
+                          float(ytob) * 0.0078125f};
     int16_t qv[3][4];
     for (int c = 0; c < 3; ++c) {
-      const float* m = B.table + slot[c] + (ccol & 63);
-      // four int16 coefficients, 8-byte aligned (at is a multiple of 4)
-      const uint2 q4 = *reinterpret_cast<const uint2*>(B.ac + c * plane + at);
-      qv[c][0] = int16_t(q4.x & 0xFFFF);
-      qv[c][1] = int16_t(q4.x >> 16);
-      qv[c][2] = int16_t(q4.y & 0xFFFF);
-      qv[c][3] = int16_t(q4.y >> 16);
-      for (int j = 0; j < 4; ++j) mv[c][j] = m[j];
+      qv[c][0] = int16_t(q4[c].x & 0xFFFF);
+      qv[c][1] = int16_t(q4[c].x >> 16);
+      qv[c][2] = int16_t(q4[c].y & 0xFFFF);
+      qv[c][3] = int16_t(q4[c].y >> 16);
     }
     float yv[4];
     for (int ci = 0; ci < 3; ++ci) {
@@ -155,21 +181,10 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
-    TileForward3(s_fwd, &s_pred[0][0], s_pix, &s_tmp[0][0]);
-    // s_pix now holds the prediction's coefficients for PredictHF blocks
-    // (channel stride kCoefPlane).
-    if (cvalid && ca != 0xFF && AcsPredictHF(craw)) {
-      for (int j = 0; j < 4; ++j) {
-        const int flat = cflat0 + j;
-        const int fi = flat / cN, fj = flat % cN;
-        if (!(fi < 2 * cs && fj < 2 * cs)) {
-          for (int c = 0; c < 3; ++c) {
-            s_coef[c][CoefAt(crow, ccol + j)] +=
-                s_pix[c * kCoefPlane + CoefAt(crow, ccol + j)];
-          }
-        }
-      }
-    }
+    // forward transform of the prediction, added to the coefficients
+    // outside each transform's low-frequency corner
+    TileForward3<false, true>(s_fwd, &s_pred[0][0], &s_coef[0][0],
+                              &s_tmp[0][0]);
   }
   __syncthreads();
   if (B.keep && cvalid) {
