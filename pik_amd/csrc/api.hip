@@ -51,7 +51,7 @@ struct pikgpu_ctx {
   DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
       dotbins;
   DevBuf desc, dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
-      dots_img, gab, out, mm_partial, mm, ar_keep, srgb8, gimg;
+      dots_img, gab, out, mm_partial, mm, ar_keep, srgb8, gimg, predlist;
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
   bool adj_ready = false;  // LLF / LF caches hold the uploaded frame's
@@ -111,7 +111,7 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->lf2x2,   &ctx->pred2x2, &ctx->coeffs, &ctx->idct,
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
                    &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
-                   &ctx->ar_keep, &ctx->gimg,   &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
+                   &ctx->ar_keep, &ctx->gimg,   &ctx->predlist, &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
                    &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss};
   for (DevBuf* b : all) {
     if (b->p) (void)hipFree(b->p);
@@ -578,6 +578,31 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
                                 dot_list.data(), dot_list.size() * 4,
                                 hipMemcpyHostToDevice, ctx->stream));
   }
+  // First blocks grouped by the class of their pred2x2 computation
+  // (KPred2x2Interior): DCT8-like, 2x2 Hadamard, DCT16, DCT32.
+  std::vector<uint32_t> plist[4];
+  for (size_t y = 0; y < by; ++y) {
+    const uint8_t* row = f->ac_strategy + y * f->ac_strategy_stride;
+    for (size_t x = 0; x < bx; ++x) {
+      if ((row[x] & 15) != 0) continue;
+      const int raw = row[x] >> 4;
+      const int cls = raw == kDCT16 ? 2
+                      : raw == kDCT32 ? 3
+                      : (raw == kDCT || raw == kDCTNoHF || raw == kLines) ? 0
+                                                                           : 1;
+      plist[cls].push_back(uint32_t(y * bx + x));
+    }
+  }
+  std::vector<uint32_t> pred_list;
+  int pred_end[4];
+  for (int k = 0; k < 4; ++k) {
+    pred_list.insert(pred_list.end(), plist[k].begin(), plist[k].end());
+    pred_end[k] = int(pred_list.size());
+  }
+  if ((rc = Grow(ctx, &ctx->predlist, pred_list.size() * 4))) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->predlist.p, pred_list.data(),
+                              pred_list.size() * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
   FrameBuffers& B = ctx->B;
@@ -595,6 +620,8 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   B.offsets = static_cast<const uint32_t*>(ctx->offsets.p);
   B.num_tables = num_tables;
   B.dots = static_cast<const pikgpu_dot*>(ctx->dots.p);
+  B.pred_list = static_cast<const uint32_t*>(ctx->predlist.p);
+  for (int k = 0; k < 4; ++k) P.pred_end[k] = pred_end[k];
   P.dot_tiles = int(dot_ids.size());
   P.dot_tiles_x = int(DivCeil(bx * 8, 32));
   if (P.dot_tiles) {
@@ -677,7 +704,6 @@ static void EnqueueDc(const FrameParams& P, const FrameBuffers& B,
 // with `hf`, the pred2x2 image of the HF prediction.
 static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
                       const DeviceConsts* K, hipStream_t st, bool hf) {
-  const int nblk = P.bx * P.by;
   const size_t ngroups = size_t(P.gx) * P.gy;
   const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
   const int g2_pos = 4 * cache_pos;
@@ -696,7 +722,10 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
     } else {
       KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
     }
-    KPred2x2Interior<<<dim3(DivCeil(nblk, 256), 3), 256, 0, st>>>(P, B, K);
+    if (P.pred_end[3] > 0) {
+      KPred2x2Interior<<<dim3(DivCeil(P.pred_end[3], 256), 3), 256, 0, st>>>(
+          P, B, K);
+    }
   }
 }
 

@@ -84,6 +84,8 @@ struct FrameParams {
   int out_flags;             // kOutLinear | kOutSrgb8
   int dot_tiles;             // 32x32 tiles touched by dots
   int dot_tiles_x;           // tiles per row (ceil(px / 32))
+  int pred_end[4];           // ends of the first-block classes in pred_list:
+                             // DCT8-like, 2x2-Hadamard, DCT16, DCT32
   float inv_global_scale;    // Quantizer::InvGlobalScale()
   float inv_quant_dc;        // Quantizer::inv_quant_dc() (decoder form)
   float mul_dc[3];           // DC dequant multipliers

@@ -45,6 +45,9 @@ struct FrameBuffers {
   const int32_t* dot_tile_ids;
   const int32_t* dot_tile_start;
   const int32_t* dot_list;
+  // first blocks grouped by the class of their pred2x2 computation (block
+  // indices; classes end at FrameParams::pred_end)
+  const uint32_t* pred_list;
   // intermediates
   // Per block two descriptors (KDcDequant): {table offset of the block's
   // dequant matrix slot for channels 0..2 (offsets[(qt * 7 + kind) * 3 + c]

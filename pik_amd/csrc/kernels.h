@@ -70,6 +70,20 @@ __device__ inline BlockDq MakeBlockDq(const FrameParams& P,
   return d;
 }
 
+// MakeBlockDq's values from the block's descriptor (KDcDequant).
+__device__ inline BlockDq DescBlockDq(const FrameParams& P,
+                                      const FrameBuffers& B, int c, int row,
+                                      int bxa) {
+  const uint4 d = B.desc[2 * (row * P.bx + bxa)];
+  const int tile = (row >> 3) * P.tx + (bxa >> 3);
+  BlockDq q;
+  q.m = B.table + (c == 0 ? d.x : (c == 1 ? d.y : d.z));
+  q.scaled = __uint_as_float(d.w);
+  q.cc = c == 0 ? float(B.ytox[tile] - 128) * 0.00390625f
+                : (c == 2 ? float(B.ytob[tile]) * 0.0078125f : 0.0f);
+  return q;
+}
+
 // Coefficient k of a block from its quantized value q, in DequantCoef's
 // operation order.
 __device__ inline float DequantWith(const BlockDq& d, int c, int16_t q, int k,

@@ -351,8 +351,8 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
   BlockDq dq[kStore], dqy[kStore];
 #pragma unroll
   for (int k = 0; k < kStore; ++k) {
-    dq[k] = MakeBlockDq(P, B, c, bya, bxa + k);
-    if (c != 1) dqy[k] = MakeBlockDq(P, B, 1, bya, bxa + k);
+    dq[k] = DescBlockDq(P, B, c, bya, bxa + k);
+    if (c != 1) dqy[k] = DescBlockDq(P, B, 1, bya, bxa + k);
   }
   const size_t plane = size_t(P.by) * P.bx * 64;
   const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
@@ -392,29 +392,30 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
   }
 }
 
+// One thread per first block and plane; the first blocks come grouped by
+// class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; pred_list, binned on the
+// host at upload) so the lanes of a wave take the same path.
 __global__ void __launch_bounds__(256)
     KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
-  if (blk >= P.bx * P.by) return;
+  if (k >= P.pred_end[3]) return;
+  const int blk = int(B.pred_list[k]);
   const int bxa = blk % P.bx, bya = blk / P.bx;
-  const uint8_t a = B.acs[blk];
-  if ((a & 15) != 0) return;
-  const int raw = a >> 4;
   const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
   const int g = gyi * P.gx + gxi;
   const int bxg = bxa - gxi * kGroupDimBlocks, byg = bya - gyi * kGroupDimBlocks;
-  if (raw == kDCT16) {
-    Pred2x2Block<2>(P, B, K, c, bxa, bya, bxg, byg, g);
-  } else if (raw == kDCT32) {
+  if (k >= P.pred_end[2]) {
     Pred2x2Block<4>(P, B, K, c, bxa, bya, bxg, byg, g);
-  } else if (raw == kDCT || raw == kDCTNoHF || raw == kLines) {
+  } else if (k >= P.pred_end[1]) {
+    Pred2x2Block<2>(P, B, K, c, bxa, bya, bxg, byg, g);
+  } else if (k < P.pred_end[0]) {
     Pred2x2Block<1>(P, B, K, c, bxa, bya, bxg, byg, g);
   } else {
     // DCT2X2, DCT4X4(_NOHF), IDENTITY: 2x2 Hadamard of the LF corner.
-    const BlockDq dq = MakeBlockDq(P, B, c, bya, bxa);
+    const BlockDq dq = DescBlockDq(P, B, c, bya, bxa);
     BlockDq dqy;
-    if (c != 1) dqy = MakeBlockDq(P, B, 1, bya, bxa);
+    if (c != 1) dqy = DescBlockDq(P, B, 1, bya, bxa);
     const size_t plane = size_t(P.by) * P.bx * 64;
     const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
     const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
