@@ -72,12 +72,14 @@ __device__ inline float ARQuantScale(int raw) {
 }
 
 // WeightFast (pik/epf_target.cc:304-376): exponent-field approximation of
-// exp(-s * sad) from the per-sigma integer multiplier.
-__device__ inline float WeightOf(int sad, int mul) {
-  const int clamped = min(sad, kClampedSAD);
-  const int unbiased = __mul24(clamped << 6, mul) >> 16;
-  const int biased = unbiased + (127 << 7);
-  return __int_as_float(biased << 16);
+// exp(-s * sad) from the per-sigma integer multiplier, for a SAD already
+// clamped to kClampedSAD and mul64 = multiplier * 64 (the reference's
+// sad << 6 folded into the multiplier; |mul64| < 2^23, |product| < 2^31).
+// ((p >> 16) + (127 << 7)) << 16 is evaluated as (p & 0xFFFF0000) +
+// (127 << 23), the same bits.
+__device__ inline float WeightOfClamped(int sad, int mul64) {
+  const int p = __mul24(sad, mul64);
+  return __uint_as_float((uint32_t(p) & 0xFFFF0000u) + (127u << 23));
 }
 
 typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
@@ -539,7 +541,9 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
         }
       }
     } else if (inside) {
-      const int mul = K->mul_table[sigma];
+      // mul * 64 as a 24-bit signed value (the table holds -32512 .. -774),
+      // so the weight multiply is a 24-bit one
+      const int mul64 = (K->mul_table[sigma] * 64 << 8) >> 8;
       // Reference patch rows of pixel j: guide bytes c0+5+j .. c0+8+j of
       // rows r+5 .. r+7 (the guide tile has a 6-pixel border).
       uint32_t ref[3][3][4];
@@ -573,12 +577,16 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
               // windows of neighbours cx = -3..0 start at byte c0+2+j,
               // those of cx = 1..4 at byte c0+6+j
               const int o = 2 + j;
-              const uint32_t a0 = __builtin_amdgcn_alignbyte(
-                  d[(o >> 2) + 1], d[o >> 2], o & 3);
-              const uint32_t a1 = __builtin_amdgcn_alignbyte(
-                  d[(o >> 2) + 2], d[(o >> 2) + 1], o & 3);
-              const uint32_t a2 = __builtin_amdgcn_alignbyte(
-                  d[(o >> 2) + 3], d[(o >> 2) + 2], o & 3);
+              // (a zero byte shift is the dword itself)
+              const int sh = o & 3, q0 = o >> 2;
+              const uint32_t a0 =
+                  sh == 0 ? d[q0] : __builtin_amdgcn_alignbyte(d[q0 + 1], d[q0], sh);
+              const uint32_t a1 = sh == 0 ? d[q0 + 1]
+                                          : __builtin_amdgcn_alignbyte(
+                                                d[q0 + 2], d[q0 + 1], sh);
+              const uint32_t a2 = sh == 0 ? d[q0 + 2]
+                                          : __builtin_amdgcn_alignbyte(
+                                                d[q0 + 3], d[q0 + 2], sh);
               const uint64_t lo = uint64_t(a0) | (uint64_t(a1) << 32);
               const uint64_t hi = uint64_t(a1) | (uint64_t(a2) << 32);
               acc[j][0] = __builtin_amdgcn_qsad_pk_u16_u8(lo, ref[ch][iy][j],
@@ -615,7 +623,7 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
           for (int j = 0; j < 4; ++j) {
             const uint32_t word = PkMinU16(msad[j][w], 0x01FB01FBu);
             const uint32_t sad = hi ? (word >> 16) : (word & 0xFFFFu);
-            float wt = WeightOf(int(sad), mul);
+            float wt = WeightOfClamped(int(sad), mul64);
             wt = wt >= 0.05f ? wt : 0.0f;
             for (int c = 0; c < 3; ++c) {
               ws[j][c] = ws[j][c] + v[c][j + cx + 3] * wt;
