@@ -35,7 +35,7 @@ namespace {
 struct CoefSlot {
   int crow, ccol, cbx, cby;
   bool valid;
-  int raw, cs, cN, cflat0, obxg, obyg;
+  int raw, cs, cN, lgn, cflat0, obxg, obyg;
   uint8_t acs;
 };
 
@@ -52,7 +52,9 @@ __device__ inline CoefSlot MakeSlot(const FrameParams& P, const uint8_t* s_acs,
   s.raw = s.acs >> 4;
   const int cidx = s.acs & 15;
   s.cs = s.acs == 0xFF ? 1 : AcsCovered(s.raw);
-  const int coby = s.crow - cidx / s.cs, cobx = (s.ccol >> 6) - cidx % s.cs;
+  const int lgs = s.cs >> 1;  // log2(cs): cs is 1, 2 or 4
+  s.lgn = 3 + lgs;
+  const int coby = s.crow - (cidx >> lgs), cobx = (s.ccol >> 6) - (cidx & (s.cs - 1));
   s.cflat0 = (s.crow - coby) * 64 * s.cs + (s.ccol - cobx * 64);
   s.cN = 8 * s.cs;
   s.obxg = tbx + cobx - gbx0;
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(256)
         float v = c == 1 ? v0 : dq.cc * yv[j] + v0;
         if (c == 1) yv[j] = v;
         const int flat = S.cflat0 + j;
-        const int fi = flat / S.cN, fj = flat % S.cN;
+        const int fi = flat >> S.lgn, fj = flat & (S.cN - 1);
         if (fi < S.cs && fj < S.cs) {
           v = llf[(S.obyg + 1 + fi) * kLlfPitch + S.obxg + 1 + fj];
         } else if (P.predict_lf && fi < 2 * S.cs && fj < 2 * S.cs) {
@@ -312,7 +314,7 @@ __global__ void __launch_bounds__(256)
     float out[3][4];
     for (int j = 0; j < 4; ++j) {
       const int flat = S.cflat0 + j;
-      const int fi = flat / S.cN, fj = flat % S.cN;
+      const int fi = flat >> S.lgn, fj = flat & (S.cN - 1);
       const int e = CoefAt(S.crow, S.ccol + j);
       if (fi < S.cs && fj < S.cs) {
         out[0][j] = out[1][j] = out[2][j] = 0.0f;

@@ -709,12 +709,13 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
       const int raw = ca >> 4, idx = ca & 15;
       const int s = AcsCovered(raw);
       const int N = 8 * s;
-      const int coby = crow - idx / s, cobx = (ccol >> 6) - idx % s;
+      const int lgs = s >> 1;  // log2(s): s is 1, 2 or 4
+      const int coby = crow - (idx >> lgs), cobx = (ccol >> 6) - (idx & (s - 1));
       const int cflat0 = (crow - coby) * 64 * s + (ccol - cobx * 64);
       const float inv_quant_ac = __uint_as_float(ar_desc.w);
       for (int j = 0; j < 4; ++j) {
         const int flat = cflat0 + j;
-        const int fi = flat / N, fj = flat % N;
+        const int fi = flat >> (3 + lgs), fj = flat & (N - 1);
         float ratio = 1.0f;
         for (int c = 0; c < 3; ++c) {
           float interval;

@@ -344,7 +344,7 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
   const int total = T.off[16];
   for (int pass = 0; pass < 2; ++pass) {
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
-      const int c = k / total, t = k - c * total;
+      const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
       const int b = TaskBlock(T, t);
       InverseTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
                   coef + c * kCoefPlane, pix + c * kTilePlane,
@@ -363,7 +363,7 @@ __device__ inline void TileForward3(const TileTasks& T, const float* pix,
   const int total = T.off[16];
   for (int pass = 0; pass < (kAddHF ? 2 : 3); ++pass) {
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
-      const int c = k / total, t = k - c * total;
+      const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
       const int b = TaskBlock(T, t);
       ForwardTask<kAdjoint, kAddHF>(pass, T.raw[b], t - T.off[b], b >> 2,
                                     b & 3, pix + c * kTilePlane,

@@ -105,7 +105,8 @@ __global__ void __launch_bounds__(256, 4)
   const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
   const int craw = ca >> 4, cidx = ca & 15;
   const int cs = ca == 0xFF ? 1 : AcsCovered(craw);
-  const int coby = crow - cidx / cs, cobx = (ccol >> 6) - cidx % cs;
+  const int lgs = cs >> 1;  // log2(cs): cs is 1, 2 or 4
+  const int coby = crow - (cidx >> lgs), cobx = (ccol >> 6) - (cidx & (cs - 1));
   const int cflat0 = (crow - coby) * 64 * cs + (ccol - cobx * 64);
   const int cN = 8 * cs;
   // group-local origin block of the transform
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(256, 4)
         float v = c == 1 ? d : ccf[c] * yv[j] + d;
         if (c == 1) yv[j] = v;
         const int flat = cflat0 + j;
-        const int fi = flat / cN, fj = flat % cN;
+        const int fi = flat >> (3 + lgs), fj = flat & (cN - 1);
         if (fi < cs && fj < cs) {
           v = llf[(obyg + 1 + fi) * kLlfPitch + obxg + 1 + fj];
         } else if (P.predict_lf && fi < 2 * cs && fj < 2 * cs) {

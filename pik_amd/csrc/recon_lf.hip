@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256)
     const int raw = a >> 4, idx = a & 15;
     const int s = AcsCovered(raw);
     const int kind = AcsQuantKind(raw);
-    const int obxa = bxa - idx % s, obya = bya - idx / s;
+    const int obxa = bxa - (idx & (s - 1)), obya = bya - (idx >> (s >> 1));
     const int oblk = obya * P.bx + obxa;
     const int qf = B.qf[i], qfo = B.qf[oblk];
     const int qt = B.dqmap[B.dcf[(bya >> 3) * P.tx + (bxa >> 3)] * 256 + qf - 1];
