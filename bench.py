@@ -39,14 +39,15 @@ HBM_PEAK_GBPS = 8000.0
 
 # Algorithmic HBM bytes per pixel of the timed kernels (inputs read once,
 # outputs written once; per-block side data amortised over 64 pixels).
-#  epf_ar (KEpf on the image): idct in 3 x f32 + AR out 3 x f32
-#     + quant field i32 + strategy u8 + sigma LUT id u8 per block
+#  epf_ar (KEpf on the image): idct in 3 x f32 + u8 guide in 3 x u8
+#     + AR out 3 x f32 + quant field i32 + strategy u8 + sigma LUT id u8
+#     per block
 #  blocks_idct (KBlocks): int16 AC 3 x 2 B + idct out 3 x f32
 #     + quant field i32, strategy u8, colour-tile data per block
 #     + LLF, LF and pred2x2 f32 per block (1 + 4 + 4 values per channel)
 #  final_opsin (KFinal): AR+dots image in 3 x f32 + linear out 3 x f32
 BYTES_PER_PX = {
-    "epf_ar": 12.0 + 12.0 + 6.0 / 64,
+    "epf_ar": 12.0 + 3.0 + 12.0 + 6.0 / 64,
     "blocks_idct": 6.0 + 12.0 + 5.0 / 64 + 3 * 9 * 4.0 / 64,
     "final_opsin": 12.0 + 12.0,
 }

@@ -439,8 +439,41 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
   //    images are at least 8 wide and high); positions further out (a tile
   //    wider than a small image reaches up to 38 past its origin) are only
   //    clamped into the image, their values are never used. All loads are
-  //    issued before the LDS stores.
-  {
+  //    issued before the LDS stores. Image-mode tiles whose window columns
+  //    all lie inside the image load aligned float4s (columns -4 .. +35,
+  //    the first one dropped) and mirror only the rows.
+  if (!kDc && x0 >= 4 && x0 + 36 <= in.W && in.vw == in.W) {
+    constexpr int kVec = 3 * kRows * 10;  // float4s per tile
+    constexpr int kPer = (kVec + 255) / 256;
+    float4 wv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      wv[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (i < kVec) {
+        const int row = i / 10, q = i - row * 10;
+        const int c = row / kRows, ly = row - c * kRows;
+        int y = y0 - kOrg + ly;
+        y = y < 0 ? -y - 1 : (y >= in.H ? 2 * in.H - 1 - y : y);
+        y = min(max(y, 0), in.vh - 1);
+        wv[u] = *reinterpret_cast<const float4*>(
+            in.p + c * in.plane + size_t(y) * in.pitch + x0 - 4 + 4 * q);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      if (i < kVec) {
+        const int row = i / 10, q = i - row * 10;
+        const int c = row / kRows, ly = row - c * kRows;
+        float* d = s_in[c] + ly * kInP + 4 * q - 1;
+        if (q > 0) d[0] = wv[u].x;
+        d[1] = wv[u].y;
+        d[2] = wv[u].z;
+        d[3] = wv[u].w;
+      }
+    }
+  } else {
     constexpr int kWin = 3 * kRows * kCols;
     constexpr int kPer = (kWin + 255) / 256;
     float wv[kPer];
