@@ -112,6 +112,23 @@ def traffic_from_profiles(kernel):
     return None
 
 
+def valu_busy_from_profiles(kernel):
+    """VALU busy fraction of `kernel` on this workload, as recorded in
+    profiles/sq_counters_recon_4096.md (rocprofv3 SQ counters); None when
+    no such record exists."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_recon_4096.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^\|\s*%s\s*\|\s*4194304\s*\|.*\|\s*([0-9.]+)%%\s*\|"
+                     r"\s*[0-9.]+\s*\|\s*$" % kernel)
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line.strip())
+            if m:
+                return round(float(m.group(1)) / 100.0, 3)
+    return None
+
+
 def cpu_baseline(frame, seconds, threads):
     """The reference's own reconstruction (AVX2 build, its ThreadPool) on
     the same frame, repeated until `seconds` have passed (at least 2
@@ -294,7 +311,10 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic_from_profiles(KERNEL_NAMES[dom]),
             "algorithmic_bytes": int(algo_bytes),
-            "avg_ms": round(fam_ms[dom], 4)}
+            "avg_ms": round(fam_ms[dom], 4),
+            # the EPF kernel is VALU-bound; its VALU busy fraction from the
+            # SQ counters of the same workload
+            "valu_busy": valu_busy_from_profiles(KERNEL_NAMES[dom])}
 
     cpu = None
     if not a.no_cpu_baseline:
