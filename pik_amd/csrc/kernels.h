@@ -127,10 +127,14 @@ __host__ __device__ inline int CoefAt(int row, int col) {
   return row * kCoefRow + (col >> 6) * kCoefBlock + (col & 63);
 }
 
+// The tile's transforms in task order: grouped by kind (DCT8, DCT16,
+// DCT32, DCT4X4, direct), so that the lanes of a wave mostly run the same
+// line code. Position p holds the transform starting at tile block blk[p].
 struct TileTasks {
-  int off[17];      // first task of each block; off[16] = tasks per channel
-  uint8_t raw[16];  // strategy of the transform starting at the block, or
-                    // 0xFF (not a first block, outside, or masked out)
+  int off[17];      // first task of each position; off[16] = tasks per
+                    // channel (positions past the last transform: off = total)
+  uint8_t raw[16];  // strategy at each position (0xFF past the last)
+  uint8_t blk[16];  // tile block (by * 4 + bx) at each position
 };
 
 __device__ inline int TaskCount(int raw) {
@@ -141,28 +145,63 @@ __device__ inline int TaskCount(int raw) {
   return 1;
 }
 
+__device__ inline int TaskKind(int raw) {
+  if (raw == kDCT || raw == kDCTNoHF) return 0;
+  if (raw == kDCT16) return 1;
+  if (raw == kDCT32) return 2;
+  if (AcsIsDCT4(raw)) return 3;
+  return 4;
+}
+
 // All threads call; `acs_t` holds the tile's 16 strategy bytes (0xFF
 // outside the image). hf_only keeps only strategies with HF prediction.
+// Lanes 0..15 of the first wave each place one block: its position is the
+// number of transforms of earlier kinds plus those of its kind at earlier
+// blocks (ballots), its first task the task count of the positions before
+// it (shuffles).
 __device__ inline void BuildTasks(const uint8_t* acs_t, bool hf_only,
                                   TileTasks* T) {
-  if (threadIdx.x == 0) {
-    int o = 0;
-    for (int b = 0; b < 16; ++b) {
-      T->off[b] = o;
+  if (threadIdx.x < 64) {
+    const int b = threadIdx.x;
+    int raw = -1, kind = 5, cnt = 0;
+    if (b < 16) {
       const uint8_t a = acs_t[b];
-      uint8_t raw = 0xFF;
       if (a != 0xFF && (a & 15) == 0 && (!hf_only || AcsPredictHF(a >> 4))) {
         raw = a >> 4;
+        kind = TaskKind(raw);
+        cnt = TaskCount(raw);
       }
-      T->raw[b] = raw;
-      o += raw == 0xFF ? 0 : TaskCount(raw);
     }
-    T->off[16] = o;
+    int pos = 0, n = 0;
+    for (int k = 0; k < 5; ++k) {
+      const unsigned long long m = __ballot(kind == k);
+      n += __popcll(m);
+      if (k < kind) pos += __popcll(m);
+      if (k == kind) pos += __popcll(m & ((1ull << b) - 1));
+    }
+    int first = 0, total = 0;
+    for (int q = 0; q < 16; ++q) {
+      const int qc = __shfl(cnt, q, 64), qp = __shfl(pos, q, 64);
+      const int qk = __shfl(kind, q, 64);
+      total += qc;
+      if (qk < 5 && qp < pos) first += qc;
+    }
+    if (b < 16 && raw >= 0) {
+      T->off[pos] = first;
+      T->raw[pos] = uint8_t(raw);
+      T->blk[pos] = uint8_t(b);
+    }
+    if (b < 16 && b >= n) {  // positions past the last transform
+      T->off[b] = total;
+      T->raw[b] = 0xFF;
+      T->blk[b] = 0;
+    }
+    if (b == 0) T->off[16] = total;
   }
   __syncthreads();
 }
 
-// Block of task t (binary search over the offsets).
+// Position of task t (binary search over the offsets).
 __device__ inline int TaskBlock(const TileTasks& T, int t) {
   int lo = 0;
   for (int step = 8; step > 0; step >>= 1) {
@@ -345,8 +384,8 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
   for (int pass = 0; pass < 2; ++pass) {
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
       const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
-      const int b = TaskBlock(T, t);
-      InverseTask(pass, T.raw[b], t - T.off[b], b >> 2, b & 3,
+      const int p = TaskBlock(T, t), b = T.blk[p];
+      InverseTask(pass, T.raw[p], t - T.off[p], b >> 2, b & 3,
                   coef + c * kCoefPlane, pix + c * kTilePlane,
                   tmp + c * kTilePlane);
     }
@@ -364,8 +403,8 @@ __device__ inline void TileForward3(const TileTasks& T, const float* pix,
   for (int pass = 0; pass < (kAddHF ? 2 : 3); ++pass) {
     for (int k = threadIdx.x; k < 3 * total; k += 256) {
       const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
-      const int b = TaskBlock(T, t);
-      ForwardTask<kAdjoint, kAddHF>(pass, T.raw[b], t - T.off[b], b >> 2,
+      const int p = TaskBlock(T, t), b = T.blk[p];
+      ForwardTask<kAdjoint, kAddHF>(pass, T.raw[p], t - T.off[p], b >> 2,
                                     b & 3, pix + c * kTilePlane,
                                     coef + c * kCoefPlane,
                                     tmp + c * kTilePlane);
