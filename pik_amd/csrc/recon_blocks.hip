@@ -89,18 +89,6 @@ __global__ void __launch_bounds__(256, 4)
     }
   }
   __syncthreads();
-  // The dequant matrix entries need the descriptor: their loads go out
-  // before the task tables are built.
-  float mv[3][4];
-  if (cvalid) {
-    const uint32_t slot[3] = {dsc.x, dsc.y, dsc.z};
-    for (int c = 0; c < 3; ++c) {
-      for (int j = 0; j < 4; ++j) mv[c][j] = B.table[slot[c] + (ccol & 63) + j];
-    }
-  }
-  BuildTasks(s_acs, false, &s_inv);
-  if (P.predict_hf) BuildTasks(s_acs, true, &s_fwd);
-
   // F-matrix coordinates of this thread's entries.
   const uint8_t ca = s_acs[crow * 4 + (ccol >> 6)];
   const int craw = ca >> 4, cidx = ca & 15;
@@ -111,6 +99,42 @@ __global__ void __launch_bounds__(256, 4)
   const int cN = 8 * cs;
   // group-local origin block of the transform
   const int obxg = tbx + cobx - gbx0, obyg = tby + coby - gby0;
+  // Entry j is an LLF position (its value comes from the LLF cache), an LF
+  // position (the LF prediction is added), or neither.
+  int fix[4];
+  for (int j = 0; j < 4; ++j) {
+    const int flat = cflat0 + j;
+    const int fi = flat >> (3 + lgs), fj = flat & (cN - 1);
+    fix[j] = (fi < cs && fj < cs)
+                 ? 1
+                 : ((P.predict_lf && fi < 2 * cs && fj < 2 * cs) ? 2 : 0);
+  }
+
+  // The dequant matrix entries need the descriptor, the LLF / LF values
+  // the entries' positions: both go out before the task tables are built.
+  float mv[3][4], fixv[3][4];
+  if (cvalid) {
+    const uint32_t slot[3] = {dsc.x, dsc.y, dsc.z};
+    for (int c = 0; c < 3; ++c) {
+      for (int j = 0; j < 4; ++j) mv[c][j] = B.table[slot[c] + (ccol & 63) + j];
+    }
+    for (int c = 0; c < 3; ++c) {
+      const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
+      const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
+      for (int j = 0; j < 4; ++j) {
+        const int flat = cflat0 + j;
+        const int fi = flat >> (3 + lgs), fj = flat & (cN - 1);
+        fixv[c][j] = 0.0f;
+        if (fix[j] == 1) {
+          fixv[c][j] = llf[(obyg + 1 + fi) * kLlfPitch + obxg + 1 + fj];
+        } else if (fix[j] == 2) {
+          fixv[c][j] = lf[(2 * (obyg + 1) + fi) * kG2Pitch + 2 * (obxg + 1) + fj];
+        }
+      }
+    }
+  }
+  BuildTasks(s_acs, false, &s_inv);
+  if (P.predict_hf) BuildTasks(s_acs, true, &s_fwd);
 
   // 1. dequantise Y, then X and B with their colour correlation
   //    (Dequant::DoAC, pik/compressed_image.cc:586-693), and 2. the LLF and
@@ -129,20 +153,16 @@ __global__ void __launch_bounds__(256, 4)
     float yv[4];
     for (int ci = 0; ci < 3; ++ci) {
       const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
-      const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
-      const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
       for (int j = 0; j < 4; ++j) {
         // DequantWith's operation order
         const float mul = mv[c][j] * scaled;
         const float d = AdjustQuantBias(c, float(qv[c][j])) * mul;
         float v = c == 1 ? d : ccf[c] * yv[j] + d;
         if (c == 1) yv[j] = v;
-        const int flat = cflat0 + j;
-        const int fi = flat >> (3 + lgs), fj = flat & (cN - 1);
-        if (fi < cs && fj < cs) {
-          v = llf[(obyg + 1 + fi) * kLlfPitch + obxg + 1 + fj];
-        } else if (P.predict_lf && fi < 2 * cs && fj < 2 * cs) {
-          v += lf[(2 * (obyg + 1) + fi) * kG2Pitch + 2 * (obxg + 1) + fj];
+        if (fix[j] == 1) {
+          v = fixv[c][j];
+        } else if (fix[j] == 2) {
+          v += fixv[c][j];
         }
         s_coef[c][CoefAt(crow, ccol + j)] = v;
       }
