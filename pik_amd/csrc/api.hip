@@ -1176,7 +1176,7 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
 // kernels of different frames overlap and the whole batch costs one launch.
 
 namespace {
-constexpr int kBranches = 8;
+constexpr int kBranches = 32;
 }  // namespace
 
 struct pikgpu_batch {
