@@ -210,18 +210,47 @@ __device__ inline int TaskBlock(const TileTasks& T, int t) {
   return lo;
 }
 
-// Coefficient index of F[i][j] (row-major N x N, flat i*N + j) of a
-// transform with origin block (oby, obx) covering s x s blocks.
-__device__ inline int CoefIdx(int oby, int obx, int s, int flat) {
-  return CoefAt(oby + flat / (64 * s), obx * 64 + flat % (64 * s));
+// Column k of an N x N transform's F matrix (row-major, flat i * N + k)
+// with origin block (oby, obx): F[i][k] sits in storage row oby + i / 8,
+// column obx * 64 + (i % 8) * N + k, so its index is
+//   CoefColumn(oby, obx, k) + CoefRowOffset<N>(i)
+// (k < N keeps (i % 8) * N % 64 + k inside one block). With the row offset
+// a compile-time constant every access of a line shares one address and
+// differs only in the instruction's immediate offset.
+__device__ inline unsigned CoefColumn(unsigned oby, unsigned obx, unsigned k) {
+  return oby * kCoefRow + obx * kCoefBlock + k;
 }
+template <int N>
+__host__ __device__ constexpr int CoefRowOffset(int i) {
+  return (i / 8) * kCoefRow + ((i % 8) * N / 64) * kCoefBlock +
+         (i % 8) * N % 64;
+}
+
+// CoefRowOffset<N>(i) + k equals the storage index of flat i * N + k
+// relative to the origin block for every i, k < N.
+template <int N>
+constexpr bool CoefRowOffsetsMatchLayout() {
+  for (int i = 0; i < N; ++i) {
+    for (int k = 0; k < N; ++k) {
+      const int flat = i * N + k;
+      const int row = flat / (8 * N), col = flat % (8 * N);
+      const int idx = row * kCoefRow + (col >> 6) * kCoefBlock + (col & 63);
+      if (CoefRowOffset<N>(i) + k != idx) return false;
+    }
+  }
+  return true;
+}
+static_assert(CoefRowOffsetsMatchLayout<8>() &&
+                  CoefRowOffsetsMatchLayout<16>() &&
+                  CoefRowOffsetsMatchLayout<32>(),
+              "coefficient row offsets");
 
 template <int N>
 __device__ inline void InvLine1(const float* coef, float* tmp, int oby,
                                 int obx, int c) {
-  constexpr int s = N / 8;
+  const float* col = coef + CoefColumn(oby, obx, c);
   float v[N], w[N];
-  for (int r = 0; r < N; ++r) v[r] = coef[CoefIdx(oby, obx, s, r * N + c)];
+  for (int r = 0; r < N; ++r) v[r] = col[CoefRowOffset<N>(r)];
   dct1d::Idct<N>(v, w);
   for (int x = 0; x < N; ++x) tmp[PixAt(oby * 8 + c, obx * 8 + x)] = w[x];
 }
@@ -252,8 +281,9 @@ __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
   for (int x = 0; x < N; ++x) v[x] = tmp[PixAt(oby * 8 + x, obx * 8 + k)];
   dct1d::Dct<N>(v, w);
   const float scale = 1.0f / (N * N);
+  float* col = coef + CoefColumn(oby, obx, k);
   for (int j = 0; j < N; ++j) {
-    float& dst = coef[CoefIdx(oby, obx, s, j * N + k)];
+    float& dst = col[CoefRowOffset<N>(j)];
     if (!kAddHF) {
       dst = w[j] * scale;
     } else if (!(j < 2 * s && k < 2 * s)) {
