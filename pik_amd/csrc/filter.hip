@@ -727,10 +727,11 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
   // (BuildTasks ends with a barrier: every thread has read s_in, so the
   // coefficient and scratch buffers may overwrite it.)
   BuildTasks(s_acs, false, &s_tasks);
+  const ThreadTasks my_tasks = AssignTasks(s_tasks);
   float (*s_coef)[kCoefPlane] = reinterpret_cast<float (*)[kCoefPlane]>(s_big);
   float (*s_tmp)[kTilePlane] =
       reinterpret_cast<float (*)[kTilePlane]>(s_big + 3 * kCoefPlane);
-  TileForward3(s_tasks, &s_out[0][0], &s_coef[0][0], &s_tmp[0][0]);
+  TileForward3(my_tasks, &s_out[0][0], &s_coef[0][0], &s_tmp[0][0]);
   {
     // coefficient entries owned by this thread; the same thread scales all
     // three channels of its entries. The quantisation interval uses the
@@ -776,7 +777,7 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
     }
   }
   __syncthreads();
-  TileInverse3(s_tasks, &s_coef[0][0], &s_out[0][0], &s_tmp[0][0]);
+  TileInverse3(my_tasks, &s_coef[0][0], &s_out[0][0], &s_tmp[0][0]);
   const int y = y0 + r;
   if (y < in.H) {
     for (int c = 0; c < 3; ++c) {

@@ -406,41 +406,88 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
   }
 }
 
-// coef (3 x kCoefPlane) -> pix (3 x kTilePlane), tmp 3 x kTilePlane. All
-// 256 threads must call.
-__device__ inline void TileInverse3(const TileTasks& T, const float* coef,
-                                    float* pix, float* tmp) {
+// A thread's share of a tile's tasks, decoded once for all passes: task
+// k = threadIdx.x + 256 * i (channel-major over the 3 channels) in slot i,
+// packed as channel | raw << 2 | line << 6 | oby << 11 | obx << 13, and
+// kNoTask past the last task. A channel has at most 16 blocks x 16 line
+// tasks (DCT4X4), so three slots cover 3 x 256 tasks.
+constexpr int kTaskSlots = 3;
+constexpr uint32_t kNoTask = 0xFFFFFFFFu;
+struct ThreadTasks {
+  uint32_t t[kTaskSlots];
+  // slot i by selects (a variable index into t would put it in scratch)
+  __device__ uint32_t Slot(int i) const {
+    return i == 0 ? t[0] : (i == 1 ? t[1] : t[2]);
+  }
+};
+
+__device__ inline ThreadTasks AssignTasks(const TileTasks& T) {
   const int total = T.off[16];
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int k = threadIdx.x; k < 3 * total; k += 256) {
+  ThreadTasks m;
+#pragma unroll
+  for (int i = 0; i < kTaskSlots; ++i) {
+    const int k = threadIdx.x + 256 * i;
+    m.t[i] = kNoTask;
+    if (k < 3 * total) {
       const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
       const int p = TaskBlock(T, t), b = T.blk[p];
-      InverseTask(pass, T.raw[p], t - T.off[p], b >> 2, b & 3,
-                  coef + c * kCoefPlane, pix + c * kTilePlane,
-                  tmp + c * kTilePlane);
+      m.t[i] = uint32_t(c) | (uint32_t(T.raw[p]) << 2) |
+               (uint32_t(t - T.off[p]) << 6) | (uint32_t(b >> 2) << 11) |
+               (uint32_t(b & 3) << 13);
+    }
+  }
+  return m;
+}
+
+// coef (3 x kCoefPlane) -> pix (3 x kTilePlane), tmp 3 x kTilePlane. All
+// 256 threads must call.
+__device__ inline void TileInverse3(const ThreadTasks& m, const float* coef,
+                                    float* pix, float* tmp) {
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll 1
+    for (int i = 0; i < kTaskSlots; ++i) {
+      const uint32_t w = m.Slot(i);
+      if (w != kNoTask) {
+        const int c = w & 3;
+        InverseTask(pass, (w >> 2) & 15, (w >> 6) & 31, (w >> 11) & 3,
+                    (w >> 13) & 3, coef + c * kCoefPlane, pix + c * kTilePlane,
+                    tmp + c * kTilePlane);
+      }
     }
     __syncthreads();
   }
+}
+__device__ inline void TileInverse3(const TileTasks& T, const float* coef,
+                                    float* pix, float* tmp) {
+  TileInverse3(AssignTasks(T), coef, pix, tmp);
 }
 
 // pix (3 x kTilePlane) -> coef (3 x kCoefPlane) for the transforms in T
 // (others keep their coefficient entries), tmp 3 x kTilePlane. All 256
 // threads must call.
 template <bool kAdjoint = false, bool kAddHF = false>
-__device__ inline void TileForward3(const TileTasks& T, const float* pix,
+__device__ inline void TileForward3(const ThreadTasks& m, const float* pix,
                                     float* coef, float* tmp) {
-  const int total = T.off[16];
   for (int pass = 0; pass < (kAddHF ? 2 : 3); ++pass) {
-    for (int k = threadIdx.x; k < 3 * total; k += 256) {
-      const int c = (k >= total) + (k >= 2 * total), t = k - c * total;
-      const int p = TaskBlock(T, t), b = T.blk[p];
-      ForwardTask<kAdjoint, kAddHF>(pass, T.raw[p], t - T.off[p], b >> 2,
-                                    b & 3, pix + c * kTilePlane,
-                                    coef + c * kCoefPlane,
-                                    tmp + c * kTilePlane);
+#pragma unroll 1
+    for (int i = 0; i < kTaskSlots; ++i) {
+      const uint32_t w = m.Slot(i);
+      if (w != kNoTask) {
+        const int c = w & 3;
+        ForwardTask<kAdjoint, kAddHF>(pass, (w >> 2) & 15, (w >> 6) & 31,
+                                      (w >> 11) & 3, (w >> 13) & 3,
+                                      pix + c * kTilePlane,
+                                      coef + c * kCoefPlane,
+                                      tmp + c * kTilePlane);
+      }
     }
     __syncthreads();
   }
+}
+template <bool kAdjoint = false, bool kAddHF = false>
+__device__ inline void TileForward3(const TileTasks& T, const float* pix,
+                                    float* coef, float* tmp) {
+  TileForward3<kAdjoint, kAddHF>(AssignTasks(T), pix, coef, tmp);
 }
 
 // ---------------------------------------------------------------------------
