@@ -723,7 +723,7 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
       KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
     }
     if (P.pred_end[3] > 0) {
-      KPred2x2Interior<<<dim3(DivCeil(P.pred_end[3], 256), 3), 256, 0, st>>>(
+      KPred2x2Interior<<<DivCeil(3 * P.pred_end[3], 256), 256, 0, st>>>(
           P, B, K);
     }
   }

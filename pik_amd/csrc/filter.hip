@@ -936,7 +936,10 @@ __global__ void __launch_bounds__(256)
   if (x >= P.px) return;
   const size_t plane = size_t(P.py) * B.ipitch;
   const int xm = Mirror(x - 1, P.px), xp = Mirror(x + 1, P.px);
-  const float* inv = K->opsin_inv;
+  // in registers: the output stores could alias K as far as the compiler
+  // knows, which would reload the matrix every row
+  float inv[9];
+  for (int i = 0; i < 9; ++i) inv[i] = K->opsin_inv[i];
   // rows above (t) and at (m) the current row: [channel][left, centre, right]
   float t[3][3], m[3][3];
   {
@@ -952,12 +955,35 @@ __global__ void __launch_bounds__(256)
       m[c][2] = src[rm + xp];
     }
   }
-  for (int y = ya; y < yb; ++y) {
-    const size_t rb = size_t(y + 1 < P.py ? y + 1 : P.py - 1) * B.ipitch;
-    float v[3];
+  // The row below is loaded one iteration ahead, before this row's stores
+  // (which the compiler must assume may alias the input).
+  float nb[3][3];
+  {
+    const size_t rb = size_t(ya + 1 < P.py ? ya + 1 : P.py - 1) * B.ipitch;
     for (int c = 0; c < 3; ++c) {
       const float* src = B.ar + c * plane;
-      const float bl = src[rb + xm], bc = src[rb + x], br = src[rb + xp];
+      nb[c][0] = src[rb + xm];
+      nb[c][1] = src[rb + x];
+      nb[c][2] = src[rb + xp];
+    }
+  }
+  for (int y = ya; y < yb; ++y) {
+    float b[3][3];
+    for (int c = 0; c < 3; ++c) {
+      for (int i = 0; i < 3; ++i) b[c][i] = nb[c][i];
+    }
+    if (y + 1 < yb) {
+      const size_t rb = size_t(y + 2 < P.py ? y + 2 : P.py - 1) * B.ipitch;
+      for (int c = 0; c < 3; ++c) {
+        const float* src = B.ar + c * plane;
+        nb[c][0] = src[rb + xm];
+        nb[c][1] = src[rb + x];
+        nb[c][2] = src[rb + xp];
+      }
+    }
+    float v[3];
+    for (int c = 0; c < 3; ++c) {
+      const float bl = b[c][0], bc = b[c][1], br = b[c][2];
       if (P.gaborish) {
         // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
         const float sum_tb = t[c][1] + bc;
@@ -972,13 +998,17 @@ __global__ void __launch_bounds__(256)
       } else {
         v[c] = m[c][1];
       }
-      if (B.keep) B.gab[c * plane + size_t(y) * B.ipitch + x] = v[c];
       t[c][0] = m[c][0];
       t[c][1] = m[c][1];
       t[c][2] = m[c][2];
       m[c][0] = bl;
       m[c][1] = bc;
       m[c][2] = br;
+    }
+    if (B.keep) {
+      for (int c = 0; c < 3; ++c) {
+        B.gab[c * plane + size_t(y) * B.ipitch + x] = v[c];
+      }
     }
     if (x >= P.xsize || y >= P.ysize) continue;
     const float gamma_r = 1.0f * (v[1] + v[0]);

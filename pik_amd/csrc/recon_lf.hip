@@ -392,13 +392,15 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
   }
 }
 
-// One thread per first block and plane; the first blocks come grouped by
-// class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; pred_list, binned on the
-// host at upload) so the lanes of a wave take the same path.
+// One thread per first block and plane, the three planes of a block in
+// adjacent lanes (X and B read the block's Y coefficients and descriptor
+// with the Y lane, in the same load instructions); the first blocks come
+// grouped by class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; pred_list,
+// binned on the host at upload) so the lanes of a wave take the same path.
 __global__ void __launch_bounds__(256)
     KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.y;
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = lane / 3, c = lane - 3 * k;
   if (k >= P.pred_end[3]) return;
   const int blk = int(B.pred_list[k]);
   const int bxa = blk % P.bx, bya = blk / P.bx;
