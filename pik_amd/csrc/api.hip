@@ -39,6 +39,9 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Most workgroups of a KMinMaxPartial launch (6 floats of partials each).
+constexpr int kMinMaxMaxGroups = 4096;
+
 }  // namespace
 
 struct pikgpu_ctx {
@@ -520,7 +523,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   const size_t gpitch = DivCeil(bx * 8, 32) * 32 + 64;
   GROW(gimg, 3 * grows * gpitch);
   GROW(out, 3 * opitch * f->ysize * 4);
-  GROW(mm_partial, 1024 * 6 * 4);
+  GROW(mm_partial, kMinMaxMaxGroups * 6 * 4);
   GROW(mm, 8 * 4);
 #undef GROW
 
@@ -653,10 +656,12 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
 }
 
 // Grid of KMinMaxPartial: 256-column strips, row ranges of >= 8 rows, at most
-// 1024 workgroups (the size of the partials buffer).
+// kMinMaxMaxGroups workgroups (the size of the partials buffer): 16 rows per
+// thread at 4096 x 4096, two waves of workgroups on 256 CUs.
 static dim3 MinMaxGrid(int w, int h) {
   const int nx = int(DivCeil(w, 256));
-  const int ny = std::max(1, std::min(int(DivCeil(h, 8)), 1024 / nx));
+  const int ny =
+      std::max(1, std::min(int(DivCeil(h, 8)), kMinMaxMaxGroups / nx));
   return dim3(nx, ny);
 }
 

@@ -157,12 +157,35 @@ __global__ void __launch_bounds__(256)
         m[c][1] = pc[r1 + xc];
         m[c][2] = pc[r1 + xp];
       }
-      for (int y = ya; y < yb; ++y) {
-        const int yn = y + 1 < in.H ? y + 1 : in.H - 1;
+      // the row below, loaded one iteration ahead
+      float nb[3][3];
+      {
+        const int yn = ya + 1 < in.H ? ya + 1 : in.H - 1;
         const size_t rb = size_t(min(yn, in.vh - 1)) * in.pitch;
         for (int c = 0; c < 3; ++c) {
           const float* pc = in.p + c * in.plane;
-          const float bl = pc[rb + xm], bc = pc[rb + xc], br = pc[rb + xp];
+          nb[c][0] = pc[rb + xm];
+          nb[c][1] = pc[rb + xc];
+          nb[c][2] = pc[rb + xp];
+        }
+      }
+      for (int y = ya; y < yb; ++y) {
+        float b[3][3];
+        for (int c = 0; c < 3; ++c) {
+          for (int i = 0; i < 3; ++i) b[c][i] = nb[c][i];
+        }
+        if (y + 1 < yb) {
+          const int yn = y + 2 < in.H ? y + 2 : in.H - 1;
+          const size_t rb = size_t(min(yn, in.vh - 1)) * in.pitch;
+          for (int c = 0; c < 3; ++c) {
+            const float* pc = in.p + c * in.plane;
+            nb[c][0] = pc[rb + xm];
+            nb[c][1] = pc[rb + xc];
+            nb[c][2] = pc[rb + xp];
+          }
+        }
+        for (int c = 0; c < 3; ++c) {
+          const float bl = b[c][0], bc = b[c][1], br = b[c][2];
           const float sum_tb = t[c][1] + bc;
           const float mul0 = m[c][1] * P.gab_w[0];
           const float sum_lr = m[c][0] + m[c][2];
@@ -259,9 +282,51 @@ __global__ void __launch_bounds__(256)
   const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
   if (tile < 0) return;
   const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
-  // The tile with a 1-pixel mirrored ring; positions past the ring (a tile
-  // wider than the image) are clamped into the image and never used.
-  {
+  // The tile with a 1-pixel mirrored ring (window row / column 0 is pixel
+  // -1 of the tile); positions past the ring (a tile wider than the image)
+  // are clamped into the image and never used. Tiles whose window columns
+  // all lie inside the image load aligned float4s (columns -4 .. +35) and
+  // mirror only the rows.
+  if (x0 >= 4 && x0 + 36 <= in.W) {
+    constexpr int kVec = 3 * 34 * 10;  // float4s per tile
+    constexpr int kPer = (kVec + 255) / 256;
+    float4 wv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      wv[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (i < kVec) {
+        const int row = i / 10, q = i - row * 10;
+        const int c = row / 34, ly = row - c * 34;
+        int y = y0 - 1 + ly;
+        y = y < 0 ? -y - 1 : (y >= in.H ? 2 * in.H - 1 - y : y);
+        y = min(max(y, 0), in.H - 1);
+        wv[u] = *reinterpret_cast<const float4*>(
+            in.p + c * in.plane + size_t(y) * in.pitch + x0 - 4 + 4 * q);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * 256;
+      if (i < kVec) {
+        const int row = i / 10, q = i - row * 10;
+        const int c = row / 34, ly = row - c * 34;
+        // float4 q holds window columns 4q - 3 .. 4q; the window is
+        // columns 0 .. 33
+        float* d = s_w[c] + ly * kWP + 4 * q - 3;
+        if (q == 0) {
+          d[3] = wv[u].w;
+        } else if (q == 9) {
+          d[0] = wv[u].x;
+        } else {
+          d[0] = wv[u].x;
+          d[1] = wv[u].y;
+          d[2] = wv[u].z;
+          d[3] = wv[u].w;
+        }
+      }
+    }
+  } else {
     constexpr int kWin = 3 * 34 * 34;
     constexpr int kPer = (kWin + 255) / 256;
     float wv[kPer];
@@ -292,38 +357,51 @@ __global__ void __launch_bounds__(256)
     }
   }
   __syncthreads();
+  // Each thread: pixels j0 .. j0 + 3 of tile row i, all three channels.
   const float all_min = mm[7];
   const float gmul = mm[6];
   const size_t gplane = size_t(B.grows) * B.gpitch;
-  for (int k = tid; k < 3 * 1024; k += 256) {
-    const int c = k >> 10, i = (k >> 5) & 31, j = k & 31;
-    const int x = x0 + j, y = y0 + i;
-    if (x >= in.W || y >= in.H) continue;
-    const float* si = s_w[c] + (i + 1) * kWP + (j + 1);
-    float g;
-    if (gaborish) {
-      // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
-      const float tc = si[-kWP], bc = si[kWP], ml = si[-1], mr = si[1];
-      const float tl = si[-kWP - 1], tr = si[-kWP + 1];
-      const float bl = si[kWP - 1], br = si[kWP + 1];
-      const float sum_tb = tc + bc;
-      const float mul0 = si[0] * P.gab_w[0];
-      const float sum_lr = ml + mr;
-      const float x1 = sum_tb + sum_lr;
-      const float mul1 = x1 * P.gab_w[1] + mul0;
-      const float sum_t2 = tl + tr;
-      const float sum_b2 = bl + br;
-      const float x2 = sum_t2 + sum_b2;
-      g = x2 * P.gab_w[2] + mul1;
+  const int i = tid >> 3, j0 = (tid & 7) << 2;
+  const int y = y0 + i, x = x0 + j0;
+  if (y >= in.H || x >= in.W) return;
+  const int n = min(4, in.W - x);
+  // guide byte of pixel (x, y) at (x + 6, y + 6); x + 6 is even
+  uint8_t* grow = B.gimg + size_t(y + 6) * B.gpitch + x + 6;
+  for (int c = 0; c < 3; ++c) {
+    float w[3][6];
+    for (int r = 0; r < 3; ++r) {
+      for (int k = 0; k < 6; ++k) w[r][k] = s_w[c][(i + r) * kWP + j0 + k];
+    }
+    uint8_t q[4];
+    for (int k = 0; k < 4; ++k) {
+      float g;
+      if (gaborish) {
+        // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
+        const float sum_tb = w[0][k + 1] + w[2][k + 1];
+        const float mul0 = w[1][k + 1] * P.gab_w[0];
+        const float sum_lr = w[1][k] + w[1][k + 2];
+        const float x1 = sum_tb + sum_lr;
+        const float mul1 = x1 * P.gab_w[1] + mul0;
+        const float sum_t2 = w[0][k] + w[0][k + 2];
+        const float sum_b2 = w[2][k] + w[2][k + 2];
+        const float x2 = sum_t2 + sum_b2;
+        g = x2 * P.gab_w[2] + mul1;
+      } else {
+        g = w[1][k + 1];
+      }
+      if (B.keep && k < n) {
+        B.guide[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x + k] = g;
+      }
+      const float scaled = (g - all_min) * gmul;
+      q[k] = static_cast<uint8_t>(static_cast<int>(scaled));
+    }
+    uint8_t* g8 = grow + c * gplane;
+    if (n == 4) {
+      *reinterpret_cast<uint16_t*>(g8) = uint16_t(q[0] | (q[1] << 8));
+      *reinterpret_cast<uint16_t*>(g8 + 2) = uint16_t(q[2] | (q[3] << 8));
     } else {
-      g = si[0];
+      for (int k = 0; k < n; ++k) g8[k] = q[k];
     }
-    if (B.keep) {
-      B.guide[size_t(c) * P.py * B.ipitch + size_t(y) * B.ipitch + x] = g;
-    }
-    const float scaled = (g - all_min) * gmul;
-    B.gimg[c * gplane + size_t(y + 6) * B.gpitch + x + 6] =
-        static_cast<uint8_t>(static_cast<int>(scaled));
   }
 }
 
