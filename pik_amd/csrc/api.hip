@@ -664,6 +664,14 @@ static dim3 MinMaxGrid(int w, int h) {
       std::max(1, std::min(int(DivCeil(h, 8)), kMinMaxMaxGroups / nx));
   return dim3(nx, ny);
 }
+// Grid of KMinMaxPartial4: 1024-column strips (four columns per thread),
+// row ranges of >= 8 rows.
+static dim3 MinMaxGrid4(int w, int h) {
+  const int nx = int(DivCeil(w, 1024));
+  const int ny =
+      std::max(1, std::min(int(DivCeil(h, 8)), kMinMaxMaxGroups / nx));
+  return dim3(nx, ny);
+}
 
 static int EnsureStageBuffers(pikgpu_ctx* ctx) {
   const FrameParams& P = ctx->P;
@@ -775,8 +783,9 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     ii.vw = P.px;
     ii.vh = P.py;
     const int gab = P.gaborish && !P.use_sharpened;
-    const dim3 mmg = MinMaxGrid(ii.W, ii.H);
-    KMinMaxPartial<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
+    // the IDCT image: width a multiple of 8, rows 32-float aligned
+    const dim3 mmg = MinMaxGrid4(ii.W, ii.H);
+    KMinMaxPartial4<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
     KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
     const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
     KGuide<<<grid, 256, 0, st>>>(ii, gab, P, B, B.mm);

@@ -226,6 +226,101 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// KMinMaxPartial for an image whose rows are float4-aligned and whose width
+// is a multiple of 4 with no replicated margin (vw == W, vh == H): thread t
+// walks columns 4t .. 4t + 3 with one float4 and the two mirrored edge
+// columns per row and channel, the next row loaded one iteration ahead.
+__global__ void __launch_bounds__(256)
+    KMinMaxPartial4(Img in, int gaborish, FrameParams P, float* partial) {
+  __shared__ float s_mm[6][256];
+  const int tid = threadIdx.x;
+  const int x = (blockIdx.x * 256 + tid) * 4;
+  const int rows = (in.H + int(gridDim.y) - 1) / int(gridDim.y);
+  const int ya = int(blockIdx.y) * rows, yb = min(in.H, ya + rows);
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  if (x < in.W && ya < yb) {
+    const int xm = Mirror(x - 1, in.W), xp = Mirror(x + 4, in.W);
+    // columns x - 1 .. x + 4 of row y, channel c
+    auto load_row = [&](int y, int c, float* v) {
+      const float* pr = in.p + c * in.plane + size_t(y) * in.pitch;
+      const float4 q = *reinterpret_cast<const float4*>(pr + x);
+      v[0] = pr[xm];
+      v[1] = q.x;
+      v[2] = q.y;
+      v[3] = q.z;
+      v[4] = q.w;
+      v[5] = pr[xp];
+    };
+    if (!gaborish) {
+      for (int y = ya; y < yb; ++y) {
+        for (int c = 0; c < 3; ++c) {
+          const float4 q = *reinterpret_cast<const float4*>(
+              in.p + c * in.plane + size_t(y) * in.pitch + x);
+          mn[c] = fminf(fminf(mn[c], q.x), fminf(q.y, fminf(q.z, q.w)));
+          mx[c] = fmaxf(fmaxf(mx[c], q.x), fmaxf(q.y, fmaxf(q.z, q.w)));
+        }
+      }
+    } else {
+      float t[3][6], m[3][6], nb[3][6];
+      const int y0 = Mirror(ya - 1, in.H);
+      const int y2 = ya + 1 < in.H ? ya + 1 : in.H - 1;
+      for (int c = 0; c < 3; ++c) {
+        load_row(y0, c, t[c]);
+        load_row(ya, c, m[c]);
+        load_row(y2, c, nb[c]);
+      }
+      for (int y = ya; y < yb; ++y) {
+        float b[3][6];
+        for (int c = 0; c < 3; ++c) {
+          for (int i = 0; i < 6; ++i) b[c][i] = nb[c][i];
+        }
+        if (y + 1 < yb) {
+          const int yn = y + 2 < in.H ? y + 2 : in.H - 1;
+          for (int c = 0; c < 3; ++c) load_row(yn, c, nb[c]);
+        }
+        for (int c = 0; c < 3; ++c) {
+          for (int k = 0; k < 4; ++k) {
+            // strategy::Symmetric3's operation order (pik/convolve.h:605-621)
+            const float sum_tb = t[c][k + 1] + b[c][k + 1];
+            const float mul0 = m[c][k + 1] * P.gab_w[0];
+            const float sum_lr = m[c][k] + m[c][k + 2];
+            const float x1 = sum_tb + sum_lr;
+            const float mul1 = x1 * P.gab_w[1] + mul0;
+            const float sum_t2 = t[c][k] + t[c][k + 2];
+            const float sum_b2 = b[c][k] + b[c][k + 2];
+            const float x2 = sum_t2 + sum_b2;
+            const float v = x2 * P.gab_w[2] + mul1;
+            mn[c] = fminf(mn[c], v);
+            mx[c] = fmaxf(mx[c], v);
+          }
+          for (int i = 0; i < 6; ++i) {
+            t[c][i] = m[c][i];
+            m[c][i] = b[c][i];
+          }
+        }
+      }
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    s_mm[c][tid] = mn[c];
+    s_mm[3 + c][tid] = mx[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      for (int c = 0; c < 3; ++c) {
+        s_mm[c][tid] = fminf(s_mm[c][tid], s_mm[c][tid + s]);
+        s_mm[3 + c][tid] = fmaxf(s_mm[3 + c][tid], s_mm[3 + c][tid + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < 6) {
+    partial[(blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = s_mm[tid][0];
+  }
+}
+
 // Reduces the partials to mm[0..5] = min[3], max[3]; mm[6] = 255 / range
 // (the guide multiplier and the sigma stretch, both 1 for a flat image),
 // mm[7] = the minimum over channels.

@@ -539,6 +539,8 @@ struct Img {
 
 __global__ void KMinMaxPartial(Img in, int gaborish, FrameParams P,
                                float* partial);
+__global__ void KMinMaxPartial4(Img in, int gaborish, FrameParams P,
+                               float* partial);
 __global__ void KMinMaxFinal(const float* partial, int n, float* mm);
 template <bool kDc>
 __global__ void KEpf(Img in, FrameParams P, FrameBuffers B,
