@@ -750,15 +750,31 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
       // mul * 64 as a 24-bit signed value (the table holds -32512 .. -774),
       // so the weight multiply is a 24-bit one
       const int mul64 = (K->mul_table[sigma] * 64 << 8) >> 8;
+      // The three patch rows iy = 0..2 of a pixel in row r are guide rows
+      // r+5+iy (the guide tile has a 6-pixel border); the SAD of a patch is
+      // the sum of its rows' SADs. Rows r = 2k and 2k+1 sit in lanes L and
+      // L+8 of one 16-lane DPP row and share patch rows 2k+5+1 and 2k+5+2:
+      // each lane sums its own row (iy = 1) with its partner's own row
+      // (a DPP rotate by 8 within the row, added as one 32-bit word per
+      // pair of u16 SADs: a three-row SAD is at most 3060, so no carry
+      // crosses the halves), then adds its outer row (iy = 0 for even r,
+      // iy = 2 for odd r). Both lanes of a pair are always active together:
+      // they share the block (sigma), the columns and, the image height
+      // being a multiple of 8, the row test.
+      const int oiy = (r & 1) ? 2 : 0;
       // Reference patch rows of pixel j: guide bytes c0+5+j .. c0+8+j of
-      // rows r+5 .. r+7 (the guide tile has a 6-pixel border).
-      uint32_t ref[3][3][4];
+      // the own row (iy = 1) and the outer row (iy = oiy).
+      uint32_t ref_own[3][4], ref_out[3][4];
       for (int ch = 0; ch < 3; ++ch) {
-        for (int iy = 0; iy < 3; ++iy) {
+        for (int k = 0; k < 2; ++k) {
+          const int iy = k == 0 ? 1 : oiy;
           const uint8_t* row = s_g[ch] + (r + 5 + iy) * kGP + c0 + 4;
           const uint32_t d0 = LdsDword(row), d1 = LdsDword(row + 4);
           for (int j = 0; j < 4; ++j) {
-            ref[ch][iy][j] = j == 3 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, j + 1);
+            const uint32_t v =
+                j == 3 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, j + 1);
+            if (k == 0) ref_own[ch][j] = v;
+            else ref_out[ch][j] = v;
           }
         }
       }
@@ -773,8 +789,9 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
         uint32_t msad[4][4];
         for (int ch = 0; ch < 3; ++ch) {
           uint64_t acc[4][2];
-          for (int j = 0; j < 4; ++j) acc[j][0] = acc[j][1] = 0;
-          for (int iy = 0; iy < 3; ++iy) {
+          for (int k = 0; k < 2; ++k) {
+            // k = 0: own patch row, then the pair's sum; k = 1: outer row
+            const int iy = k == 0 ? 1 : oiy;
             const uint8_t* row = s_g[ch] + (r + 5 + cy + iy) * kGP + c0;
             const uint32_t d[5] = {LdsDword(row), LdsDword(row + 4),
                                    LdsDword(row + 8), LdsDword(row + 12),
@@ -795,10 +812,24 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
                                                 d[q0 + 3], d[q0 + 2], sh);
               const uint64_t lo = uint64_t(a0) | (uint64_t(a1) << 32);
               const uint64_t hi = uint64_t(a1) | (uint64_t(a2) << 32);
-              acc[j][0] = __builtin_amdgcn_qsad_pk_u16_u8(lo, ref[ch][iy][j],
-                                                          acc[j][0]);
-              acc[j][1] = __builtin_amdgcn_qsad_pk_u16_u8(hi, ref[ch][iy][j],
-                                                          acc[j][1]);
+              if (k == 0) {
+                acc[j][0] = __builtin_amdgcn_qsad_pk_u16_u8(lo, ref_own[ch][j], 0);
+                acc[j][1] = __builtin_amdgcn_qsad_pk_u16_u8(hi, ref_own[ch][j], 0);
+                for (int h = 0; h < 2; ++h) {
+                  uint32_t w[2] = {uint32_t(acc[j][h]), uint32_t(acc[j][h] >> 32)};
+                  for (int e = 0; e < 2; ++e) {
+                    // + the partner lane's word (row_ror:8)
+                    w[e] += uint32_t(__builtin_amdgcn_update_dpp(
+                        0, int(w[e]), 0x128, 0xF, 0xF, false));
+                  }
+                  acc[j][h] = uint64_t(w[0]) | (uint64_t(w[1]) << 32);
+                }
+              } else {
+                acc[j][0] = __builtin_amdgcn_qsad_pk_u16_u8(lo, ref_out[ch][j],
+                                                            acc[j][0]);
+                acc[j][1] = __builtin_amdgcn_qsad_pk_u16_u8(hi, ref_out[ch][j],
+                                                            acc[j][1]);
+              }
             }
           }
           for (int j = 0; j < 4; ++j) {
