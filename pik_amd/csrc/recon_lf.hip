@@ -397,11 +397,14 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
 // with the Y lane, in the same load instructions); the first blocks come
 // grouped by class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; pred_list,
 // binned on the host at upload) so the lanes of a wave take the same path.
+// The grid walks the list from its end: the DCT32 threads, by far the
+// longest, start first instead of forming the tail of the launch.
 __global__ void __launch_bounds__(256)
     KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = lane / 3, c = lane - 3 * k;
-  if (k >= P.pred_end[3]) return;
+  const int kr = lane / 3, c = lane - 3 * kr;
+  if (kr >= P.pred_end[3]) return;
+  const int k = P.pred_end[3] - 1 - kr;
   const int blk = int(B.pred_list[k]);
   const int bxa = blk % P.bx, bya = blk / P.bx;
   const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
