@@ -736,8 +736,12 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
       KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
     }
     if (P.pred_end[3] > 0) {
-      KPred2x2Interior<<<DivCeil(3 * P.pred_end[3], 256), 256, 0, st>>>(
-          P, B, K);
+      // segments: DCT32 (32 units of 8 lanes per workgroup), DCT16 (64 of
+      // 4), then one lane per unit
+      const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
+      const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+      const int wg1 = int(DivCeil(3 * P.pred_end[1], 256));
+      KPred2x2Interior<<<wg4 + wg2 + wg1, 256, 0, st>>>(P, B, K, wg4, wg2);
     }
   }
 }

@@ -520,7 +520,8 @@ __global__ void KLfBlur(FrameParams P, FrameBuffers B);
 __global__ void KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KZeroPred(FrameParams P, FrameBuffers B);
 __global__ void KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K);
-__global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K);
+__global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                                 int wg4, int wg2);
 __global__ void KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KAdjForward(FrameParams P, FrameBuffers B,
                             const DeviceConsts* K, AdjArgs A);

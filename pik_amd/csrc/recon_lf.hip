@@ -392,62 +392,153 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
   }
 }
 
-// One thread per first block and plane, the three planes of a block in
-// adjacent lanes (X and B read the block's Y coefficients and descriptor
-// with the Y lane, in the same load instructions); the first blocks come
-// grouped by class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; pred_list,
-// binned on the host at upload) so the lanes of a wave take the same path.
-// The grid walks the list from its end: the DCT32 threads, by far the
-// longest, start first instead of forming the tail of the launch.
-__global__ void __launch_bounds__(256)
-    KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-  const int kr = lane / 3, c = lane - 3 * kr;
-  if (kr >= P.pred_end[3]) return;
-  const int k = P.pred_end[3] - 1 - kr;
-  const int blk = int(B.pred_list[k]);
-  const int bxa = blk % P.bx, bya = blk / P.bx;
-  const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
-  const int g = gyi * P.gx + gxi;
-  const int bxg = bxa - gxi * kGroupDimBlocks, byg = bya - gyi * kGroupDimBlocks;
-  if (k >= P.pred_end[2]) {
-    Pred2x2Block<4>(P, B, K, c, bxa, bya, bxg, byg, g);
-  } else if (k >= P.pred_end[1]) {
-    Pred2x2Block<2>(P, B, K, c, bxa, bya, bxg, byg, g);
-  } else if (k < P.pred_end[0]) {
-    Pred2x2Block<1>(P, B, K, c, bxa, bya, bxg, byg, g);
-  } else {
-    // DCT2X2, DCT4X4(_NOHF), IDENTITY: 2x2 Hadamard of the LF corner.
-    const BlockDq dq = DescBlockDq(P, B, c, bya, bxa);
-    BlockDq dqy;
-    if (c != 1) dqy = DescBlockDq(P, B, 1, bya, bxa);
+// Location of a first block in its group.
+struct FirstBlock {
+  int bxa, bya, bxg, byg, g;
+};
+__device__ inline FirstBlock LocateBlock(const FrameParams& P, int blk) {
+  FirstBlock f;
+  f.bxa = blk % P.bx;
+  f.bya = blk / P.bx;
+  const int gxi = f.bxa / kGroupDimBlocks, gyi = f.bya / kGroupDimBlocks;
+  f.g = gyi * P.gx + gxi;
+  f.bxg = f.bxa - gxi * kGroupDimBlocks;
+  f.byg = f.bya - gyi * kGroupDimBlocks;
+  return f;
+}
+
+// Pred2x2Block<S> for S = 2, 4 on N = 2S lanes of a workgroup: lane l
+// builds column l of the corner F, scales it and runs the column pass of
+// Idct2D into row l of the unit's LDS matrix t (pitch N + 1); after the
+// barrier lane l runs the row pass over column l of t and writes column l
+// of the output. Each element sees the operations of the one-lane form in
+// the same order. Every thread of the workgroup calls (the barrier);
+// `active` is false for lanes past the last unit.
+template <int S>
+__device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
+                                    const DeviceConsts* K, bool active, int c,
+                                    int blk, int l, float* t) {
+  constexpr int N = 2 * S;
+  constexpr int ai = ScaleIdx<N>(), ti = ScaleIdx<8 * S>();
+  FirstBlock f = {0, 0, 0, 0, 0};
+  float v[N], w[N];
+  if (active) {
+    f = LocateBlock(P, blk);
+    const float* llf = B.llf + (size_t(f.g) * 3 + c) * kLlfPlane;
+    const float* lf = B.lf2x2 + (size_t(f.g) * 3 + c) * kG2Plane;
     const size_t plane = size_t(P.by) * P.bx * 64;
-    const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
-    const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
-    const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
-    const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
-    float F[4];
-    for (int i = 0; i < 2; ++i) {
-      for (int j = 0; j < 2; ++j) {
-        float v;
-        if (i == 0 && j == 0) {
-          v = llf[(byg + 1) * kLlfPitch + bxg + 1];
-        } else {
-          const int flat = i * 8 + j;
-          float yv = 0.0f;
-          if (c != 1) yv = DequantWith(dqy, 1, acy[flat], flat, 0.0f);
-          v = DequantWith(dq, c, acrow[flat], flat, yv);
-          if (P.predict_lf) {
-            v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
-          }
+    const size_t row0 = size_t(f.bya) * P.bx * 64 + size_t(f.bxa) * 64;
+    const int16_t* acrow = B.ac + size_t(c) * plane + row0;
+    const int16_t* acy = B.ac + plane + row0;
+#pragma unroll
+    for (int y = 0; y < N; ++y) {
+      float x;
+      if (y < S && l < S) {
+        x = llf[(f.byg + 1 + y) * kLlfPitch + f.bxg + 1 + l];
+      } else {
+        // F[y][l] is flat y * 8S + l, in storage block bxa + flat / 64
+        const int flat = y * 8 * S + l;
+        const int sb = flat >> 6, k = flat & 63;
+        float yv = 0.0f;
+        if (c != 1) {
+          yv = DequantWith(DescBlockDq(P, B, 1, f.bya, f.bxa + sb), 1,
+                           acy[flat], k, 0.0f);
         }
-        F[i * 2 + j] = v;
+        x = DequantWith(DescBlockDq(P, B, c, f.bya, f.bxa + sb), c,
+                        acrow[flat], k, yv);
+        if (P.predict_lf) {
+          x += lf[(2 * (f.byg + 1) + y) * kG2Pitch + 2 * (f.bxg + 1) + l];
+        }
       }
+      // ReinterpretingIDCT<8S, 2S, 2S>'s scaling
+      v[y] = (x * K->ascale[ai][y][l]) * K->tscale[ti][y][l];
     }
-    float* o = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane +
-               2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
-    Idct2x2(F[0], F[1], F[2], F[3], o, kG2Pitch);
+    dct1d::Idct<N>(v, w);
+#pragma unroll
+    for (int x = 0; x < N; ++x) t[l * (N + 1) + x] = w[x];
   }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int cc = 0; cc < N; ++cc) v[cc] = t[cc * (N + 1) + l];
+  dct1d::Idct<N>(v, w);
+  float* o = B.pred2x2 + (size_t(f.g) * 3 + c) * kG2Plane +
+             2 * (f.byg + 1) * kG2Pitch + 2 * (f.bxg + 1);
+#pragma unroll
+  for (int y = 0; y < N; ++y) o[y * kG2Pitch + l] = w[y];
+}
+
+// Interior of pred2x2 over the first blocks of pred_list, binned on the
+// host by class (DCT8-like, 2x2 Hadamard, DCT16, DCT32; P.pred_end holds
+// the class ends). One launch in three segments, the longest work first:
+// workgroups [0, wg4) take the DCT32 blocks, 32 units (block x plane) of 8
+// lanes; [wg4, wg4 + wg2) the DCT16 blocks, 64 units of 4 lanes; the rest
+// one lane per unit for the DCT8-like and Hadamard classes. The three
+// planes of a block are adjacent units (X and B read the block's Y
+// coefficients and descriptors with the Y unit).
+__global__ void __launch_bounds__(256)
+    KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                     int wg4, int wg2) {
+  __shared__ float s_t[32 * 8 * 9];
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x;
+  if (wg < wg4) {
+    const int unit = wg * 32 + (tid >> 3);
+    const int kb = unit / 3, c = unit - 3 * kb;
+    const bool active = kb < P.pred_end[3] - P.pred_end[2];
+    const int blk = active ? int(B.pred_list[P.pred_end[2] + kb]) : 0;
+    Pred2x2Lanes<4>(P, B, K, active, c, blk, tid & 7,
+                    s_t + (tid >> 3) * 8 * 9);
+    return;
+  }
+  if (wg < wg4 + wg2) {
+    const int unit = (wg - wg4) * 64 + (tid >> 2);
+    const int kb = unit / 3, c = unit - 3 * kb;
+    const bool active = kb < P.pred_end[2] - P.pred_end[1];
+    const int blk = active ? int(B.pred_list[P.pred_end[1] + kb]) : 0;
+    Pred2x2Lanes<2>(P, B, K, active, c, blk, tid & 3,
+                    s_t + (tid >> 2) * 4 * 5);
+    return;
+  }
+  const int lane = (wg - wg4 - wg2) * 256 + tid;
+  const int k = lane / 3, c = lane - 3 * k;
+  if (k >= P.pred_end[1]) return;
+  const FirstBlock f = LocateBlock(P, int(B.pred_list[k]));
+  if (k < P.pred_end[0]) {
+    Pred2x2Block<1>(P, B, K, c, f.bxa, f.bya, f.bxg, f.byg, f.g);
+    return;
+  }
+  // DCT2X2, DCT4X4(_NOHF), IDENTITY: 2x2 Hadamard of the LF corner.
+  const int bxa = f.bxa, bya = f.bya, bxg = f.bxg, byg = f.byg, g = f.g;
+  const BlockDq dq = DescBlockDq(P, B, c, bya, bxa);
+  BlockDq dqy;
+  if (c != 1) dqy = DescBlockDq(P, B, 1, bya, bxa);
+  const size_t plane = size_t(P.by) * P.bx * 64;
+  const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
+  const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
+  float F[4];
+  for (int i = 0; i < 2; ++i) {
+    for (int j = 0; j < 2; ++j) {
+      float v;
+      if (i == 0 && j == 0) {
+        v = llf[(byg + 1) * kLlfPitch + bxg + 1];
+      } else {
+        const int flat = i * 8 + j;
+        float yv = 0.0f;
+        if (c != 1) yv = DequantWith(dqy, 1, acy[flat], flat, 0.0f);
+        v = DequantWith(dq, c, acrow[flat], flat, yv);
+        if (P.predict_lf) {
+          v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
+        }
+      }
+      F[i * 2 + j] = v;
+    }
+  }
+  float* o = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane +
+             2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
+  Idct2x2(F[0], F[1], F[2], F[3], o, kG2Pitch);
 }
 
 }  // namespace pikgpu
