@@ -606,6 +606,11 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->predlist.p, pred_list.data(),
                               pred_list.size() * 4, hipMemcpyHostToDevice,
                               ctx->stream));
+  // pred2x2 positions that no block writes (ZeroFillImage,
+  // pik/compressed_image.cc:976): the same ones on every run of this frame,
+  // so once here rather than per run
+  HIP_TRY(ctx, hipMemsetAsync(ctx->pred2x2.p, 0, ngroups * 3 * kG2Plane * 4,
+                              ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
   FrameBuffers& B = ctx->B;
@@ -722,18 +727,17 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
   const int g2_pos = 4 * cache_pos;
   dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
   dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
-  KComputeLlf<<<gc, 256, 0, st>>>(P, B, K);
+  KComputeLlf<<<gc, 256, 0, st>>>(P, B, K, P.predict_lf);
   if (P.predict_lf) {
-    KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 0);
     KLfBlur<<<g2, 256, 0, st>>>(P, B);
     KLfFromDc2x2<<<gc, 256, 0, st>>>(P, B, K);
   }
   if (hf) {
-    KZeroPred<<<DivCeil(ngroups * 3 * kG2Plane, 256), 256, 0, st>>>(P, B);
+    // (positions of pred2x2 no block writes were zeroed at upload)
     if (P.predict_lf) {
       KPred2x2Border<<<gc, 256, 0, st>>>(P, B, K);
     } else {
-      KDc2x2FromLlf<<<gc, 256, 0, st>>>(P, B, K, 1);
+      KDc2x2Border<<<gc, 256, 0, st>>>(P, B, K);
     }
     if (P.pred_end[3] > 0) {
       // segments: DCT32 (32 units of 8 lanes per workgroup), DCT16 (64 of

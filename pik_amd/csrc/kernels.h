@@ -513,12 +513,12 @@ __device__ inline int XcdTile(int wg, int num_tiles) {
 // ---------------------------------------------------------------------------
 // Kernels.
 __global__ void KDcDequant(FrameParams P, FrameBuffers B);
-__global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K);
-__global__ void KDc2x2FromLlf(FrameParams P, FrameBuffers B,
-                              const DeviceConsts* K, int to_pred2x2);
+__global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                            int dc2x2);
+__global__ void KDc2x2Border(FrameParams P, FrameBuffers B,
+                             const DeviceConsts* K);
 __global__ void KLfBlur(FrameParams P, FrameBuffers B);
 __global__ void KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K);
-__global__ void KZeroPred(FrameParams P, FrameBuffers B);
 __global__ void KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                                  int wg4, int wg2);

@@ -155,19 +155,12 @@ __device__ inline void ReinterpIdct(const DeviceConsts* K, const float* in,
 }
 
 // ---------------------------------------------------------------------------
-// ComputeLlf: DC -> lowest-frequency coefficients per cache position
-// (LowestFrequenciesFromDC, pik/ac_strategy.cc:850-887).
-__global__ void __launch_bounds__(256)
-    KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
-  const GroupGeom G = Geom(P, g);
-  const int W = G.gw + 2, H = G.gh + 2;
-  if (pos >= W * H) return;
-  const int x = pos % W, y = pos / W;
-  const uint8_t a = CacheAcs(P, B, G, x, y);
-  if ((a & 15) != 0) return;  // written by the first block
-  const int raw = a >> 4;
+// ComputeLlf: DC -> lowest-frequency coefficients of the first block at
+// cache position (x, y) (LowestFrequenciesFromDC, pik/ac_strategy.cc:
+// 850-887).
+__device__ inline void LlfAt(const FrameParams& P, const FrameBuffers& B,
+                             const DeviceConsts* K, const GroupGeom& G, int g,
+                             int c, int x, int y, int raw) {
   const float* dcp = B.dc + c * P.bx * P.by;
   float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
   if (raw == kDCT16 || raw == kDCT32) {
@@ -200,24 +193,11 @@ __global__ void __launch_bounds__(256)
   llf[0] = GroupDc(P, dcp, G, x, y);
 }
 
-// DC2x2FromLowestFrequencies into tmp2x2 or the border of pred2x2
-// (pik/ac_strategy.cc:928-969).
-__global__ void __launch_bounds__(256)
-    KDc2x2FromLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                  int to_pred2x2) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
-  const GroupGeom G = Geom(P, g);
-  const int W = G.gw + 2, H = G.gh + 2;
-  if (pos >= W * H) return;
-  const int x = pos % W, y = pos / W;
-  const uint8_t a = CacheAcs(P, B, G, x, y);
-  // Without LF prediction only the border is wanted (the interior of
-  // pred2x2 is filled by KPred2x2Interior).
-  const bool border = x == 0 || y == 0 || x == W - 1 || y == H - 1;
-  if (to_pred2x2 && !border) return;
-  if ((a & 15) != 0) return;
-  const int raw = a >> 4;
+// DC2x2FromLowestFrequencies of the first block at cache position (x, y)
+// into tmp2x2 or pred2x2 (pik/ac_strategy.cc:928-969).
+__device__ inline void Dc2x2At(const FrameParams& P, const FrameBuffers& B,
+                               const DeviceConsts* K, int g, int c, int x,
+                               int y, int raw, bool to_pred2x2) {
   const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
   float* o = (to_pred2x2 ? B.pred2x2 : B.tmp2x2) + (size_t(g) * 3 + c) * kG2Plane +
              2 * y * kG2Pitch + 2 * x;
@@ -238,6 +218,40 @@ __global__ void __launch_bounds__(256)
     o[kG2Pitch] = v;
     o[kG2Pitch + 1] = v;
   }
+}
+
+// ComputeLlf per cache position; with dc2x2 the same thread then turns its
+// block's LLF values into tmp2x2 (the first step of PredictLf), reading
+// back what it wrote.
+__global__ void __launch_bounds__(256)
+    KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                int dc2x2) {
+  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y, c = blockIdx.z;
+  const GroupGeom G = Geom(P, g);
+  const int W = G.gw + 2, H = G.gh + 2;
+  if (pos >= W * H) return;
+  const int x = pos % W, y = pos / W;
+  const uint8_t a = CacheAcs(P, B, G, x, y);
+  if ((a & 15) != 0) return;  // written by the first block
+  LlfAt(P, B, K, G, g, c, x, y, a >> 4);
+  if (dc2x2) Dc2x2At(P, B, K, g, c, x, y, a >> 4, false);
+}
+
+// The border of pred2x2 from the LLF cache without LF prediction
+// (pik/compressed_image.cc:980-1039; the interior is KPred2x2Interior's).
+__global__ void __launch_bounds__(256)
+    KDc2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y, c = blockIdx.z;
+  const GroupGeom G = Geom(P, g);
+  const int W = G.gw + 2, H = G.gh + 2;
+  if (pos >= W * H) return;
+  const int x = pos % W, y = pos / W;
+  if (!(x == 0 || y == 0 || x == W - 1 || y == H - 1)) return;
+  const uint8_t a = CacheAcs(P, B, G, x, y);
+  if ((a & 15) != 0) return;
+  Dc2x2At(P, B, K, g, c, x, y, a >> 4, true);
 }
 
 // 3x3 LF prediction blur with mirrored borders (Symmetric3,
@@ -297,14 +311,6 @@ __global__ void __launch_bounds__(256)
     p[kG2Pitch] = d[2];
     p[kG2Pitch + 1] = d[3];
   }
-}
-
-// Zero-fills pred2x2 (ZeroFillImage, pik/compressed_image.cc:976).
-__global__ void __launch_bounds__(256)
-    KZeroPred(FrameParams P, FrameBuffers B) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t n = size_t(P.gx) * P.gy * 3 * kG2Plane;
-  if (i < n) B.pred2x2[i] = 0.0f;
 }
 
 // Border of pred2x2 from llf and lf2x2 as DCT8 blocks
