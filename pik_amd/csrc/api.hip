@@ -728,9 +728,14 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
   dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
   dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
   KComputeLlf<<<gc, 256, 0, st>>>(P, B, K, P.predict_lf);
+  // DCT32 / DCT16 first blocks on 8 / 4 lanes (32 / 64 units of block x
+  // plane per workgroup), then the cache positions
+  const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
+  const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
   if (P.predict_lf) {
     KLfBlur<<<g2, 256, 0, st>>>(P, B);
-    KLfFromDc2x2<<<gc, 256, 0, st>>>(P, B, K);
+    KLfFromDc2x2<<<wg4 + wg2 + gc.x * gc.y * gc.z, 256, 0, st>>>(
+        P, B, K, wg4, wg2, int(gc.x), int(ngroups));
   }
   if (hf) {
     // (positions of pred2x2 no block writes were zeroed at upload)
@@ -740,10 +745,7 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
       KDc2x2Border<<<gc, 256, 0, st>>>(P, B, K);
     }
     if (P.pred_end[3] > 0) {
-      // segments: DCT32 (32 units of 8 lanes per workgroup), DCT16 (64 of
-      // 4), then one lane per unit
-      const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
-      const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+      // segments as above, then one lane per unit
       const int wg1 = int(DivCeil(3 * P.pred_end[1], 256));
       KPred2x2Interior<<<wg4 + wg2 + wg1, 256, 0, st>>>(P, B, K, wg4, wg2);
     }

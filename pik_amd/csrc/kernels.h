@@ -518,7 +518,8 @@ __global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K
 __global__ void KDc2x2Border(FrameParams P, FrameBuffers B,
                              const DeviceConsts* K);
 __global__ void KLfBlur(FrameParams P, FrameBuffers B);
-__global__ void KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K);
+__global__ void KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                             int wg4, int wg2, int pos_blocks, int ngroups);
 __global__ void KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                                  int wg4, int wg2);

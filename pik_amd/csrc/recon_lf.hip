@@ -271,11 +271,81 @@ __global__ void __launch_bounds__(256)
                  P.lfblur_w[2]);
 }
 
+// ReinterpDct<8S, 2S> in place on the 2S x 2S block at p (pitch kG2Pitch)
+// on N = 2S lanes: lane l runs the first pass of Dct2D over column l into
+// row l of the unit's LDS matrix t (pitch N + 1), then, after the barrier,
+// the second pass over column l of t, scaling and writing column l. Every
+// element sees the one-lane form's operations in the same order. All
+// threads of the workgroup call; `active` is false past the last unit.
+template <int S>
+__device__ inline void ReinterpDctLanes(const DeviceConsts* K, bool active,
+                                        float* p, int l, float* t) {
+  constexpr int N = 2 * S;
+  constexpr int ti = ScaleIdx<N>(), ai = ScaleIdx<8 * S>();
+  float v[N], w[N];
+  if (active) {
+#pragma unroll
+    for (int y = 0; y < N; ++y) v[y] = p[y * kG2Pitch + l];
+    dct1d::Dct<N>(v, w);
+#pragma unroll
+    for (int k = 0; k < N; ++k) t[l * (N + 1) + k] = w[k];
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int x = 0; x < N; ++x) v[x] = t[x * (N + 1) + l];
+  dct1d::Dct<N>(v, w);
+  const float scale = 1.0f / (N * N);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const float cf = w[j] * scale;
+    const float u = cf * K->tscale[ti][j][l];
+    p[j * kG2Pitch + l] = u * K->ascale[ai][j][l];
+  }
+}
+
 // LowFrequenciesFromDC2x2, in place on lf2x2 (pik/ac_strategy.cc:1019-1064).
+// One launch: workgroups [0, wg4) take the DCT32 first blocks of pred_list
+// (32 units of block x plane on 8 lanes each), [wg4, wg4 + wg2) the DCT16
+// ones (64 units of 4 lanes), the rest one lane per cache position of
+// (position block, group, plane) for the other strategies.
 __global__ void __launch_bounds__(256)
-    KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
+    KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                 int wg4, int wg2, int pos_blocks, int ngroups) {
+  __shared__ float s_t[32 * 8 * 9];
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x;
+  if (wg < wg4 + wg2) {
+    const bool big = wg < wg4;
+    const int lanes = big ? 8 : 4;
+    const int unit = big ? wg * 32 + (tid >> 3) : (wg - wg4) * 64 + (tid >> 2);
+    const int kb = unit / 3, c = unit - 3 * kb;
+    const int first = big ? P.pred_end[2] : P.pred_end[1];
+    const int n = big ? P.pred_end[3] - P.pred_end[2]
+                      : P.pred_end[2] - P.pred_end[1];
+    const bool active = kb < n;
+    float* p = nullptr;
+    if (active) {
+      const int blk = int(B.pred_list[first + kb]);
+      const int bxa = blk % P.bx, bya = blk / P.bx;
+      const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
+      const int g = gyi * P.gx + gxi;
+      const int x = bxa - gxi * kGroupDimBlocks + 1;
+      const int y = bya - gyi * kGroupDimBlocks + 1;
+      p = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane + 2 * y * kG2Pitch + 2 * x;
+    }
+    if (big) {
+      ReinterpDctLanes<4>(K, active, p, tid & 7, s_t + (tid >> 3) * 8 * 9);
+    } else {
+      ReinterpDctLanes<2>(K, active, p, tid & 3, s_t + (tid >> 2) * 4 * 5);
+    }
+    (void)lanes;
+    return;
+  }
+  const int r = wg - wg4 - wg2;
+  const int pb = r % pos_blocks, g = (r / pos_blocks) % ngroups;
+  const int c = r / (pos_blocks * ngroups);
+  const int pos = pb * 256 + tid;
   const GroupGeom G = Geom(P, g);
   const int W = G.gw + 2, H = G.gh + 2;
   if (pos >= W * H) return;
@@ -283,15 +353,8 @@ __global__ void __launch_bounds__(256)
   const uint8_t a = CacheAcs(P, B, G, x, y);
   if ((a & 15) != 0) return;
   const int raw = a >> 4;
+  if (raw == kDCT16 || raw == kDCT32) return;  // the lane segments
   float* p = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane + 2 * y * kG2Pitch + 2 * x;
-  if (raw == kDCT16) {
-    ReinterpDct<16, 4>(K, p, kG2Pitch, p, kG2Pitch);
-    return;
-  }
-  if (raw == kDCT32) {
-    ReinterpDct<32, 8>(K, p, kG2Pitch, p, kG2Pitch);
-    return;
-  }
   const float b00 = p[0], b01 = p[1], b10 = p[kG2Pitch], b11 = p[kG2Pitch + 1];
   const float d[4] = {(b00 + b01 + b10 + b11) * 0.25f,
                       (b00 + b01 - b10 - b11) * 0.25f,
