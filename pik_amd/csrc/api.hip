@@ -727,11 +727,12 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
   const int g2_pos = 4 * cache_pos;
   dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
   dim3 g2(DivCeil(g2_pos, 256), ngroups, 3);
-  KComputeLlf<<<gc, 256, 0, st>>>(P, B, K, P.predict_lf);
   // DCT32 / DCT16 first blocks on 8 / 4 lanes (32 / 64 units of block x
   // plane per workgroup), then the cache positions
   const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
   const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+  KComputeLlf<<<wg4 + wg2 + gc.x * gc.y * gc.z, 256, 0, st>>>(
+      P, B, K, P.predict_lf, wg4, wg2, int(gc.x), int(ngroups));
   if (P.predict_lf) {
     KLfBlur<<<g2, 256, 0, st>>>(P, B);
     KLfFromDc2x2<<<wg4 + wg2 + gc.x * gc.y * gc.z, 256, 0, st>>>(

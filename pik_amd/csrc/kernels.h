@@ -514,7 +514,8 @@ __device__ inline int XcdTile(int wg, int num_tiles) {
 // Kernels.
 __global__ void KDcDequant(FrameParams P, FrameBuffers B);
 __global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                            int dc2x2);
+                            int dc2x2, int wg4, int wg2, int pos_blocks,
+                            int ngroups);
 __global__ void KDc2x2Border(FrameParams P, FrameBuffers B,
                              const DeviceConsts* K);
 __global__ void KLfBlur(FrameParams P, FrameBuffers B);

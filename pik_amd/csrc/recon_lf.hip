@@ -220,22 +220,147 @@ __device__ inline void Dc2x2At(const FrameParams& P, const FrameBuffers& B,
   }
 }
 
-// ComputeLlf per cache position; with dc2x2 the same thread then turns its
-// block's LLF values into tmp2x2 (the first step of PredictLf), reading
-// back what it wrote.
+// LlfAt and, with dc2x2, Dc2x2At of a DCT16 (S = 2) or DCT32 (S = 4) first
+// block on N = 2S lanes, each element with the one-lane form's operations
+// in the same order. LLF: for S = 4 the 4x4 ReinterpretingDCT runs its
+// column pass on lanes 0..3 into the unit's LDS matrix t, then its row pass
+// there, lane l ending with LLF column l; for S = 2 every lane recomputes
+// the 2x2 Hadamard and keeps its column. The 2x image: the column pass of
+// Idct2D<N> over the zero-padded LLF, lane l on column l, into t, then the
+// row pass on column l of t. All threads of the workgroup call (barriers);
+// `active` is false past the last unit.
+template <int S>
+__device__ inline void LlfDc2x2Lanes(const FrameParams& P,
+                                     const FrameBuffers& B,
+                                     const DeviceConsts* K, bool active, int c,
+                                     int blk, int l, bool dc2x2, float* t) {
+  constexpr int N = 2 * S;
+  int g = 0, x = 0, y = 0;
+  GroupGeom G = {0, 0, 0, 0};
+  if (active) {
+    const int bxa = blk % P.bx, bya = blk / P.bx;
+    const int gxi = bxa / kGroupDimBlocks, gyi = bya / kGroupDimBlocks;
+    g = gyi * P.gx + gxi;
+    x = bxa - gxi * kGroupDimBlocks + 1;
+    y = bya - gyi * kGroupDimBlocks + 1;
+    G = Geom(P, g);
+  }
+  const float* dcp = B.dc + c * P.bx * P.by;
+  float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
+  float col[S];  // LLF column l (lanes l < S)
+  if (S == 4) {
+    // ReinterpretingDCT<32, 4, 4> of the 4 x 4 DC values
+    float v[4], w[4];
+    if (active && l < 4) {
+#pragma unroll
+      for (int iy = 0; iy < 4; ++iy) v[iy] = GroupDc(P, dcp, G, x + l, y + iy);
+      dct1d::Dct<4>(v, w);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t[l * 5 + k] = w[k];
+    }
+    __syncthreads();
+    if (active && l < 4) {
+#pragma unroll
+      for (int xx = 0; xx < 4; ++xx) v[xx] = t[xx * 5 + l];
+      dct1d::Dct<4>(v, w);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float cf = w[j] * (1.0f / 16);
+        const float u = cf * K->tscale[1][j][l];
+        col[j] = u * K->ascale[4][j][l];
+        llf[j * kLlfPitch + l] = col[j];
+      }
+    }
+    __syncthreads();  // t is reused below
+  } else {
+    if (active) {
+      const float a00 = GroupDc(P, dcp, G, x, y);
+      const float a01 = GroupDc(P, dcp, G, x + 1, y);
+      const float a10 = GroupDc(P, dcp, G, x, y + 1);
+      const float a11 = GroupDc(P, dcp, G, x + 1, y + 1);
+      const float d[4] = {(a00 + a01 + a10 + a11) * 0.25f,
+                          (a00 + a01 - a10 - a11) * 0.25f,
+                          (a00 - a01 + a10 - a11) * 0.25f,
+                          (a00 - a01 - a10 + a11) * 0.25f};
+      if (l < 2) {
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy) {
+          const float v = d[iy * 2 + l] * K->tscale[0][iy][l];
+          col[iy] = v * K->ascale[3][iy][l];
+          llf[iy * kLlfPitch + l] = col[iy];
+        }
+      }
+    }
+  }
+  if (!dc2x2) return;  // the same for the whole launch
+  // ReinterpretingIDCT<8S, S, 2S>
+  constexpr int ai = ScaleIdx<N>(), ti = ScaleIdx<8 * S>();
+  float v[N], w[N];
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      v[r] = (r < S && l < S) ? (col[r < S ? r : 0] * K->ascale[ai][r][l]) *
+                                    K->tscale[ti][r][l]
+                              : 0.0f;
+    }
+    dct1d::Idct<N>(v, w);
+#pragma unroll
+    for (int k = 0; k < N; ++k) t[l * (N + 1) + k] = w[k];
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll
+  for (int cc = 0; cc < N; ++cc) v[cc] = t[cc * (N + 1) + l];
+  dct1d::Idct<N>(v, w);
+  float* o = B.tmp2x2 + (size_t(g) * 3 + c) * kG2Plane + 2 * y * kG2Pitch + 2 * x;
+#pragma unroll
+  for (int yy = 0; yy < N; ++yy) o[yy * kG2Pitch + l] = w[yy];
+}
+
+// ComputeLlf (and with dc2x2 the LLF's 2x image in tmp2x2, the first step
+// of PredictLf). One launch: workgroups [0, wg4) take the DCT32 first
+// blocks of pred_list (32 units of block x plane on 8 lanes), [wg4, wg4 +
+// wg2) the DCT16 ones (64 units of 4 lanes), the rest one lane per cache
+// position of (position block, group, plane); there the thread turns its
+// block's LLF values into the 2x image reading back what it wrote.
 __global__ void __launch_bounds__(256)
     KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                int dc2x2) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
+                int dc2x2, int wg4, int wg2, int pos_blocks, int ngroups) {
+  __shared__ float s_t[32 * 8 * 9];
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x;
+  if (wg < wg4 + wg2) {
+    const bool big = wg < wg4;
+    const int unit = big ? wg * 32 + (tid >> 3) : (wg - wg4) * 64 + (tid >> 2);
+    const int kb = unit / 3, c = unit - 3 * kb;
+    const int first = big ? P.pred_end[2] : P.pred_end[1];
+    const int n = big ? P.pred_end[3] - P.pred_end[2]
+                      : P.pred_end[2] - P.pred_end[1];
+    const bool active = kb < n;
+    const int blk = active ? int(B.pred_list[first + kb]) : 0;
+    if (big) {
+      LlfDc2x2Lanes<4>(P, B, K, active, c, blk, tid & 7, dc2x2 != 0,
+                       s_t + (tid >> 3) * 8 * 9);
+    } else {
+      LlfDc2x2Lanes<2>(P, B, K, active, c, blk, tid & 3, dc2x2 != 0,
+                       s_t + (tid >> 2) * 4 * 5);
+    }
+    return;
+  }
+  const int r = wg - wg4 - wg2;
+  const int pb = r % pos_blocks, g = (r / pos_blocks) % ngroups;
+  const int c = r / (pos_blocks * ngroups);
+  const int pos = pb * 256 + tid;
   const GroupGeom G = Geom(P, g);
   const int W = G.gw + 2, H = G.gh + 2;
   if (pos >= W * H) return;
   const int x = pos % W, y = pos / W;
   const uint8_t a = CacheAcs(P, B, G, x, y);
   if ((a & 15) != 0) return;  // written by the first block
-  LlfAt(P, B, K, G, g, c, x, y, a >> 4);
-  if (dc2x2) Dc2x2At(P, B, K, g, c, x, y, a >> 4, false);
+  const int raw = a >> 4;
+  if (raw == kDCT16 || raw == kDCT32) return;  // the lane segments
+  LlfAt(P, B, K, G, g, c, x, y, raw);
+  if (dc2x2) Dc2x2At(P, B, K, g, c, x, y, raw, false);
 }
 
 // The border of pred2x2 from the LLF cache without LF prediction
