@@ -969,9 +969,14 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
           const float corr = s_coef[c][CoefAt(crow, ccol + j)];
           const float m1 = lower < corr ? corr : lower;
           const float clamped = m1 < upper ? m1 : upper;
-          const float divisor = __float_as_uint(corr) == 0u ? 1E-7f : corr;
-          const float cr = clamped / divisor;
-          ratio = cr < ratio ? cr : ratio;
+          // An entry inside the interval (clamped == corr != 0) has the
+          // ratio corr / corr = 1 exactly, which leaves ratio (<= 1) as it
+          // is: only clamped and zero entries need the division.
+          if (!(clamped == corr && corr != 0.0f)) {
+            const float divisor = __float_as_uint(corr) == 0u ? 1E-7f : corr;
+            const float cr = clamped / divisor;
+            ratio = cr < ratio ? cr : ratio;
+          }
         }
         for (int c = 0; c < 3; ++c) {
           s_coef[c][CoefAt(crow, ccol + j)] =

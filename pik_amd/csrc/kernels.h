@@ -135,6 +135,7 @@ struct TileTasks {
                     // channel (positions past the last transform: off = total)
   uint8_t raw[16];  // strategy at each position (0xFF past the last)
   uint8_t blk[16];  // tile block (by * 4 + bx) at each position
+  int has_dct4;     // a DCT4X4 transform is among them
 };
 
 __device__ inline int TaskCount(int raw) {
@@ -196,7 +197,11 @@ __device__ inline void BuildTasks(const uint8_t* acs_t, bool hf_only,
       T->raw[b] = 0xFF;
       T->blk[b] = 0;
     }
-    if (b == 0) T->off[16] = total;
+    const unsigned long long m4 = __ballot(kind == 3);
+    if (b == 0) {
+      T->off[16] = total;
+      T->has_dct4 = m4 != 0;
+    }
   }
   __syncthreads();
 }
@@ -415,6 +420,7 @@ constexpr int kTaskSlots = 3;
 constexpr uint32_t kNoTask = 0xFFFFFFFFu;
 struct ThreadTasks {
   uint32_t t[kTaskSlots];
+  bool has_dct4;  // TileTasks::has_dct4 (the same for the workgroup)
   // slot i by selects (a variable index into t would put it in scratch)
   __device__ uint32_t Slot(int i) const {
     return i == 0 ? t[0] : (i == 1 ? t[1] : t[2]);
@@ -424,6 +430,7 @@ struct ThreadTasks {
 __device__ inline ThreadTasks AssignTasks(const TileTasks& T) {
   const int total = T.off[16];
   ThreadTasks m;
+  m.has_dct4 = T.has_dct4 != 0;
 #pragma unroll
   for (int i = 0; i < kTaskSlots; ++i) {
     const int k = threadIdx.x + 256 * i;
@@ -468,7 +475,9 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
 template <bool kAdjoint = false, bool kAddHF = false>
 __device__ inline void TileForward3(const ThreadTasks& m, const float* pix,
                                     float* coef, float* tmp) {
-  for (int pass = 0; pass < (kAddHF ? 2 : 3); ++pass) {
+  // pass 2 (the DCT4X4 DC Hadamard) only when the tile has a DCT4X4
+  const int passes = kAddHF ? 2 : (m.has_dct4 ? 3 : 2);
+  for (int pass = 0; pass < passes; ++pass) {
 #pragma unroll 1
     for (int i = 0; i < kTaskSlots; ++i) {
       const uint32_t w = m.Slot(i);
