@@ -235,6 +235,18 @@ enum {
  * instead of 12. */
 PIKGPU_API int pikgpu_set_output(pikgpu_ctx* ctx, int outputs);
 
+/* Selects the downsampling factor of the following pikgpu_run calls: 1
+ * (full size, the default), 2, 4 or 8. Replaces the `downsample` argument
+ * of ReconOpsinImage and FinalizeFrameDecoding (pik/compressed_image.h:
+ * 95-110; InverseIntegralTransform, pik/compressed_image.cc:869-934, and
+ * FinalizeFrameDecoding, :1144-1181): each transform's pixels are averaged
+ * over factor x factor squares, without adaptive reconstruction or Gaborish,
+ * and the outputs of pikgpu_download / pikgpu_download_srgb8 are
+ * ceil(xsize / factor) x ceil(ysize / factor). As in the reference a frame
+ * with dots cannot be downsampled (pikgpu_run returns PIKGPU_EUNSUPPORTED),
+ * and stage dumps need factor 1 (PIKGPU_EINVAL). */
+PIKGPU_API int pikgpu_set_downsample(pikgpu_ctx* ctx, int factor);
+
 /* Copies the 8-bit sRGB output of the last run (xsize x ysize pixels, rows
  * of 3 * xsize bytes) to host memory with the given row stride. */
 PIKGPU_API int pikgpu_download_srgb8(pikgpu_ctx* ctx, uint8_t* rgb,

@@ -1673,3 +1673,75 @@ done:
   free(mem);
   return rc;
 }
+
+/* Downsampled decode (ReconOpsinImage / FinalizeFrameDecoding with
+ * downsample = ds, pik/compressed_image.cc:869-934, 1144-1181): the
+ * reconstruction up to the IDCT as at full size, each transform's pixels
+ * averaged over ds x ds squares (the running sum in row order, times
+ * 1 / ds^2; the squares are aligned to the transform, which is 8-aligned),
+ * no AR and no Gaborish, dots refused (BlockDictionary::Apply checks there
+ * are none), then ShrinkTo(ceil(xsize / ds), ceil(ysize / ds)) and
+ * OpsinToLinear. `out`: 3 planes of ceil(xsize / ds) x ceil(ysize / ds). */
+int pikoracle_recon_downsampled(const pikgpu_frame* f, int ds,
+                                float* const out[3]) {
+  init_tables();
+  int rc = check_frame(f);
+  if (rc) return rc;
+  if (ds != 2 && ds != 4 && ds != 8) {
+    return fail(PIKGPU_EINVAL, "downsampling factor must be 2, 4 or 8");
+  }
+  if (f->num_dots != 0) {
+    return fail(PIKGPU_EUNSUPPORTED, "dots with downsampling");
+  }
+  Ctx k;
+  k.f = f;
+  k.bx = (int)div_ceil(f->xsize, 8);
+  k.by = (int)div_ceil(f->ysize, 8);
+  k.px = 8 * k.bx;
+  k.py = 8 * k.by;
+  k.tx = (int)div_ceil(f->xsize, 64);
+  k.inv_global_scale = (float)(1.0 * 65536 / f->global_scale);
+  k.inv_quant_dc = k.inv_global_scale / f->quant_dc;
+  k.table = f->dequant_table;
+  k.offsets = f->dequant_offsets;
+
+  const size_t nb = (size_t)k.bx * k.by;
+  const size_t np = (size_t)k.px * k.py;
+  float* mem = (float*)calloc(6 * nb + 3 * nb * 64 + 3 * np, sizeof(float));
+  if (!mem) return fail(PIKGPU_ENOMEM, "out of memory");
+  float *dcq[3], *dc[3], *ac[3], *idct[3];
+  for (int c = 0; c < 3; ++c) {
+    dcq[c] = mem + c * nb;
+    dc[c] = mem + (3 + c) * nb;
+    ac[c] = mem + 6 * nb + c * nb * 64;
+    idct[c] = mem + 6 * nb + 3 * nb * 64 + c * np;
+  }
+  if ((rc = reconstruct_dc(&k, dcq, dc))) goto done;
+  dequant_ac(&k, ac);
+  for (int gy = 0; gy < (int)div_ceil(k.by, kGroupBlocks); ++gy) {
+    for (int gx = 0; gx < (int)div_ceil(k.bx, kGroupBlocks); ++gx) {
+      if ((rc = reconstruct_group(&k, gx, gy, dc, ac, idct))) goto done;
+    }
+  }
+  {
+    const size_t ox = div_ceil(f->xsize, ds), oy = div_ceil(f->ysize, ds);
+    const float mean_mul = 1.0f / (float)(ds * ds);
+    for (int c = 0; c < 3; ++c) {
+      for (size_t y = 0; y < oy; ++y) {
+        for (size_t x = 0; x < ox; ++x) {
+          float sum = 0.0f;
+          for (int iy = 0; iy < ds; ++iy) {
+            for (int ix = 0; ix < ds; ++ix) {
+              sum += idct[c][(y * ds + iy) * k.px + x * ds + ix];
+            }
+          }
+          out[c][y * ox + x] = sum * mean_mul;
+        }
+      }
+    }
+    opsin_to_linear(out, ox * oy);
+  }
+done:
+  free(mem);
+  return rc;
+}

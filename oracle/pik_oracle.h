@@ -28,6 +28,11 @@ extern "C" {
 int pikoracle_recon(const pikgpu_frame* frame, float* const* stages,
                     float* const out[3]);
 
+/* The downsampled decode for ds = 2, 4, 8 (no AR, no Gaborish, no dots):
+ * `out` receives 3 planes of ceil(xsize / ds) x ceil(ysize / ds) floats. */
+int pikoracle_recon_downsampled(const pikgpu_frame* frame, int ds,
+                                float* const out[3]);
+
 const char* pikoracle_last_error(void);
 
 /* One block transform of raw strategy `raw`: dir 0 = TransformToPixels,

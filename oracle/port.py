@@ -27,6 +27,8 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.pikoracle_recon.argtypes = [ctypes.POINTER(abi.FrameC),
                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.pikoracle_recon_downsampled.argtypes = [
+            ctypes.POINTER(abi.FrameC), ctypes.c_int, ctypes.c_void_p]
         L.pikoracle_last_error.restype = ctypes.c_char_p
         L.pikoracle_block_transform.argtypes = [ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_void_p,
@@ -69,6 +71,22 @@ def recon(frame, stages=True):
         raise RuntimeError("pikoracle_recon: %s"
                            % L.pikoracle_last_error().decode())
     return out, st
+
+
+def recon_downsampled(frame, ds):
+    """Downsampled decode (ds = 2, 4, 8): (3, ceil(H/ds), ceil(W/ds))
+    float32."""
+    L = lib()
+    fc, keep = frame.to_c()
+    oy, ox = -(-frame.ysize // ds), -(-frame.xsize // ds)
+    out = np.zeros((3, oy, ox), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    rc = L.pikoracle_recon_downsampled(ctypes.byref(fc), ds,
+                                       ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikoracle_recon_downsampled: %s"
+                           % L.pikoracle_last_error().decode())
+    return out
 
 
 def transform(raw, direction, block):

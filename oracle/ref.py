@@ -35,6 +35,9 @@ def lib(kind="scalar"):
     L.pikref_default_tables.argtypes = [
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    L.pikref_recon_downsampled.argtypes = [ctypes.POINTER(abi.FrameC),
+                                           ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p]
     L.pikref_epf_weight.restype = ctypes.c_float
     L.pikref_epf_weight.argtypes = [ctypes.c_int, ctypes.c_int]
     L.pikref_transform.argtypes = [ctypes.c_int, ctypes.c_int,
@@ -93,6 +96,22 @@ def recon(frame, threads=0, stages=True, kind="scalar"):
     if rc != 0:
         raise RuntimeError("pikref_recon: %s" % L.pikref_last_error().decode())
     return out, st
+
+
+def recon_downsampled(frame, ds, threads=0, kind="scalar"):
+    """The reference's downsampled decode (ds = 2, 4, 8): (3, ceil(H/ds),
+    ceil(W/ds)) float32."""
+    L = lib(kind)
+    fc, keep = frame.to_c()
+    oy, ox = -(-frame.ysize // ds), -(-frame.xsize // ds)
+    out = np.zeros((3, oy, ox), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    rc = L.pikref_recon_downsampled(ctypes.byref(fc), threads, ds,
+                                    ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikref_recon_downsampled: %s"
+                           % L.pikref_last_error().decode())
+    return out
 
 
 def transform(raw, direction, block):

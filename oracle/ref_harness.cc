@@ -198,9 +198,19 @@ API int pikref_default_tables(float* table, size_t table_cap,
 // PIKGPU_NUM_STAGES entries of 3 plane pointers (each may be NULL) receiving
 // dense images of the sizes documented in pikgpu.h. `out` receives the final
 // linear image (xsize x ysize per plane, dense). Returns 0 on success.
-API int pikref_recon(const pikgpu_frame* f, int num_threads,
-                     float* const* stages, float* const* out) {
+// The reference reconstruction; downsample 2, 4 or 8 takes
+// ReconOpsinImage's downsampled IDCT and FinalizeFrameDecoding's
+// downsampled branch (no AR, no Gaborish, ShrinkTo the downsampled size).
+static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
+                 float* const* stages, float* const* out) {
   if (f == nullptr) return Fail("null frame");
+  if (downsample != 1 && downsample != 2 && downsample != 4 &&
+      downsample != 8) {
+    return Fail("downsample must be 1, 2, 4 or 8");
+  }
+  if (downsample != 1 && (stages != nullptr || f->num_dots != 0)) {
+    return Fail("downsampled decode: no stage dumps, no dots");
+  }
   InitEpfOnce();
   std::unique_ptr<ThreadPool> pool = MakePool(num_threads);
   const size_t xsize = f->xsize, ysize = f->ysize;
@@ -335,7 +345,8 @@ API int pikref_recon(const pikgpu_frame* f, int num_threads,
   }
 
   // Groups.
-  Image3F idct(bx * kBlockDim, by * kBlockDim);
+  Image3F idct(DivCeil(bx * kBlockDim, downsample),
+               DivCeil(by * kBlockDim, downsample));
   Image3F coeffs_dump;
   const bool want_coeffs =
       stage(PIKGPU_STAGE_COEFFS) != nullptr;
@@ -377,8 +388,12 @@ API int pikref_recon(const pikgpu_frame* f, int num_threads,
         InitializeDecCache(fdc, padded_rect, gdc);
         DequantImageAC(quant, cmap, group_in_color_tiles, group_ac, &fdc, gdc,
                        padded_rect, nullptr);
+        const Rect opsin_rect(padded_rect.x0() / downsample,
+                              padded_rect.y0() / downsample,
+                              DivCeil(padded_rect.xsize(), downsample),
+                              DivCeil(padded_rect.ysize(), downsample));
         ReconOpsinImage(frame_header, group_header, quant, block_group_rect,
-                        &fdc, gdc, &idct, padded_rect, nullptr);
+                        &fdc, gdc, &idct, opsin_rect, nullptr, downsample);
         if (want_coeffs) {
           for (int c = 0; c < 3; ++c) {
             for (size_t y = 0; y < block_group_rect.ysize(); ++y) {
@@ -395,6 +410,17 @@ API int pikref_recon(const pikgpu_frame* f, int num_threads,
     DumpImage3F(coeffs_dump, stage(PIKGPU_STAGE_COEFFS), bx * 64, by);
   }
   DumpImage3F(idct, stage(PIKGPU_STAGE_IDCT), bx * 8, by * 8);
+
+  if (downsample != 1) {
+    // FinalizeFrameDecoding's downsampled branch: the dictionary is empty
+    // (checked above), no AR, no Gaborish.
+    const size_t ox = DivCeil<size_t>(xsize, downsample);
+    const size_t oy = DivCeil<size_t>(ysize, downsample);
+    idct.ShrinkTo(ox, oy);
+    OpsinToLinear(&idct, pool.get());
+    DumpImage3F(idct, out, ox, oy);
+    return 0;
+  }
 
   // FinalizeFrameDecoding steps (pik/compressed_image.cc:1110-1181).
   if (frame_header.have_adaptive_reconstruction) {
@@ -439,6 +465,18 @@ API int pikref_recon(const pikgpu_frame* f, int num_threads,
   DumpImage3F(idct, stage(PIKGPU_STAGE_LINEAR), xsize, ysize);
   DumpImage3F(idct, out, xsize, ysize);
   return 0;
+}
+
+API int pikref_recon(const pikgpu_frame* f, int num_threads,
+                     float* const* stages, float* const* out) {
+  return Recon(f, num_threads, 1, stages, out);
+}
+
+// Downsampled decode: `out` receives 3 planes of ceil(xsize / downsample) x
+// ceil(ysize / downsample).
+API int pikref_recon_downsampled(const pikgpu_frame* f, int num_threads,
+                                 int downsample, float* const* out) {
+  return Recon(f, num_threads, size_t(downsample), nullptr, out);
 }
 
 // Reference EPF weight for (sigma, sad) (WeightFast via MulTable,

@@ -63,8 +63,11 @@ struct pikgpu_ctx {
   bool ran = false;
   bool kept = false;
   int out_flags = kOutLinear;
+  int downsample = 1;
   int ran_flags = 0;
   int spitch = 0;
+  int out_xsize = 0, out_ysize = 0;  // output size of the last run
+  bool ran_downsampled = false;
   bool timed = false;   // the last run recorded timing events
   FrameParams P;
   FrameBuffers B;
@@ -469,6 +472,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   P.ar = f->adaptive_reconstruction != 0;
   P.use_sharpened = f->epf_use_sharpened != 0;
   P.num_dots = int(f->num_dots);
+  P.downsample = 1;  // per run (pikgpu_set_downsample)
   // Quantizer::Decode (pik/quantizer.cc:63-71).
   P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
   P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
@@ -766,9 +770,22 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
   FrameBuffers B = ctx->B;
   B.keep = keep_stages != 0;
   P.out_flags = ctx->out_flags;
+  P.downsample = ctx->downsample;
+  const int ds = P.downsample;
+  if (ds != 1) {
+    if (keep_stages) {
+      return Fail(ctx, PIKGPU_EINVAL, "stage dumps need downsampling factor 1");
+    }
+    if (P.num_dots != 0) {
+      return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                  "a frame with dots cannot be downsampled");
+    }
+  }
+  // output size (ShrinkTo of the reference's final stage)
+  const int out_xs = int(DivCeil(P.xsize, ds)), out_ys = int(DivCeil(P.ysize, ds));
   if (P.out_flags & kOutSrgb8) {
-    B.spitch = int(DivCeil(size_t(P.xsize) * 3, 64) * 64);
-    if ((rc = Grow(ctx, &ctx->srgb8, size_t(B.spitch) * P.ysize))) return rc;
+    B.spitch = int(DivCeil(size_t(out_xs) * 3, 64) * 64);
+    if ((rc = Grow(ctx, &ctx->srgb8, size_t(B.spitch) * out_ys))) return rc;
     B.srgb8 = static_cast<uint8_t*>(ctx->srgb8.p);
   }
   const DeviceConsts* K = ctx->consts;
@@ -784,6 +801,35 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     KBlocks<<<grid, 256, 0, st>>>(P, B, K);
   }
   if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
+  if (ds != 1) {
+    // The downsampled branch of FinalizeFrameDecoding: KBlocks wrote the
+    // averaged image into B.idct (planes of (py / ds) x ipitch); no AR, no
+    // Gaborish, no dots; ShrinkTo and OpsinToLinear in KFinal.
+    if (timing) {
+      for (int e = 4; e <= 6; ++e) HIP_TRY(ctx, hipEventRecord(ctx->ev[e], st));
+    }
+    FrameParams Pd = P;
+    Pd.px = P.px / ds;
+    Pd.py = P.py / ds;
+    Pd.xsize = out_xs;
+    Pd.ysize = out_ys;
+    Pd.gaborish = 0;
+    FrameBuffers Bd = B;
+    Bd.ar = B.idct;
+    const int grid =
+        XcdGrid(int(DivCeil(Pd.px, 256) * DivCeil(Pd.py, kFinalRows)));
+    KFinal<<<grid, 256, 0, st>>>(Pd, Bd, K);
+    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
+    HIP_TRY(ctx, hipGetLastError());
+    ctx->ran = true;
+    ctx->kept = false;
+    ctx->ran_flags = P.out_flags;
+    ctx->spitch = B.spitch;
+    ctx->out_xsize = out_xs;
+    ctx->out_ysize = out_ys;
+    ctx->ran_downsampled = true;
+    return PIKGPU_OK;
+  }
   if (P.ar) {
     Img ii;
     ii.p = B.idct;
@@ -837,6 +883,9 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
   ctx->kept = B.keep;
   ctx->ran_flags = P.out_flags;
   ctx->spitch = B.spitch;
+  ctx->out_xsize = out_xs;
+  ctx->out_ysize = out_ys;
+  ctx->ran_downsampled = false;
   return PIKGPU_OK;
 }
 
@@ -884,20 +933,28 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
       !out_planes[2]) {
     return Fail(ctx, PIKGPU_EINVAL, "null output plane");
   }
-  const FrameParams& P = ctx->P;
-  if (out_stride_bytes < size_t(P.xsize) * 4) {
+  const size_t xs = size_t(ctx->out_xsize), ys = size_t(ctx->out_ysize);
+  if (out_stride_bytes < xs * 4) {
     return Fail(ctx, PIKGPU_EINVAL, "output stride smaller than a row");
   }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const size_t oplane = size_t(ctx->B.opitch) * P.ysize;
+  const size_t oplane = size_t(ctx->B.opitch) * ys;
   for (int c = 0; c < 3; ++c) {
     HIP_TRY(ctx, hipMemcpy2DAsync(out_planes[c], out_stride_bytes,
                                   ctx->B.out + c * oplane,
-                                  size_t(ctx->B.opitch) * 4,
-                                  size_t(P.xsize) * 4, P.ysize,
+                                  size_t(ctx->B.opitch) * 4, xs * 4, ys,
                                   hipMemcpyDeviceToHost, ctx->stream));
   }
   return pikgpu_sync(ctx);
+}
+
+int pikgpu_set_downsample(pikgpu_ctx* ctx, int factor) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (factor != 1 && factor != 2 && factor != 4 && factor != 8) {
+    return Fail(ctx, PIKGPU_EINVAL, "downsampling factor must be 1, 2, 4 or 8");
+  }
+  ctx->downsample = factor;
+  return PIKGPU_OK;
 }
 
 int pikgpu_set_output(pikgpu_ctx* ctx, int outputs) {
@@ -915,14 +972,14 @@ int pikgpu_download_srgb8(pikgpu_ctx* ctx, uint8_t* rgb, size_t stride_bytes) {
   if (!(ctx->ran_flags & kOutSrgb8)) {
     return Fail(ctx, PIKGPU_ESTATE, "the last run produced no 8-bit output");
   }
-  const FrameParams& P = ctx->P;
-  if (rgb == nullptr || stride_bytes < size_t(P.xsize) * 3) {
+  const size_t xs = size_t(ctx->out_xsize), ys = size_t(ctx->out_ysize);
+  if (rgb == nullptr || stride_bytes < xs * 3) {
     return Fail(ctx, PIKGPU_EINVAL, "bad output buffer or stride");
   }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipMemcpy2DAsync(rgb, stride_bytes, ctx->srgb8.p,
-                                size_t(ctx->spitch), size_t(P.xsize) * 3,
-                                P.ysize, hipMemcpyDeviceToHost, ctx->stream));
+                                size_t(ctx->spitch), xs * 3, ys,
+                                hipMemcpyDeviceToHost, ctx->stream));
   return pikgpu_sync(ctx);
 }
 
@@ -971,6 +1028,10 @@ int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
   uint32_t xs, ys;
   int rc = pikgpu_stage_size(ctx, stage, &xs, &ys);
   if (rc) return rc;
+  if (ctx->ran && ctx->ran_downsampled && stage != PIKGPU_STAGE_DC_DEQUANT &&
+      stage != PIKGPU_STAGE_DC) {
+    return Fail(ctx, PIKGPU_ESTATE, "the last run was downsampled");
+  }
   if (!ctx->ran || (!ctx->kept && stage != PIKGPU_STAGE_DC_DEQUANT &&
                     stage != PIKGPU_STAGE_DC && stage != PIKGPU_STAGE_IDCT &&
                     stage != PIKGPU_STAGE_AR && stage != PIKGPU_STAGE_LINEAR)) {

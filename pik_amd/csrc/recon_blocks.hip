@@ -216,16 +216,42 @@ __global__ void __launch_bounds__(256, 4)
   }
   // 4. IDCT (InverseIntegralTransform, pik/compressed_image.cc:869-895)
   TileInverse3(s_inv, &s_coef[0][0], s_pix, &s_tmp[0][0]);
-  const int py = tby * 8 + pr;
-  if (py < P.py) {
-    for (int c = 0; c < 3; ++c) {
-      float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + pc0;
-      for (int j = 0; j < 4; ++j) {
-        if (tbx * 8 + pc0 + j < P.px) {
-          dst[j] = s_pix[c * kTilePlane + PixAt(pr, pc0 + j)];
+  if (P.downsample == 1) {
+    const int py = tby * 8 + pr;
+    if (py < P.py) {
+      for (int c = 0; c < 3; ++c) {
+        float* dst = B.idct + (size_t(c) * P.py + py) * B.ipitch + tbx * 8 + pc0;
+        for (int j = 0; j < 4; ++j) {
+          if (tbx * 8 + pc0 + j < P.px) {
+            dst[j] = s_pix[c * kTilePlane + PixAt(pr, pc0 + j)];
+          }
         }
       }
     }
+    return;
+  }
+  // Downsampled decode (InverseIntegralTransform with downsample,
+  // pik/compressed_image.cc:896-933): each ds x ds square of a transform's
+  // pixels (transforms are 8-aligned, so the squares are tile-aligned) is
+  // summed in row order and scaled by 1 / ds^2, into planes of
+  // (py / ds) x ipitch.
+  const int ds = P.downsample, n = 32 / ds;
+  const float mean_mul = 1.0f / float(ds * ds);
+  const int ox0 = tbx * 8 / ds, oy0 = tby * 8 / ds;
+  const int opx = P.px / ds, opy = P.py / ds;
+  const size_t oplane = size_t(opy) * B.ipitch;
+  for (int k = tid; k < 3 * n * n; k += 256) {
+    const int c = k / (n * n), rem = k - c * (n * n);
+    const int oy = rem / n, ox = rem - oy * n;
+    if (ox0 + ox >= opx || oy0 + oy >= opy) continue;
+    float sum = 0.0f;
+    for (int iy = 0; iy < ds; ++iy) {
+      for (int ix = 0; ix < ds; ++ix) {
+        sum += s_pix[c * kTilePlane + PixAt(oy * ds + iy, ox * ds + ix)];
+      }
+    }
+    B.idct[c * oplane + size_t(oy0 + oy) * B.ipitch + ox0 + ox] =
+        sum * mean_mul;
   }
 }
 

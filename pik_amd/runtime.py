@@ -41,6 +41,7 @@ class Reconstructor:
                               % device)
         self._frame = None
         self._keep = None
+        self._ds = 1
 
     def close(self):
         if self._ctx:
@@ -80,14 +81,25 @@ class Reconstructor:
     def sync(self):
         self._check(self._lib.pikgpu_sync(self._ctx))
 
+    def _out_size(self):
+        f, ds = self._frame, self._ds
+        return -(-f.xsize // ds), -(-f.ysize // ds)
+
     def download(self):
-        f = self._frame
-        out = np.empty((3, f.ysize, f.xsize), np.float32)
+        """(3, ysize, xsize) float32 (ceil(size / factor) when
+        downsampled)."""
+        xs, ys = self._out_size()
+        out = np.empty((3, ys, xs), np.float32)
         planes = (abi.c_float_p * 3)(
             *[out[c].ctypes.data_as(abi.c_float_p) for c in range(3)])
-        self._check(self._lib.pikgpu_download(self._ctx, planes,
-                                              f.xsize * 4))
+        self._check(self._lib.pikgpu_download(self._ctx, planes, xs * 4))
         return out
+
+    def set_downsample(self, factor):
+        """Downsampling factor (1, 2, 4, 8) of the following runs
+        (pikgpu_set_downsample)."""
+        self._check(self._lib.pikgpu_set_downsample(self._ctx, int(factor)))
+        self._ds = int(factor)
 
     def set_output(self, outputs):
         """abi.OUTPUT_LINEAR | abi.OUTPUT_SRGB8 for the following runs."""
@@ -95,10 +107,10 @@ class Reconstructor:
 
     def download_srgb8(self):
         """(ysize, xsize, 3) uint8 sRGB of the last run."""
-        f = self._frame
-        out = np.empty((f.ysize, f.xsize, 3), np.uint8)
+        xs, ys = self._out_size()
+        out = np.empty((ys, xs, 3), np.uint8)
         self._check(self._lib.pikgpu_download_srgb8(
-            self._ctx, out.ctypes.data, f.xsize * 3))
+            self._ctx, out.ctypes.data, xs * 3))
         return out
 
     def reconstruct(self, frame):

@@ -1,4 +1,5 @@
-"""Writes the golden reconstruction fixtures tests/golden/recon_*.npz.
+"""Writes the golden reconstruction fixtures tests/golden/recon_*.npz and
+the downsampled-decode fixtures tests/golden/downsample_*.npz.
 
 Each fixture holds a synthetic frame (the decoder state after entropy
 decoding, as pik_amd.frame.Frame fields) and what the reference's own
@@ -37,6 +38,18 @@ CASES = {
 }
 STAGES = ("dc_dequant", "dc", "idct", "ar")
 
+# Downsampled decode (factors 2, 4, 8; frames without dots, as the
+# reference requires): name -> (xsize, ysize, make_frame kwargs, overrides).
+# Written to downsample_<name>.npz with the outputs ds2, ds4, ds8.
+DS_CASES = {
+    "default": (200, 136, dict(seed=21, num_dots=0), {}),
+    "ragged_no_lf": (83, 45, dict(seed=22, num_dots=0),
+                     dict(predict_lf=0)),
+    "no_predict": (120, 72, dict(seed=23, predict=False, num_dots=0),
+                   dict(gaborish=abi.GABORISH_1000)),
+}
+DS_FACTORS = (2, 4, 8)
+
 
 def make(name):
     xs, ys, kw, hdr = CASES[name]
@@ -47,7 +60,27 @@ def make(name):
     return fr, out, st
 
 
+def make_downsampled(name):
+    xs, ys, kw, hdr = DS_CASES[name]
+    fr = synth.make_frame(xs, ys, **kw)
+    for k, v in hdr.items():
+        setattr(fr, k, v)
+    outs = {ds: ref.recon_downsampled(fr, ds, threads=1, kind="scalar")
+            for ds in DS_FACTORS}
+    return fr, outs
+
+
 def main():
+    for name in DS_CASES:
+        fr, outs = make_downsampled(name)
+        path = os.path.join(HERE, "downsample_%s.npz" % name)
+        d = {}
+        for k, v in vars(fr).items():
+            d["frame_" + k] = v if isinstance(v, np.ndarray) else np.array(v)
+        for ds, out in outs.items():
+            d["ds%d" % ds] = out
+        np.savez_compressed(path, **d)
+        print("wrote", path, os.path.getsize(path), "bytes")
     for name in CASES:
         fr, out, st = make(name)
         path = os.path.join(HERE, "recon_%s.npz" % name)

@@ -29,6 +29,25 @@ def load_golden(name):
     return Frame(**kw).contiguous(), want
 
 
+def downsample_names():
+    return sorted(os.path.basename(p)[len("downsample_"):-len(".npz")]
+                  for p in glob.glob(os.path.join(GOLDEN, "downsample_*.npz")))
+
+
+def load_downsample(name):
+    """Returns (Frame, {factor: (3, ceil(H / factor), ceil(W / factor))})
+    of a downsampled-decode fixture."""
+    z = np.load(os.path.join(GOLDEN, "downsample_%s.npz" % name),
+                allow_pickle=False)
+    kw = {}
+    for k in z.files:
+        if k.startswith("frame_"):
+            v = z[k]
+            kw[k[len("frame_"):]] = v if v.ndim else v.item()
+    want = {int(k[2:]): z[k] for k in z.files if k.startswith("ds")}
+    return Frame(**kw).contiguous(), want
+
+
 def checker_recon(frame, stages=True, threads=1):
     """The oracle's reconstruction of `frame`: the reference's own code
     (scalar build under oracle/_ref) where it is built, else the C
