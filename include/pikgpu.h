@@ -235,6 +235,19 @@ enum {
  * instead of 12. */
 PIKGPU_API int pikgpu_set_output(pikgpu_ctx* ctx, int outputs);
 
+/* Adds an AC pass to the uploaded frame (multi-pass accumulation): after
+ * pikgpu_frame_upload, call once per further pass, in pass order. ac[c] is
+ * plane c of the pass's quantized coefficients in the layout of
+ * pikgpu_frame.ac (rows of ac_stride int16). Replaces the decoding of
+ * passes 1 .. num_passes - 1 with Dequant<false>::DoAC
+ * (DecodeFromBitstream<false>, pik/pik_frame.cc:903-917;
+ * pik/compressed_image.cc:575-703): each coefficient is the first pass's
+ * dequantised value plus, in order, every later pass's own (the colour
+ * correlation applied per pass). At most 16 passes; a new upload starts
+ * again from one. pikgpu_fwd_adj takes single-pass frames only. */
+PIKGPU_API int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
+                                     size_t ac_stride);
+
 /* Selects the downsampling factor of the following pikgpu_run calls: 1
  * (full size, the default), 2, 4 or 8. Replaces the `downsample` argument
  * of ReconOpsinImage and FinalizeFrameDecoding (pik/compressed_image.h:

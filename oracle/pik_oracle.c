@@ -942,6 +942,10 @@ typedef struct {
   float inv_global_scale, inv_quant_dc;
   const float* table;
   const uint32_t* offsets;
+  /* passes after the frame's own AC: extra[3 * p + c], rows of extra_stride */
+  int num_extra;
+  const int16_t* const* extra;
+  size_t extra_stride;
 } Ctx;
 
 static const float* dequant_matrix(const Ctx* k, int table, int kind,
@@ -983,18 +987,27 @@ static void dequant_ac(const Ctx* k, float* const ac[3]) {
       for (int c = 0; c < 3; ++c) m[c] = dequant_matrix(k, table, kind, c) + idx * 64;
       for (int i = 0; i < 64; ++i) {
         const size_t at = (size_t)bx * 64 + i;
-        float q[3];
-        for (int c = 0; c < 3; ++c) {
-          q[c] = (float)f->ac[c][by * f->ac_stride + at];
+        /* Dequant<first>::DoAC per pass (pik/compressed_image.cc:575-703):
+         * the first pass stores, later ones add their own dequantised
+         * value to what is there. */
+        for (int p = 0; p <= k->num_extra; ++p) {
+          float q[3];
+          for (int c = 0; c < 3; ++c) {
+            q[c] = p == 0 ? (float)f->ac[c][by * f->ac_stride + at]
+                          : (float)k->extra[3 * (p - 1) + c]
+                                           [by * k->extra_stride + at];
+          }
+          const float xm = m[0][i] * scaled, ym = m[1][i] * scaled,
+                      bm = m[2][i] * scaled;
+          const float dx = adjust_quant_bias(0, q[0]) * xm;
+          const float dy = adjust_quant_bias(1, q[1]) * ym;
+          const float db = adjust_quant_bias(2, q[2]) * bm;
+          const float v[3] = {x_cc * dy + dx, dy, b_cc * dy + db};
+          for (int c = 0; c < 3; ++c) {
+            float* dst = &ac[c][by * row + at];
+            *dst = p == 0 ? v[c] : v[c] + *dst;
+          }
         }
-        const float xm = m[0][i] * scaled, ym = m[1][i] * scaled,
-                    bm = m[2][i] * scaled;
-        const float dx = adjust_quant_bias(0, q[0]) * xm;
-        const float dy = adjust_quant_bias(1, q[1]) * ym;
-        const float db = adjust_quant_bias(2, q[2]) * bm;
-        ac[0][by * row + at] = x_cc * dy + dx;
-        ac[1][by * row + at] = dy;
-        ac[2][by * row + at] = b_cc * dy + db;
       }
     }
   }
@@ -1581,13 +1594,36 @@ static int check_frame(const pikgpu_frame* f) {
   return 0;
 }
 
+static int recon(const pikgpu_frame* f, int num_extra,
+                 const int16_t* const* extra, size_t extra_stride,
+                 float* const* stages, float* const out[3]);
+
 int pikoracle_recon(const pikgpu_frame* f, float* const* stages,
                     float* const out[3]) {
+  return recon(f, 0, NULL, 0, stages, out);
+}
+
+int pikoracle_recon_passes(const pikgpu_frame* f, int num_extra,
+                           const int16_t* const* extra_ac,
+                           size_t extra_stride, float* const* stages,
+                           float* const out[3]) {
+  if (num_extra < 0 || (num_extra > 0 && extra_ac == NULL)) {
+    return fail(PIKGPU_EINVAL, "bad extra passes");
+  }
+  return recon(f, num_extra, extra_ac, extra_stride, stages, out);
+}
+
+static int recon(const pikgpu_frame* f, int num_extra,
+                 const int16_t* const* extra, size_t extra_stride,
+                 float* const* stages, float* const out[3]) {
   init_tables();
   int rc = check_frame(f);
   if (rc) return rc;
   Ctx k;
   k.f = f;
+  k.num_extra = num_extra;
+  k.extra = extra;
+  k.extra_stride = extra_stride;
   k.bx = (int)div_ceil(f->xsize, 8);
   k.by = (int)div_ceil(f->ysize, 8);
   k.px = 8 * k.bx;
@@ -1695,6 +1731,9 @@ int pikoracle_recon_downsampled(const pikgpu_frame* f, int ds,
   }
   Ctx k;
   k.f = f;
+  k.num_extra = 0;
+  k.extra = NULL;
+  k.extra_stride = 0;
   k.bx = (int)div_ceil(f->xsize, 8);
   k.by = (int)div_ceil(f->ysize, 8);
   k.px = 8 * k.bx;

@@ -28,6 +28,13 @@ extern "C" {
 int pikoracle_recon(const pikgpu_frame* frame, float* const* stages,
                     float* const out[3]);
 
+/* pikoracle_recon with AC passes after the frame's own (multi-pass
+ * accumulation): extra_ac[3 * p + c] is plane c of extra pass p, rows of
+ * extra_stride int16 in the frame's AC layout. */
+int pikoracle_recon_passes(const pikgpu_frame* frame, int num_extra,
+                           const int16_t* const* extra_ac, size_t extra_stride,
+                           float* const* stages, float* const out[3]);
+
 /* The downsampled decode for ds = 2, 4, 8 (no AR, no Gaborish, no dots):
  * `out` receives 3 planes of ceil(xsize / ds) x ceil(ysize / ds) floats. */
 int pikoracle_recon_downsampled(const pikgpu_frame* frame, int ds,

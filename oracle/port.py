@@ -27,6 +27,9 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.pikoracle_recon.argtypes = [ctypes.POINTER(abi.FrameC),
                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.pikoracle_recon_passes.argtypes = [
+            ctypes.POINTER(abi.FrameC), ctypes.c_int, ctypes.c_void_p,
+            ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
         L.pikoracle_recon_downsampled.argtypes = [
             ctypes.POINTER(abi.FrameC), ctypes.c_int, ctypes.c_void_p]
         L.pikoracle_last_error.restype = ctypes.c_char_p
@@ -71,6 +74,28 @@ def recon(frame, stages=True):
         raise RuntimeError("pikoracle_recon: %s"
                            % L.pikoracle_last_error().decode())
     return out, st
+
+
+def recon_passes(frame, extra):
+    """Reconstruction with AC passes after the frame's own: extra is a list
+    of (3, by, bx * 64) int16 arrays. Returns the linear (3, H, W)."""
+    L = lib()
+    fc, keep = frame.to_c()
+    extra = [np.ascontiguousarray(e, np.int16) for e in extra]
+    ptrs = (ctypes.c_void_p * max(1, 3 * len(extra)))()
+    for p, e in enumerate(extra):
+        for c in range(3):
+            ptrs[3 * p + c] = e[c].ctypes.data
+    stride = extra[0].shape[2] if extra else 0
+    out = np.zeros((3, frame.ysize, frame.xsize), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    rc = L.pikoracle_recon_passes(ctypes.byref(fc), len(extra),
+                                  ctypes.cast(ptrs, ctypes.c_void_p), stride,
+                                  None, ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikoracle_recon_passes: %s"
+                           % L.pikoracle_last_error().decode())
+    return out
 
 
 def recon_downsampled(frame, ds):

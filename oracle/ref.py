@@ -35,6 +35,10 @@ def lib(kind="scalar"):
     L.pikref_default_tables.argtypes = [
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    L.pikref_recon_passes.argtypes = [ctypes.POINTER(abi.FrameC), ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_size_t, ctypes.c_void_p,
+                                      ctypes.c_void_p]
     L.pikref_recon_downsampled.argtypes = [ctypes.POINTER(abi.FrameC),
                                            ctypes.c_int, ctypes.c_int,
                                            ctypes.c_void_p]
@@ -96,6 +100,28 @@ def recon(frame, threads=0, stages=True, kind="scalar"):
     if rc != 0:
         raise RuntimeError("pikref_recon: %s" % L.pikref_last_error().decode())
     return out, st
+
+
+def recon_passes(frame, extra, threads=0, kind="scalar"):
+    """The reference reconstruction with AC passes after the frame's own
+    (extra: list of (3, by, bx * 64) int16). Returns the linear (3, H, W)."""
+    L = lib(kind)
+    fc, keep = frame.to_c()
+    extra = [np.ascontiguousarray(e, np.int16) for e in extra]
+    ptrs = (ctypes.c_void_p * max(1, 3 * len(extra)))()
+    for p, e in enumerate(extra):
+        for c in range(3):
+            ptrs[3 * p + c] = e[c].ctypes.data
+    stride = extra[0].shape[2] if extra else 0
+    out = np.zeros((3, frame.ysize, frame.xsize), np.float32)
+    out_ptrs = (ctypes.c_void_p * 3)(*[out[c].ctypes.data for c in range(3)])
+    rc = L.pikref_recon_passes(ctypes.byref(fc), threads, len(extra),
+                               ctypes.cast(ptrs, ctypes.c_void_p), stride,
+                               None, ctypes.cast(out_ptrs, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikref_recon_passes: %s"
+                           % L.pikref_last_error().decode())
+    return out
 
 
 def recon_downsampled(frame, ds, threads=0, kind="scalar"):

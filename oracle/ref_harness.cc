@@ -202,7 +202,9 @@ API int pikref_default_tables(float* table, size_t table_cap,
 // ReconOpsinImage's downsampled IDCT and FinalizeFrameDecoding's
 // downsampled branch (no AR, no Gaborish, ShrinkTo the downsampled size).
 static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
-                 float* const* stages, float* const* out) {
+                 float* const* stages, float* const* out, int num_extra = 0,
+                 const int16_t* const* extra_ac = nullptr,
+                 size_t extra_stride = 0) {
   if (f == nullptr) return Fail("null frame");
   if (downsample != 1 && downsample != 2 && downsample != 4 &&
       downsample != 8) {
@@ -388,6 +390,34 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
         InitializeDecCache(fdc, padded_rect, gdc);
         DequantImageAC(quant, cmap, group_in_color_tiles, group_ac, &fdc, gdc,
                        padded_rect, nullptr);
+        // Later passes: Dequant<false>::DoAC stores each coefficient's
+        // dequantised value of the pass plus the value already there
+        // (pik/compressed_image.cc:695-703); DequantImageAC (the first-pass
+        // form) computes the pass's values into a second cache.
+        for (int p = 0; p < num_extra; ++p) {
+          for (int c = 0; c < 3; ++c) {
+            for (size_t y = 0; y < block_group_rect.ysize(); ++y) {
+              memcpy(group_ac.PlaneRow(c, y),
+                     extra_ac[3 * p + c] +
+                         (block_group_rect.y0() + y) * extra_stride +
+                         block_group_rect.x0() * 64,
+                     block_group_rect.xsize() * 64 * sizeof(int16_t));
+            }
+          }
+          GroupDecCache pass_cache;
+          InitializeDecCache(fdc, padded_rect, &pass_cache);
+          DequantImageAC(quant, cmap, group_in_color_tiles, group_ac, &fdc,
+                         &pass_cache, padded_rect, nullptr);
+          for (int c = 0; c < 3; ++c) {
+            for (size_t y = 0; y < block_group_rect.ysize(); ++y) {
+              const float* src = pass_cache.ac.ConstPlaneRow(c, y);
+              float* dst = gdc->ac.PlaneRow(c, y);
+              for (size_t x = 0; x < block_group_rect.xsize() * 64; ++x) {
+                dst[x] = src[x] + dst[x];
+              }
+            }
+          }
+        }
         const Rect opsin_rect(padded_rect.x0() / downsample,
                               padded_rect.y0() / downsample,
                               DivCeil(padded_rect.xsize(), downsample),
@@ -470,6 +500,19 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
 API int pikref_recon(const pikgpu_frame* f, int num_threads,
                      float* const* stages, float* const* out) {
   return Recon(f, num_threads, 1, stages, out);
+}
+
+// With AC passes after the frame's own: extra_ac[3 * p + c] is plane c of
+// extra pass p (rows of extra_stride int16 in the frame's AC layout).
+API int pikref_recon_passes(const pikgpu_frame* f, int num_threads,
+                            int num_extra, const int16_t* const* extra_ac,
+                            size_t extra_stride, float* const* stages,
+                            float* const* out) {
+  if (num_extra < 0 || (num_extra > 0 && extra_ac == nullptr)) {
+    return Fail("bad extra passes");
+  }
+  return Recon(f, num_threads, 1, stages, out, num_extra, extra_ac,
+               extra_stride);
 }
 
 // Downsampled decode: `out` receives 3 planes of ceil(xsize / downsample) x

@@ -169,6 +169,7 @@ def load():
                                              ctypes.c_void_p, ctypes.c_void_p]
     lib.pikgpu_set_output.argtypes = [vp, ctypes.c_int]
     lib.pikgpu_set_downsample.argtypes = [vp, ctypes.c_int]
+    lib.pikgpu_frame_add_pass.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
     lib.pikgpu_download_srgb8.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
     lib.pikgpu_default_dequant_tables.argtypes = [
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
@@ -190,4 +191,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample",
+    "pikgpu_frame_add_pass",
 ]

@@ -42,6 +42,9 @@ struct DevBuf {
 // Most workgroups of a KMinMaxPartial launch (6 floats of partials each).
 constexpr int kMinMaxMaxGroups = 4096;
 
+// Most AC passes of a frame (pikgpu_frame_add_pass).
+constexpr int kMaxPasses = 16;
+
 }  // namespace
 
 struct pikgpu_ctx {
@@ -52,7 +55,7 @@ struct pikgpu_ctx {
   HostTables host;
   // device buffers (grown on demand)
   DevBuf ac, dc16, qf, acs, lut, dcf, ytox, ytob, dqmap, table, offsets, dots,
-      dotbins;
+      dotbins, ac_extra;
   DevBuf desc, dcq, dcv, llf, tmp2x2, lf2x2, pred2x2, coeffs, idct, guide, epf, ar,
       dots_img, gab, out, mm_partial, mm, ar_keep, srgb8, gimg, predlist;
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
@@ -109,7 +112,8 @@ int Grow(pikgpu_ctx* ctx, DevBuf* b, size_t bytes) {
 }
 
 void FreeAll(pikgpu_ctx* ctx) {
-  DevBuf* all[] = {&ctx->ac,      &ctx->dc16,   &ctx->qf,      &ctx->acs,
+  DevBuf* all[] = {&ctx->ac,      &ctx->ac_extra, &ctx->dc16, &ctx->qf,
+                   &ctx->acs,
                    &ctx->lut,     &ctx->dcf,    &ctx->ytox,    &ctx->ytob,
                    &ctx->dqmap,   &ctx->table,  &ctx->offsets, &ctx->dots,
                    &ctx->dotbins, &ctx->srgb8,
@@ -473,6 +477,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   P.use_sharpened = f->epf_use_sharpened != 0;
   P.num_dots = int(f->num_dots);
   P.downsample = 1;  // per run (pikgpu_set_downsample)
+  P.num_passes = 1;  // more with pikgpu_frame_add_pass
   // Quantizer::Decode (pik/quantizer.cc:63-71).
   P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
   P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
@@ -620,6 +625,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   FrameBuffers& B = ctx->B;
   memset(&B, 0, sizeof(B));
   B.ac = static_cast<const int16_t*>(ctx->ac.p);
+  B.ac_extra = nullptr;
   B.dc16 = static_cast<const int16_t*>(ctx->dc16.p);
   B.qf = static_cast<const int32_t*>(ctx->qf.p);
   B.acs = static_cast<const uint8_t*>(ctx->acs.p);
@@ -948,6 +954,56 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
   return pikgpu_sync(ctx);
 }
 
+int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
+                          size_t ac_stride) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  FrameParams& P = ctx->P;
+  if (ac == nullptr || !ac[0] || !ac[1] || !ac[2]) {
+    return Fail(ctx, PIKGPU_EINVAL, "null AC plane");
+  }
+  const size_t row = size_t(P.bx) * 64;
+  if (ac_stride < row) return Fail(ctx, PIKGPU_EINVAL, "AC stride too small");
+  if (P.num_passes >= kMaxPasses) {
+    return Fail(ctx, PIKGPU_EINVAL, "too many passes");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t pass_bytes = 3 * row * P.by * sizeof(int16_t);
+  const size_t extra = size_t(P.num_passes - 1);
+  const size_t need = (extra + 1) * pass_bytes;
+  if (ctx->ac_extra.cap < need) {
+    // grow keeping the passes already there
+    DevBuf nb;
+    hipError_t e = hipMalloc(&nb.p, need);
+    if (e != hipSuccess) {
+      return Fail(ctx, PIKGPU_ENOMEM,
+                  std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    nb.cap = need;
+    if (extra > 0) {
+      HIP_TRY(ctx, hipMemcpyAsync(nb.p, ctx->ac_extra.p, extra * pass_bytes,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (ctx->ac_extra.p) (void)hipFree(ctx->ac_extra.p);
+    ctx->ac_extra = nb;
+  }
+  int16_t* dst = static_cast<int16_t*>(ctx->ac_extra.p) + extra * 3 * row * P.by;
+  int rc;
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = Upload2D(ctx, dst + c * row * P.by, ac[c],
+                       ac_stride * sizeof(int16_t), row * sizeof(int16_t),
+                       P.by))) {
+      return rc;
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  P.num_passes += 1;
+  ctx->B.ac_extra = static_cast<const int16_t*>(ctx->ac_extra.p);
+  ctx->ran = false;
+  return PIKGPU_OK;
+}
+
 int pikgpu_set_downsample(pikgpu_ctx* ctx, int factor) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
   if (factor != 1 && factor != 2 && factor != 4 && factor != 8) {
@@ -1096,6 +1152,10 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
   if (io == nullptr) return Fail(ctx, PIKGPU_EINVAL, "null io");
   if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
   const FrameParams& P0 = ctx->P;
+  if (P0.num_passes != 1) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "the forward model takes single-pass frames");
+  }
   const size_t bx = size_t(P0.bx), by = size_t(P0.by);
   const size_t xs = size_t(P0.xsize), ys = size_t(P0.ysize);
   const bool dev = (io->flags & PIKGPU_ADJ_DEVICE_POINTERS) != 0;

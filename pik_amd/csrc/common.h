@@ -83,6 +83,7 @@ struct FrameParams {
   int num_dots;
   int out_flags;             // kOutLinear | kOutSrgb8
   int downsample;            // 1, 2, 4 or 8
+  int num_passes;            // AC passes accumulated (1 + pikgpu_frame_add_pass)
   int dot_tiles;             // 32x32 tiles touched by dots
   int dot_tiles_x;           // tiles per row (ceil(px / 32))
   int pred_end[4];           // ends of the first-block classes in pred_list:

@@ -28,6 +28,7 @@ constexpr int kG2Plane = kG2Pitch * kG2Pitch;
 struct FrameBuffers {
   // inputs
   const int16_t* ac;      // 3 planes, each by x (bx*64)
+  const int16_t* ac_extra;  // passes 1 .. num_passes - 1, 3 planes each
   const int16_t* dc16;    // 3 planes, each by x bx
   const int32_t* qf;      // by x bx
   const uint8_t* acs;     // by x bx

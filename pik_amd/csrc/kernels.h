@@ -93,6 +93,32 @@ __device__ inline float DequantWith(const BlockDq& d, int c, int16_t q, int k,
   return c == 1 ? v : d.cc * yval + v;
 }
 
+// AC plane c of pass p (0: the frame's own, then pikgpu_frame_add_pass's).
+__device__ inline const int16_t* AcPlane(const FrameParams& P,
+                                         const FrameBuffers& B, int p, int c) {
+  const size_t plane = size_t(P.by) * P.bx * 64;
+  return p == 0 ? B.ac + c * plane
+                : B.ac_extra + (size_t(p - 1) * 3 + c) * plane;
+}
+
+// DequantWith summed over the frame's passes: the first pass's value, then
+// each later pass's own value (its colour correlation from its own Y) plus
+// the sum so far (Dequant<false>::DoAC, pik/compressed_image.cc:695-703).
+// `at` indexes the coefficient within a plane; dy is read only for c != 1.
+__device__ inline float DequantPasses(const FrameParams& P,
+                                      const FrameBuffers& B, const BlockDq& d,
+                                      const BlockDq& dy, int c, size_t at,
+                                      int k) {
+  float acc = 0.0f;
+  for (int p = 0; p < P.num_passes; ++p) {
+    float yv = 0.0f;
+    if (c != 1) yv = DequantWith(dy, 1, AcPlane(P, B, p, 1)[at], k, 0.0f);
+    const float v = DequantWith(d, c, AcPlane(P, B, p, c)[at], k, yv);
+    acc = p == 0 ? v : v + acc;
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // Cooperative transforms of the three channels of a 32x32 tile (256
 // threads). The 2-D transforms are the reference's two column passes with a

@@ -143,22 +143,40 @@ __global__ void __launch_bounds__(256, 4)
     const float scaled = __uint_as_float(dsc.w);
     const float ccf[3] = {float(ytox - 128) * 0.00390625f, 0.0f,
                           float(ytob) * 0.0078125f};
-    int16_t qv[3][4];
-    for (int c = 0; c < 3; ++c) {
-      qv[c][0] = int16_t(q4[c].x & 0xFFFF);
-      qv[c][1] = int16_t(q4[c].x >> 16);
-      qv[c][2] = int16_t(q4[c].y & 0xFFFF);
-      qv[c][3] = int16_t(q4[c].y >> 16);
+    // Every pass in order: the first stores, the later ones add their own
+    // dequantised values (Dequant<false>::DoAC, pik/compressed_image.cc:
+    // 695-703); then the LLF / LF positions.
+    float acc[3][4];
+    for (int p = 0; p < P.num_passes; ++p) {
+      uint2 qp[3] = {q4[0], q4[1], q4[2]};
+      if (p > 0) {
+        for (int c = 0; c < 3; ++c) {
+          qp[c] = *reinterpret_cast<const uint2*>(AcPlane(P, B, p, c) + at);
+        }
+      }
+      int16_t qv[3][4];
+      for (int c = 0; c < 3; ++c) {
+        qv[c][0] = int16_t(qp[c].x & 0xFFFF);
+        qv[c][1] = int16_t(qp[c].x >> 16);
+        qv[c][2] = int16_t(qp[c].y & 0xFFFF);
+        qv[c][3] = int16_t(qp[c].y >> 16);
+      }
+      float yv[4];
+      for (int ci = 0; ci < 3; ++ci) {
+        const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+        for (int j = 0; j < 4; ++j) {
+          // DequantWith's operation order
+          const float mul = mv[c][j] * scaled;
+          const float d = AdjustQuantBias(c, float(qv[c][j])) * mul;
+          const float v = c == 1 ? d : ccf[c] * yv[j] + d;
+          if (c == 1) yv[j] = v;
+          acc[c][j] = p == 0 ? v : v + acc[c][j];
+        }
+      }
     }
-    float yv[4];
-    for (int ci = 0; ci < 3; ++ci) {
-      const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+    for (int c = 0; c < 3; ++c) {
       for (int j = 0; j < 4; ++j) {
-        // DequantWith's operation order
-        const float mul = mv[c][j] * scaled;
-        const float d = AdjustQuantBias(c, float(qv[c][j])) * mul;
-        float v = c == 1 ? d : ccf[c] * yv[j] + d;
-        if (c == 1) yv[j] = v;
+        float v = acc[c][j];
         if (fix[j] == 1) {
           v = fixv[c][j];
         } else if (fix[j] == 2) {

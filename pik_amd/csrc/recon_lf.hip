@@ -548,9 +548,7 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
     dq[k] = DescBlockDq(P, B, c, bya, bxa + k);
     if (c != 1) dqy[k] = DescBlockDq(P, B, 1, bya, bxa + k);
   }
-  const size_t plane = size_t(P.by) * P.bx * 64;
-  const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
-  const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  const size_t row0 = size_t(bya) * P.bx * 64 + size_t(bxa) * 64;
   float F[N * N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -562,9 +560,7 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
       } else {
         const int flat = i * 8 * S + j;
         const int sb = S == 4 ? flat >> 6 : 0, k = flat & 63;
-        float yv = 0.0f;
-        if (c != 1) yv = DequantWith(dqy[sb], 1, acy[flat], k, 0.0f);
-        v = DequantWith(dq[sb], c, acrow[flat], k, yv);
+        v = DequantPasses(P, B, dq[sb], dqy[sb], c, row0 + flat, k);
         if (P.predict_lf) {
           v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
         }
@@ -620,10 +616,7 @@ __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
     f = LocateBlock(P, blk);
     const float* llf = B.llf + (size_t(f.g) * 3 + c) * kLlfPlane;
     const float* lf = B.lf2x2 + (size_t(f.g) * 3 + c) * kG2Plane;
-    const size_t plane = size_t(P.by) * P.bx * 64;
     const size_t row0 = size_t(f.bya) * P.bx * 64 + size_t(f.bxa) * 64;
-    const int16_t* acrow = B.ac + size_t(c) * plane + row0;
-    const int16_t* acy = B.ac + plane + row0;
 #pragma unroll
     for (int y = 0; y < N; ++y) {
       float x;
@@ -633,13 +626,10 @@ __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
         // F[y][l] is flat y * 8S + l, in storage block bxa + flat / 64
         const int flat = y * 8 * S + l;
         const int sb = flat >> 6, k = flat & 63;
-        float yv = 0.0f;
-        if (c != 1) {
-          yv = DequantWith(DescBlockDq(P, B, 1, f.bya, f.bxa + sb), 1,
-                           acy[flat], k, 0.0f);
-        }
-        x = DequantWith(DescBlockDq(P, B, c, f.bya, f.bxa + sb), c,
-                        acrow[flat], k, yv);
+        const BlockDq d = DescBlockDq(P, B, c, f.bya, f.bxa + sb);
+        BlockDq dy = d;
+        if (c != 1) dy = DescBlockDq(P, B, 1, f.bya, f.bxa + sb);
+        x = DequantPasses(P, B, d, dy, c, row0 + flat, k);
         if (P.predict_lf) {
           x += lf[(2 * (f.byg + 1) + y) * kG2Pitch + 2 * (f.bxg + 1) + l];
         }
@@ -707,9 +697,7 @@ __global__ void __launch_bounds__(256)
   const BlockDq dq = DescBlockDq(P, B, c, bya, bxa);
   BlockDq dqy;
   if (c != 1) dqy = DescBlockDq(P, B, 1, bya, bxa);
-  const size_t plane = size_t(P.by) * P.bx * 64;
-  const int16_t* acrow = B.ac + size_t(c) * plane + size_t(bya) * P.bx * 64 + bxa * 64;
-  const int16_t* acy = B.ac + plane + size_t(bya) * P.bx * 64 + bxa * 64;
+  const size_t row0 = size_t(bya) * P.bx * 64 + size_t(bxa) * 64;
   const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
   const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
   float F[4];
@@ -720,9 +708,7 @@ __global__ void __launch_bounds__(256)
         v = llf[(byg + 1) * kLlfPitch + bxg + 1];
       } else {
         const int flat = i * 8 + j;
-        float yv = 0.0f;
-        if (c != 1) yv = DequantWith(dqy, 1, acy[flat], flat, 0.0f);
-        v = DequantWith(dq, c, acrow[flat], flat, yv);
+        v = DequantPasses(P, B, dq, dqy, c, row0 + flat, flat);
         if (P.predict_lf) {
           v += lf[(2 * (byg + 1) + i) * kG2Pitch + 2 * (bxg + 1) + j];
         }

@@ -95,6 +95,14 @@ class Reconstructor:
         self._check(self._lib.pikgpu_download(self._ctx, planes, xs * 4))
         return out
 
+    def add_pass(self, ac):
+        """Adds an AC pass (3 planes of int16, the frame's AC layout) to the
+        uploaded frame (pikgpu_frame_add_pass)."""
+        planes = [np.ascontiguousarray(a, np.int16) for a in ac]
+        ptrs = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in planes])
+        self._check(self._lib.pikgpu_frame_add_pass(
+            self._ctx, ctypes.cast(ptrs, ctypes.c_void_p), planes[0].shape[1]))
+
     def set_downsample(self, factor):
         """Downsampling factor (1, 2, 4, 8) of the following runs
         (pikgpu_set_downsample)."""

@@ -419,3 +419,36 @@ def make_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
     rgb = synthetic_srgb(xsize, ysize, seed, num_rects=num_rects,
                          line_art=line_art)
     return encode_frame(rgb, distance=distance, seed=seed, **kw)
+
+
+def make_extra_passes(frame, num_passes, seed=0, density=0.02, amplitude=1):
+    """AC refinement passes for `frame` (multi-pass accumulation): each a
+    (3, by, bx * 64) int16 array of sparse small values on the frame's
+    coefficient layout, zero at every block's DC position (the AC image
+    never carries the DC)."""
+    rng = np.random.default_rng(seed)
+    by, bx = frame.ysize_blocks, frame.xsize_blocks
+    out = []
+    for _ in range(num_passes):
+        v = rng.integers(-amplitude, amplitude + 1, (3, by, bx * 64))
+        v[rng.random(v.shape) > density] = 0
+        v[:, :, ::64] = 0
+        out.append(v.astype(np.int16))
+    return out
+
+
+def split_passes(frame, num_passes, seed=0):
+    """Splits the frame's AC over num_passes passes, every coefficient in
+    exactly one of them: returns (frame with the first pass's AC, list of
+    the later passes). Each position then accumulates one dequantised value
+    and zeros (x + 0 = x), and the colour correlation only reorders a
+    commutative addition, so the multi-pass decode equals the single-pass
+    one bit for bit."""
+    rng = np.random.default_rng(seed)
+    ac = np.asarray(frame.ac)
+    which = rng.integers(0, num_passes, ac.shape)
+    first = frame.copy()
+    first.ac = np.where(which == 0, ac, 0).astype(np.int16)
+    extra = [np.where(which == p, ac, 0).astype(np.int16)
+             for p in range(1, num_passes)]
+    return first, extra
