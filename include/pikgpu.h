@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define PIKGPU_ABI_VERSION 1
+#define PIKGPU_ABI_VERSION 2
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PIKGPU_API __attribute__((visibility("default")))
@@ -143,6 +143,13 @@ typedef struct pikgpu_frame {
 
   /* Dot dictionary placements, applied in order. */
   const pikgpu_dot* dots; size_t num_dots;
+
+  /* FrameHeader::kGrayscaleOpt (pik/headers.h): the X and B DC of every
+   * group are rebuilt from Y (GrayXyb::RestoreXB, pik/compressed_image.cc:
+   * 94-209, 953-956) before the LF predictions, and the linear output is
+   * set to its luma in all three channels (pik/pik_frame.cc:1155-1166).
+   * bool. */
+  int32_t grayscale;
 } pikgpu_frame;
 
 typedef struct pikgpu_config {

@@ -1120,6 +1120,112 @@ static int acs_first_at(const Ctx* k, int bx, int by) {
   return (k->f->ac_strategy[by * k->f->ac_strategy_stride + bx] & 15) == 0;
 }
 
+/* GrayXyb (pik/compressed_image.cc:94-209): the X and B of a grayscale
+ * frame's DC from its Y, by 16 line pieces fitted over 1024 gray levels of
+ * LinearToXyb (pik/opsin_image.cc:34-47, with ApproxCubeRoot,
+ * pik/approx_cube_root.h). */
+enum { kGrayM = 16 };
+static double g_gray_sub, g_gray_mul;
+static double g_gray_xs[kGrayM], g_gray_xc[kGrayM], g_gray_bs[kGrayM],
+    g_gray_bc[kGrayM];
+static int g_gray_init;
+
+static float approx_cbrt(float y) {
+  int32_t ix;
+  memcpy(&ix, &y, 4);
+  ix = 0x2a50f200 + ix / 3;
+  float x;
+  memcpy(&x, &ix, 4);
+  const float third = 1.0f / 3.0f;
+  x = third * (2.0f * x + y / (x * x));
+  x = third * (2.0f * x + y / (x * x));
+  return x;
+}
+
+static void linear_to_xyb(float r, float g, float b, float* x, float* y,
+                          float* z) {
+  static const double m[9] = {0.29956550340058319, 0.63373087833825936,
+                              0.077705617820981968, 0.22158691104574774,
+                              0.68491388044116142, 0.096254234043612538,
+                              0.20062661225219422, 0.070366199217588729,
+                              0.5571760754215358};
+  static const double bias[3] = {0.26786006338144885, 0.24494032763907073,
+                                 0.14255999980363571};
+  float mixed[3];
+  for (int c = 0; c < 3; ++c) {
+    const float m0 = (float)(m[3 * c] / 255.0f);
+    const float m1 = (float)(m[3 * c + 1] / 255.0f);
+    const float m2 = (float)(m[3 * c + 2] / 255.0f);
+    const float bc = (float)(bias[c] / 255.0f);
+    float v = m0 * r + m1 * g + m2 * b + bc;
+    v = 0.0f < v ? v : 0.0f;
+    mixed[c] = approx_cbrt(v);
+  }
+  *x = (1.0f * mixed[0] - 1.0f * mixed[1]) * 0.5f;
+  *y = (1.0f * mixed[0] + 1.0f * mixed[1]) * 0.5f;
+  *z = mixed[2];
+}
+
+static void gray_regression(const float* x, const float* y, size_t size,
+                            double* a, double* b) {
+  double mx = 0, my = 0, mx2 = 0, my2 = 0, mxy = 0;
+  for (size_t i = 0; i < size; i++) {
+    const double inv = 1.0 / (i + 1);
+    const double dx = x[i] - mx;
+    const double xn = dx * inv;
+    mx += xn;
+    mx2 += dx * xn * i;
+    const double dy = y[i] - my;
+    const double yn = dy * inv;
+    my += yn;
+    my2 += dy * yn * i;
+    mxy += i * xn * yn - mxy * inv;
+  }
+  const double sx = sqrt(mx2 / (size - 1));
+  const double sy = sqrt(my2 / (size - 1));
+  const double sumxy = mxy * size + my * mx * size;
+  const double r = (sumxy - size * mx * my) / ((size - 1.0) * sx * sy);
+  *b = r * sy / sx;
+  *a = my - *b * mx;
+}
+
+static void init_gray(void) {
+  if (g_gray_init) return;
+  enum { kN = 1024 };
+  static float gx[kN], gy[kN], gz[kN];
+  for (int i = 0; i < kN; i++) {
+    const float gray = (float)(256.0f * i / kN);
+    linear_to_xyb(gray, gray, gray, &gx[i], &gy[i], &gz[i]);
+  }
+  const float mn = gy[0], mxv = gy[kN - 1];
+  int m = 0, border[kGrayM + 1];
+  for (int i = 0; i < kN; i++) {
+    if (gy[i] >= gy[0] + (mxv - mn) * m / kGrayM) {
+      border[m] = i;
+      m++;
+    }
+  }
+  border[kGrayM] = kN;
+  g_gray_sub = mn;
+  g_gray_mul = 1.0 / (mxv - mn);
+  for (int i = 0; i < kGrayM; i++) {
+    const size_t n = (size_t)(border[i + 1] - border[i]);
+    gray_regression(gy + border[i], gx + border[i], n, &g_gray_xc[i],
+                    &g_gray_xs[i]);
+    gray_regression(gy + border[i], gz + border[i], n, &g_gray_bc[i],
+                    &g_gray_bs[i]);
+  }
+  g_gray_init = 1;
+}
+
+/* GrayXyb::YToXyb (pik/compressed_image.cc:99-104). */
+static void y_to_xb(float y, float* x, float* b) {
+  int i = (int)((y - g_gray_sub) * g_gray_mul * kGrayM);
+  i = i < 0 ? 0 : (i > kGrayM - 1 ? kGrayM - 1 : i);
+  *x = (float)(y * g_gray_xs[i] + g_gray_xc[i]);
+  *b = (float)(y * g_gray_bs[i] + g_gray_bc[i]);
+}
+
 static int reconstruct_group(const Ctx* k, int gx, int gy, float* const dc[3],
                              float* const ac[3], float* const idct[3]) {
   const pikgpu_frame* f = k->f;
@@ -1150,6 +1256,12 @@ static int reconstruct_group(const Ctx* k, int gx, int gy, float* const dc[3],
         cdc[c][y * cw + x] = dc[c][sy * k->bx + sx];
       }
     }
+  }
+  /* kGrayscaleOpt: GrayXyb::RestoreXB on the group's DC
+   * (pik/compressed_image.cc:953-956). */
+  if (f->grayscale) {
+    init_gray();
+    for (int i = 0; i < cw * ch; ++i) y_to_xb(cdc[1][i], &cdc[0][i], &cdc[2][i]);
   }
   /* Strategy at cache position (x, y): border positions are DCT8. */
 #define CACHE_RAW(x, y)                                                  \
@@ -1558,6 +1670,16 @@ static void opsin_to_linear(float* const img[3], size_t n) {
   }
 }
 
+/* kGrayscaleOpt frames after FinalizeFrameDecoding: every channel set to
+ * the luma of linear RGB, evaluated in double (pik/pik_frame.cc:1155-1166). */
+static void force_gray(float* const img[3], size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    const float gray = (float)(img[0][i] * 0.299 + img[1][i] * 0.587 +
+                               img[2][i] * 0.114);
+    img[0][i] = img[1][i] = img[2][i] = gray;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 
 static void dump(float* const* stages, int s, float* const src[3], size_t w,
@@ -1704,6 +1826,7 @@ static int recon(const pikgpu_frame* f, int num_extra,
     }
   }
   opsin_to_linear(out, (size_t)f->xsize * f->ysize);
+  if (f->grayscale) force_gray(out, (size_t)f->xsize * f->ysize);
   dump(stages, PIKGPU_STAGE_LINEAR, out, f->xsize, f->ysize, f->xsize);
 done:
   free(mem);
@@ -1779,6 +1902,7 @@ int pikoracle_recon_downsampled(const pikgpu_frame* f, int ds,
       }
     }
     opsin_to_linear(out, ox * oy);
+    if (f->grayscale) force_gray(out, ox * oy);
   }
 done:
   free(mem);

@@ -194,11 +194,26 @@ API int pikref_default_tables(float* table, size_t table_cap,
   return 0;
 }
 
+// DecodeFrame's step after FinalizeFrameDecoding for kGrayscaleOpt frames
+// (pik/pik_frame.cc:1155-1166): DecodeFrame itself needs a bitstream, so the
+// loop is restated here on the finalized image.
+static void ForceGray(Image3F* img) {
+  for (size_t y = 0; y < img->ysize(); ++y) {
+    float* PIK_RESTRICT r = img->PlaneRow(0, y);
+    float* PIK_RESTRICT g = img->PlaneRow(1, y);
+    float* PIK_RESTRICT b = img->PlaneRow(2, y);
+    for (size_t x = 0; x < img->xsize(); ++x) {
+      const float gray = r[x] * 0.299 + g[x] * 0.587 + b[x] * 0.114;
+      r[x] = g[x] = b[x] = gray;
+    }
+  }
+}
+
 // Reference reconstruction. `stages` may be NULL, or point to
 // PIKGPU_NUM_STAGES entries of 3 plane pointers (each may be NULL) receiving
 // dense images of the sizes documented in pikgpu.h. `out` receives the final
 // linear image (xsize x ysize per plane, dense). Returns 0 on success.
-// The reference reconstruction; downsample 2, 4 or 8 takes
+// Downsample 2, 4 or 8 takes
 // ReconOpsinImage's downsampled IDCT and FinalizeFrameDecoding's
 // downsampled branch (no AR, no Gaborish, ShrinkTo the downsampled size).
 static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
@@ -238,7 +253,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   }
 
   FrameHeader frame_header;
-  frame_header.flags = 0;
+  frame_header.flags = f->grayscale ? FrameHeader::kGrayscaleOpt : 0;
   frame_header.gaborish = GaborishStrength(f->gaborish);
   frame_header.predict_lf = f->predict_lf != 0;
   frame_header.predict_hf = f->predict_hf != 0;
@@ -268,7 +283,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   cmap.ytob_dc = f->ytob_dc;
 
   FrameDecCache fdc;
-  fdc.grayscale = false;
+  fdc.grayscale = f->grayscale != 0;
   // AC strategy: the frame holds combined bytes; SetFromRaw expands raw
   // types (pik/ac_strategy.cc:1066-1099).
   {
@@ -448,6 +463,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
     const size_t oy = DivCeil<size_t>(ysize, downsample);
     idct.ShrinkTo(ox, oy);
     OpsinToLinear(&idct, pool.get());
+    if (f->grayscale) ForceGray(&idct);
     DumpImage3F(idct, out, ox, oy);
     return 0;
   }
@@ -492,6 +508,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   DumpImage3F(idct, stage(PIKGPU_STAGE_GABORISH), bx * 8, by * 8);
   idct.ShrinkTo(xsize, ysize);
   OpsinToLinear(&idct, pool.get());
+  if (f->grayscale) ForceGray(&idct);
   DumpImage3F(idct, stage(PIKGPU_STAGE_LINEAR), xsize, ysize);
   DumpImage3F(idct, out, xsize, ysize);
   return 0;

@@ -10,7 +10,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpikgpu.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 EINVAL = -1
@@ -88,6 +88,7 @@ class FrameC(ctypes.Structure):
         ("dequant_offsets", ctypes.c_void_p),
         ("num_dequant_offsets", ctypes.c_size_t),
         ("dots", ctypes.c_void_p), ("num_dots", ctypes.c_size_t),
+        ("grayscale", ctypes.c_int32),
     ]
 
 

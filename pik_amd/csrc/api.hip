@@ -198,6 +198,7 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
   memcpy(h->mul_table, ctx->host.mul_table, sizeof(h->mul_table));
   OpsinInverse(h->opsin_inv);
   AdjointNorms(h->adj_norm);
+  ComputeGrayXyb(&h->gray);
   if (hipMalloc(&ctx->consts, sizeof(DeviceConsts)) != hipSuccess ||
       hipMemcpy(ctx->consts, h, sizeof(DeviceConsts), hipMemcpyHostToDevice) !=
           hipSuccess) {
@@ -478,6 +479,7 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   P.num_dots = int(f->num_dots);
   P.downsample = 1;  // per run (pikgpu_set_downsample)
   P.num_passes = 1;  // more with pikgpu_frame_add_pass
+  P.grayscale = f->grayscale != 0;
   // Quantizer::Decode (pik/quantizer.cc:63-71).
   P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
   P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
@@ -1152,9 +1154,9 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
   if (io == nullptr) return Fail(ctx, PIKGPU_EINVAL, "null io");
   if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
   const FrameParams& P0 = ctx->P;
-  if (P0.num_passes != 1) {
+  if (P0.num_passes != 1 || P0.grayscale) {
     return Fail(ctx, PIKGPU_EUNSUPPORTED,
-                "the forward model takes single-pass frames");
+                "the forward model takes single-pass colour frames");
   }
   const size_t bx = size_t(P0.bx), by = size_t(P0.by);
   const size_t xs = size_t(P0.xsize), ys = size_t(P0.ysize);

@@ -84,6 +84,7 @@ struct FrameParams {
   int out_flags;             // kOutLinear | kOutSrgb8
   int downsample;            // 1, 2, 4 or 8
   int num_passes;            // AC passes accumulated (1 + pikgpu_frame_add_pass)
+  int grayscale;             // rebuild the X / B DC from Y (GrayXyb)
   int dot_tiles;             // 32x32 tiles touched by dots
   int dot_tiles_x;           // tiles per row (ceil(px / 32))
   int pred_end[4];           // ends of the first-block classes in pred_list:
@@ -106,6 +107,21 @@ struct HostTables {
 };
 
 void ComputeHostTables(HostTables* t);
+
+// GrayXyb (pik/compressed_image.cc:94-209): 16 line pieces mapping a
+// grayscale Y to X and B, fitted by linear regression over 1024 gray
+// levels; index i = (int)((y - sub) * mul * 16), clamped to 0..15.
+constexpr int kGrayPieces = 16;
+struct GrayXybTable {
+  double sub, mul;
+  double x_slope[kGrayPieces], x_const[kGrayPieces];
+  double b_slope[kGrayPieces], b_const[kGrayPieces];
+};
+void ComputeGrayXyb(GrayXybTable* t);
+// LinearToXyb (pik/opsin_image.cc:34-47) with the reference's approximate
+// cube root (pik/approx_cube_root.h).
+void LinearToXybApprox(float r, float g, float b, float* x, float* y,
+                       float* z);
 // The library dequantisation tables (quant_tables.cpp); returns the number
 // of floats, writes at most table_cap floats and offsets_cap offsets.
 size_t DefaultDequantTables(float* table, size_t table_cap, uint32_t* offsets,

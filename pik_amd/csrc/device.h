@@ -15,6 +15,7 @@ struct DeviceConsts {
   int mul_table[kMaxSigma + 1];  // EPF WeightFast multipliers
   float opsin_inv[9];            // inverse opsin absorbance matrix
   float adj_norm[kAdjNormFloats];  // AdjointNorms
+  GrayXybTable gray;               // ComputeGrayXyb
 };
 
 // LF-domain per-group caches: (gw+2) x (gh+2) values per plane for llf and

@@ -1241,6 +1241,13 @@ __global__ void __launch_bounds__(256)
     lr = lr + tr;
     lg = lg + tg;
     lb = lb + tb;
+    if (P.grayscale) {
+      // kGrayscaleOpt frames: every channel set to the luma of linear RGB,
+      // evaluated in double (pik/pik_frame.cc:1155-1166)
+      const float gray = (float)(double(lr) * 0.299 + double(lg) * 0.587 +
+                                 double(lb) * 0.114);
+      lr = lg = lb = gray;
+    }
     if (P.out_flags & kOutLinear) {
       const size_t oplane = size_t(P.ysize) * B.opitch;
       B.out[size_t(y) * B.opitch + x] = lr;

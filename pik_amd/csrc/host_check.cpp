@@ -54,3 +54,22 @@ HAPI void pikgpu_host_ub4(float* k) {
   pikgpu::Ub4Kernel(t);
   memcpy(k, t, sizeof(t));
 }
+
+// GrayXyb's table: out[0] = sub, out[1] = mul, then 16 each of x_slope,
+// x_const, b_slope, b_const (66 doubles).
+HAPI void pikgpu_host_gray_xyb(double* out) {
+  pikgpu::GrayXybTable t;
+  pikgpu::ComputeGrayXyb(&t);
+  out[0] = t.sub;
+  out[1] = t.mul;
+  for (int i = 0; i < pikgpu::kGrayPieces; ++i) {
+    out[2 + i] = t.x_slope[i];
+    out[18 + i] = t.x_const[i];
+    out[34 + i] = t.b_slope[i];
+    out[50 + i] = t.b_const[i];
+  }
+}
+
+HAPI void pikgpu_host_linear_to_xyb(float r, float g, float b, float* xyb) {
+  pikgpu::LinearToXybApprox(r, g, b, &xyb[0], &xyb[1], &xyb[2]);
+}

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <string.h>
 
+#include <vector>
+
 #include "common.h"
 
 namespace pikgpu {
@@ -227,6 +229,104 @@ void AdjointNorms(float* d) {
                               : 2.0 * std::cos(k * M_PI / (2 * n));
       o[k] = float(0.5 * n * w * w);
     }
+  }
+}
+
+namespace {
+
+// ApproxCubeRoot (pik/approx_cube_root.h): initial guess 0x2a50f200 + ix / 3
+// on the float's bits, then two Newton steps x = 1/3 (2 x + y / x^2).
+float CubeRootApprox(float y) {
+  int32_t ix;
+  memcpy(&ix, &y, 4);
+  ix = 0x2a50f200 + ix / 3;
+  float x;
+  memcpy(&x, &ix, 4);
+  const float kOneThird = 1.0f / 3.0f;
+  for (int step = 0; step < 2; ++step) x = kOneThird * (2.0f * x + y / (x * x));
+  return x;
+}
+
+// y ~= b * x + a over `size` points (GrayXyb::LinearRegression,
+// pik/compressed_image.cc:164-193, its running-moment form).
+void LinearRegression(const float* x, const float* y, size_t size, double* a,
+                      double* b) {
+  double mx = 0, my = 0;
+  double mx2 = 0, my2 = 0;
+  double mxy = 0;
+  for (size_t i = 0; i < size; i++) {
+    const double inv = 1.0 / (i + 1);
+    const double dx = x[i] - mx;
+    const double xn = dx * inv;
+    mx += xn;
+    mx2 += dx * xn * i;
+    const double dy = y[i] - my;
+    const double yn = dy * inv;
+    my += yn;
+    my2 += dy * yn * i;
+    mxy += i * xn * yn - mxy * inv;
+  }
+  const double sx = std::sqrt(mx2 / (size - 1));
+  const double sy = std::sqrt(my2 / (size - 1));
+  const double sumxy = mxy * size + my * mx * size;
+  const double r = (sumxy - size * mx * my) / ((size - 1.0) * sx * sy);
+  *b = r * sy / sx;
+  *a = my - *b * mx;
+}
+
+}  // namespace
+
+void LinearToXybApprox(float r, float g, float b, float* x, float* y,
+                       float* z) {
+  // OpsinAbsorbance (pik/opsin_image.h:23-30), the matrix and bias of
+  // pik/opsin_params.h:21-41 rounded from double.
+  const float kScale = 255.0f;
+  const float mix[9] = {
+      float(0.29956550340058319 / kScale), float(0.63373087833825936 / kScale),
+      float(0.077705617820981968 / kScale), float(0.22158691104574774 / kScale),
+      float(0.68491388044116142 / kScale), float(0.096254234043612538 / kScale),
+      float(0.20062661225219422 / kScale), float(0.070366199217588729 / kScale),
+      float(0.5571760754215358 / kScale)};
+  const float bias[3] = {float(0.26786006338144885 / kScale),
+                         float(0.24494032763907073 / kScale),
+                         float(0.14255999980363571 / kScale)};
+  float mixed[3];
+  for (int c = 0; c < 3; ++c) {
+    const float v = mix[3 * c] * r + mix[3 * c + 1] * g + mix[3 * c + 2] * b +
+                    bias[c];
+    // std::max(0.0f, v): (0 < v) ? v : 0 (so -0 and NaN give +0)
+    mixed[c] = CubeRootApprox(0.0f < v ? v : 0.0f);
+  }
+  // LinearXybTransform with kScaleR = kScaleG = 1 (pik/opsin_image.cc:25-30)
+  *x = (1.0f * mixed[0] - 1.0f * mixed[1]) * 0.5f;
+  *y = (1.0f * mixed[0] + 1.0f * mixed[1]) * 0.5f;
+  *z = mixed[2];
+}
+
+void ComputeGrayXyb(GrayXybTable* t) {
+  const int kN = 1024;
+  std::vector<float> x(kN), y(kN), z(kN);
+  for (int i = 0; i < kN; i++) {
+    const float gray = (float)(256.0f * i / kN);
+    LinearToXybApprox(gray, gray, gray, &x[i], &y[i], &z[i]);
+  }
+  const float mn = y[0], mx = y[kN - 1];
+  int m = 0;
+  int border[kGrayPieces + 1];
+  for (int i = 0; i < kN; i++) {
+    if (y[i] >= y[0] + (mx - mn) * m / kGrayPieces) {
+      border[m] = i;
+      m++;
+    }
+  }
+  border[kGrayPieces] = kN;
+  t->sub = mn;
+  t->mul = 1.0 / (mx - mn);
+  for (int i = 0; i < kGrayPieces; i++) {
+    LinearRegression(y.data() + border[i], x.data() + border[i],
+                     border[i + 1] - border[i], &t->x_const[i], &t->x_slope[i]);
+    LinearRegression(y.data() + border[i], z.data() + border[i],
+                     border[i + 1] - border[i], &t->b_const[i], &t->b_slope[i]);
   }
 }
 

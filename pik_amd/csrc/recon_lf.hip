@@ -72,13 +72,25 @@ __device__ inline GroupGeom Geom(const FrameParams& P, int g) {
   return G;
 }
 
-// DC of the group cache at cache coordinates (x, y): SourceCoord clamping
-// (pik/compressed_dc.h:46-49) is clamp(c - 1, 0, size - 1).
-__device__ inline float GroupDc(const FrameParams& P, const float* dcp,
-                                const GroupGeom& G, int x, int y) {
+// DC of plane c of the group cache at cache coordinates (x, y): SourceCoord
+// clamping (pik/compressed_dc.h:46-49) is clamp(c - 1, 0, size - 1). For a
+// grayscale frame X and B come from Y (GrayXyb::YToXyb, pik/compressed_image.cc:
+// 99-104, in double as there: the piece index truncates (y - sub) * mul * 16,
+// the value y * slope + constant is rounded to float), as RestoreXB rebuilds
+// them in the group cache before the LF predictions (:953-956).
+__device__ inline float GroupDc(const FrameParams& P, const DeviceConsts* K,
+                                const float* dc, int c, const GroupGeom& G,
+                                int x, int y) {
   const int sx = min(max(G.gbx0 + x - 1, 0), P.bx - 1);
   const int sy = min(max(G.gby0 + y - 1, 0), P.by - 1);
-  return dcp[sy * P.bx + sx];
+  const size_t plane = size_t(P.bx) * P.by, at = size_t(sy) * P.bx + sx;
+  if (!P.grayscale || c == 1) return dc[c * plane + at];
+  const float yv = dc[plane + at];
+  const GrayXybTable& T = K->gray;
+  int i = (int)((yv - T.sub) * T.mul * kGrayPieces);
+  i = min(max(0, i), kGrayPieces - 1);
+  return c == 0 ? (float)(yv * T.x_slope[i] + T.x_const[i])
+                : (float)(yv * T.b_slope[i] + T.b_const[i]);
 }
 
 // Strategy of cache position (x, y); border positions count as DCT8 first
@@ -161,7 +173,6 @@ __device__ inline void ReinterpIdct(const DeviceConsts* K, const float* in,
 __device__ inline void LlfAt(const FrameParams& P, const FrameBuffers& B,
                              const DeviceConsts* K, const GroupGeom& G, int g,
                              int c, int x, int y, int raw) {
-  const float* dcp = B.dc + c * P.bx * P.by;
   float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
   if (raw == kDCT16 || raw == kDCT32) {
     // ReinterpretingDCT<8s, s, s> of the s x s DC values.
@@ -170,7 +181,7 @@ __device__ inline void LlfAt(const FrameParams& P, const FrameBuffers& B,
     for (int iy = 0; iy < 4; ++iy) {
       for (int ix = 0; ix < 4; ++ix) {
         dc[iy * 4 + ix] =
-            (iy < s && ix < s) ? GroupDc(P, dcp, G, x + ix, y + iy) : 0.0f;
+            (iy < s && ix < s) ? GroupDc(P, K, B.dc, c, G, x + ix, y + iy) : 0.0f;
       }
     }
     if (raw == kDCT16) {
@@ -190,7 +201,7 @@ __device__ inline void LlfAt(const FrameParams& P, const FrameBuffers& B,
     }
     return;
   }
-  llf[0] = GroupDc(P, dcp, G, x, y);
+  llf[0] = GroupDc(P, K, B.dc, c, G, x, y);
 }
 
 // DC2x2FromLowestFrequencies of the first block at cache position (x, y)
@@ -245,7 +256,6 @@ __device__ inline void LlfDc2x2Lanes(const FrameParams& P,
     y = bya - gyi * kGroupDimBlocks + 1;
     G = Geom(P, g);
   }
-  const float* dcp = B.dc + c * P.bx * P.by;
   float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane + y * kLlfPitch + x;
   float col[S];  // LLF column l (lanes l < S)
   if (S == 4) {
@@ -253,7 +263,7 @@ __device__ inline void LlfDc2x2Lanes(const FrameParams& P,
     float v[4], w[4];
     if (active && l < 4) {
 #pragma unroll
-      for (int iy = 0; iy < 4; ++iy) v[iy] = GroupDc(P, dcp, G, x + l, y + iy);
+      for (int iy = 0; iy < 4; ++iy) v[iy] = GroupDc(P, K, B.dc, c, G, x + l, y + iy);
       dct1d::Dct<4>(v, w);
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[l * 5 + k] = w[k];
@@ -274,10 +284,10 @@ __device__ inline void LlfDc2x2Lanes(const FrameParams& P,
     __syncthreads();  // t is reused below
   } else {
     if (active) {
-      const float a00 = GroupDc(P, dcp, G, x, y);
-      const float a01 = GroupDc(P, dcp, G, x + 1, y);
-      const float a10 = GroupDc(P, dcp, G, x, y + 1);
-      const float a11 = GroupDc(P, dcp, G, x + 1, y + 1);
+      const float a00 = GroupDc(P, K, B.dc, c, G, x, y);
+      const float a01 = GroupDc(P, K, B.dc, c, G, x + 1, y);
+      const float a10 = GroupDc(P, K, B.dc, c, G, x, y + 1);
+      const float a11 = GroupDc(P, K, B.dc, c, G, x + 1, y + 1);
       const float d[4] = {(a00 + a01 + a10 + a11) * 0.25f,
                           (a00 + a01 - a10 - a11) * 0.25f,
                           (a00 - a01 + a10 - a11) * 0.25f,

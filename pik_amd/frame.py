@@ -49,6 +49,7 @@ class Frame:
     adaptive_reconstruction: int = 1
     epf_enable_adaptive: int = 1
     epf_use_sharpened: int = 0
+    grayscale: int = 0
 
     @property
     def xsize_blocks(self):
@@ -85,7 +86,7 @@ class Frame:
         for name in ("gaborish", "predict_lf", "predict_hf",
                      "adaptive_reconstruction", "epf_enable_adaptive",
                      "epf_use_sharpened", "global_scale", "quant_dc",
-                     "ytox_dc", "ytob_dc"):
+                     "ytox_dc", "ytob_dc", "grayscale"):
             setattr(f, name, int(getattr(self, name)))
         for c in range(3):
             f.ac[c] = self.ac[c].ctypes.data

@@ -266,9 +266,16 @@ def choose_strategies(xyb_pad, seed):
 
 
 def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
-                 lut_ids="zero", predict=True):
-    """Encodes an (H, W, 3) uint8 sRGB image into a Frame."""
+                 lut_ids="zero", predict=True, grayscale=False):
+    """Encodes an (H, W, 3) uint8 sRGB image into a Frame. With
+    `grayscale` the image is first made gray (channel mean) and the frame
+    is a kGrayscaleOpt frame: only the Y DC is coded, the X and B DC planes
+    hold zeros as Image3SDecompress leaves them (pik/compressed_dc.cc:
+    107-121), and the decoder rebuilds them from Y."""
     rgb = np.asarray(rgb)
+    if grayscale:
+        g = np.rint(rgb.astype(np.float64).mean(axis=2)).astype(np.uint8)
+        rgb = np.repeat(g[:, :, None], 3, axis=2)
     ysize, xsize = rgb.shape[:2]
     bx, by = div_ceil(xsize, 8), div_ceil(ysize, 8)
     tx, ty = div_ceil(xsize, 64), div_ceil(ysize, 64)
@@ -374,6 +381,9 @@ def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
                                 / max(float((ydec * ydec).sum()), 1e-9)), 0, 255))
     for c, f in ((0, (ytox_dc - 128) / 256.0), (2, ytob_dc / 128.0)):
         dc[c] = np.clip(np.rint((dcimg[c] - f * ydec) / mul_dc[c]), -32767, 32767)
+    if grayscale:
+        dc[0] = 0
+        dc[2] = 0
 
     if lut_ids == "zero":
         lut = np.zeros((by, bx), np.uint8)
@@ -389,7 +399,8 @@ def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
         dequant_map=np.zeros((16, 256), np.uint8), dequant_table=table,
         dequant_offsets=offsets, dots=dots, global_scale=global_scale,
         quant_dc=quant_dc, ytox_dc=ytox_dc, ytob_dc=ytob_dc,
-        predict_lf=int(predict), predict_hf=int(predict)).contiguous()
+        predict_lf=int(predict), predict_hf=int(predict),
+        grayscale=int(grayscale)).contiguous()
 
 
 def make_dots(xsize, ysize, n, seed):

@@ -9,7 +9,7 @@ AdaptiveReconstruction output and the final linear Image3F that
 FinalizeFrameDecoding leaves. The reference code is the scalar build under
 oracle/_ref (make -C oracle ref); run from the repository root:
 
-    python3 tests/golden/make_golden.py
+    python3 tests/golden/make_golden.py [recon_<name> | downsample_<name> ...]
 """
 import os
 import sys
@@ -35,6 +35,7 @@ CASES = {
                         gaborish=abi.GABORISH_500)),
     "sharpened_d3": (72, 72, dict(seed=12, distance=3.0),
                      dict(epf_use_sharpened=1)),
+    "grayscale": (104, 80, dict(seed=31, num_dots=2, grayscale=True), {}),
 }
 STAGES = ("dc_dequant", "dc", "idct", "ar")
 
@@ -47,6 +48,7 @@ DS_CASES = {
                      dict(predict_lf=0)),
     "no_predict": (120, 72, dict(seed=23, predict=False, num_dots=0),
                    dict(gaborish=abi.GABORISH_1000)),
+    "grayscale": (120, 88, dict(seed=32, num_dots=0, grayscale=True), {}),
 }
 DS_FACTORS = (2, 4, 8)
 
@@ -70,8 +72,11 @@ def make_downsampled(name):
     return fr, outs
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or those named in `only`."""
     for name in DS_CASES:
+        if only and "downsample_" + name not in only:
+            continue
         fr, outs = make_downsampled(name)
         path = os.path.join(HERE, "downsample_%s.npz" % name)
         d = {}
@@ -82,6 +87,8 @@ def main():
         np.savez_compressed(path, **d)
         print("wrote", path, os.path.getsize(path), "bytes")
     for name in CASES:
+        if only and "recon_" + name not in only:
+            continue
         fr, out, st = make(name)
         path = os.path.join(HERE, "recon_%s.npz" % name)
         d = {}
@@ -95,4 +102,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -1,0 +1,208 @@
+"""Grayscale frames (FrameHeader::kGrayscaleOpt) on the CPU.
+
+The decoder rebuilds the X and B DC of every group from Y with GrayXyb's
+16 line pieces (pik/compressed_image.cc:94-209, applied at 953-956) and
+sets all three linear channels to their luma after FinalizeFrameDecoding
+(pik/pik_frame.cc:1155-1166). Checked here:
+
+  - the host LinearToXyb the tables are fitted to, bit for bit against the
+    reference's LinearToXyb where oracle/_ref is built, and against the
+    closed form (cube roots of the opsin mixture) in float64;
+  - the table itself: the segment borders and each piece's slope and
+    intercept against numpy's least-squares fit of the same samples, and
+    the fitted pieces against the samples they approximate;
+  - the oracle's C restatement against the reference on seeded grayscale
+    frames (full size, downsampled, every stage), bit for bit;
+  - the output is gray: three equal channels.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from pik_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_LIB = os.path.join(ROOT, "pik_amd", "libpikgpu_host.so")
+
+# kOpsinAbsorbanceMatrix and kOpsinAbsorbanceBias (pik/opsin_params.h)
+OPSIN = np.array([[0.29956550340058319, 0.63373087833825936,
+                   0.077705617820981968],
+                  [0.22158691104574774, 0.68491388044116142,
+                   0.096254234043612538],
+                  [0.20062661225219422, 0.070366199217588729,
+                   0.5571760754215358]])
+BIAS = np.array([0.26786006338144885, 0.24494032763907073,
+                 0.14255999980363571])
+
+
+@pytest.fixture(scope="module")
+def host():
+    L = ctypes.CDLL(HOST_LIB)
+    L.pikgpu_host_linear_to_xyb.argtypes = [ctypes.c_float] * 3 + [
+        ctypes.c_void_p]
+    L.pikgpu_host_gray_xyb.argtypes = [ctypes.c_void_p]
+    return L
+
+
+def _port():
+    from oracle import port
+    if not port.available():
+        pytest.skip("oracle port not built (make -C oracle port)")
+    return port
+
+
+def _ref():
+    from oracle import ref
+    if not ref.available("scalar"):
+        pytest.skip("reference harness not built (oracle/_ref)")
+    return ref
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def host_xyb(host, r, g, b):
+    out = np.zeros(3, np.float32)
+    host.pikgpu_host_linear_to_xyb(r, g, b, out.ctypes.data)
+    return out
+
+
+def gray_table(host):
+    t = np.zeros(66, np.float64)
+    host.pikgpu_host_gray_xyb(t.ctypes.data)
+    return dict(sub=t[0], mul=t[1], x_slope=t[2:18], x_const=t[18:34],
+                b_slope=t[34:50], b_const=t[50:66])
+
+
+def _samples():
+    rng = np.random.default_rng(5)
+    grays = [np.float32(256.0 * i / 1024) for i in range(0, 1024, 7)]
+    pts = [(g, g, g) for g in grays] + [(0.0, 0.0, 0.0), (255.0, 255.0, 255.0)]
+    pts += [tuple(v) for v in rng.uniform(0, 255, (64, 3)).astype(np.float32)]
+    return pts
+
+
+def test_linear_to_xyb_closed_form(host):
+    # ApproxCubeRoot after two Newton steps is within a few ulp of cbrt.
+    for r, g, b in _samples():
+        got = host_xyb(host, r, g, b).astype(np.float64)
+        mixed = np.cbrt(np.maximum(OPSIN @ (np.array([r, g, b], np.float64)
+                                             / 255.0) + BIAS / 255.0, 0.0))
+        want = [(mixed[0] - mixed[1]) * 0.5, (mixed[0] + mixed[1]) * 0.5,
+                mixed[2]]
+        np.testing.assert_allclose(got, want, rtol=0, atol=2e-6)
+
+
+def test_linear_to_xyb_matches_reference(host):
+    ref = _ref()
+    L = ref.lib()
+    for r, g, b in _samples():
+        want = np.zeros(3, np.float32)
+        L.pikref_linear_to_xyb(r, g, b, want.ctypes.data)
+        assert np.array_equal(_bits(host_xyb(host, r, g, b)), _bits(want)), \
+            (r, g, b)
+
+
+def _gray_samples(host):
+    xyb = np.array([host_xyb(host, *(3 * [np.float32(256.0 * i / 1024)]))
+                    for i in range(1024)], np.float32)
+    return xyb[:, 0], xyb[:, 1], xyb[:, 2]
+
+
+def _borders(y):
+    # y >= y[0] + (max - min) * m / 16, in float as the table is built
+    lo, hi = y[0], y[-1]
+    border, m = [], 0
+    for i in range(len(y)):
+        t = np.float32(lo + np.float32(np.float32((hi - lo) * np.float32(m))
+                                       / np.float32(16)))
+        if y[i] >= t:
+            border.append(i)
+            m += 1
+    return border[:16] + [len(y)]
+
+
+def test_gray_table_is_least_squares(host):
+    T = gray_table(host)
+    x, y, z = _gray_samples(host)
+    assert T["sub"] == float(y[0])
+    assert T["mul"] == 1.0 / float(np.float32(y[-1] - y[0]))
+    assert np.all(np.diff(y) > 0)
+    border = _borders(y)
+    assert len(border) == 17 and border[0] == 0
+    assert all(b1 - b0 >= 2 for b0, b1 in zip(border, border[1:]))
+    for i in range(16):
+        s = slice(border[i], border[i + 1])
+        yy = y[s].astype(np.float64)
+        for tgt, slope, const in ((x, "x_slope", "x_const"),
+                                  (z, "b_slope", "b_const")):
+            A = np.stack([yy, np.ones_like(yy)], axis=1)
+            (ls, lc), *_ = np.linalg.lstsq(A, tgt[s].astype(np.float64),
+                                           rcond=None)
+            np.testing.assert_allclose(T[slope][i], ls, rtol=1e-7, atol=1e-9)
+            np.testing.assert_allclose(T[const][i], lc, rtol=1e-7, atol=1e-9)
+
+
+def test_gray_pieces_approximate_samples(host):
+    T = gray_table(host)
+    x, y, z = _gray_samples(host)
+    yd = y.astype(np.float64)
+    i = np.clip(((yd - T["sub"]) * T["mul"] * 16).astype(np.int64), 0, 15)
+    ex = np.abs(yd * T["x_slope"][i] + T["x_const"][i] - x)
+    eb = np.abs(yd * T["b_slope"][i] + T["b_const"][i] - z)
+    assert ex.max() < 2e-3 and eb.max() < 2e-2
+    # the middle of the range is smooth: much tighter there
+    mid = slice(256, 1024)
+    assert ex[mid].max() < 2e-4 and eb[mid].max() < 2e-3
+
+
+@pytest.mark.parametrize("w,h,seed,kw", [
+    (64, 64, 1, {}),
+    (136, 72, 4, dict(num_dots=3)),
+    (200, 136, 21, dict(predict=False)),
+    (83, 45, 22, dict(lut_ids="rand")),
+])
+def test_port_matches_reference(w, h, seed, kw):
+    ref, port = _ref(), _port()
+    fr = synth.make_frame(w, h, seed=seed, grayscale=True, **kw)
+    want, wst = ref.recon(fr, threads=1, stages=True)
+    got, gst = port.recon(fr, stages=True)
+    assert np.array_equal(_bits(got), _bits(want))
+    for k in wst:
+        assert np.array_equal(_bits(gst[k]), _bits(wst[k])), k
+    if not kw.get("num_dots"):
+        for ds in (2, 4, 8):
+            assert np.array_equal(_bits(port.recon_downsampled(fr, ds)),
+                                  _bits(ref.recon_downsampled(fr, ds))), ds
+
+
+def test_output_is_gray_and_flag_matters():
+    port = _port()
+    fr = synth.make_frame(96, 64, seed=3, grayscale=True)
+    out, _ = port.recon(fr, stages=False)
+    assert np.array_equal(_bits(out[0]), _bits(out[1]))
+    assert np.array_equal(_bits(out[1]), _bits(out[2]))
+    fr.grayscale = 0
+    colour, _ = port.recon(fr, stages=False)
+    assert not np.array_equal(_bits(colour[0]), _bits(colour[1]))
+
+
+def test_nonzero_coded_xb_matches_reference():
+    # The X and B DC of the frame are overwritten only in the groups' DC
+    # caches: the DC smoothing before them (AdaptiveDCReconstruction,
+    # pik/adaptive_reconstruction.cc:300) still reads all three planes, so
+    # coded X / B values reach Y. Both implementations must agree on that.
+    ref, port = _ref(), _port()
+    fr = synth.make_frame(72, 56, seed=8, grayscale=True)
+    rng = np.random.default_rng(1)
+    fr.dc = fr.dc.copy()
+    fr.dc[0] = rng.integers(-50, 50, fr.dc[0].shape)
+    fr.dc[2] = rng.integers(-50, 50, fr.dc[2].shape)
+    fr = fr.contiguous()
+    want, wst = ref.recon(fr, threads=1, stages=True)
+    got, gst = port.recon(fr, stages=True)
+    assert np.array_equal(_bits(got), _bits(want))
+    assert np.array_equal(_bits(gst["dc"]), _bits(wst["dc"]))
