@@ -255,6 +255,26 @@ PIKGPU_API int pikgpu_set_output(pikgpu_ctx* ctx, int outputs);
 PIKGPU_API int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
                                      size_t ac_stride);
 
+/* Encoder-side quantization of an XYB image under the uploaded frame's
+ * side information (quant field, AC strategy, colour correlation, dequant
+ * tables / control field / map, global_scale, quant_dc): the coefficient
+ * half of RoundtripImage (pik/adaptive_quantization.cc:580-651), i.e.
+ * InitializeFrameEncCache (pik/compressed_image.cc:210-304) and, per group,
+ * ComputeCoefficients (:351-455). opsin[c]: plane c of the XYB image padded
+ * to whole blocks (bx * 8 columns, by * 8 rows; rows of opsin_stride
+ * floats; host memory). The quantized AC and DC replace the uploaded
+ * frame's (later passes are dropped), so a following pikgpu_run decodes the
+ * round trip; ac_out / dc_out (host, may be NULL) also receive them in the
+ * layouts of pikgpu_frame.ac / .dc (rows of ac_stride / dc_stride int16).
+ * Frames with LF / HF prediction, dots or the grayscale flag return
+ * PIKGPU_EUNSUPPORTED, as does a quant value the encoder's dequant-map
+ * lookup (indexed by the quant value itself, :417, 445) would read past the
+ * map with. */
+PIKGPU_API int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
+                               size_t opsin_stride, int16_t* ac_out[3],
+                               size_t ac_stride, int16_t* dc_out[3],
+                               size_t dc_stride);
+
 /* Selects the downsampling factor of the following pikgpu_run calls: 1
  * (full size, the default), 2, 4 or 8. Replaces the `downsample` argument
  * of ReconOpsinImage and FinalizeFrameDecoding (pik/compressed_image.h:

@@ -1908,3 +1908,279 @@ done:
   free(mem);
   return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Encoder-side quantization, as RoundtripImage runs it                      */
+/* (pik/adaptive_quantization.cc:580-651): InitializeFrameEncCache           */
+/* (pik/compressed_image.cc:210-304) and ComputeCoefficients (:351-455) for  */
+/* frames without LF / HF prediction, dots or the grayscale flag.            */
+
+/* Quantizer::QuantizeBlockAC (pik/quantizer.h:220-285) of one 8x8
+ * sub-block (ix, iy) of an xs x ys transform: the 64 positions interleave
+ * with the other sub-blocks', and the rounding error is carried forward
+ * in double along each row and down from the previous row. `in` and `out`
+ * index the transform's coefficients as pos = y * 8 xs + x. */
+static void quantize_sub_block(const float* in, const float* qm, float qac,
+                               float thres, int xs, int ys, int ix, int iy,
+                               int16_t* out) {
+  double prev[8] = {0};
+  double err = 0;
+  for (int k = 0; k < 64; ++k) {
+    if ((k & 7) == 0) {
+      err = prev[0];
+    } else {
+      err += prev[k & 7];
+    }
+    const int x = xs * (k % 8) + ix, y = ys * (k / 8) + iy;
+    const int pos = y * 8 * xs + x;
+    float val = in[pos] * (qm[pos] * qac);
+    if (err > 0) {
+      if (val > 0) {
+        val += 0.5 * err;
+      } else {
+        val -= 0.5 * err;
+      }
+      err = 0;
+    }
+    if (fabsf(val) > 1) err = 0;
+    double v = (fabsf(val) < thres) ? 0 : (double)roundf(val);
+    if (fabs(v) < fabs((double)val)) {
+      err += fabs(v - val);
+    } else {
+      err -= fabs(v - val);
+    }
+    if (k == 0) err = 0;
+    if ((k & 7) == 7) {
+      prev[k & 7] = 1.0 * err;
+      err = 0;
+    } else {
+      err *= 0.5;
+      prev[k & 7] = err;
+    }
+    if (v > 32767) v = 32767;
+    if (v < -32767) v = -32767;
+    out[pos] = (int16_t)v;
+  }
+}
+
+/* AdjustQuantBiasVar (pik/quantizer.h:66-74) of an integer. */
+static float adjust_quant_bias_int(int c, int q) {
+  static const float kOneBias[3] = {0.05465007330715401f, 0.07005449891748593f,
+                                    0.049935103337343655f};
+  if (q == 0) return 0;
+  if (q == 1) return 1 - kOneBias[c];
+  if (q == -1) return kOneBias[c] - 1;
+  return q - 0.145f / q;
+}
+
+int pikoracle_quantize(const pikgpu_frame* f, const float* const* opsin,
+                       int16_t* const* ac_out, int16_t* const* dc_out) {
+  init_tables();
+  if (!f || !opsin) return fail(PIKGPU_EINVAL, "null argument");
+  if (f->predict_lf || f->predict_hf || f->num_dots || f->grayscale) {
+    return fail(PIKGPU_EUNSUPPORTED,
+                "quantization of frames with predictions, dots or grayscale");
+  }
+  pikgpu_frame g = *f;
+  const int bx = (int)div_ceil(f->xsize, 8), by = (int)div_ceil(f->ysize, 8);
+  const size_t nb = (size_t)bx * by, row = (size_t)bx * 64;
+  const size_t px = (size_t)bx * 8;
+  int16_t* q16 = (int16_t*)calloc(3 * nb * 65, sizeof(int16_t));
+  float* mem = (float*)calloc(3 * nb * 64 + 9 * nb, sizeof(float));
+  int rc = 0;
+  if (!q16 || !mem) {
+    rc = fail(PIKGPU_ENOMEM, "out of memory");
+    goto done;
+  }
+  {
+    float *coef[3], *dcf[3], *dcq[3], *dc[3];
+    int16_t *ac16[3], *dc16[3];
+    for (int c = 0; c < 3; ++c) {
+      coef[c] = mem + c * nb * 64;
+      dcf[c] = mem + 3 * nb * 64 + c * nb;
+      dcq[c] = mem + 3 * nb * 64 + (3 + c) * nb;
+      dc[c] = mem + 3 * nb * 64 + (6 + c) * nb;
+      ac16[c] = q16 + c * nb * 64;
+      dc16[c] = q16 + 3 * nb * 64 + c * nb;
+      g.ac[c] = ac16[c];
+      g.dc[c] = dc16[c];
+    }
+    g.ac_stride = row;
+    g.dc_stride = (size_t)bx;
+    if ((rc = check_frame(&g))) goto done;
+    Ctx k;
+    memset(&k, 0, sizeof(k));
+    k.f = &g;
+    k.bx = bx;
+    k.by = by;
+    k.px = 8 * bx;
+    k.py = 8 * by;
+    k.tx = (int)div_ceil(f->xsize, 64);
+    k.inv_global_scale = (float)(1.0 * 65536 / f->global_scale);
+    k.inv_quant_dc = k.inv_global_scale / f->quant_dc;
+    k.table = f->dequant_table;
+    k.offsets = f->dequant_offsets;
+    /* TransformFromPixels + DCFromLowestFrequencies
+     * (pik/compressed_image.cc:234-247, pik/ac_strategy.cc:889-927). */
+    for (int c = 0; c < 3; ++c) {
+      for (int y = 0; y < by; ++y) {
+        for (int x = 0; x < bx; ++x) {
+          if (!acs_first_at(&k, x, y)) continue;
+          const int raw = acs_raw_at(&k, x, y);
+          float* cb = coef[c] + y * row + (size_t)x * 64;
+          from_pixels(raw, opsin[c] + (size_t)y * 8 * px + (size_t)x * 8, px,
+                      cb, row);
+          float* d = dcf[c] + (size_t)y * bx + x;
+          if (raw == kDCT16) {
+            reinterp_idct(16, 2, 2, cb, 16, d, (size_t)bx);
+          } else if (raw == kDCT32) {
+            reinterp_idct(32, 4, 4, cb, 32, d, (size_t)bx);
+          } else {
+            d[0] = cb[0];
+          }
+        }
+      }
+    }
+    /* QuantizeRoundtripDC of Y, ApplyColorCorrelationDC<false>,
+     * QuantizeCoeffsDC (pik/compressed_image.cc:262-271,
+     * pik/quantizer.cc:86-117, pik/color_correlation.cc:180-205). */
+    {
+      const float gsf = (float)(f->global_scale * (1.0 / 65536));
+      float inv0[3], mul[3];
+      for (int c = 0; c < 3; ++c) {
+        const float m0 = dequant_matrix(&k, 0, 0, c)[0];
+        inv0[c] = 1.0f / m0;
+        mul[c] = m0 * k.inv_quant_dc;
+      }
+      const float qdc = gsf * f->quant_dc;
+      const float ytox = 1.0f * (f->ytox_dc - 128) * (1.0f / 256);
+      const float ytob = 1.0f * (f->ytob_dc - 0) * (1.0f / 128);
+      for (size_t i = 0; i < nb; ++i) {
+        const int qy = (int)roundf(dcf[1][i] * (inv0[1] * qdc));
+        const float ydec = qy * mul[1];
+        const float xv = dcf[0][i] - ytox * ydec;
+        const float bv = dcf[2][i] - ytob * ydec;
+        dc16[1][i] = (int16_t)qy;
+        dc16[0][i] = (int16_t)(int)roundf(xv * (inv0[0] * qdc));
+        dc16[2][i] = (int16_t)(int)roundf(bv * (inv0[2] * qdc));
+      }
+    }
+    /* dc_dec: the decoder's DequantDC and AdaptiveDCReconstruction do the
+     * same arithmetic as the encoder's (pik/compressed_image.cc:272-298). */
+    if ((rc = reconstruct_dc(&k, dcq, dc))) goto done;
+    /* Per group: InitializeEncCache's DC with a replicated border, then
+     * ComputeLlf and CopyLlf (pik/ac_predictions.cc:380-405, 581-607). */
+    for (int gy = 0; gy < (int)div_ceil(by, kGroupBlocks); ++gy) {
+      for (int gx = 0; gx < (int)div_ceil(bx, kGroupBlocks); ++gx) {
+        const int x0 = gx * kGroupBlocks, y0 = gy * kGroupBlocks;
+        const int gw = bx - x0 < kGroupBlocks ? bx - x0 : kGroupBlocks;
+        const int gh = by - y0 < kGroupBlocks ? by - y0 : kGroupBlocks;
+        const int cw = gw + 2, ch = gh + 2;
+        float* cdc = (float*)malloc(sizeof(float) * cw * ch);
+        float llf[16];
+        if (!cdc) {
+          rc = fail(PIKGPU_ENOMEM, "out of memory");
+          goto done;
+        }
+        for (int c = 0; c < 3; ++c) {
+          for (int y = 0; y < ch; ++y) {
+            const size_t sy = source_coord(y + y0, by);
+            for (int x = 0; x < cw; ++x) {
+              cdc[y * cw + x] = dc[c][sy * bx + source_coord(x + x0, bx)];
+            }
+          }
+          for (int y = 0; y < gh; ++y) {
+            for (int x = 0; x < gw; ++x) {
+              if (!acs_first_at(&k, x0 + x, y0 + y)) continue;
+              const int raw = acs_raw_at(&k, x0 + x, y0 + y), s = covered(raw);
+              llf_from_dc(raw, cdc + (y + 1) * cw + x + 1, cw, llf, 4);
+              float* cb = coef[c] + (y0 + y) * row + (size_t)(x0 + x) * 64;
+              for (int iy = 0; iy < s; ++iy) {
+                for (int ix = 0; ix < s; ++ix) cb[s * 8 * iy + ix] = llf[iy * 4 + ix];
+              }
+            }
+          }
+        }
+        free(cdc);
+      }
+    }
+    /* QuantizeRoundtripBlockAC of Y, UnapplyColorCorrelationAC, then
+     * QuantizeBlockAC of every channel (pik/compressed_image.cc:396-453). */
+    {
+      static const float kZeroBias[3] = {0.65f, 0.6f, 0.7f};
+      const float gsf = (float)(f->global_scale * (1.0 / 65536));
+      float inv_m[64 * 16];
+      int16_t qy[64 * 16];
+      float ydec[64 * 16], vals[64 * 16];
+      for (int y = 0; y < by; ++y) {
+        for (int x = 0; x < bx; ++x) {
+          if (!acs_first_at(&k, x, y)) continue;
+          const int raw = acs_raw_at(&k, x, y), s = covered(raw);
+          const int kind = quant_kind(raw);
+          const int quant = f->quant_field[y * f->quant_field_stride + x];
+          const int tx = x / 8, ty = y / 8;
+          const int cf = f->dequant_cf[ty * f->dequant_cf_stride + tx];
+          /* indexed by the quant value itself, as the encoder does */
+          if (cf * 256 + quant >= 16 * 256) {
+            rc = fail(PIKGPU_EUNSUPPORTED, "quant value past the dequant map");
+            goto done;
+          }
+          const int table = f->dequant_map[cf * 256 + quant];
+          const float qac = gsf * quant;
+          const float inv_qac = k.inv_global_scale / quant;
+          const int n = 64 * s * s;
+          /* Y first: X and B subtract its dequantised values */
+          for (int ci = 0; ci < 3; ++ci) {
+            const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+            const float* dm = dequant_matrix(&k, table, kind, c);
+            for (int i = 0; i < n; ++i) inv_m[i] = 1.0f / dm[i];
+            /* the transform's coefficients as pos = y * 8s + x, with the
+             * colour correlation of Y removed from X and B */
+            for (int r = 0; r < s; ++r) {
+              for (int i = 0; i < 64 * s; ++i) {
+                const size_t at = (size_t)(y + r) * row + (size_t)x * 64 + i;
+                const int pos = r * 64 * s + i;
+                float v = coef[c][at];
+                if (c != 1) {
+                  const int ttx = (x + i / 64) / 8, tty = (y + r) / 8;
+                  const float fac =
+                      c == 0 ? 1.0f * (f->ytox_map[tty * f->ytox_stride + ttx] -
+                                       128) * (1.0f / 256)
+                             : 1.0f * (f->ytob_map[tty * f->ytob_stride + ttx] -
+                                       0) * (1.0f / 128);
+                  v = v - fac * ydec[pos];
+                }
+                vals[pos] = v;
+              }
+            }
+            int16_t qbuf[64 * 16];
+            int16_t* qout = c == 1 ? qy : qbuf;
+            for (int iy = 0; iy < s; ++iy) {
+              for (int ix = 0; ix < s; ++ix) {
+                quantize_sub_block(vals, inv_m, qac, kZeroBias[c], s, s, ix,
+                                   iy, qout);
+              }
+            }
+            if (c == 1) {
+              for (int i = 0; i < n; ++i) {
+                ydec[i] = adjust_quant_bias_int(1, qy[i]) * dm[i] * inv_qac;
+              }
+            }
+            for (int r = 0; r < s; ++r) {
+              memcpy(ac16[c] + (size_t)(y + r) * row + (size_t)x * 64,
+                     qout + r * 64 * s, sizeof(int16_t) * 64 * s);
+            }
+          }
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      if (ac_out && ac_out[c]) memcpy(ac_out[c], ac16[c], sizeof(int16_t) * nb * 64);
+      if (dc_out && dc_out[c]) memcpy(dc_out[c], dc16[c], sizeof(int16_t) * nb);
+    }
+  }
+done:
+  free(q16);
+  free(mem);
+  return rc;
+}

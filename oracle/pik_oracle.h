@@ -37,6 +37,15 @@ int pikoracle_recon_passes(const pikgpu_frame* frame, int num_extra,
 
 /* The downsampled decode for ds = 2, 4, 8 (no AR, no Gaborish, no dots):
  * `out` receives 3 planes of ceil(xsize / ds) x ceil(ysize / ds) floats. */
+/* Encoder-side quantization of an 8-padded XYB image (three dense
+ * bx*8 x by*8 planes) under the frame's side information, as
+ * InitializeFrameEncCache + ComputeCoefficients compute it
+ * (pik/compressed_image.cc:210-455): quantized AC (three dense planes of
+ * by rows x bx*64) and DC (three dense bx x by planes). Frames without
+ * predictions, dots or the grayscale flag. */
+int pikoracle_quantize(const pikgpu_frame* frame, const float* const* opsin,
+                       int16_t* const* ac_out, int16_t* const* dc_out);
+
 int pikoracle_recon_downsampled(const pikgpu_frame* frame, int ds,
                                 float* const out[3]);
 

@@ -33,6 +33,9 @@ def lib():
         L.pikoracle_recon_downsampled.argtypes = [
             ctypes.POINTER(abi.FrameC), ctypes.c_int, ctypes.c_void_p]
         L.pikoracle_last_error.restype = ctypes.c_char_p
+        L.pikoracle_quantize.argtypes = [ctypes.POINTER(abi.FrameC),
+                                         ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]
         L.pikoracle_block_transform.argtypes = [ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_void_p,
                                                 ctypes.c_void_p]
@@ -123,3 +126,26 @@ def transform(raw, direction, block):
                                          out.ctypes.data)
     assert rc == 0
     return out.reshape(n, n)
+
+
+def quantize(frame, opsin):
+    """The C restatement of the encoder's quantization (frames without
+    predictions): (ac (3, by, bx * 64) int16, dc (3, by, bx) int16) of the
+    8-padded XYB image `opsin` (3, by * 8, bx * 8)."""
+    L = lib()
+    fc, keep = frame.to_c()
+    bx, by = frame.xsize_blocks, frame.ysize_blocks
+    img = np.ascontiguousarray(opsin, np.float32)
+    assert img.shape == (3, by * 8, bx * 8), img.shape
+    ac = np.zeros((3, by, bx * 64), np.int16)
+    dc = np.zeros((3, by, bx), np.int16)
+    ip = (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
+    ap = (ctypes.c_void_p * 3)(*[ac[c].ctypes.data for c in range(3)])
+    dp = (ctypes.c_void_p * 3)(*[dc[c].ctypes.data for c in range(3)])
+    rc = L.pikoracle_quantize(ctypes.byref(fc), ctypes.cast(ip, ctypes.c_void_p),
+                              ctypes.cast(ap, ctypes.c_void_p),
+                              ctypes.cast(dp, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikoracle_quantize: %s"
+                           % L.pikoracle_last_error().decode())
+    return ac, dc

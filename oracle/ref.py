@@ -52,6 +52,8 @@ def lib(kind="scalar"):
     L.pikref_opsin_to_linear.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
                                          ctypes.c_size_t]
     L.pikref_linear_to_xyb.argtypes = [ctypes.c_float] * 3 + [ctypes.c_void_p]
+    L.pikref_quantize.argtypes = [ctypes.POINTER(abi.FrameC), ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
     _libs[kind] = L
     return L
 
@@ -171,3 +173,27 @@ def opsin_to_linear(img):
     p = (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
     lib().pikref_opsin_to_linear(p, xs, ys)
     return img
+
+
+def quantize(frame, opsin, kind="scalar"):
+    """The reference encoder's quantized coefficients of the 8-padded XYB
+    image `opsin` (3, by * 8, bx * 8) under the frame's side information
+    (InitializeFrameEncCache + ComputeCoefficients per group, as
+    RoundtripImage runs them). Returns (ac (3, by, bx * 64) int16,
+    dc (3, by, bx) int16)."""
+    L = lib(kind)
+    fc, keep = frame.to_c()
+    bx, by = frame.xsize_blocks, frame.ysize_blocks
+    img = np.ascontiguousarray(opsin, np.float32)
+    assert img.shape == (3, by * 8, bx * 8), img.shape
+    ac = np.zeros((3, by, bx * 64), np.int16)
+    dc = np.zeros((3, by, bx), np.int16)
+    ip = (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
+    ap = (ctypes.c_void_p * 3)(*[ac[c].ctypes.data for c in range(3)])
+    dp = (ctypes.c_void_p * 3)(*[dc[c].ctypes.data for c in range(3)])
+    rc = L.pikref_quantize(ctypes.byref(fc), ctypes.cast(ip, ctypes.c_void_p),
+                           ctypes.cast(ap, ctypes.c_void_p),
+                           ctypes.cast(dp, ctypes.c_void_p))
+    if rc != 0:
+        raise RuntimeError("pikref_quantize: %s" % L.pikref_last_error().decode())
+    return ac, dc

@@ -592,6 +592,127 @@ API int pikref_opsin_to_linear(float* const* planes, size_t xs, size_t ys) {
   return 0;
 }
 
+// The encoder's coefficient computation as RoundtripImage runs it
+// (pik/adaptive_quantization.cc:580-651): InitializeFrameEncCache on the
+// whole XYB image, then per group InitializeEncCache and ComputeCoefficients
+// with the group's Quantizer copy and colour-tile rect. `opsin` holds three
+// dense planes of the 8-padded image (bx * 8 by by * 8); the frame supplies
+// the side information (its ac / dc are ignored). `ac_out` receives the
+// quantized AC (three dense planes of by rows of bx * 64), `dc_out` the
+// quantized DC (three dense bx by by planes); either may be NULL.
+API int pikref_quantize(const pikgpu_frame* f, const float* const* opsin,
+                        int16_t* const* ac_out, int16_t* const* dc_out) {
+  if (f == nullptr || opsin == nullptr) return Fail("null argument");
+  if (f->num_dots != 0 || f->grayscale != 0) {
+    return Fail("quantize: frames without dots, not grayscale");
+  }
+  InitEpfOnce();
+  const size_t xsize = f->xsize, ysize = f->ysize;
+  const size_t bx = DivCeil<size_t>(xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(ysize, kBlockDim);
+  DequantMatrices matrices(/*need_inv_matrices=*/true);
+  {
+    std::vector<float> t;
+    std::vector<uint32_t> o;
+    ExportTables(matrices, &t, &o);
+    if (f->dequant_table_size != t.size() ||
+        memcmp(f->dequant_table, t.data(), t.size() * 4) != 0 ||
+        f->num_dequant_offsets != o.size() ||
+        memcmp(f->dequant_offsets, o.data(), o.size() * 4) != 0) {
+      return Fail("frame does not use the default dequant tables");
+    }
+  }
+  FrameHeader frame_header;
+  frame_header.flags = 0;
+  frame_header.predict_lf = f->predict_lf != 0;
+  frame_header.predict_hf = f->predict_hf != 0;
+  GroupHeader group_header;
+  Quantizer quantizer =
+      DecoderQuantizer(&matrices, bx, by, f->quant_dc, f->global_scale);
+  {
+    ImageI qf(bx, by);
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(qf.Row(y), f->quant_field + y * f->quant_field_stride, bx * 4);
+    }
+    quantizer.SetRawQuantField(std::move(qf));
+  }
+  ColorCorrelationMap cmap(xsize, ysize);
+  for (size_t y = 0; y < cmap.ytox_map.ysize(); ++y) {
+    memcpy(cmap.ytox_map.Row(y), f->ytox_map + y * f->ytox_stride,
+           cmap.ytox_map.xsize() * 4);
+    memcpy(cmap.ytob_map.Row(y), f->ytob_map + y * f->ytob_stride,
+           cmap.ytob_map.xsize() * 4);
+  }
+  cmap.ytox_dc = f->ytox_dc;
+  cmap.ytob_dc = f->ytob_dc;
+  AcStrategyImage ac_strategy;
+  {
+    ImageB raw(bx, by);
+    for (size_t y = 0; y < by; ++y) {
+      for (size_t x = 0; x < bx; ++x) {
+        raw.Row(y)[x] = f->ac_strategy[y * f->ac_strategy_stride + x] >> 4;
+      }
+    }
+    ac_strategy = AcStrategyImage(bx, by);
+    ac_strategy.SetFromRaw(Rect(0, 0, bx, by), raw);
+  }
+  const size_t tx = DivCeil<size_t>(xsize, kColorTileDim);
+  const size_t ty = DivCeil<size_t>(ysize, kColorTileDim);
+  FrameEncCache fec;
+  fec.dequant_control_field = ImageB(tx, ty);
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(fec.dequant_control_field.Row(y),
+           f->dequant_cf + y * f->dequant_cf_stride, tx);
+  }
+  memcpy(fec.dequant_map, f->dequant_map, sizeof(fec.dequant_map));
+  Image3F img(bx * kBlockDim, by * kBlockDim);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < by * kBlockDim; ++y) {
+      memcpy(img.PlaneRow(c, y), opsin[c] + y * bx * kBlockDim,
+             bx * kBlockDim * 4);
+    }
+  }
+  BlockDictionary dictionary;
+  InitializeFrameEncCache(frame_header, img, ac_strategy, quantizer, cmap,
+                          dictionary, nullptr, &fec, nullptr);
+  const size_t xsize_groups = DivCeil<size_t>(xsize, kGroupDim);
+  const size_t ysize_groups = DivCeil<size_t>(ysize, kGroupDim);
+  for (size_t gy = 0; gy < ysize_groups; ++gy) {
+    for (size_t gx = 0; gx < xsize_groups; ++gx) {
+      const Rect group_rect(gx * kGroupDim, gy * kGroupDim, kGroupDim,
+                            kGroupDim, bx * kBlockDim, by * kBlockDim);
+      const Rect block_rect(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                            kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+      EncCache cache;
+      InitializeEncCache(frame_header, group_header, fec, group_rect, &cache);
+      Quantizer quant = quantizer.Copy(block_rect);
+      const Rect tiles(block_rect.x0() / kColorTileDimInBlocks,
+                       block_rect.y0() / kColorTileDimInBlocks,
+                       DivCeil(block_rect.xsize(), kColorTileDimInBlocks),
+                       DivCeil(block_rect.ysize(), kColorTileDimInBlocks));
+      ComputeCoefficients(quant, cmap, tiles, nullptr, fec, &cache, nullptr);
+      if (ac_out == nullptr) continue;
+      for (int c = 0; c < 3; ++c) {
+        if (ac_out[c] == nullptr) continue;
+        for (size_t y = 0; y < block_rect.ysize(); ++y) {
+          memcpy(ac_out[c] + (block_rect.y0() + y) * bx * 64 +
+                     block_rect.x0() * 64,
+                 cache.ac.ConstPlaneRow(c, y), block_rect.xsize() * 64 * 2);
+        }
+      }
+    }
+  }
+  if (dc_out != nullptr) {
+    for (int c = 0; c < 3; ++c) {
+      if (dc_out[c] == nullptr) continue;
+      for (size_t y = 0; y < by; ++y) {
+        memcpy(dc_out[c] + y * bx, fec.dc.ConstPlaneRow(c, y), bx * 2);
+      }
+    }
+  }
+  return 0;
+}
+
 // Reference LinearToXyb for one pixel (for generator tests).
 API void pikref_linear_to_xyb(float r, float g, float b, float* xyb) {
   LinearToXyb(r, g, b, &xyb[0], &xyb[1], &xyb[2]);

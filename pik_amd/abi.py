@@ -170,6 +170,8 @@ def load():
                                              ctypes.c_void_p, ctypes.c_void_p]
     lib.pikgpu_set_output.argtypes = [vp, ctypes.c_int]
     lib.pikgpu_set_downsample.argtypes = [vp, ctypes.c_int]
+    lib.pikgpu_quantize.argtypes = [vp, vp, ctypes.c_size_t, vp,
+                                    ctypes.c_size_t, vp, ctypes.c_size_t]
     lib.pikgpu_frame_add_pass.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
     lib.pikgpu_download_srgb8.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
     lib.pikgpu_default_dequant_tables.argtypes = [
@@ -191,6 +193,6 @@ EXPORTED_SYMBOLS = [
     "pikgpu_download_srgb8", "pikgpu_batch_create", "pikgpu_batch_destroy",
     "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
-    "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample",
+    "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
     "pikgpu_frame_add_pass",
 ]

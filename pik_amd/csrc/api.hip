@@ -60,6 +60,11 @@ struct pikgpu_ctx {
       dots_img, gab, out, mm_partial, mm, ar_keep, srgb8, gimg, predlist;
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
+  // pikgpu_quantize: XYB input, coefficients, DC, the inverse dequant table
+  DevBuf eopsin, ecoef, edcf, itable;
+  bool enc_map_ok = false;  // every quant_table index of the frame in range
+  int global_scale = 0, quant_dc = 0;  // the uploaded frame's
+  float dc_matrix0[3] = {0, 0, 0};     // DequantMatrix(0, DCT8, c)[0]
   bool adj_ready = false;  // LLF / LF caches hold the uploaded frame's
   bool adj_timed = false;
   bool uploaded = false;
@@ -574,6 +579,35 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->offsets.p, f->dequant_offsets,
                               f->num_dequant_offsets * 4,
                               hipMemcpyHostToDevice, ctx->stream));
+  // The encoder's inverse table (DequantMatrices, pik/quant_weights.cc:
+  // 918-923) and whether its quant_table lookups, which index the map by
+  // the quant value itself (pik/compressed_image.cc:417, 445), stay inside
+  // the 16 x 256 map.
+  {
+    std::vector<float> inv(f->dequant_table_size);
+    for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0f / f->dequant_table[i];
+    if ((rc = Grow(ctx, &ctx->itable, inv.size() * 4))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->itable.p, inv.data(), inv.size() * 4,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    bool ok = true;
+    for (size_t y = 0; y < by && ok; ++y) {
+      for (size_t x = 0; x < bx; ++x) {
+        const int q = f->quant_field[y * f->quant_field_stride + x];
+        const int cf = f->dequant_cf[(y / 8) * f->dequant_cf_stride + x / 8];
+        if (cf * 256 + q >= 16 * 256) {
+          ok = false;
+          break;
+        }
+      }
+    }
+    ctx->enc_map_ok = ok;
+    ctx->global_scale = int(f->global_scale);
+    ctx->quant_dc = int(f->quant_dc);
+    for (int c = 0; c < 3; ++c) {
+      ctx->dc_matrix0[c] = f->dequant_table[f->dequant_offsets[c]];
+    }
+  }
   std::vector<int32_t> dot_ids, dot_starts, dot_list;
   if (f->num_dots) {
     HIP_TRY(ctx, hipMemcpyAsync(ctx->dots.p, f->dots,
@@ -1003,6 +1037,99 @@ int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
   P.num_passes += 1;
   ctx->B.ac_extra = static_cast<const int16_t*>(ctx->ac_extra.p);
   ctx->ran = false;
+  return PIKGPU_OK;
+}
+
+// Encoder-side quantization of an XYB image under the uploaded frame's side
+// information (see pikgpu.h). The quantized AC and DC replace the frame's,
+// so pikgpu_run then decodes the round trip.
+int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
+                    size_t opsin_stride, int16_t* ac_out[3], size_t ac_stride,
+                    int16_t* dc_out[3], size_t dc_stride) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  if (opsin == nullptr || !opsin[0] || !opsin[1] || !opsin[2]) {
+    return Fail(ctx, PIKGPU_EINVAL, "need three opsin planes");
+  }
+  FrameParams& P = ctx->P;
+  if (opsin_stride < size_t(P.px)) {
+    return Fail(ctx, PIKGPU_EINVAL, "opsin stride below the padded width");
+  }
+  if (P.predict_lf || P.predict_hf || P.num_dots != 0 || P.grayscale) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "quantization of frames with predictions, dots or grayscale");
+  }
+  if (!ctx->enc_map_ok) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED, "quant value past the dequant map");
+  }
+  const size_t nb = size_t(P.bx) * P.by, row = size_t(P.bx) * 64;
+  if ((ac_out && (!ac_out[0] || !ac_out[1] || !ac_out[2] || ac_stride < row)) ||
+      (dc_out && (!dc_out[0] || !dc_out[1] || !dc_out[2] ||
+                  dc_stride < size_t(P.bx)))) {
+    return Fail(ctx, PIKGPU_EINVAL, "bad output planes or strides");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t plane = size_t(P.px) * P.py;
+  int rc;
+  if ((rc = Grow(ctx, &ctx->eopsin, 3 * plane * 4))) return rc;
+  if ((rc = Grow(ctx, &ctx->ecoef, 3 * nb * 64 * 4))) return rc;
+  if ((rc = Grow(ctx, &ctx->edcf, 3 * nb * 4))) return rc;
+  float* dev_opsin = static_cast<float*>(ctx->eopsin.p);
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = Upload2D(ctx, dev_opsin + c * plane, opsin[c], opsin_stride * 4,
+                       size_t(P.px) * 4, size_t(P.py))))
+      return rc;
+  }
+  // the quantized values replace the frame's own (and any later passes)
+  P.num_passes = 1;
+  ctx->B.ac_extra = nullptr;
+  ctx->ran = false;
+  ctx->adj_ready = false;
+  const FrameBuffers& B = ctx->B;
+  const DeviceConsts* K = ctx->consts;
+  hipStream_t st = ctx->stream;
+  float* coef = static_cast<float*>(ctx->ecoef.p);
+  float* dcf = static_cast<float*>(ctx->edcf.p);
+  int16_t* ac16 = static_cast<int16_t*>(ctx->ac.p);
+  int16_t* dc16 = static_cast<int16_t*>(ctx->dc16.p);
+  const int tiles = int(DivCeil(P.bx, 4) * DivCeil(P.by, 4));
+  KEncForward<<<tiles, 256, 0, st>>>(P, B, K, dev_opsin, coef, dcf);
+  // Quantizer::QuantizeDC's factors (float arithmetic as the reference's)
+  const float gsf = static_cast<float>(ctx->global_scale * (1.0 / 65536));
+  const float qdc = gsf * ctx->quant_dc;
+  EncDcScale S;
+  for (int c = 0; c < 3; ++c) {
+    const float inv0 = 1.0f / ctx->dc_matrix0[c];
+    S.scale[c] = inv0 * qdc;
+  }
+  KEncDc<<<DivCeil(nb, 256), 256, 0, st>>>(P, dcf, S, dc16);
+  // dc_dec and the LLF exactly as the decoder computes them
+  EnqueueDc(P, B, K, st);
+  {
+    const size_t ngroups = size_t(P.gx) * P.gy;
+    const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
+    dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
+    const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
+    const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+    KComputeLlf<<<wg4 + wg2 + gc.x * gc.y * gc.z, 256, 0, st>>>(
+        P, B, K, 0, wg4, wg2, int(gc.x), int(ngroups));
+  }
+  KEncQuant<<<DivCeil(nb, kEncQuantThreads), kEncQuantThreads, 0, st>>>(
+      P, B, coef, static_cast<const float*>(ctx->itable.p), gsf, ac16);
+  HIP_TRY(ctx, hipGetLastError());
+  for (int c = 0; c < 3; ++c) {
+    if (ac_out) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(ac_out[c], ac_stride * 2,
+                                    ac16 + c * nb * 64, row * 2, row * 2,
+                                    size_t(P.by), hipMemcpyDeviceToHost, st));
+    }
+    if (dc_out) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(dc_out[c], dc_stride * 2, dc16 + c * nb,
+                                    size_t(P.bx) * 2, size_t(P.bx) * 2,
+                                    size_t(P.by), hipMemcpyDeviceToHost, st));
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(st));
   return PIKGPU_OK;
 }
 

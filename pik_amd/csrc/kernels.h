@@ -546,8 +546,87 @@ __device__ inline int XcdTile(int wg, int num_tiles) {
 }
 
 // ---------------------------------------------------------------------------
+// Small reinterpreting transforms with compile-time sizes: every array index
+// is static after unrolling, so the blocks stay in registers.
+
+// 2-point transforms in the reference's explicit form (pik/dct.h:478-504).
+__device__ inline void Idct2x2(float a00, float a01, float a10, float a11,
+                               float* o, int os) {
+  o[0] = a00 + a01 + a10 + a11;
+  o[1] = a00 + a01 - a10 - a11;
+  o[os] = a00 - a01 + a10 - a11;
+  o[os + 1] = a00 - a01 - a10 + a11;
+}
+
+// Index of the scale tables for size n: log2(n) - 1.
+template <int N>
+constexpr int ScaleIdx() {
+  return N == 2 ? 0 : (N == 4 ? 1 : (N == 8 ? 2 : (N == 16 ? 3 : 4)));
+}
+
+// ReinterpretingDCT<DCTN, N, N> (pik/ac_strategy.cc:272-293), N >= 4:
+// N x N input at `in` (pitch ip) -> N x N output at `out` (pitch op); in and
+// out may alias.
+template <int DCTN, int N>
+__device__ inline void ReinterpDct(const DeviceConsts* K, const float* in,
+                                   int ip, float* out, int op) {
+  float blk[N * N], coef[N * N];
+  for (int y = 0; y < N; ++y) {
+    for (int x = 0; x < N; ++x) blk[y * N + x] = in[y * ip + x];
+  }
+  dct1d::Dct2D<N>(blk, N, coef, N);
+  constexpr int ti = ScaleIdx<N>(), ai = ScaleIdx<DCTN>();
+  for (int y = 0; y < N; ++y) {
+    for (int x = 0; x < N; ++x) {
+      const float v = coef[y * N + x] * K->tscale[ti][y][x];
+      out[y * op + x] = v * K->ascale[ai][y][x];
+    }
+  }
+}
+
+// ReinterpretingIDCT<DCTN, LF, N> (pik/ac_strategy.cc:296-315), N >= 4:
+// the LF x LF input (pitch ip) zero-padded to N x N.
+template <int DCTN, int LF, int N>
+__device__ inline void ReinterpIdct(const DeviceConsts* K, const float* in,
+                                    int ip, float* out, int op) {
+  float blk[N * N], pix[N * N];
+  constexpr int ai = ScaleIdx<N>(), ti = ScaleIdx<DCTN>();
+#pragma unroll
+  for (int y = 0; y < N; ++y) {
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+      float v = 0.0f;
+      if (y < LF && x < LF) {
+        v = (in[y * ip + x] * K->ascale[ai][y][x]) * K->tscale[ti][y][x];
+      }
+      blk[y * N + x] = v;
+    }
+  }
+  dct1d::Idct2D<N>(blk, N, pix, N);
+#pragma unroll
+  for (int y = 0; y < N; ++y) {
+#pragma unroll
+    for (int x = 0; x < N; ++x) out[y * op + x] = pix[y * N + x];
+  }
+}
+
+// Quantizer::QuantizeDC's factor per channel: InvMatrix(0, DCT8, c)[0] *
+// (Scale() * quant_dc) (pik/quantizer.h:350-353).
+struct EncDcScale {
+  float scale[3];
+};
+// KEncQuant's workgroup size (its LDS holds 64 floats per thread).
+constexpr int kEncQuantThreads = 128;
+
+// ---------------------------------------------------------------------------
 // Kernels.
 __global__ void KDcDequant(FrameParams P, FrameBuffers B);
+__global__ void KEncForward(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                            const float* opsin, float* coef, float* dcf);
+__global__ void KEncDc(FrameParams P, const float* dcf, EncDcScale S,
+                       int16_t* dc16);
+__global__ void KEncQuant(FrameParams P, FrameBuffers B, const float* coef,
+                          const float* itable, float gsf, int16_t* ac);
 __global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                             int dc2x2, int wg4, int wg2, int pos_blocks,
                             int ngroups);

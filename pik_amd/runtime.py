@@ -103,6 +103,28 @@ class Reconstructor:
         self._check(self._lib.pikgpu_frame_add_pass(
             self._ctx, ctypes.cast(ptrs, ctypes.c_void_p), planes[0].shape[1]))
 
+    def quantize(self, opsin):
+        """Encoder-side quantization of the 8-padded XYB image `opsin`
+        (3, by * 8, bx * 8) under the uploaded frame's side information
+        (pikgpu_quantize). The result replaces the frame's coefficients on
+        the device, so run() then decodes the round trip. Returns
+        (ac (3, by, bx * 64) int16, dc (3, by, bx) int16)."""
+        f = self._frame
+        bx, by = f.xsize_blocks, f.ysize_blocks
+        img = np.ascontiguousarray(opsin, np.float32)
+        if img.shape != (3, by * 8, bx * 8):
+            raise ValueError("opsin must be (3, %d, %d)" % (by * 8, bx * 8))
+        ac = np.zeros((3, by, bx * 64), np.int16)
+        dc = np.zeros((3, by, bx), np.int16)
+        vp = ctypes.c_void_p
+        ip = (vp * 3)(*[img[c].ctypes.data for c in range(3)])
+        ap = (vp * 3)(*[ac[c].ctypes.data for c in range(3)])
+        dp = (vp * 3)(*[dc[c].ctypes.data for c in range(3)])
+        self._check(self._lib.pikgpu_quantize(
+            self._ctx, ctypes.cast(ip, vp), bx * 8, ctypes.cast(ap, vp),
+            bx * 64, ctypes.cast(dp, vp), bx))
+        return ac, dc
+
     def set_downsample(self, factor):
         """Downsampling factor (1, 2, 4, 8) of the following runs
         (pikgpu_set_downsample)."""

@@ -432,6 +432,23 @@ def make_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
     return encode_frame(rgb, distance=distance, seed=seed, **kw)
 
 
+def make_opsin_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
+                     line_art=False, predict=True, lut_ids="zero"):
+    """A frame's side information together with the XYB image it was made
+    from, padded to whole blocks by edge replication as PadImageToMultiple
+    does (pik/image.cc:45-80): (frame, opsin (3, by * 8, bx * 8) float32).
+    The encoder-side quantization takes both."""
+    rgb = synthetic_srgb(xsize, ysize, seed, num_rects=num_rects,
+                         line_art=line_art)
+    frame = encode_frame(rgb, distance=distance, seed=seed, predict=predict,
+                         lut_ids=lut_ids)
+    xyb = linear_to_xyb(srgb8_to_linear255(rgb))
+    bx, by = div_ceil(xsize, 8), div_ceil(ysize, 8)
+    xyb = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
+                 mode="edge")
+    return frame, np.ascontiguousarray(xyb, np.float32)
+
+
 def make_extra_passes(frame, num_passes, seed=0, density=0.02, amplitude=1):
     """AC refinement passes for `frame` (multi-pass accumulation): each a
     (3, by, bx * 64) int16 array of sparse small values on the frame's
