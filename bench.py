@@ -22,6 +22,11 @@ Two further figures ride on the same line:
            2048x2048 frame, loss sum (linear - target)^2, gradient with
            respect to the 256x256 float quant field and the dequantised
            coefficients; MP/s.
+  quantize encoder-side quantization (pikgpu_quantize) of a 4096x4096 XYB
+           image without predictions: forward transforms, DC and dithered
+           AC quantization; MP/s of the kernels (HIP events), and the rate
+           with the host-to-device copy of the float image and the int16
+           downloads included.
 """
 import argparse
 import json
@@ -84,6 +89,8 @@ def parse_args():
                     help="distinct synthetic frames cycled over the batch")
     ap.add_argument("--adj-size", type=int, default=2048,
                     help="frame size of the fwd_adj figure (0: skip)")
+    ap.add_argument("--quant-size", type=int, default=4096,
+                    help="frame size of the quantize figure (0: skip)")
     return ap.parse_args()
 
 
@@ -257,6 +264,33 @@ def bench_fwd_adj(size, steps, warmup, seed):
                         % (size, size, by, bx)}
 
 
+def bench_quantize(size, steps, warmup, seed):
+    """MP/s of pikgpu_quantize on a size x size XYB image."""
+    from pik_amd import synth
+    from pik_amd.runtime import Reconstructor
+    frame, opsin = synth.make_opsin_frame(size, size, seed=seed + 29,
+                                          distance=1.0, num_rects=100,
+                                          predict=False)
+    with Reconstructor(0) as R:
+        R.upload(frame)
+        for _ in range(max(1, warmup)):
+            R.quantize(opsin)
+        kms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.quantize(opsin)
+            kms += R.timings()["enc_quantize"]
+        el = time.perf_counter() - t0
+    mp = frame.xsize * frame.ysize / 1e6
+    return {"value": round(mp * steps / (kms / 1e3), 2), "unit": "MP/s",
+            "kernel_ms": round(kms / steps, 4),
+            "with_copies_MPps": round(mp * steps / el, 2),
+            "steps": steps,
+            "workload": "quantize %dx%d d=1.0 XYB image, no predictions "
+                        "(forward transforms + DC + dithered AC)"
+                        % (size, size)}
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -355,6 +389,11 @@ def main():
     if a.adj_size > 0:
         out["fwd_adj"] = bench_fwd_adj(a.adj_size, a.steps, a.warmup, a.seed)
         print("fwd_adj: " + json.dumps(out["fwd_adj"]), file=sys.stderr,
+              flush=True)
+    if a.quant_size > 0:
+        out["quantize"] = bench_quantize(a.quant_size, max(1, a.steps // 4),
+                                         a.warmup, a.seed)
+        print("quantize: " + json.dumps(out["quantize"]), file=sys.stderr,
               flush=True)
     print(json.dumps(out), flush=True)
 

@@ -23,16 +23,16 @@ using namespace pikgpu;
 
 namespace {
 
-// Kernel families timed by pikgpu_run (events ev[0..kNumRunTimers]) and by
-// pikgpu_fwd_adj (adj_ev[0..2]).
+// Kernel families timed by pikgpu_run (events ev[0..kNumRunTimers]), by
+// pikgpu_fwd_adj (adj_ev[0..2]) and by pikgpu_quantize (enc_ev[0..1]).
 enum Timer {
   kTDc, kTLf, kTBlocks, kTMinMax, kTEpf, kTDots, kTFinal, kNumRunTimers,
-  kTAdjForward = kNumRunTimers, kTAdjBackward, kNumTimers
+  kTAdjForward = kNumRunTimers, kTAdjBackward, kTEncQuantize, kNumTimers
 };
 const char* kTimerNames[kNumTimers] = {"dc",     "lf_predict", "blocks_idct",
                                        "guide",  "epf_ar",     "dots",
                                        "final_opsin", "adj_forward",
-                                       "adj_backward"};
+                                       "adj_backward", "enc_quantize"};
 
 struct DevBuf {
   void* p = nullptr;
@@ -82,6 +82,7 @@ struct pikgpu_ctx {
   int num_tables = 0;
   hipEvent_t ev[kNumRunTimers + 1] = {};
   hipEvent_t adj_ev[3] = {};
+  hipEvent_t enc_ev[2] = {};
   float timings[kNumTimers] = {};
 };
 
@@ -195,6 +196,12 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
       return nullptr;
     }
   }
+  for (hipEvent_t& e : ctx->enc_ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return nullptr;
+    }
+  }
   // Constant tables.
   DeviceConsts* h = new DeviceConsts;
   memset(h, 0, sizeof(*h));
@@ -226,6 +233,9 @@ void pikgpu_destroy(pikgpu_ctx* ctx) {
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   }
   for (hipEvent_t e : ctx->adj_ev) {
+    if (e) (void)hipEventDestroy(e);
+  }
+  for (hipEvent_t e : ctx->enc_ev) {
     if (e) (void)hipEventDestroy(e);
   }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1093,6 +1103,7 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
   int16_t* ac16 = static_cast<int16_t*>(ctx->ac.p);
   int16_t* dc16 = static_cast<int16_t*>(ctx->dc16.p);
   const int tiles = int(DivCeil(P.bx, 4) * DivCeil(P.by, 4));
+  HIP_TRY(ctx, hipEventRecord(ctx->enc_ev[0], st));
   KEncForward<<<tiles, 256, 0, st>>>(P, B, K, dev_opsin, coef, dcf);
   // Quantizer::QuantizeDC's factors (float arithmetic as the reference's)
   const float gsf = static_cast<float>(ctx->global_scale * (1.0 / 65536));
@@ -1117,6 +1128,7 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
   KEncQuant<<<DivCeil(nb, kEncQuantThreads), kEncQuantThreads, 0, st>>>(
       P, B, coef, static_cast<const float*>(ctx->itable.p), gsf, ac16);
   HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->enc_ev[1], st));
   for (int c = 0; c < 3; ++c) {
     if (ac_out) {
       HIP_TRY(ctx, hipMemcpy2DAsync(ac_out[c], ac_stride * 2,
@@ -1130,6 +1142,10 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
     }
   }
   HIP_TRY(ctx, hipStreamSynchronize(st));
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->enc_ev[0], ctx->enc_ev[1]) == hipSuccess) {
+    ctx->timings[kTEncQuantize] = ms;
+  }
   return PIKGPU_OK;
 }
 
