@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define PIKGPU_ABI_VERSION 2
+#define PIKGPU_ABI_VERSION 3
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PIKGPU_API __attribute__((visibility("default")))
@@ -150,7 +150,24 @@ typedef struct pikgpu_frame {
    * set to its luma in all three channels (pik/pik_frame.cc:1155-1166).
    * bool. */
   int32_t grayscale;
+
+  /* FrameHeader::flags verbatim (pik/headers.h:185-196, read at :209).
+   * PIKGPU_FLAG_GRAYSCALE_OPT has the meaning of `grayscale` above (either
+   * one selects it). PIKGPU_FLAG_GRADIENT_MAP (the DC is predicted from a
+   * gradient map instead of AdaptiveDCReconstruction, pik/compressed_dc.cc:
+   * 515-522) and PIKGPU_FLAG_NOISE (AddNoise after the final Gaborish,
+   * pik/compressed_image.cc:1171-1174) are refused with
+   * PIKGPU_EUNSUPPORTED, as is any bit the format does not define, so such
+   * a frame never decodes silently without them. */
+  uint32_t flags;
 } pikgpu_frame;
+
+/* Bits of pikgpu_frame.flags (FrameHeader::Flags, pik/headers.h:185-196). */
+enum {
+  PIKGPU_FLAG_GRADIENT_MAP = 1,
+  PIKGPU_FLAG_GRAYSCALE_OPT = 2,
+  PIKGPU_FLAG_NOISE = 4
+};
 
 typedef struct pikgpu_config {
   uint32_t abi_version; /* must be PIKGPU_ABI_VERSION */

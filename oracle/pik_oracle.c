@@ -60,6 +60,12 @@ static long mirror(long x, long n) {
 
 static size_t div_ceil(size_t a, size_t b) { return (a + b - 1) / b; }
 
+/* FrameHeader::kGrayscaleOpt (pik/headers.h:185-196) from either of the
+ * frame's two spellings. */
+static int is_gray(const pikgpu_frame* f) {
+  return f->grayscale != 0 || (f->flags & PIKGPU_FLAG_GRAYSCALE_OPT) != 0;
+}
+
 /* AcStrategy properties, pik/ac_strategy.h:41-170. */
 static int covered(int raw) {
   return raw == kDCT32 ? 4 : (raw == kDCT16 ? 2 : 1);
@@ -1259,7 +1265,7 @@ static int reconstruct_group(const Ctx* k, int gx, int gy, float* const dc[3],
   }
   /* kGrayscaleOpt: GrayXyb::RestoreXB on the group's DC
    * (pik/compressed_image.cc:953-956). */
-  if (f->grayscale) {
+  if (is_gray(f)) {
     init_gray();
     for (int i = 0; i < cw * ch; ++i) y_to_xb(cdc[1][i], &cdc[0][i], &cdc[2][i]);
   }
@@ -1703,6 +1709,9 @@ static int check_frame(const pikgpu_frame* f) {
   if (f->adaptive_reconstruction && !f->epf_enable_adaptive) {
     return fail(PIKGPU_EUNSUPPORTED, "fixed-sigma EPF");
   }
+  if (f->flags & ~(uint32_t)PIKGPU_FLAG_GRAYSCALE_OPT) {
+    return fail(PIKGPU_EUNSUPPORTED, "gradient map, noise or unknown flag");
+  }
   const size_t bx = div_ceil(f->xsize, 8), by = div_ceil(f->ysize, 8);
   for (size_t y = 0; y < by; ++y) {
     for (size_t x = 0; x < bx; ++x) {
@@ -1826,7 +1835,7 @@ static int recon(const pikgpu_frame* f, int num_extra,
     }
   }
   opsin_to_linear(out, (size_t)f->xsize * f->ysize);
-  if (f->grayscale) force_gray(out, (size_t)f->xsize * f->ysize);
+  if (is_gray(f)) force_gray(out, (size_t)f->xsize * f->ysize);
   dump(stages, PIKGPU_STAGE_LINEAR, out, f->xsize, f->ysize, f->xsize);
 done:
   free(mem);
@@ -1902,7 +1911,7 @@ int pikoracle_recon_downsampled(const pikgpu_frame* f, int ds,
       }
     }
     opsin_to_linear(out, ox * oy);
-    if (f->grayscale) force_gray(out, ox * oy);
+    if (is_gray(f)) force_gray(out, ox * oy);
   }
 done:
   free(mem);
@@ -1977,7 +1986,7 @@ int pikoracle_quantize(const pikgpu_frame* f, const float* const* opsin,
                        int16_t* const* ac_out, int16_t* const* dc_out) {
   init_tables();
   if (!f || !opsin) return fail(PIKGPU_EINVAL, "null argument");
-  if (f->predict_lf || f->predict_hf || f->num_dots || f->grayscale) {
+  if (f->predict_lf || f->predict_hf || f->num_dots || is_gray(f)) {
     return fail(PIKGPU_EUNSUPPORTED,
                 "quantization of frames with predictions, dots or grayscale");
   }

@@ -101,6 +101,11 @@ namespace {
 
 thread_local std::string g_error;
 
+// FrameHeader::kGrayscaleOpt from either of the frame's two spellings.
+bool IsGray(const pikgpu_frame* f) {
+  return f->grayscale != 0 || (f->flags & PIKGPU_FLAG_GRAYSCALE_OPT) != 0;
+}
+
 int Fail(const char* msg) {
   g_error = msg;
   return -1;
@@ -253,7 +258,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   }
 
   FrameHeader frame_header;
-  frame_header.flags = f->grayscale ? FrameHeader::kGrayscaleOpt : 0;
+  frame_header.flags = IsGray(f) ? FrameHeader::kGrayscaleOpt : 0;
   frame_header.gaborish = GaborishStrength(f->gaborish);
   frame_header.predict_lf = f->predict_lf != 0;
   frame_header.predict_hf = f->predict_hf != 0;
@@ -283,7 +288,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   cmap.ytob_dc = f->ytob_dc;
 
   FrameDecCache fdc;
-  fdc.grayscale = f->grayscale != 0;
+  fdc.grayscale = IsGray(f);
   // AC strategy: the frame holds combined bytes; SetFromRaw expands raw
   // types (pik/ac_strategy.cc:1066-1099).
   {
@@ -463,7 +468,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
     const size_t oy = DivCeil<size_t>(ysize, downsample);
     idct.ShrinkTo(ox, oy);
     OpsinToLinear(&idct, pool.get());
-    if (f->grayscale) ForceGray(&idct);
+    if (IsGray(f)) ForceGray(&idct);
     DumpImage3F(idct, out, ox, oy);
     return 0;
   }
@@ -508,7 +513,7 @@ static int Recon(const pikgpu_frame* f, int num_threads, size_t downsample,
   DumpImage3F(idct, stage(PIKGPU_STAGE_GABORISH), bx * 8, by * 8);
   idct.ShrinkTo(xsize, ysize);
   OpsinToLinear(&idct, pool.get());
-  if (f->grayscale) ForceGray(&idct);
+  if (IsGray(f)) ForceGray(&idct);
   DumpImage3F(idct, stage(PIKGPU_STAGE_LINEAR), xsize, ysize);
   DumpImage3F(idct, out, xsize, ysize);
   return 0;
@@ -603,7 +608,7 @@ API int pikref_opsin_to_linear(float* const* planes, size_t xs, size_t ys) {
 API int pikref_quantize(const pikgpu_frame* f, const float* const* opsin,
                         int16_t* const* ac_out, int16_t* const* dc_out) {
   if (f == nullptr || opsin == nullptr) return Fail("null argument");
-  if (f->num_dots != 0 || f->grayscale != 0) {
+  if (f->num_dots != 0 || IsGray(f)) {
     return Fail("quantize: frames without dots, not grayscale");
   }
   InitEpfOnce();

@@ -10,7 +10,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpikgpu.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 EINVAL = -1
@@ -29,6 +29,11 @@ ACS_DCT32X32 = 5
 ACS_LINES = 6
 ACS_DCT_NOHF = 7
 ACS_DCT4X4_NOHF = 8
+
+# FrameHeader::flags bits (pik/headers.h:185-196).
+FLAG_GRADIENT_MAP = 1
+FLAG_GRAYSCALE_OPT = 2
+FLAG_NOISE = 4
 
 OUTPUT_LINEAR = 1
 OUTPUT_SRGB8 = 2
@@ -89,6 +94,7 @@ class FrameC(ctypes.Structure):
         ("num_dequant_offsets", ctypes.c_size_t),
         ("dots", ctypes.c_void_p), ("num_dots", ctypes.c_size_t),
         ("grayscale", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
     ]
 
 

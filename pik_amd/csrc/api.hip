@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "host_api.h"
 #include "kernels.h"
 
 using namespace pikgpu;
@@ -39,8 +40,6 @@ struct DevBuf {
   size_t cap = 0;
 };
 
-// Most workgroups of a KMinMaxPartial launch (6 floats of partials each).
-constexpr int kMinMaxMaxGroups = 4096;
 
 // Most AC passes of a frame (pikgpu_frame_add_pass).
 constexpr int kMaxPasses = 16;
@@ -146,6 +145,10 @@ int Upload2D(pikgpu_ctx* ctx, void* dst, const void* src, size_t src_stride_b,
 
 float ToF(double v) { return static_cast<float>(v); }
 
+}  // namespace
+
+namespace pikgpu {
+
 // Inverse of the opsin absorbance matrix as Inv3x3Matrix<float> computes it
 // (pik/linalg.h:153-170, pik/opsin_params.cc:13-26).
 void OpsinInverse(float out[9]) {
@@ -169,6 +172,10 @@ void OpsinInverse(float out[9]) {
   const float idet = 1.0 / (m[0] * t[0] + m[1] * t[3] + m[2] * t[6]);
   for (int i = 0; i < 9; ++i) out[i] = t[i] * idet;
 }
+
+}  // namespace pikgpu
+
+namespace {
 
 size_t DivCeil(size_t a, size_t b) { return (a + b - 1) / b; }
 
@@ -251,12 +258,18 @@ void* pikgpu_stream(pikgpu_ctx* ctx) {
   return ctx ? static_cast<void*>(ctx->stream) : nullptr;
 }
 
+}  // extern "C"
+
 namespace {
 
 int VFail(std::string* err, int code, const char* msg) {
   if (err) *err = msg;
   return code;
 }
+
+}  // namespace
+
+namespace pikgpu {
 
 // Checks that a frame is complete and self-consistent (what the reference's
 // decoder establishes while reading the bitstream; a violation there is a
@@ -273,6 +286,18 @@ int ValidateFrame(const pikgpu_frame* f, std::string* err) {
   }
   if (f->gaborish < 0 || f->gaborish > 4) {
     return VFail(err, PIKGPU_EINVAL, "invalid gaborish strength");
+  }
+  if (f->flags & PIKGPU_FLAG_GRADIENT_MAP) {
+    return VFail(err, PIKGPU_EUNSUPPORTED,
+                 "gradient-map DC (FrameHeader::kGradientMap) is not supported");
+  }
+  if (f->flags & PIKGPU_FLAG_NOISE) {
+    return VFail(err, PIKGPU_EUNSUPPORTED,
+                 "noise synthesis (FrameHeader::kNoise) is not supported");
+  }
+  if (f->flags & ~uint32_t(PIKGPU_FLAG_GRADIENT_MAP | PIKGPU_FLAG_GRAYSCALE_OPT |
+                           PIKGPU_FLAG_NOISE)) {
+    return VFail(err, PIKGPU_EUNSUPPORTED, "unknown frame header flag");
   }
   if (f->adaptive_reconstruction && !f->epf_enable_adaptive) {
     return VFail(err, PIKGPU_EUNSUPPORTED,
@@ -370,6 +395,77 @@ int ValidateFrame(const pikgpu_frame* f, std::string* err) {
   return PIKGPU_OK;
 }
 
+// Geometry and derived scalars of a validated frame.
+void FillParams(const pikgpu_frame* f, FrameParams* Pp) {
+  FrameParams& P = *Pp;
+  memset(&P, 0, sizeof(P));
+  const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
+  const size_t tx = DivCeil(f->xsize, 64), ty = DivCeil(f->ysize, 64);
+  P.xsize = f->xsize;
+  P.ysize = f->ysize;
+  P.bx = int(bx);
+  P.by = int(by);
+  P.px = int(bx * 8);
+  P.py = int(by * 8);
+  P.tx = int(tx);
+  P.ty = int(ty);
+  P.gx = int(DivCeil(bx, kGroupDimBlocks));
+  P.gy = int(DivCeil(by, kGroupDimBlocks));
+  P.predict_lf = f->predict_lf != 0;
+  P.predict_hf = f->predict_hf != 0;
+  P.gaborish = f->gaborish != 0;
+  P.ar = f->adaptive_reconstruction != 0;
+  P.use_sharpened = f->epf_use_sharpened != 0;
+  P.num_dots = int(f->num_dots);
+  P.downsample = 1;  // per run (pikgpu_set_downsample)
+  P.num_passes = 1;  // more with pikgpu_frame_add_pass
+  P.grayscale = f->grayscale != 0 || (f->flags & PIKGPU_FLAG_GRAYSCALE_OPT);
+  // Quantizer::Decode (pik/quantizer.cc:63-71).
+  P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
+  P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
+  for (int c = 0; c < 3; ++c) {
+    const float m0 = f->dequant_table[f->dequant_offsets[c]];
+    P.mul_dc[c] = m0 * P.inv_quant_dc;
+    P.dc_half_step[c] = P.inv_quant_dc * m0 * 0.5f;
+  }
+  P.ytox_dc = float(f->ytox_dc - 128) * (1.0f / 256);
+  P.ytob_dc = float(f->ytob_dc) * (1.0f / 128);
+  P.dc_qf = f->quant_dc / 2;
+  GaborishWeights(f->gaborish, P.gab_w);
+  LfBlurWeights(P.lfblur_w);
+  Ub4Kernel(P.ub4);
+  const float kScale = 255.0f;
+  P.neg_bias[0] = -ToF(0.26786006338144885 / kScale);
+  P.neg_bias[1] = -ToF(0.24494032763907073 / kScale);
+  P.neg_bias[2] = -ToF(0.14255999980363571 / kScale);
+}
+
+// First blocks grouped by the class of their pred2x2 computation
+// (KPred2x2Interior): DCT8-like, 2x2 Hadamard, DCT16, DCT32; `end` receives
+// the end of each class in `list`.
+void PredClasses(const pikgpu_frame* f, std::vector<uint32_t>* list,
+                 int end[4]) {
+  const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
+  std::vector<uint32_t> plist[4];
+  for (size_t y = 0; y < by; ++y) {
+    const uint8_t* row = f->ac_strategy + y * f->ac_strategy_stride;
+    for (size_t x = 0; x < bx; ++x) {
+      if ((row[x] & 15) != 0) continue;
+      const int raw = row[x] >> 4;
+      const int cls = raw == kDCT16 ? 2
+                      : raw == kDCT32 ? 3
+                      : (raw == kDCT || raw == kDCTNoHF || raw == kLines) ? 0
+                                                                           : 1;
+      plist[cls].push_back(uint32_t(y * bx + x));
+    }
+  }
+  list->clear();
+  for (int k = 0; k < 4; ++k) {
+    list->insert(list->end(), plist[k].begin(), plist[k].end());
+    end[k] = int(list->size());
+  }
+}
+
 // Bins the dots into 32x32 tiles of the padded image. Every dot goes to
 // each tile its footprint may touch (transformed dots: the bounding box of
 // the splat's four corners, widened by two pixels), and each tile's list
@@ -438,7 +534,9 @@ void BinDots(const pikgpu_frame* f, int px, int py, std::vector<int32_t>* ids,
   starts->push_back(int32_t(entries.size()));
 }
 
-}  // namespace
+}  // namespace pikgpu
+
+extern "C" {
 
 int pikgpu_default_dequant_tables(float* table, size_t table_cap,
                                   uint32_t* offsets, size_t offsets_cap,
@@ -473,46 +571,8 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   const size_t bx = DivCeil(f->xsize, 8), by = DivCeil(f->ysize, 8);
   const size_t tx = DivCeil(f->xsize, 64), ty = DivCeil(f->ysize, 64);
   const int num_tables = int(f->num_dequant_offsets / (kNumQuantKinds * 3));
-  // --- geometry and derived scalars ---
   FrameParams& P = ctx->P;
-  memset(&P, 0, sizeof(P));
-  P.xsize = f->xsize;
-  P.ysize = f->ysize;
-  P.bx = int(bx);
-  P.by = int(by);
-  P.px = int(bx * 8);
-  P.py = int(by * 8);
-  P.tx = int(tx);
-  P.ty = int(ty);
-  P.gx = int(DivCeil(bx, kGroupDimBlocks));
-  P.gy = int(DivCeil(by, kGroupDimBlocks));
-  P.predict_lf = f->predict_lf != 0;
-  P.predict_hf = f->predict_hf != 0;
-  P.gaborish = f->gaborish != 0;
-  P.ar = f->adaptive_reconstruction != 0;
-  P.use_sharpened = f->epf_use_sharpened != 0;
-  P.num_dots = int(f->num_dots);
-  P.downsample = 1;  // per run (pikgpu_set_downsample)
-  P.num_passes = 1;  // more with pikgpu_frame_add_pass
-  P.grayscale = f->grayscale != 0;
-  // Quantizer::Decode (pik/quantizer.cc:63-71).
-  P.inv_global_scale = static_cast<float>(1.0 * 65536 / f->global_scale);
-  P.inv_quant_dc = P.inv_global_scale / f->quant_dc;
-  for (int c = 0; c < 3; ++c) {
-    const float m0 = f->dequant_table[f->dequant_offsets[c]];
-    P.mul_dc[c] = m0 * P.inv_quant_dc;
-    P.dc_half_step[c] = P.inv_quant_dc * m0 * 0.5f;
-  }
-  P.ytox_dc = float(f->ytox_dc - 128) * (1.0f / 256);
-  P.ytob_dc = float(f->ytob_dc) * (1.0f / 128);
-  P.dc_qf = f->quant_dc / 2;
-  GaborishWeights(f->gaborish, P.gab_w);
-  LfBlurWeights(P.lfblur_w);
-  Ub4Kernel(P.ub4);
-  const float kScale = 255.0f;
-  P.neg_bias[0] = -ToF(0.26786006338144885 / kScale);
-  P.neg_bias[1] = -ToF(0.24494032763907073 / kScale);
-  P.neg_bias[2] = -ToF(0.14255999980363571 / kScale);
+  FillParams(f, &P);
 
   // --- buffers ---
   const size_t nblk = bx * by;
@@ -636,27 +696,9 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
                                 dot_list.data(), dot_list.size() * 4,
                                 hipMemcpyHostToDevice, ctx->stream));
   }
-  // First blocks grouped by the class of their pred2x2 computation
-  // (KPred2x2Interior): DCT8-like, 2x2 Hadamard, DCT16, DCT32.
-  std::vector<uint32_t> plist[4];
-  for (size_t y = 0; y < by; ++y) {
-    const uint8_t* row = f->ac_strategy + y * f->ac_strategy_stride;
-    for (size_t x = 0; x < bx; ++x) {
-      if ((row[x] & 15) != 0) continue;
-      const int raw = row[x] >> 4;
-      const int cls = raw == kDCT16 ? 2
-                      : raw == kDCT32 ? 3
-                      : (raw == kDCT || raw == kDCTNoHF || raw == kLines) ? 0
-                                                                           : 1;
-      plist[cls].push_back(uint32_t(y * bx + x));
-    }
-  }
   std::vector<uint32_t> pred_list;
   int pred_end[4];
-  for (int k = 0; k < 4; ++k) {
-    pred_list.insert(pred_list.end(), plist[k].begin(), plist[k].end());
-    pred_end[k] = int(pred_list.size());
-  }
+  PredClasses(f, &pred_list, pred_end);
   if ((rc = Grow(ctx, &ctx->predlist, pred_list.size() * 4))) return rc;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->predlist.p, pred_list.data(),
                               pred_list.size() * 4, hipMemcpyHostToDevice,
@@ -715,6 +757,8 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   ctx->uploaded = true;
   return PIKGPU_OK;
 }
+
+}  // extern "C"
 
 // Grid of KMinMaxPartial: 256-column strips, row ranges of >= 8 rows, at most
 // kMinMaxMaxGroups workgroups (the size of the partials buffer): 16 rows per
@@ -809,6 +853,96 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
   }
 }
 
+// DC, LF predictions and the block kernel: everything before the pixel
+// filters (the same for full-size and downsampled runs). `ev` (may be
+// null) receives events 0..3 around the families.
+static hipError_t EnqueuePrefix(const FrameParams& P, const FrameBuffers& B,
+                                const DeviceConsts* K, hipStream_t st,
+                                const hipEvent_t* ev) {
+  hipError_t e = hipSuccess;
+#define REC(i) \
+  if (ev && (e = hipEventRecord(ev[i], st)) != hipSuccess) return e
+  REC(0);
+  EnqueueDc(P, B, K, st);
+  REC(1);
+  EnqueueLf(P, B, K, st, P.predict_hf != 0);
+  REC(2);
+  {
+    const int grid = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
+    KBlocks<<<grid, 256, 0, st>>>(P, B, K);
+  }
+  REC(3);
+#undef REC
+  return hipGetLastError();
+}
+
+// The full-size reconstruction of a frame whose inputs are in device
+// memory (B): DC, predictions, IDCT, EPF + AR (or the plain copy without
+// AR), dots, Gaborish and OpsinToLinear into B.out; with B.keep the stage
+// copies for pikgpu_stage_dump. Allocates nothing. `ev` (may be null)
+// receives events 0..7 between the kernel families.
+hipError_t pikgpu::EnqueueFullSize(const FrameParams& P,
+                                   const FrameBuffers& B,
+                                   const DeviceConsts* K, hipStream_t st,
+                                   const hipEvent_t* ev) {
+  hipError_t e = EnqueuePrefix(P, B, K, st, ev);
+  if (e != hipSuccess) return e;
+#define REC(i) \
+  if (ev && (e = hipEventRecord(ev[i], st)) != hipSuccess) return e
+#define TRY(expr) \
+  if ((e = (expr)) != hipSuccess) return e
+  const size_t iplane = size_t(B.ipitch) * P.py;
+  if (P.ar) {
+    Img ii;
+    ii.p = B.idct;
+    ii.plane = iplane;
+    ii.pitch = B.ipitch;
+    ii.W = P.px;
+    ii.H = P.py;
+    ii.vw = P.px;
+    ii.vh = P.py;
+    const int gab = P.gaborish && !P.use_sharpened;
+    // the IDCT image: width a multiple of 8, rows 32-float aligned
+    const dim3 mmg = MinMaxGrid4(ii.W, ii.H);
+    KMinMaxPartial4<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
+    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
+    const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
+    KGuide<<<grid, 256, 0, st>>>(ii, gab, P, B, B.mm);
+    const int ring = 3 * (2 * 6 * (P.px + 12) + 2 * 6 * P.py);
+    KGuideRing<<<DivCeil(ring, 256), 256, 0, st>>>(P, B);
+    REC(4);
+    KEpf<false><<<grid, 256, 0, st>>>(ii, P, B, K, B.mm, B.ar, B.ipitch,
+                                      iplane);
+  } else {
+    REC(4);
+    KCopyPlanes<<<DivCeil(3 * iplane, 256), 256, 0, st>>>(B.idct, B.ar,
+                                                          3 * iplane);
+  }
+  REC(5);
+  if (B.keep) {
+    if (!P.ar) {
+      TRY(hipMemcpyAsync(B.guide, B.idct, 3 * iplane * 4,
+                         hipMemcpyDeviceToDevice, st));
+      TRY(hipMemcpyAsync(B.epf, B.idct, 3 * iplane * 4,
+                         hipMemcpyDeviceToDevice, st));
+    }
+    // B.dots_img keeps the AR output before the dots modify B.ar in place.
+    TRY(hipMemcpyAsync(B.dots_img, B.ar, 3 * iplane * 4,
+                       hipMemcpyDeviceToDevice, st));
+  }
+  if (P.dot_tiles) KDots<<<P.dot_tiles, 256, 0, st>>>(P, B);
+  REC(6);
+  {
+    const int grid =
+        XcdGrid(int(DivCeil(P.px, 256) * DivCeil(P.py, kFinalRows)));
+    KFinal<<<grid, 256, 0, st>>>(P, B, K);
+  }
+  REC(7);
+#undef REC
+#undef TRY
+  return hipGetLastError();
+}
+
 // Enqueues the whole reconstruction of the uploaded frame on stream `st`.
 // With `timing`, HIP events between the kernel families record their
 // durations (pikgpu_last_timings). Allocates nothing unless stage dumps or
@@ -841,19 +975,10 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     B.srgb8 = static_cast<uint8_t*>(ctx->srgb8.p);
   }
   const DeviceConsts* K = ctx->consts;
-  const size_t iplane = size_t(B.ipitch) * P.py;
+  const hipEvent_t* ev = timing ? ctx->ev : nullptr;
 
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
-  EnqueueDc(P, B, K, st);
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
-  EnqueueLf(P, B, K, st, P.predict_hf != 0);
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
-  {
-    const int grid = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
-    KBlocks<<<grid, 256, 0, st>>>(P, B, K);
-  }
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   if (ds != 1) {
+    HIP_TRY(ctx, EnqueuePrefix(P, B, K, st, ev));
     // The downsampled branch of FinalizeFrameDecoding: KBlocks wrote the
     // averaged image into B.idct (planes of (py / ds) x ipitch); no AR, no
     // Gaborish, no dots; ShrinkTo and OpsinToLinear in KFinal.
@@ -873,73 +998,20 @@ static int Enqueue(pikgpu_ctx* ctx, hipStream_t st, bool keep_stages,
     KFinal<<<grid, 256, 0, st>>>(Pd, Bd, K);
     if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
     HIP_TRY(ctx, hipGetLastError());
-    ctx->ran = true;
-    ctx->kept = false;
-    ctx->ran_flags = P.out_flags;
-    ctx->spitch = B.spitch;
-    ctx->out_xsize = out_xs;
-    ctx->out_ysize = out_ys;
-    ctx->ran_downsampled = true;
-    return PIKGPU_OK;
-  }
-  if (P.ar) {
-    Img ii;
-    ii.p = B.idct;
-    ii.plane = iplane;
-    ii.pitch = B.ipitch;
-    ii.W = P.px;
-    ii.H = P.py;
-    ii.vw = P.px;
-    ii.vh = P.py;
-    const int gab = P.gaborish && !P.use_sharpened;
-    // the IDCT image: width a multiple of 8, rows 32-float aligned
-    const dim3 mmg = MinMaxGrid4(ii.W, ii.H);
-    KMinMaxPartial4<<<mmg, 256, 0, st>>>(ii, gab, P, B.mm_partial);
-    KMinMaxFinal<<<1, 256, 0, st>>>(B.mm_partial, int(mmg.x * mmg.y), B.mm);
-    const int grid = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
-    KGuide<<<grid, 256, 0, st>>>(ii, gab, P, B, B.mm);
-    const int ring = 3 * (2 * 6 * (P.px + 12) + 2 * 6 * P.py);
-    KGuideRing<<<DivCeil(ring, 256), 256, 0, st>>>(P, B);
-    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
-    KEpf<false><<<grid, 256, 0, st>>>(ii, P, B, K, B.mm, B.ar, B.ipitch,
-                                      iplane);
   } else {
-    if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
-    KCopyPlanes<<<DivCeil(3 * iplane, 256), 256, 0, st>>>(B.idct, B.ar,
-                                                          3 * iplane);
+    HIP_TRY(ctx, EnqueueFullSize(P, B, K, st, ev));
   }
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
-  if (B.keep) {
-    if (!P.ar) {
-      HIP_TRY(ctx, hipMemcpyAsync(B.guide, B.idct, 3 * iplane * 4,
-                                  hipMemcpyDeviceToDevice, st));
-      HIP_TRY(ctx, hipMemcpyAsync(B.epf, B.idct, 3 * iplane * 4,
-                                  hipMemcpyDeviceToDevice, st));
-    }
-  }
-  if (B.keep) {
-    // B.dots_img keeps the AR output before the dots modify B.ar in place.
-    HIP_TRY(ctx, hipMemcpyAsync(B.dots_img, B.ar, 3 * iplane * 4,
-                                hipMemcpyDeviceToDevice, st));
-  }
-  if (P.dot_tiles) KDots<<<P.dot_tiles, 256, 0, st>>>(P, B);
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], st));
-  {
-    const int grid =
-        XcdGrid(int(DivCeil(P.px, 256) * DivCeil(P.py, kFinalRows)));
-    KFinal<<<grid, 256, 0, st>>>(P, B, K);
-  }
-  if (timing) HIP_TRY(ctx, hipEventRecord(ctx->ev[7], st));
-  HIP_TRY(ctx, hipGetLastError());
   ctx->ran = true;
   ctx->kept = B.keep;
   ctx->ran_flags = P.out_flags;
   ctx->spitch = B.spitch;
   ctx->out_xsize = out_xs;
   ctx->out_ysize = out_ys;
-  ctx->ran_downsampled = false;
+  ctx->ran_downsampled = ds != 1;
   return PIKGPU_OK;
 }
+
+extern "C" {
 
 int pikgpu_run(pikgpu_ctx* ctx, int keep_stages) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
@@ -1018,7 +1090,10 @@ int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
   const size_t extra = size_t(P.num_passes - 1);
   const size_t need = (extra + 1) * pass_bytes;
   if (ctx->ac_extra.cap < need) {
-    // grow keeping the passes already there
+    // Grow keeping the passes already there. The old buffer is freed only
+    // once its contents are in the new one, and the frame's buffer table
+    // points at the new one from then on, so a failure at any step leaves a
+    // consistent context (the passes so far, in a live buffer).
     DevBuf nb;
     hipError_t e = hipMalloc(&nb.p, need);
     if (e != hipSuccess) {
@@ -1027,12 +1102,22 @@ int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
     }
     nb.cap = need;
     if (extra > 0) {
-      HIP_TRY(ctx, hipMemcpyAsync(nb.p, ctx->ac_extra.p, extra * pass_bytes,
-                                  hipMemcpyDeviceToDevice, ctx->stream));
+      e = hipMemcpyAsync(nb.p, ctx->ac_extra.p, extra * pass_bytes,
+                         hipMemcpyDeviceToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) {
+        (void)hipFree(nb.p);
+        return Fail(ctx, PIKGPU_EHIP,
+                    std::string("copying the earlier passes: ") +
+                        hipGetErrorString(e));
+      }
+    } else {
+      // nothing in flight may still read the old buffer
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
     if (ctx->ac_extra.p) (void)hipFree(ctx->ac_extra.p);
     ctx->ac_extra = nb;
+    ctx->B.ac_extra = extra > 0 ? static_cast<const int16_t*>(nb.p) : nullptr;
   }
   int16_t* dst = static_cast<int16_t*>(ctx->ac_extra.p) + extra * 3 * row * P.by;
   int rc;

@@ -50,6 +50,7 @@ class Frame:
     epf_enable_adaptive: int = 1
     epf_use_sharpened: int = 0
     grayscale: int = 0
+    flags: int = 0            # FrameHeader::flags (abi.FLAG_*)
 
     @property
     def xsize_blocks(self):
@@ -78,33 +79,64 @@ class Frame:
                 setattr(self, f.name, np.ascontiguousarray(v))
         return self
 
+    def rows(self):
+        """Makes every array row-addressable for the C ABI: unit stride
+        along a row, rows at a positive stride that is a whole number of
+        elements. Views into padded buffers (the reference's Image layout,
+        rows of BytesPerRow bytes, pik/image.h:47-68) keep their stride;
+        anything else is copied to a contiguous array."""
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if not isinstance(v, np.ndarray):
+                continue
+            ok = v.ndim >= 1 and (v.shape[-1] <= 1 or
+                                  v.strides[-1] == v.itemsize)
+            if ok and v.ndim >= 2:
+                rs = v.strides[-2]
+                ok = (rs > 0 and rs % v.itemsize == 0 and
+                      rs >= v.shape[-1] * v.itemsize)
+            if ok and v.ndim == 1:
+                ok = v.flags.c_contiguous
+            if not ok:
+                setattr(self, f.name, np.ascontiguousarray(v))
+        return self
+
+    @staticmethod
+    def _stride(a):
+        """Row stride of a row-addressable array, in elements."""
+        return a.strides[-2] // a.itemsize if a.shape[-2] > 1 else a.shape[-1]
+
     def to_c(self):
-        """Returns (FrameC, keepalive) with pointers into this frame."""
-        self.contiguous()
+        """Returns (FrameC, keepalive) with pointers into this frame; row
+        strides are taken from the arrays (rows())."""
+        self.rows()
         f = abi.FrameC()
         f.xsize, f.ysize = self.xsize, self.ysize
         for name in ("gaborish", "predict_lf", "predict_hf",
                      "adaptive_reconstruction", "epf_enable_adaptive",
                      "epf_use_sharpened", "global_scale", "quant_dc",
-                     "ytox_dc", "ytob_dc", "grayscale"):
+                     "ytox_dc", "ytob_dc", "grayscale", "flags"):
             setattr(f, name, int(getattr(self, name)))
         for c in range(3):
             f.ac[c] = self.ac[c].ctypes.data
             f.dc[c] = self.dc[c].ctypes.data
-        f.ac_stride = self.ac.shape[2]
-        f.dc_stride = self.dc.shape[2]
+        st = self._stride
+        f.ac_stride = st(self.ac)
+        f.dc_stride = st(self.dc)
         f.quant_field = self.quant_field.ctypes.data
-        f.quant_field_stride = self.quant_field.shape[1]
+        f.quant_field_stride = st(self.quant_field)
         f.ac_strategy = self.ac_strategy.ctypes.data
-        f.ac_strategy_stride = self.ac_strategy.shape[1]
+        f.ac_strategy_stride = st(self.ac_strategy)
         f.sigma_lut_ids = self.sigma_lut_ids.ctypes.data
-        f.sigma_lut_stride = self.sigma_lut_ids.shape[1]
+        f.sigma_lut_stride = st(self.sigma_lut_ids)
         f.dequant_cf = self.dequant_cf.ctypes.data
-        f.dequant_cf_stride = self.dequant_cf.shape[1]
+        f.dequant_cf_stride = st(self.dequant_cf)
         f.ytox_map = self.ytox_map.ctypes.data
-        f.ytox_stride = self.ytox_map.shape[1]
+        f.ytox_stride = st(self.ytox_map)
         f.ytob_map = self.ytob_map.ctypes.data
-        f.ytob_stride = self.ytob_map.shape[1]
+        f.ytob_stride = st(self.ytob_map)
+        if not self.dequant_map.flags.c_contiguous:
+            self.dequant_map = np.ascontiguousarray(self.dequant_map)
         f.dequant_map = self.dequant_map.ctypes.data
         f.dequant_table = self.dequant_table.ctypes.data
         f.dequant_table_size = self.dequant_table.size
@@ -127,6 +159,8 @@ class Frame:
         z = np.load(path, allow_pickle=False)
         kw = {}
         for fld in dataclasses.fields(cls):
+            if fld.name not in z.files:
+                continue  # written before the field existed: its default
             v = z[fld.name]
             kw[fld.name] = v if v.ndim else v.item()
         return cls(**kw).contiguous()

@@ -85,20 +85,36 @@ class Reconstructor:
         f, ds = self._frame, self._ds
         return -(-f.xsize // ds), -(-f.ysize // ds)
 
-    def download(self):
+    def download(self, out=None):
         """(3, ysize, xsize) float32 (ceil(size / factor) when
-        downsampled)."""
+        downsampled). `out`, if given, receives it: a (3, ysize, xsize)
+        float32 array whose rows may be padded (a view into wider rows, as
+        the reference's Image3F holds them), written in place."""
         xs, ys = self._out_size()
-        out = np.empty((3, ys, xs), np.float32)
+        if out is None:
+            out = np.empty((3, ys, xs), np.float32)
+        if (out.dtype != np.float32 or out.shape != (3, ys, xs) or
+                out.strides[2] != 4 or out.strides[1] < 4 * xs):
+            raise ValueError("out must be (3, %d, %d) float32 with unit-"
+                             "stride rows" % (ys, xs))
         planes = (abi.c_float_p * 3)(
             *[out[c].ctypes.data_as(abi.c_float_p) for c in range(3)])
-        self._check(self._lib.pikgpu_download(self._ctx, planes, xs * 4))
+        self._check(self._lib.pikgpu_download(self._ctx, planes,
+                                              out.strides[1]))
         return out
 
     def add_pass(self, ac):
         """Adds an AC pass (3 planes of int16, the frame's AC layout) to the
         uploaded frame (pikgpu_frame_add_pass)."""
+        f = self._frame
+        if len(ac) != 3:
+            raise ValueError("a pass has three planes")
         planes = [np.ascontiguousarray(a, np.int16) for a in ac]
+        want = None if f is None else (f.ysize_blocks, f.xsize_blocks * 64)
+        for a in planes:
+            if want is not None and a.shape != want:
+                raise ValueError("AC pass planes must be %s int16, got %s"
+                                 % (want, a.shape))
         ptrs = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in planes])
         self._check(self._lib.pikgpu_frame_add_pass(
             self._ctx, ctypes.cast(ptrs, ctypes.c_void_p), planes[0].shape[1]))

@@ -265,9 +265,42 @@ def choose_strategies(xyb_pad, seed):
     return out
 
 
+# Quantizer::SetQuantField's target for the median quant value
+# (pik/quantizer.h:155).
+QUANT_FIELD_TARGET = 3.80987740592518214386
+
+
+def set_quant_field(qf_float, quant_dc_float, global_scale=None):
+    """Integer quantizer state from a float quant field the way
+    Quantizer::SetQuantField (pik/quantizer.h:142-199) derives it: the
+    global scale from the field's median minus its median absolute deviation
+    (capped by quant_dc * 4096 and to [1, 32768]), then every value times
+    65536 / global_scale rounded and clamped to [1, 256]. An explicit
+    `global_scale` replaces the derived one. Returns (global_scale,
+    quant_dc, int32 field)."""
+    q = np.asarray(qf_float, np.float32)
+    if global_scale is None:
+        med = np.float32(np.median(q))
+        mad = np.float32(np.median(np.abs(q - med)))
+        gs = int(np.float32(65536.0) * (med - mad)
+                 / np.float32(QUANT_FIELD_TARGET))
+        gs = min(gs, int(quant_dc_float * 4096))
+        global_scale = max(1, min(gs, 1 << 15))
+    inv_gs = np.float32(65536.0 / global_scale)
+    quant_dc = int(np.clip(int(np.float32(quant_dc_float) * inv_gs + 0.5),
+                           1, 256))
+    field = np.clip((q * inv_gs + np.float32(0.5)).astype(np.int64), 1,
+                    256).astype(np.int32)
+    return int(global_scale), quant_dc, field
+
+
 def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
-                 lut_ids="zero", predict=True, grayscale=False):
-    """Encodes an (H, W, 3) uint8 sRGB image into a Frame. With
+                 lut_ids="zero", predict=True, grayscale=False,
+                 global_scale=None):
+    """Encodes an (H, W, 3) uint8 sRGB image into a Frame. The quantizer
+    scalars come from a float quant field as the reference encoder derives
+    them (set_quant_field), so global_scale is in general no power of two;
+    an explicit `global_scale` overrides it. With
     `grayscale` the image is first made gray (channel mean) and the frame
     is a kGrayscaleOpt frame: only the Y DC is coded, the X and B DC planes
     hold zeros as Image3SDecompress leaves them (pik/compressed_dc.cc:
@@ -292,17 +325,18 @@ def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
     yb = xyb[1].reshape(by, 8, bx, 8)
     act = yb.std(axis=(1, 3))
     rng = np.random.default_rng(seed + 17)
-    qf = np.clip(np.rint((5.0 + 60.0 * act + rng.uniform(0, 4, act.shape))
-                         / max(distance, 0.05)), 1, 256).astype(np.int32)
+    qf_float = ((5.0 + 60.0 * act + rng.uniform(0, 4, act.shape))
+                / (8.0 * max(distance, 0.05))).astype(np.float32)
     # AdjustQuantField semantics: a multi-block transform uses its maximum.
     for code in (abi.ACS_DCT16X16, abi.ACS_DCT32X32):
         s = covered(code)
         ys, xs = np.nonzero(first & (raw == code))
         for y0, x0 in zip(ys, xs):
-            qf[y0:y0 + s, x0:x0 + s] = qf[y0:y0 + s, x0:x0 + s].max()
-    global_scale = 8192
+            qf_float[y0:y0 + s, x0:x0 + s] = \
+                qf_float[y0:y0 + s, x0:x0 + s].max()
+    global_scale, quant_dc, qf = set_quant_field(
+        qf_float, 6.0 / max(distance, 0.05), global_scale)
     inv_gs = np.float32(65536.0 / global_scale)
-    quant_dc = int(np.clip(round(12.0 / max(distance, 0.05)), 1, 256))
     inv_quant_dc = np.float32(inv_gs / np.float32(quant_dc))
 
     # Coefficients per strategy, in float64.

@@ -68,7 +68,7 @@ int main(void) {
   F(pikgpu_frame, ytox_dc) F(pikgpu_frame, dequant_map)
   F(pikgpu_frame, dequant_table) F(pikgpu_frame, dequant_offsets)
   F(pikgpu_frame, num_dequant_offsets) F(pikgpu_frame, dots)
-  F(pikgpu_frame, num_dots)
+  F(pikgpu_frame, num_dots) F(pikgpu_frame, grayscale) F(pikgpu_frame, flags)
   F(pikgpu_dot, transform) F(pikgpu_dot, width) F(pikgpu_dot, pixels)
   return 0;
 }
@@ -116,6 +116,21 @@ def test_rejects_bad_sizes_and_scalars(frame):
     _expect(frame, abi.EINVAL, "invalid quantizer", quant_dc=257)
     _expect(frame, abi.EINVAL, "gaborish", gaborish=5)
     _expect(frame, abi.EUNSUPPORTED, "fixed-sigma", epf_enable_adaptive=0)
+
+
+def test_header_flags(frame):
+    # FrameHeader::kNoise and kGradientMap change the decode; a frame with
+    # either is refused rather than decoded without them
+    _expect(frame, abi.EUNSUPPORTED, "kNoise", flags=abi.FLAG_NOISE)
+    _expect(frame, abi.EUNSUPPORTED, "kGradientMap",
+            flags=abi.FLAG_GRADIENT_MAP)
+    _expect(frame, abi.EUNSUPPORTED, "kGradientMap",
+            flags=abi.FLAG_GRADIENT_MAP | abi.FLAG_GRAYSCALE_OPT)
+    _expect(frame, abi.EUNSUPPORTED, "unknown", flags=8)
+    _expect(frame, abi.EUNSUPPORTED, "unknown", flags=1 << 31)
+    fr = frame.copy()
+    fr.flags = abi.FLAG_GRAYSCALE_OPT
+    validate(fr)
 
 
 def test_rejects_quant_field_and_lut(frame):

@@ -43,6 +43,19 @@ def test_stages_match_oracle(rec, w, h, seed, kw):
         assert np.array_equal(_bits(rec.stage(s)), _bits(wst[name])), name
 
 
+def test_header_flag_spelling(rec):
+    # FrameHeader::kGrayscaleOpt in the flag word decodes as grayscale = 1
+    from oracle import port
+    print("grayscale via header flags", flush=True)
+    fr = synth.make_frame(200, 136, seed=14, grayscale=True)
+    want, _ = port.recon(fr, stages=False)
+    fr.grayscale = 0
+    fr.flags = abi.FLAG_GRAYSCALE_OPT
+    rec.upload(fr)
+    rec.run()
+    assert np.array_equal(_bits(rec.download()), _bits(want))
+
+
 @pytest.mark.parametrize("ds", (2, 4, 8))
 def test_downsampled_matches_oracle(rec, ds):
     from oracle import port

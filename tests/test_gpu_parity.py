@@ -74,6 +74,11 @@ def _cases():
                                  predict_hf=0)
     c["hf_only"] = lambda: _with(synth.make_frame(160, 128, seed=23),
                                  predict_lf=0)
+    # global_scale as the encoder derives it is no power of two; these
+    # also pin the extremes of Quantizer::Decode's range [1, 32768]
+    for gs in (1, 3, 1000, 12345, 32767, 32768):
+        c["gs%d" % gs] = (lambda gs=gs: synth.make_frame(
+            136, 104, seed=90 + gs % 7, num_dots=2, global_scale=gs))
     for raw in (abi.ACS_DCT, abi.ACS_IDENTITY, abi.ACS_DCT2X2,
                 abi.ACS_DCT4X4, abi.ACS_DCT16X16, abi.ACS_DCT32X32,
                 abi.ACS_DCT_NOHF, abi.ACS_DCT4X4_NOHF):
@@ -143,6 +148,50 @@ def test_full_size_4096(rec):
                                                                  kind))
     assert rel <= TOL
     assert eq == 1.0
+
+
+def bytes_per_row(valid_bytes):
+    """Row stride of the reference's images (BytesPerRow<128>,
+    pik/image.h:47-68, with kMaxVectorSize = 32 of its AVX2 build)."""
+    row = (valid_bytes + 2 * 32 + 127) & ~127
+    if row % 2048 == 0:
+        row += 128
+    return row
+
+
+def padded(a):
+    """Copy of `a` (2-D or 3-D) whose rows are BytesPerRow apart, as a view
+    into the padded buffer; the padding holds garbage."""
+    a = np.asarray(a)
+    stride = bytes_per_row(a.shape[-1] * a.itemsize) // a.itemsize
+    buf = np.full(a.shape[:-1] + (stride,), 77, a.dtype)
+    view = buf[..., :a.shape[-1]]
+    view[...] = a
+    return view
+
+
+def test_padded_image_strides(rec):
+    """Inputs and output in the reference's Image layout (rows padded to
+    BytesPerRow, every plane and side-information image), as a drop-in
+    caller hands them over: same bits as the tightly packed frame."""
+    frame = synth.make_frame(600, 520, seed=95, num_dots=3, lut_ids="rand")
+    want, _, _ = checker_recon(frame, stages=False)
+    fr = frame.copy()
+    for name in ("ac", "dc", "quant_field", "ac_strategy", "sigma_lut_ids",
+                 "dequant_cf", "ytox_map", "ytob_map"):
+        setattr(fr, name, padded(getattr(frame, name)))
+    fc, _ = fr.to_c()
+    assert fc.ac_stride == bytes_per_row(frame.ac.shape[2] * 2) // 2
+    assert fc.quant_field_stride > frame.quant_field.shape[1]
+    rec.upload(fr)
+    rec.run()
+    out = padded(np.zeros((3, frame.ysize, frame.xsize), np.float32))
+    base = out.base.copy()
+    rec.download(out=out)
+    assert bit_equal_fraction(out, want) == 1.0
+    # the padding of the output rows is left as it was
+    pad = out.base[..., frame.xsize:]
+    assert np.array_equal(pad, base[..., frame.xsize:])
 
 
 def test_context_reuse_shrinking_and_growing(rec):

@@ -21,7 +21,7 @@ import os
 import numpy as np
 import pytest
 
-from pik_amd import synth
+from pik_amd import abi, synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST_LIB = os.path.join(ROOT, "pik_amd", "libpikgpu_host.so")
@@ -188,6 +188,10 @@ def test_output_is_gray_and_flag_matters():
     fr.grayscale = 0
     colour, _ = port.recon(fr, stages=False)
     assert not np.array_equal(_bits(colour[0]), _bits(colour[1]))
+    # the header's flag word selects the same decode
+    fr.flags = abi.FLAG_GRAYSCALE_OPT
+    flagged, _ = port.recon(fr, stages=False)
+    assert np.array_equal(_bits(flagged), _bits(out))
 
 
 def test_nonzero_coded_xb_matches_reference():
