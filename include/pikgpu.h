@@ -420,6 +420,68 @@ typedef struct pikgpu_adj_io {
  * context stream (pikgpu_sync waits); qf values are not checked. */
 PIKGPU_API int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io);
 
+/* ------------------------------------------------------------------------
+ * Gradient quant-field search. Replaces the reference encoder's
+ * Butteraugli-driven search (FindBestQuantization, pik/adaptive_quantization.
+ * cc:677-853: twelve round trips, each a quantization, a decode and a
+ * Butteraugli comparison, rescaling the field by tile distances) by Adam
+ * steps on the log quant field driven by the forward + adjoint of the decode
+ * (pikgpu_fwd_adj's model), all device-resident on the context stream.
+ *
+ * Each step quantizes `opsin` with the current integer field (as
+ * pikgpu_quantize), evaluates the squared linear-RGB error against `target`
+ * and its gradient dL/dd with respect to the dequantized coefficients, and
+ * attributes the error to the blocks to first order: D_b = 1/2 sum_k g_k e_k
+ * (e: each coefficient's quantization error). With the noise model
+ * D ~ qf^-2 the gradient of the rate-distortion objective with respect to a
+ * transform's log field is sum over its blocks of (lambda - 2 w_b D_b);
+ * Adam follows it, the field being rounded to integers in [1, qf_max] for
+ * every quantization. The stationary point equalizes w_b D_b at lambda / 2.
+ *
+ * The uploaded frame supplies everything but the field's values: strategy,
+ * colour correlation, dequant tables / control field / map, global_scale,
+ * quant_dc, header flags (frames without LF / HF prediction, dots or the
+ * grayscale flag, as pikgpu_quantize) and the initial integer field. On
+ * return the context holds the frame quantized with the final field, so
+ * pikgpu_run decodes it. */
+typedef struct pikgpu_search_params {
+  int32_t steps;         /* Adam steps (the field is quantized steps + 1
+                            times) */
+  float learning_rate;   /* Adam step size on log qf */
+  float beta1, beta2, eps;
+  float lambda;          /* rate weight per block; <= 0 selects the mean of
+                            2 w_b D_b over the blocks at the initial field
+                            times -lambda (0: that mean itself), i.e. the
+                            distortion level of the start is kept */
+  int32_t qf_max;        /* largest integer quant value, <= 256 */
+} pikgpu_search_params;
+
+typedef struct pikgpu_search_io {
+  const float* opsin[3];   /* XYB image as pikgpu_quantize takes it (padded
+                              to whole blocks; host) */
+  size_t opsin_stride;     /* floats per row */
+  const float* target[3];  /* linear RGB original, xsize x ysize (host) */
+  size_t target_stride;    /* floats per row */
+  const float* weight;     /* per-block perceptual weights w_b (ysize_blocks
+                              rows of xsize_blocks; host), or NULL for 1 */
+  size_t weight_stride;
+  int32_t* quant_field;    /* out (host, may be NULL): final integer field */
+  size_t quant_field_stride;
+  int16_t* ac[3];          /* out (host, all or none): final quantized AC, */
+  size_t ac_stride;        /* layout of pikgpu_frame.ac */
+  int16_t* dc[3];          /* out (host, all or none): quantized DC */
+  size_t dc_stride;
+  double* loss;            /* out (host, steps + 1 entries, may be NULL):
+                              squared linear-RGB error at each quantization */
+  double* rate;            /* out (host, steps + 1, may be NULL): sum of
+                              log(qf) over the blocks at each quantization */
+  float* lambda_used;      /* out (may be NULL): the rate weight applied */
+} pikgpu_search_io;
+
+PIKGPU_API int pikgpu_search_quant_field(pikgpu_ctx* ctx,
+                                         const pikgpu_search_params* params,
+                                         const pikgpu_search_io* io);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

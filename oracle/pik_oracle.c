@@ -423,6 +423,195 @@ static void idct2_top(int S, const float* block, size_t stride_out,
   }
 }
 
+/* LINES, the diagonal-lines transform (pik/ac_strategy.cc:380-530 tables
+ * and 1-D transforms, 545-568 forward, 680-724 inverse), with pixels and
+ * coefficient slots numbered row-major (y * 8 + x): the reference numbers
+ * both column-major through its P1 / C1 macros, so each of its indices i is
+ * (i & 7) * 8 + (i >> 3) here. */
+static const int kLn[13] = {3, 3, 3, 3, 4, 4, 4, 4, 6, 6, 6, 6, 8};
+static const int kLpix[13][8] = {
+    {48, 56, 57}, {40, 49, 58}, {5, 14, 23}, {6, 7, 15},
+    {32, 41, 50, 59}, {24, 33, 51, 60}, {3, 12, 30, 39}, {4, 13, 22, 31},
+    {16, 25, 34, 43, 52, 61}, {8, 17, 26, 44, 53, 62},
+    {1, 10, 19, 37, 46, 55}, {2, 11, 20, 29, 38, 47},
+    {0, 9, 18, 27, 36, 45, 54, 63}};
+static const int kLcoef[13][8] = {
+    {48, 56, 57}, {40, 49, 58}, {5, 14, 23}, {6, 7, 15},
+    {32, 41, 50, 59}, {24, 33, 42, 51}, {3, 12, 21, 30}, {4, 13, 22, 31},
+    {16, 25, 34, 43, 52, 61}, {17, 26, 35, 44, 53, 62},
+    {10, 19, 28, 37, 46, 55}, {2, 11, 20, 29, 38, 47},
+    {18, 27, 36, 45, 54, 60, 39, 63}};
+static const int kLgn[4] = {21, 15, 15, 13};
+static const int kLg[4][21] = {
+    {0, 1, 9, 2, 10, 18, 11, 19, 27, 20, 28, 36, 29, 37, 45, 38, 46, 54, 47,
+     55, 63},
+    {3, 4, 12, 5, 13, 21, 6, 14, 22, 30, 7, 15, 23, 31, 39},
+    {24, 32, 40, 48, 56, 33, 41, 49, 57, 42, 50, 58, 51, 59, 60},
+    {8, 16, 17, 25, 26, 34, 35, 43, 44, 52, 53, 61, 62}};
+static const int kLgslot[4] = {0, 8, 1, 9};
+
+/* float * double constant, rounded to float (the reference assigns such a
+ * product to a float). */
+static float mul_d(float v, double k) { return (float)((double)v * k); }
+
+static void lines_idct(int n, float* v) {
+  if (n == 3) {
+    float t0 = v[0], t1 = v[1] * 0.86602540f, t2a = v[2] * 0.5f, t2b = v[2];
+    v[0] = t0 + t2a + t1;
+    v[1] = t0 - t2b;
+    v[2] = t0 + t2a - t1;
+  } else if (n == 4) {
+    float i0 = v[0] * (0.5f * 2), i1 = v[1] * (0.382683432365089772f * 2);
+    float i2 = v[2] * (0.5f * 2), i3 = v[3] * (0.923879532511286756f * 2);
+    float t0 = i0 + i2, t1 = i0 - i2, t2 = i1 + i3, t3 = i1 - i3;
+    float t4 = t3 * 0.7071067811865475244f, t5 = t2 + t4;
+    v[0] = t0 + t5;
+    v[1] = t1 + t4;
+    v[2] = t1 - t4;
+    v[3] = t0 - t5;
+  } else if (n == 6) {
+    float a0 = v[0], a1 = mul_d(v[1], 0.96592583), b1 = mul_d(v[1], 0.70710678);
+    float c1 = mul_d(v[1], 0.25881905), a2 = mul_d(v[2], 0.86602540);
+    float a3 = mul_d(v[3], 0.70710678), a4 = mul_d(v[4], 0.5), b4 = v[4];
+    float a5 = mul_d(v[5], 0.25881905), b5 = mul_d(v[5], 0.70710678);
+    float c5 = mul_d(v[5], 0.96592583);
+    v[0] = a0 + a1 + a3 + a4 + a5 + a2;
+    v[1] = a0 + b1 - a3 - b4 - b5;
+    v[2] = a0 + c1 - a3 + a4 + c5 - a2;
+    v[3] = a0 - c1 + a3 + a4 - c5 - a2;
+    v[4] = a0 - b1 + a3 - b4 + b5;
+    v[5] = a0 - a1 - a3 + a4 - a5 + a2;
+  } else {
+    const double k = (double)2.8284271247461903f;
+    float c1 = (float)1.41421356237310, c2 = (float)2.61312592975275;
+    float c3 = (float)0.76536686473018, c4 = (float)1.08239220029239;
+    float i0 = mul_d(v[0], 0.353553390593273762 * k);
+    float i1 = mul_d(v[1], 0.490392640201615225 * k);
+    float i2 = mul_d(v[2], 0.461939766255643378 * k);
+    float i3 = mul_d(v[3], 0.415734806151272619 * k);
+    float i4 = mul_d(v[4], 0.353553390593273762 * k);
+    float i5 = mul_d(v[5], 0.277785116509801112 * k);
+    float i6 = mul_d(v[6], 0.191341716182544886 * k);
+    float i7 = mul_d(v[7], 0.0975451610080641339 * k);
+    float t00 = i0 + i4, t01 = i0 - i4, t02 = i6 + i2, t03 = i6 - i2;
+    float t04 = i7 + i1, t05 = i7 - i1, t06 = i5 + i3, t07 = i5 - i3;
+    float t08 = t04 + t06, t09 = t04 - t06, t10 = t00 + t02, t11 = t00 - t02;
+    float t12 = t07 - t05, t13 = c3 * t12, t14 = c1 * t03 + t02;
+    float t15 = t01 - t14, t16 = t01 + t14, t17 = c2 * t05 + t13;
+    float t18 = c4 * t07 + t13, t19 = t08 + t17, t20 = c1 * t09 + t19;
+    float t21 = t18 - t20;
+    v[0] = t10 + t08;
+    v[1] = t15 - t19;
+    v[2] = t16 + t20;
+    v[3] = t11 + t21;
+    v[4] = t11 - t21;
+    v[5] = t16 - t20;
+    v[6] = t15 + t19;
+    v[7] = t10 - t08;
+  }
+}
+
+static void lines_dct(int n, float* v) {
+  if (n == 3) {
+    float i0 = v[0], i1 = v[1], i2 = v[2];
+    v[0] = (i0 + i1 + i2) * (1 / 3.0f);
+    v[1] = mul_d(i0 - i2, 0.57735029);
+    v[2] = (i0 + i2) * (1 / 3.0f) - i1 * (2 / 3.0f);
+  } else if (n == 4) {
+    float t0 = v[0] + v[3], t1 = v[1] + v[2], t2 = v[0] - v[3];
+    float t3 = v[1] - v[2];
+    float t4 = t0 + t1, t5 = t0 - t1, t6 = t2 - t3;
+    float t7 = t3 * 1.414213562373095048f;
+    float t8 = t6 + t7, t9 = t6 - t7;
+    v[0] = t4 * (0.5f / 2);
+    v[1] = t8 * (0.653281482438188264f / 2);
+    v[2] = t5 * (0.5f / 2);
+    v[3] = t9 * (0.270598050073098492f / 2);
+  } else if (n == 6) {
+    float i0 = v[0], i1 = v[1], i2 = v[2], i3 = v[3], i4 = v[4], i5 = v[5];
+    float t0 = mul_d(i1 - i4, 0.23570227);
+    v[0] = (i0 + i1 + i2 + i3 + i4 + i5) * (1 / 6.0f);
+    v[1] = (float)((double)(i0 - i5) * 0.32197529 + (double)t0 +
+                   (double)(i2 - i3) * 0.08627302);
+    v[2] = mul_d(i0 - i2 - i3 + i5, 0.28867514);
+    v[3] = mul_d(i0 - i1 - i2 + i3 + i4 - i5, 0.23570227);
+    v[4] = (i0 + i2 + i3 + i5) * (1 / 6.0f) - (i1 + i4) * (1 / 3.0f);
+    v[5] = (float)((double)(i0 - i5) * 0.08627302 - (double)t0 +
+                   (double)(i2 - i3) * 0.32197529);
+  } else {
+    const float c1 = 0.707106781186548f, c2 = 0.382683432365090f;
+    const float c3 = 1.30656296487638f, c4 = 0.541196100146197f;
+    const float k = 2.8284271247461903f;
+    float t00 = v[0] + v[7], t01 = v[0] - v[7], t02 = v[3] + v[4];
+    float t03 = v[3] - v[4], t04 = v[2] + v[5], t05 = v[2] - v[5];
+    float t06 = v[1] + v[6], t07 = v[1] - v[6];
+    float t08 = t00 + t02, t09 = t00 - t02, t10 = t06 + t04, t11 = t06 - t04;
+    float t12 = t07 + t05, t13 = t01 + t07, t14 = t05 + t03;
+    float t15 = t11 + t09, t16 = t14 - t13;
+    float t17 = c1 * t15, t18 = c1 * t12, t19 = c2 * t16;
+    float t20 = t01 + t18, t21 = t01 - t18;
+    float t22 = c3 * t13 + t19, t23 = c4 * t14 + t19;
+    v[0] = (t08 + t10) * (0.353553390593273762f / k);
+    v[1] = (t20 + t22) * (0.254897789552079584f / k);
+    v[2] = (t09 + t17) * (0.270598050073098492f / k);
+    v[3] = (t21 - t23) * (0.30067244346752264f / k);
+    v[4] = (t08 - t10) * (0.353553390593273762f / k);
+    v[5] = (t21 + t23) * (0.449988111568207852f / k);
+    v[6] = (t09 - t17) * (0.653281482438188264f / k);
+    v[7] = (t20 - t22) * (1.28145772387075309f / k);
+  }
+}
+
+static void lines_to_pixels(const float* coef, float* pix, size_t ps) {
+  float t[64], v[8], h[4];
+  memset(t, 0, sizeof(t));
+  for (int l = 0; l < 13; ++l) {
+    for (int j = 0; j < kLn[l]; ++j) v[j] = coef[kLcoef[l][j]];
+    lines_idct(kLn[l], v);
+    for (int j = 0; j < kLn[l]; ++j) t[kLpix[l][j]] = v[j];
+  }
+  {
+    const float i0 = coef[0], i1 = coef[8], i2 = coef[1], i3 = coef[9];
+    h[0] = i0 + i1 + i2 + i3;
+    h[1] = i0 - i1 + i2 - i3;
+    h[2] = i0 + i1 - i2 - i3;
+    h[3] = i0 - i1 - i2 + i3;
+  }
+  for (int g = 0; g < 4; ++g) {
+    for (int j = 0; j < kLgn[g]; ++j) t[kLg[g][j]] += h[g];
+  }
+  t[21] = (t[12] + t[30]) * 0.5f;
+  t[28] = (t[19] + t[37]) * 0.5f;
+  t[35] = (t[26] + t[44]) * 0.5f;
+  t[42] = (t[33] + t[51]) * 0.5f;
+  for (int y = 0; y < 8; ++y) {
+    for (int x = 0; x < 8; ++x) pix[y * ps + x] = t[y * 8 + x];
+  }
+}
+
+static void lines_from_pixels(const float* pix, size_t ps, float* coef) {
+  float t[64], v[8], m[4];
+  for (int y = 0; y < 8; ++y) {
+    for (int x = 0; x < 8; ++x) t[y * 8 + x] = pix[y * ps + x];
+  }
+  for (int g = 0; g < 4; ++g) {
+    float sum = 0;
+    for (int j = 0; j < kLgn[g]; ++j) sum += t[kLg[g][j]];
+    sum /= (float)kLgn[g];
+    for (int j = 0; j < kLgn[g]; ++j) t[kLg[g][j]] -= sum;
+    m[g] = sum;
+  }
+  coef[kLgslot[0]] = (m[0] + m[1] + m[2] + m[3]) * 0.25f;
+  coef[kLgslot[1]] = (m[0] - m[1] + m[2] - m[3]) * 0.25f;
+  coef[kLgslot[2]] = (m[0] + m[1] - m[2] - m[3]) * 0.25f;
+  coef[kLgslot[3]] = (m[0] - m[1] - m[2] + m[3]) * 0.25f;
+  for (int l = 0; l < 13; ++l) {
+    for (int j = 0; j < kLn[l]; ++j) v[j] = t[kLpix[l][j]];
+    lines_dct(kLn[l], v);
+    for (int j = 0; j < kLn[l]; ++j) coef[kLcoef[l][j]] = v[j];
+  }
+}
+
 /* AcStrategy::TransformToPixels, pik/ac_strategy.cc:674-840. */
 static void to_pixels(int raw, const float* coef, size_t cs, float* pix,
                       size_t ps) {
@@ -498,6 +687,9 @@ static void to_pixels(int raw, const float* coef, size_t cs, float* pix,
       }
       break;
     }
+    case kLines:
+      lines_to_pixels(coef, pix, ps);
+      break;
     default:
       break;
   }
@@ -567,6 +759,9 @@ static void from_pixels(int raw, const float* pix, size_t ps, float* coef,
       quarter_hadamard_in_place(coef);
       break;
     }
+    case kLines:
+      lines_from_pixels(pix, ps, coef);
+      break;
     default:
       break;
   }
@@ -575,7 +770,7 @@ static void from_pixels(int raw, const float* pix, size_t ps, float* coef,
 int pikoracle_block_transform(int raw, int dir, const float* in,
                               float* out) {
   init_tables();
-  if (raw < 0 || raw > 8 || raw == kLines) return -1;
+  if (raw < 0 || raw > 8) return -1;
   const int n = 8 * covered(raw);
   if (dir == 0) {
     to_pixels(raw, in, 64 * covered(raw), out, n);
@@ -1716,7 +1911,6 @@ static int check_frame(const pikgpu_frame* f) {
   for (size_t y = 0; y < by; ++y) {
     for (size_t x = 0; x < bx; ++x) {
       const int raw = f->ac_strategy[y * f->ac_strategy_stride + x] >> 4;
-      if (raw == kLines) return fail(PIKGPU_EUNSUPPORTED, "LINES strategy");
       if (raw > 8) return fail(PIKGPU_EINVAL, "invalid AC strategy");
       const int q = f->quant_field[y * f->quant_field_stride + x];
       if (q < 1 || q > 256) return fail(PIKGPU_EINVAL, "quant field range");

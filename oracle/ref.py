@@ -197,3 +197,129 @@ def quantize(frame, opsin, kind="scalar"):
     if rc != 0:
         raise RuntimeError("pikref_quantize: %s" % L.pikref_last_error().decode())
     return ac, dc
+
+
+def _planes(img):
+    img = np.ascontiguousarray(img, np.float32)
+    return img, (ctypes.c_void_p * 3)(*[img[c].ctypes.data for c in range(3)])
+
+
+def butteraugli(opsin, linear, distmap=False, kind="scalar"):
+    """The reference Butteraugli distance of `linear` (3, H, W) from the
+    image whose unpadded XYB planes are `opsin` (3, H, W), as the encoder's
+    comparator computes it. Returns the distance, or (distance, map) with
+    distmap=True."""
+    L = lib(kind)
+    L.pikref_butteraugli.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_size_t, ctypes.c_size_t,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+    o, op = _planes(opsin)
+    li, lp = _planes(linear)
+    assert o.shape == li.shape, (o.shape, li.shape)
+    _, ys, xs = o.shape
+    dm = np.zeros((ys, xs), np.float32) if distmap else None
+    d = ctypes.c_double()
+    rc = L.pikref_butteraugli(ctypes.cast(op, ctypes.c_void_p),
+                              ctypes.cast(lp, ctypes.c_void_p), xs, ys,
+                              dm.ctypes.data if distmap else None,
+                              ctypes.byref(d))
+    if rc != 0:
+        raise RuntimeError("pikref_butteraugli: %s"
+                           % L.pikref_last_error().decode())
+    return (d.value, dm) if distmap else d.value
+
+
+def initial_quant_field(opsin, target, kind="scalar"):
+    """The reference's InitialQuantField of the unpadded XYB image (3, H, W)
+    at distance `target`: (ceil(H/8), ceil(W/8)) float32."""
+    L = lib(kind)
+    L.pikref_initial_quant_field.argtypes = [
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_float,
+        ctypes.c_void_p]
+    o, op = _planes(opsin)
+    _, ys, xs = o.shape
+    qf = np.zeros((div_ceil(ys, 8), div_ceil(xs, 8)), np.float32)
+    rc = L.pikref_initial_quant_field(ctypes.cast(op, ctypes.c_void_p), xs,
+                                      ys, target, qf.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("pikref_initial_quant_field: %s"
+                           % L.pikref_last_error().decode())
+    return qf
+
+
+def initial_quant_dc(target, kind="scalar"):
+    L = lib(kind)
+    L.pikref_initial_quant_dc.argtypes = [ctypes.c_float]
+    L.pikref_initial_quant_dc.restype = ctypes.c_float
+    return float(L.pikref_initial_quant_dc(target))
+
+
+def color_correlation(opsin, kind="scalar"):
+    """The reference's FindBestColorCorrelationMap of the padded XYB image
+    (3, H8, W8): (ytox (ty, tx) int32, ytob, ytox_dc, ytob_dc)."""
+    L = lib(kind)
+    L.pikref_color_correlation.argtypes = [
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_void_p]
+    o, op = _planes(opsin)
+    _, ys, xs = o.shape
+    tx, ty = div_ceil(xs, 64), div_ceil(ys, 64)
+    ytox = np.zeros((ty, tx), np.int32)
+    ytob = np.zeros((ty, tx), np.int32)
+    dc = np.zeros(2, np.int32)
+    rc = L.pikref_color_correlation(ctypes.cast(op, ctypes.c_void_p), xs, ys,
+                                    ytox.ctypes.data, ytob.ctypes.data,
+                                    dc.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("pikref_color_correlation: %s"
+                           % L.pikref_last_error().decode())
+    return ytox, ytob, int(dc[0]), int(dc[1])
+
+
+def encoded_bytes(frame, kind="scalar"):
+    """(AC bytes, DC + control-field bytes) of the frame's quantized data
+    with the reference's entropy coder."""
+    L = lib(kind)
+    L.pikref_encoded_bytes.argtypes = [ctypes.POINTER(abi.FrameC),
+                                       ctypes.c_void_p, ctypes.c_void_p]
+    fc, keep = frame.to_c()
+    a, d = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = L.pikref_encoded_bytes(ctypes.byref(fc), ctypes.byref(a),
+                                ctypes.byref(d))
+    if rc != 0:
+        raise RuntimeError("pikref_encoded_bytes: %s"
+                           % L.pikref_last_error().decode())
+    return a.value, d.value
+
+
+def find_best_quantization(frame, opsin, opsin_orig, init, target,
+                           rounds=12, threads=1, kind="scalar"):
+    """The reference encoder's Butteraugli-driven quant-field search over
+    the frame's side information (restated over the reference's own
+    functions). opsin: padded XYB (3, by*8, bx*8) the encoder quantizes;
+    opsin_orig: unpadded XYB (3, H, W); init: initial float field (by, bx).
+    Returns dict(quant_field (by, bx) int32, global_scale, quant_dc,
+    distance, seconds)."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_find_best_quantization.argtypes = [
+        ctypes.POINTER(abi.FrameC), vp, vp, vp, ctypes.c_float, ctypes.c_int,
+        ctypes.c_int, vp, vp, vp, vp]
+    fc, keep = frame.to_c()
+    o, op = _planes(opsin)
+    oo, oop = _planes(opsin_orig)
+    init = np.ascontiguousarray(init, np.float32)
+    by, bx = frame.ysize_blocks, frame.xsize_blocks
+    assert init.shape == (by, bx)
+    qf = np.zeros((by, bx), np.int32)
+    sc = np.zeros(2, np.int32)
+    d, sec = ctypes.c_double(), ctypes.c_double()
+    rc = L.pikref_find_best_quantization(
+        ctypes.byref(fc), ctypes.cast(op, vp), ctypes.cast(oop, vp),
+        init.ctypes.data, target, rounds, threads, qf.ctypes.data,
+        sc.ctypes.data, ctypes.byref(d), ctypes.byref(sec))
+    if rc != 0:
+        raise RuntimeError("pikref_find_best_quantization: %s"
+                           % L.pikref_last_error().decode())
+    return dict(quant_field=qf, global_scale=int(sc[0]), quant_dc=int(sc[1]),
+                distance=d.value, seconds=sec.value)

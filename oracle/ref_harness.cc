@@ -37,6 +37,7 @@
 #include <array>
 #include <atomic>
 #include <cassert>
+#include <chrono>
 #include <cinttypes>
 #include <cmath>
 #include <complex>
@@ -70,10 +71,12 @@
 #include "pik/ar_control_field.h"
 #include "pik/bilinear_transform.h"
 #include "pik/bit_reader.h"
+#include "pik/butteraugli_comparator.h"
 #include "pik/color_correlation.h"
 #include "pik/compressed_dc.h"
 #include "pik/compressed_image.h"
 #include "pik/data_parallel.h"
+#include "pik/entropy_coder.h"
 #include "pik/epf.h"
 #include "pik/gaborish.h"
 #include "pik/headers.h"
@@ -721,4 +724,522 @@ API int pikref_quantize(const pikgpu_frame* f, const float* const* opsin,
 // Reference LinearToXyb for one pixel (for generator tests).
 API void pikref_linear_to_xyb(float r, float g, float b, float* xyb) {
   LinearToXyb(r, g, b, &xyb[0], &xyb[1], &xyb[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Quant-field search: the judge (Butteraugli), the byte count and the
+// reference's own search loop as the comparator of the GPU gradient search.
+
+namespace {
+
+// The reference objects of a frame's side information (its AC / DC / quant
+// field are not used).
+struct Side {
+  std::unique_ptr<DequantMatrices> matrices;
+  FrameHeader header;
+  GroupHeader group_header;
+  std::unique_ptr<ColorCorrelationMap> cmap;
+  AcStrategyImage acs;
+  ImageB luts;
+  ImageB dcf;
+  uint8_t dqmap[kMaxQuantControlFieldValue][256];
+};
+
+int ImportSide(const pikgpu_frame* f, Side* s) {
+  if (f == nullptr) return Fail("null frame");
+  if (f->num_dots != 0 || IsGray(f)) {
+    return Fail("search: frames without dots, not grayscale");
+  }
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  s->matrices.reset(new DequantMatrices(/*need_inv_matrices=*/true));
+  std::vector<float> t;
+  std::vector<uint32_t> o;
+  ExportTables(*s->matrices, &t, &o);
+  if (f->dequant_table_size != t.size() ||
+      memcmp(f->dequant_table, t.data(), t.size() * 4) != 0 ||
+      f->num_dequant_offsets != o.size() ||
+      memcmp(f->dequant_offsets, o.data(), o.size() * 4) != 0) {
+    return Fail("frame does not use the default dequant tables");
+  }
+  s->header.flags = 0;
+  s->header.gaborish = GaborishStrength(f->gaborish);
+  s->header.predict_lf = f->predict_lf != 0;
+  s->header.predict_hf = f->predict_hf != 0;
+  s->header.have_adaptive_reconstruction = f->adaptive_reconstruction != 0;
+  s->header.epf_params.enable_adaptive = f->epf_enable_adaptive != 0;
+  s->header.epf_params.use_sharpened = f->epf_use_sharpened != 0;
+  s->cmap.reset(new ColorCorrelationMap(f->xsize, f->ysize));
+  for (size_t y = 0; y < s->cmap->ytox_map.ysize(); ++y) {
+    memcpy(s->cmap->ytox_map.Row(y), f->ytox_map + y * f->ytox_stride,
+           s->cmap->ytox_map.xsize() * 4);
+    memcpy(s->cmap->ytob_map.Row(y), f->ytob_map + y * f->ytob_stride,
+           s->cmap->ytob_map.xsize() * 4);
+  }
+  s->cmap->ytox_dc = f->ytox_dc;
+  s->cmap->ytob_dc = f->ytob_dc;
+  ImageB raw(bx, by);
+  s->luts = ImageB(bx, by);
+  for (size_t y = 0; y < by; ++y) {
+    for (size_t x = 0; x < bx; ++x) {
+      raw.Row(y)[x] = f->ac_strategy[y * f->ac_strategy_stride + x] >> 4;
+      s->luts.Row(y)[x] = f->sigma_lut_ids[y * f->sigma_lut_stride + x];
+    }
+  }
+  s->acs = AcStrategyImage(bx, by);
+  s->acs.SetFromRaw(Rect(0, 0, bx, by), raw);
+  const size_t tx = DivCeil<size_t>(f->xsize, kColorTileDim);
+  const size_t ty = DivCeil<size_t>(f->ysize, kColorTileDim);
+  s->dcf = ImageB(tx, ty);
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(s->dcf.Row(y), f->dequant_cf + y * f->dequant_cf_stride, tx);
+  }
+  memcpy(s->dqmap, f->dequant_map, sizeof(s->dqmap));
+  return 0;
+}
+
+Image3F ImportPlanes(const float* const* p, size_t xs, size_t ys) {
+  Image3F img(xs, ys);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < ys; ++y) {
+      memcpy(img.PlaneRow(c, y), p[c] + y * xs, xs * 4);
+    }
+  }
+  return img;
+}
+
+CompressParams SearchParams(float target) {
+  CompressParams cp;
+  cp.butteraugli_distance = target;
+  return cp;
+}
+
+}  // namespace
+
+// Butteraugli distance of `linear` (dense xs x ys planes, linear RGB in
+// [0, 255]) from the image whose XYB planes are `opsin` (dense, unpadded),
+// as the reference encoder's comparator measures it (ButteraugliComparator,
+// pik/butteraugli_comparator.cc:23-44, hf_asymmetry 1 and the default
+// intensity multiplier, pik/pik_params.h:116-136). `distmap` (dense xs x ys,
+// may be NULL) receives the per-pixel map.
+API int pikref_butteraugli(const float* const* opsin, const float* const* linear,
+                           size_t xs, size_t ys, float* distmap,
+                           double* distance) {
+  if (opsin == nullptr || linear == nullptr || distance == nullptr) {
+    return Fail("null argument");
+  }
+  const CompressParams cp;
+  ButteraugliComparator comparator(ImportPlanes(opsin, xs, ys),
+                                   cp.hf_asymmetry,
+                                   cp.GetIntensityMultiplier());
+  comparator.Compare(ImportPlanes(linear, xs, ys));
+  *distance = comparator.distance();
+  if (distmap != nullptr) {
+    for (size_t y = 0; y < ys; ++y) {
+      memcpy(distmap + y * xs, comparator.distmap().ConstRow(y), xs * 4);
+    }
+  }
+  return 0;
+}
+
+// The reference's initial quant field (InitialQuantField,
+// pik/adaptive_quantization.cc:1047-1060) of the unpadded XYB image `opsin`
+// (dense xs x ys planes) at Butteraugli distance `target`: by x bx floats.
+API int pikref_initial_quant_field(const float* const* opsin, size_t xs,
+                                   size_t ys, float target, float* qf) {
+  if (opsin == nullptr || qf == nullptr) return Fail("null argument");
+  const CompressParams cp = SearchParams(target);
+  const ImageF field =
+      InitialQuantField(target, cp.GetIntensityMultiplier(),
+                        ImportPlanes(opsin, xs, ys), cp, nullptr, 1.0);
+  for (size_t y = 0; y < field.ysize(); ++y) {
+    memcpy(qf + y * field.xsize(), field.ConstRow(y), field.xsize() * 4);
+  }
+  return 0;
+}
+
+// InitialQuantDC (pik/adaptive_quantization.cc:1040-1045) at `target`.
+API float pikref_initial_quant_dc(float target) {
+  return InitialQuantDC(target, SearchParams(target).GetIntensityMultiplier());
+}
+
+// The reference's colour correlation map of the padded XYB image
+// (FindBestColorCorrelationMap, pik/color_correlation.h:101): ytox / ytob
+// per 64x64 tile (ty x tx each) and the DC factors.
+API int pikref_color_correlation(const float* const* opsin, size_t xs,
+                                 size_t ys, int32_t* ytox, int32_t* ytob,
+                                 int32_t* dc) {
+  if (opsin == nullptr || ytox == nullptr || ytob == nullptr || dc == nullptr) {
+    return Fail("null argument");
+  }
+  DequantMatrices matrices(/*need_inv_matrices=*/true);
+  ColorCorrelationMap cmap(xs, ys);
+  FindBestColorCorrelationMap(ImportPlanes(opsin, xs, ys), matrices, &cmap);
+  const size_t tx = cmap.ytox_map.xsize(), ty = cmap.ytox_map.ysize();
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(ytox + y * tx, cmap.ytox_map.ConstRow(y), tx * 4);
+    memcpy(ytob + y * tx, cmap.ytob_map.ConstRow(y), tx * 4);
+  }
+  dc[0] = cmap.ytox_dc;
+  dc[1] = cmap.ytob_dc;
+  return 0;
+}
+
+// Encoded size of a frame's quantized data with the reference's entropy
+// coder: *ac_bytes sums EncodeToBitstream over the 512-px groups (coefficient
+// orders, histograms and AC tokens, pik/compressed_image.cc:457-573, as
+// PixelsToPikGroup encodes each group, pik/pik_frame.cc:345-411);
+// *dc_bytes the DC groups (quantized DC, AC strategy, quant field and sigma
+// LUT ids as CompressDCGroup codes them, pik/compressed_dc.cc:233-273, group
+// TOC excluded) plus the quantizer's three bytes and the colour correlation
+// map (pik/pik_frame.cc:560-572).
+API int pikref_encoded_bytes(const pikgpu_frame* f, size_t* ac_bytes,
+                             size_t* dc_bytes) {
+  Side side;
+  if (ImportSide(f, &side) != 0) return -1;
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  Quantizer quantizer = DecoderQuantizer(side.matrices.get(), bx, by,
+                                         f->quant_dc, f->global_scale);
+  {
+    ImageI qf(bx, by);
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(qf.Row(y), f->quant_field + y * f->quant_field_stride, bx * 4);
+    }
+    quantizer.SetRawQuantField(std::move(qf));
+  }
+  size_t ac = 0;
+  const size_t gxs = DivCeil<size_t>(f->xsize, kGroupDim);
+  const size_t gys = DivCeil<size_t>(f->ysize, kGroupDim);
+  for (size_t gy = 0; gy < gys; ++gy) {
+    for (size_t gx = 0; gx < gxs; ++gx) {
+      const Rect group(gx * kGroupDim, gy * kGroupDim, kGroupDim, kGroupDim,
+                       bx * kBlockDim, by * kBlockDim);
+      const Rect blocks(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                        kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+      EncCache cache;
+      cache.xsize_blocks = blocks.xsize();
+      cache.ysize_blocks = blocks.ysize();
+      cache.ac_strategy = side.acs.Copy(blocks);
+      cache.ac = Image3S(blocks.xsize() * 64, blocks.ysize());
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < blocks.ysize(); ++y) {
+          memcpy(cache.ac.PlaneRow(c, y),
+                 f->ac[c] + (blocks.y0() + y) * f->ac_stride + blocks.x0() * 64,
+                 blocks.xsize() * 64 * 2);
+        }
+      }
+      const PaddedBytes bytes = EncodeToBitstream(
+          cache, Rect(0, 0, group.xsize(), group.ysize()),
+          quantizer.Copy(blocks), NoiseParams(), /*fast_mode=*/false,
+          /*handler=*/nullptr);
+      ac += bytes.size();
+    }
+  }
+  // DC groups as CompressDCGroup codes them without the lossless DC coder
+  // (use_new_dc off, the default, pik/compressed_dc.cc:233-273; the
+  // single-image manager gives no hints): DC residuals, then the control
+  // fields (strategy, quant field, sigma LUT ids) with their histograms.
+  Image3S dc(bx, by);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(dc.PlaneRow(c, y), f->dc[c] + y * f->dc_stride, bx * 2);
+    }
+  }
+  size_t dcb = 0;
+  const size_t dgx = DivCeil<size_t>(bx, kDcGroupDimInBlocks);
+  const size_t dgy = DivCeil<size_t>(by, kDcGroupDimInBlocks);
+  for (size_t gy = 0; gy < dgy; ++gy) {
+    for (size_t gx = 0; gx < dgx; ++gx) {
+      const Rect rect(gx * kDcGroupDimInBlocks, gy * kDcGroupDimInBlocks,
+                      kDcGroupDimInBlocks, kDcGroupDimInBlocks, bx, by);
+      Image3S residuals(rect.xsize(), rect.ysize());
+      ShrinkDC(rect, dc, &residuals);
+      const std::string dc_code =
+          EncodeImageData(Rect(residuals), residuals, nullptr);
+      std::vector<std::vector<Token>> tokens(1);
+      TokenizeAcStrategy(rect, side.acs, nullptr, &tokens[0]);
+      TokenizeQuantField(rect, quantizer.RawQuantField(), nullptr, side.acs,
+                         &tokens[0]);
+      TokenizeARParameters(rect, side.luts, side.acs, &tokens[0]);
+      std::vector<uint8_t> context_map;
+      std::vector<ANSEncodingData> codes;
+      const std::string histo = BuildAndEncodeHistograms(
+          kNumControlFieldContexts, tokens, &codes, &context_map, nullptr);
+      const std::string fields =
+          WriteTokens(tokens[0], codes, context_map, nullptr);
+      dcb += dc_code.size() + histo.size() + fields.size();
+    }
+  }
+  const std::string qcode = quantizer.Encode(nullptr);
+  const std::string cmap_code =
+      EncodeColorMap(side.cmap->ytob_map, Rect(side.cmap->ytob_map),
+                     side.cmap->ytob_dc, nullptr) +
+      EncodeColorMap(side.cmap->ytox_map, Rect(side.cmap->ytox_map),
+                     side.cmap->ytox_dc, nullptr);
+  if (ac_bytes) *ac_bytes = ac;
+  if (dc_bytes) *dc_bytes = dcb + qcode.size() + cmap_code.size();
+  return 0;
+}
+
+namespace {
+
+// TileDistMap (pik/adaptive_quantization.cc:420-482; in an unnamed
+// namespace there): per transform, the 16-norm of the distance map over its
+// pixels widened by `margin` (edge pixels weighted 0.98, corners 0.7 when
+// margin != 0), times 1.2, written to every block the transform covers.
+ImageF TileDistances(const ImageF& dist, int margin,
+                     const AcStrategyImage& acs) {
+  const int tx = int(DivCeil<size_t>(dist.xsize(), 8));
+  const int ty = int(DivCeil<size_t>(dist.ysize(), 8));
+  ImageF out(tx, ty);
+  for (int by = 0; by < ty; ++by) {
+    for (int bx = 0; bx < tx; ++bx) {
+      const AcStrategy a = acs.ConstRow(by)[bx];
+      if (!a.IsFirstBlock()) continue;
+      const int y0 = std::max(0, 8 * by - margin);
+      const int y1 = std::min<int>(dist.ysize(),
+                                   8 * (by + int(a.covered_blocks_y())) + margin);
+      const int x0 = std::max(0, 8 * bx - margin);
+      const int x1 = std::min<int>(dist.xsize(),
+                                   8 * (bx + int(a.covered_blocks_x())) + margin);
+      float norm = 0.0f;
+      double weight = 0.0;
+      for (int y = y0; y < y1; ++y) {
+        const bool yedge = margin != 0 && (y == y0 || y == y1 - 1);
+        const float* row = dist.ConstRow(y);
+        for (int x = x0; x < x1; ++x) {
+          const bool xedge = margin != 0 && (x == x0 || x == x1 - 1);
+          const float w = yedge ? (xedge ? 0.7f : 0.98f) : (xedge ? 0.98f : 1.0f);
+          float v = row[x];
+          v *= v;
+          v *= v;
+          v *= v;
+          v *= v;
+          norm += w * v;
+          weight += w;
+        }
+      }
+      if (weight == 0) weight = 1;
+      const double d = 1.2 * std::pow(norm / weight, 1.0 / 16);
+      for (size_t iy = 0; iy < a.covered_blocks_y(); ++iy) {
+        for (size_t ix = 0; ix < a.covered_blocks_x(); ++ix) {
+          out.Row(by + iy)[bx + ix] = d;
+        }
+      }
+    }
+  }
+  return out;
+}
+
+// AdjustQuantField (pik/adaptive_quantization.cc:554-578): a multi-block
+// transform takes the maximum of the values over its blocks.
+void MaxOverTransforms(const AcStrategyImage& acs, ImageF* field) {
+  for (size_t y = 0; y < field->ysize(); ++y) {
+    for (size_t x = 0; x < field->xsize(); ++x) {
+      const AcStrategy a = acs.ConstRow(y)[x];
+      if (!a.IsFirstBlock()) continue;
+      float m = field->Row(y)[x];
+      for (size_t iy = 0; iy < a.covered_blocks_y(); ++iy) {
+        for (size_t ix = 0; ix < a.covered_blocks_x(); ++ix) {
+          m = std::max(m, field->Row(y + iy)[x + ix]);
+        }
+      }
+      for (size_t iy = 0; iy < a.covered_blocks_y(); ++iy) {
+        for (size_t ix = 0; ix < a.covered_blocks_x(); ++ix) {
+          field->Row(y + iy)[x + ix] = m;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+
+// One round trip of the search: the frame with `quantizer`'s serialized
+// scalars and field, its coefficients quantized from `opsin` by the
+// reference encoder (pikref_quantize), then decoded by the reference
+// (Recon) into `linear`.
+int Roundtrip(const pikgpu_frame* f, const Quantizer& quantizer,
+              const float* const* opsin, int num_threads,
+              std::vector<int16_t>* ac, std::vector<int16_t>* dc,
+              Image3F* linear) {
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  pikgpu_frame g = *f;
+  const std::string b = quantizer.Encode(nullptr);
+  g.global_scale = ((uint8_t(b[0]) << 8) | uint8_t(b[1])) + 1;
+  g.quant_dc = uint8_t(b[2]) + 1;
+  g.quant_field = quantizer.RawQuantField().ConstRow(0);
+  g.quant_field_stride = quantizer.RawQuantField().PixelsPerRow();
+  ac->assign(3 * by * bx * 64, 0);
+  dc->assign(3 * by * bx, 0);
+  int16_t* acp[3];
+  int16_t* dcp[3];
+  for (int c = 0; c < 3; ++c) {
+    acp[c] = ac->data() + c * by * bx * 64;
+    dcp[c] = dc->data() + c * by * bx;
+    g.ac[c] = acp[c];
+    g.dc[c] = dcp[c];
+  }
+  g.ac_stride = bx * 64;
+  g.dc_stride = bx;
+  if (pikref_quantize(&g, opsin, acp, dcp) != 0) return -1;
+  float* out[3];
+  std::vector<float> buf(3 * f->xsize * f->ysize);
+  for (int c = 0; c < 3; ++c) out[c] = buf.data() + c * f->xsize * f->ysize;
+  if (Recon(&g, num_threads, 1, nullptr, out) != 0) return -1;
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < f->ysize; ++y) {
+      memcpy(linear->PlaneRow(c, y), out[c] + y * f->xsize, f->xsize * 4);
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// The reference encoder's quant-field search (FindBestQuantization,
+// pik/adaptive_quantization.cc:677-853), restated over the reference's own
+// functions because the function itself cannot be linked here (its debug
+// heat maps reach the colour-management code). Each round sets the field
+// through Quantizer::SetQuantField, round-trips the image through the
+// reference's RoundtripImage, compares with the reference Butteraugli, tiles
+// the distance map (TileDistMap) and rescales every block's quant value by
+// its tile distance relative to the target: upward where the distance is
+// above the target, downward by a round-dependent power of the ratio below
+// it, within bounds derived from the initial field; from round 6 on, an
+// increase that made the nearby distance worse by more than 1.5 % is taken
+// back; at round 5 the field is pulled 60 % of the way back toward the
+// initial one where it dropped below that.
+//
+// Inputs: the frame's side information (strategy, sigma LUTs, colour
+// correlation, dequant control field / map, header flags); `opsin` the
+// padded XYB image the encoder quantizes, `opsin_orig` the unpadded one the
+// comparator holds (dense planes); `init` the initial float field (by x bx).
+// Outputs: the final integer field (qf_out, by x bx), global scale and
+// DC quant (scalars[0..1]), the last measured distance and the loop's wall
+// time (excluding set-up). `rounds` Butteraugli rounds (the reference runs
+// max_butteraugli_iters + 1 = 12).
+API int pikref_find_best_quantization(const pikgpu_frame* f,
+                                      const float* const* opsin,
+                                      const float* const* opsin_orig,
+                                      const float* init, float target,
+                                      int rounds, int num_threads,
+                                      int32_t* qf_out, int32_t* scalars,
+                                      double* distance, double* seconds) {
+  if (opsin == nullptr || opsin_orig == nullptr || init == nullptr ||
+      qf_out == nullptr || scalars == nullptr) {
+    return Fail("null argument");
+  }
+  if (rounds < 1) return Fail("rounds < 1");
+  InitEpfOnce();
+  Side side;
+  if (ImportSide(f, &side) != 0) return -1;
+  const size_t xs = f->xsize, ys = f->ysize;
+  const size_t bx = DivCeil<size_t>(xs, kBlockDim);
+  const size_t by = DivCeil<size_t>(ys, kBlockDim);
+  const Image3F orig = ImportPlanes(opsin_orig, xs, ys);
+  const Image3F padded = ImportPlanes(opsin, bx * kBlockDim, by * kBlockDim);
+  std::unique_ptr<ThreadPool> pool = MakePool(num_threads);
+  const CompressParams cp = SearchParams(target);
+  Quantizer quantizer(side.matrices.get(), bx, by);
+  std::vector<int16_t> q_ac, q_dc;
+  Image3F linear(xs, ys);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  ButteraugliComparator comparator(orig, cp.hf_asymmetry,
+                                   cp.GetIntensityMultiplier());
+  const float quant_dc = InitialQuantDC(target, cp.GetIntensityMultiplier());
+  ImageF field(bx, by);
+  for (size_t y = 0; y < by; ++y) memcpy(field.Row(y), init + y * bx, bx * 4);
+  MaxOverTransforms(side.acs, &field);
+  const ImageF first = CopyImage(field);
+  float lo0, hi0;
+  ImageMinMax(first, &lo0, &hi0);
+  const float dev = std::sqrt(250.0f / (hi0 / lo0));
+  const float asym = std::min(2.0f, dev);
+  const float qf_lo = lo0 / (asym * dev), qf_hi = hi0 * (dev / asym);
+  ImageF tiles, local, prev_field = CopyImage(first), prev_local;
+  const int kMargin[12] = {0, 0, 0, 1, 2, 1, 1, 1, 0, 0, 0, 0};
+  const double kPowBase[3] = {0.97524596113492301, 1.0424361904568509,
+                              0.64984804448911193};
+  const double kPowSlope[4] = {0.011236980155043978, 0.0061256294105472651,
+                               -0.0030115055086858242, 0.06929488142351059};
+  double last_distance = 0.0;
+  const int last = rounds - 1;
+  for (int r = 0; r <= last; ++r) {
+    if (quantizer.SetQuantField(quant_dc, QuantField(field))) {
+      // RoundtripImage (pik/adaptive_quantization.cc:580-671, unnamed
+      // namespace): the frame quantized with this field, then decoded
+      if (Roundtrip(f, quantizer, opsin, num_threads, &q_ac, &q_dc, &linear)) {
+        return -1;
+      }
+      comparator.Compare(linear);
+      last_distance = comparator.distance();
+      tiles = TileDistances(comparator.distmap(), kMargin[std::min(r, 11)],
+                            side.acs);
+      local = TileDistances(comparator.distmap(), 2, side.acs);
+    }
+    if (r > 5) {
+      for (size_t y = 0; y < by; ++y) {
+        float* q = field.Row(y);
+        for (size_t x = 0; x < bx; ++x) {
+          if (q[x] > prev_field.Row(y)[x] &&
+              local.Row(y)[x] > 1.015f * prev_local.Row(y)[x]) {
+            q[x] = prev_field.Row(y)[x];
+          }
+        }
+      }
+    }
+    prev_field = CopyImage(field);
+    prev_local = CopyImage(local);
+    if (r == last) break;
+    if (r == 5) {
+      for (size_t y = 0; y < by; ++y) {
+        float* q = field.Row(y);
+        for (size_t x = 0; x < bx; ++x) {
+          const double floor_v = 0.4 * q[x] + 0.6 * first.Row(y)[x];
+          if (q[x] < floor_v) q[x] = std::min(std::max(float(floor_v), qf_lo), qf_hi);
+        }
+      }
+    }
+    double p = 0.0;
+    if (r < 7) {
+      p = (r < 3 ? kPowBase[r] : 0.0) +
+          (target - 1.0) * (r < 4 ? kPowSlope[r] : 0.0);
+      if (p < 0) p = 0;
+    }
+    for (size_t y = 0; y < by; ++y) {
+      float* q = field.Row(y);
+      for (size_t x = 0; x < bx; ++x) {
+        const float ratio = tiles.Row(y)[x] / target;
+        if (ratio >= 1.0f) {
+          q[x] *= ratio;
+        } else if (p != 0.0) {
+          q[x] *= std::pow(ratio, p);
+        }
+        q[x] = std::min(std::max(q[x], qf_lo), qf_hi);
+      }
+    }
+  }
+  quantizer.SetQuantField(quant_dc, QuantField(field));
+  const auto t1 = std::chrono::steady_clock::now();
+  for (size_t y = 0; y < by; ++y) {
+    memcpy(qf_out + y * bx, quantizer.RawQuantField().ConstRow(y), bx * 4);
+  }
+  int gs = 0, qdc = 0;
+  {
+    // the serialized scalars (Quantizer::Encode, pik/quantizer.cc:51-61)
+    const std::string b = quantizer.Encode(nullptr);
+    gs = ((uint8_t(b[0]) << 8) | uint8_t(b[1])) + 1;
+    qdc = uint8_t(b[2]) + 1;
+  }
+  scalars[0] = gs;
+  scalars[1] = qdc;
+  if (distance) *distance = last_distance;
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
 }

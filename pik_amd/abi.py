@@ -127,6 +127,40 @@ class AdjIoC(ctypes.Structure):
     ]
 
 
+class SearchParamsC(ctypes.Structure):
+    """pikgpu_search_params."""
+    _fields_ = [
+        ("steps", ctypes.c_int32),
+        ("learning_rate", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("lambda_", ctypes.c_float),
+        ("qf_max", ctypes.c_int32),
+    ]
+
+
+class SearchIoC(ctypes.Structure):
+    """pikgpu_search_io."""
+    _fields_ = [
+        ("opsin", ctypes.c_void_p * 3),
+        ("opsin_stride", ctypes.c_size_t),
+        ("target", ctypes.c_void_p * 3),
+        ("target_stride", ctypes.c_size_t),
+        ("weight", ctypes.c_void_p),
+        ("weight_stride", ctypes.c_size_t),
+        ("quant_field", ctypes.c_void_p),
+        ("quant_field_stride", ctypes.c_size_t),
+        ("ac", ctypes.c_void_p * 3),
+        ("ac_stride", ctypes.c_size_t),
+        ("dc", ctypes.c_void_p * 3),
+        ("dc_stride", ctypes.c_size_t),
+        ("loss", ctypes.c_void_p),
+        ("rate", ctypes.c_void_p),
+        ("lambda_used", ctypes.c_void_p),
+    ]
+
+
 def load():
     """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
     global _lib
@@ -159,6 +193,8 @@ def load():
                                         c_float_p, ctypes.c_int]
     lib.pikgpu_stream.argtypes = [vp]
     lib.pikgpu_stream.restype = vp
+    lib.pikgpu_search_quant_field.argtypes = [
+        vp, ctypes.POINTER(SearchParamsC), ctypes.POINTER(SearchIoC)]
     lib.pikgpu_batch_create.restype = vp
     lib.pikgpu_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ConfigC)]
     lib.pikgpu_batch_destroy.argtypes = [vp]
@@ -200,5 +236,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
-    "pikgpu_frame_add_pass",
+    "pikgpu_frame_add_pass", "pikgpu_search_quant_field",
 ]

@@ -61,7 +61,11 @@ struct pikgpu_ctx {
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
   // pikgpu_quantize: XYB input, coefficients, DC, the inverse dequant table
   DevBuf eopsin, ecoef, edcf, itable;
+  // pikgpu_search_quant_field: log field, Adam moments, weights, per-block
+  // distortion, rate partials, the rate weight, loss / rate histories
+  DevBuf srch;
   bool enc_map_ok = false;  // every quant_table index of the frame in range
+  bool has_lines = false;   // a LINES block (no adjoint: fwd_adj, search)
   int global_scale = 0, quant_dc = 0;  // the uploaded frame's
   float dc_matrix0[3] = {0, 0, 0};     // DequantMatrix(0, DCT8, c)[0]
   bool adj_ready = false;  // LLF / LF caches hold the uploaded frame's
@@ -127,7 +131,9 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->guide,   &ctx->epf,    &ctx->ar,      &ctx->dots_img,
                    &ctx->gab,     &ctx->out,    &ctx->mm_partial, &ctx->mm,
                    &ctx->ar_keep, &ctx->gimg,   &ctx->predlist, &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
-                   &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss};
+                   &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss,
+                   &ctx->eopsin, &ctx->ecoef, &ctx->edcf, &ctx->itable,
+                   &ctx->srch};
   for (DevBuf* b : all) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
@@ -360,9 +366,6 @@ int ValidateFrame(const pikgpu_frame* f, std::string* err) {
         return VFail(err, PIKGPU_EINVAL, "sigma lut id > 1");
       }
       if (raw > 8) return VFail(err, PIKGPU_EINVAL, "invalid AC strategy");
-      if (raw == kLines) {
-        return VFail(err, PIKGPU_EUNSUPPORTED, "LINES AC strategy");
-      }
       const int s = AcsCovered(raw);
       if (idx >= s * s) return VFail(err, PIKGPU_EINVAL, "invalid block index");
       const size_t ox = x - idx % s, oy = y - idx / s;
@@ -699,6 +702,15 @@ int pikgpu_frame_upload(pikgpu_ctx* ctx, const pikgpu_frame* f) {
   std::vector<uint32_t> pred_list;
   int pred_end[4];
   PredClasses(f, &pred_list, pred_end);
+  ctx->has_lines = false;
+  for (size_t y = 0; y < by && !ctx->has_lines; ++y) {
+    for (size_t x = 0; x < bx; ++x) {
+      if ((f->ac_strategy[y * f->ac_strategy_stride + x] >> 4) == kLines) {
+        ctx->has_lines = true;
+        break;
+      }
+    }
+  }
   if ((rc = Grow(ctx, &ctx->predlist, pred_list.size() * 4))) return rc;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->predlist.p, pred_list.data(),
                               pred_list.size() * 4, hipMemcpyHostToDevice,
@@ -1386,6 +1398,10 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
     return Fail(ctx, PIKGPU_EUNSUPPORTED,
                 "the forward model takes single-pass colour frames");
   }
+  if (ctx->has_lines) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "the forward model has no adjoint of the LINES transform");
+  }
   const size_t bx = size_t(P0.bx), by = size_t(P0.by);
   const size_t xs = size_t(P0.xsize), ys = size_t(P0.ysize);
   const bool dev = (io->flags & PIKGPU_ADJ_DEVICE_POINTERS) != 0;
@@ -1540,6 +1556,216 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
                                 st));
   }
   return pikgpu_sync(ctx);
+}
+
+
+// Gradient quant-field search (see pikgpu.h): device-resident loop of
+// quantization (KEncQuant), forward + adjoint (KAdjForward / KAdjBackward),
+// distortion attribution (KSearchDist) and Adam (KSearchUpdate).
+int pikgpu_search_quant_field(pikgpu_ctx* ctx,
+                              const pikgpu_search_params* prm,
+                              const pikgpu_search_io* io) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  if (prm == nullptr || io == nullptr) {
+    return Fail(ctx, PIKGPU_EINVAL, "null params or io");
+  }
+  if (!ctx->uploaded) return Fail(ctx, PIKGPU_ESTATE, "no frame uploaded");
+  FrameParams& P = ctx->P;
+  if (P.predict_lf || P.predict_hf || P.num_dots != 0 || P.grayscale) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "search of frames with predictions, dots or grayscale");
+  }
+  if (ctx->has_lines) {
+    return Fail(ctx, PIKGPU_EUNSUPPORTED,
+                "the search has no adjoint of the LINES transform");
+  }
+  if (prm->steps < 0 || prm->steps > 1000 || !(prm->learning_rate > 0.0f) ||
+      !(prm->beta1 >= 0.0f && prm->beta1 < 1.0f) ||
+      !(prm->beta2 >= 0.0f && prm->beta2 < 1.0f) || !(prm->eps > 0.0f) ||
+      prm->qf_max < 1 || prm->qf_max > 256 || !std::isfinite(prm->lambda)) {
+    return Fail(ctx, PIKGPU_EINVAL, "invalid search parameters");
+  }
+  if (!io->opsin[0] || !io->opsin[1] || !io->opsin[2] || !io->target[0] ||
+      !io->target[1] || !io->target[2]) {
+    return Fail(ctx, PIKGPU_EINVAL, "need three opsin and three target planes");
+  }
+  const size_t bx = size_t(P.bx), by = size_t(P.by), nb = bx * by;
+  const size_t xs = size_t(P.xsize), ys = size_t(P.ysize);
+  const int nac = (io->ac[0] != nullptr) + (io->ac[1] != nullptr) +
+                  (io->ac[2] != nullptr);
+  const int ndc = (io->dc[0] != nullptr) + (io->dc[1] != nullptr) +
+                  (io->dc[2] != nullptr);
+  if ((nac != 0 && nac != 3) || (ndc != 0 && ndc != 3)) {
+    return Fail(ctx, PIKGPU_EINVAL, "ac and dc outputs take all three planes");
+  }
+  if (io->opsin_stride < size_t(P.px) || io->target_stride < xs ||
+      (io->weight && io->weight_stride < bx) ||
+      (io->quant_field && io->quant_field_stride < bx) ||
+      (nac && io->ac_stride < bx * 64) || (ndc && io->dc_stride < bx)) {
+    return Fail(ctx, PIKGPU_EINVAL, "stride smaller than a row");
+  }
+  // the encoder's dequant-map lookup (indexed by the quant value itself)
+  // must stay inside the map for every value the search may take
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int qmax = prm->qf_max;
+  {
+    std::vector<uint8_t> cf(size_t(P.tx) * P.ty);
+    HIP_TRY(ctx, hipMemcpy(cf.data(), ctx->B.dcf, cf.size(),
+                           hipMemcpyDeviceToHost));
+    for (uint8_t v : cf) {
+      if (v == 15) qmax = std::min(qmax, 255);
+    }
+  }
+  hipStream_t st = ctx->stream;
+  const size_t plane = size_t(P.px) * P.py;
+  const size_t tplane = xs * ys;
+  const int steps = prm->steps;
+  const int round_groups = int(DivCeil(nb, 256));
+  const int tiles = int(DivCeil(bx, 4) * DivCeil(by, 4));
+  // search state: u, m, v, weight, dist (nb floats each), lambda, rate
+  // partials (round_groups floats), loss / rate histories (steps + 1 doubles)
+  const size_t hist_off = DivCeil((5 * nb + 1 + size_t(round_groups)) * 4, 8) * 8;
+  int rc;
+  if ((rc = Grow(ctx, &ctx->eopsin, 3 * plane * 4)) ||
+      (rc = Grow(ctx, &ctx->ecoef, 3 * nb * 64 * 4)) ||
+      (rc = Grow(ctx, &ctx->edcf, 3 * nb * 4)) ||
+      (rc = Grow(ctx, &ctx->adj_qf, nb * 4)) ||
+      (rc = Grow(ctx, &ctx->adj_in, 3 * tplane * 4)) ||
+      (rc = Grow(ctx, &ctx->adj_dqf, nb * 4)) ||
+      (rc = Grow(ctx, &ctx->adj_dc, 3 * nb * 64 * 4)) ||
+      (rc = Grow(ctx, &ctx->adj_loss, size_t(tiles) * 4 + 16)) ||
+      (rc = Grow(ctx, &ctx->srch, hist_off + 2 * size_t(steps + 1) * 8))) {
+    return rc;
+  }
+  float* dev_opsin = static_cast<float*>(ctx->eopsin.p);
+  float* target = static_cast<float*>(ctx->adj_in.p);
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = Upload2D(ctx, dev_opsin + c * plane, io->opsin[c],
+                       io->opsin_stride * 4, size_t(P.px) * 4, size_t(P.py))) ||
+        (rc = Upload2D(ctx, target + c * tplane, io->target[c],
+                       io->target_stride * 4, xs * 4, ys))) {
+      return rc;
+    }
+  }
+  float* st_u = static_cast<float*>(ctx->srch.p);
+  float* st_m = st_u + nb;
+  float* st_v = st_m + nb;
+  float* st_w = st_v + nb;
+  float* st_d = st_w + nb;
+  float* st_lam = st_d + nb;
+  float* st_rate = st_lam + 1;
+  double* h_loss = reinterpret_cast<double*>(
+      static_cast<char*>(ctx->srch.p) + hist_off);
+  double* h_rate = h_loss + (steps + 1);
+  if (io->weight) {
+    if ((rc = Upload2D(ctx, st_w, io->weight, io->weight_stride * 4, bx * 4,
+                       by))) {
+      return rc;
+    }
+  }
+  // the quantized values replace the frame's own (and any later passes)
+  P.num_passes = 1;
+  ctx->B.ac_extra = nullptr;
+  ctx->ran = false;
+  FrameBuffers& B = ctx->B;
+  const DeviceConsts* K = ctx->consts;
+  float* coef = static_cast<float*>(ctx->ecoef.p);
+  float* dcf = static_cast<float*>(ctx->edcf.p);
+  int16_t* ac16 = static_cast<int16_t*>(ctx->ac.p);
+  int16_t* dc16 = static_cast<int16_t*>(ctx->dc16.p);
+  int32_t* qf = static_cast<int32_t*>(ctx->qf.p);
+  float* qfr = static_cast<float*>(ctx->adj_qf.p);
+  float* grad = static_cast<float*>(ctx->adj_dc.p);
+  // forward transforms and the DC: independent of the AC field
+  KEncForward<<<tiles, 256, 0, st>>>(P, B, K, dev_opsin, coef, dcf);
+  const float gsf = static_cast<float>(ctx->global_scale * (1.0 / 65536));
+  {
+    EncDcScale S;
+    const float qdc = gsf * ctx->quant_dc;
+    for (int c = 0; c < 3; ++c) S.scale[c] = (1.0f / ctx->dc_matrix0[c]) * qdc;
+    KEncDc<<<DivCeil(nb, 256), 256, 0, st>>>(P, dcf, S, dc16);
+  }
+  EnqueueDc(P, B, K, st);
+  EnqueueLf(P, B, K, st, false);  // the LLF (CopyLlf) and the LF caches
+  ctx->adj_ready = true;
+  KSearchInit<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, st_u, st_m, st_v);
+
+  AdjArgs A;
+  memset(&A, 0, sizeof(A));
+  A.qf = qfr;
+  A.qf_pitch = int(bx);
+  for (int c = 0; c < 3; ++c) {
+    A.target[c] = target + c * tplane;
+    A.dcoef[c] = grad + c * nb * 64;
+  }
+  A.in_pitch = int(xs);
+  A.dqf = static_cast<float*>(ctx->adj_dqf.p);
+  A.dqf_pitch = int(bx);
+  A.dc_pitch = bx * 64;
+  A.loss_part = static_cast<float*>(ctx->adj_loss.p);
+  SearchArgs S;
+  S.weight = io->weight ? st_w : nullptr;
+  S.lambda = st_lam;
+  S.lr = prm->learning_rate;
+  S.beta1 = prm->beta1;
+  S.beta2 = prm->beta2;
+  S.eps = prm->eps;
+  S.ulo = 0.0f;
+  S.uhi = std::log(float(qmax));
+  const float* itable = static_cast<const float*>(ctx->itable.p);
+  const int grid = XcdGrid(tiles);
+  for (int t = 0; t <= steps; ++t) {
+    KSearchRound<<<round_groups, 256, 0, st>>>(P, B, st_u, qmax, qf, qfr,
+                                               st_rate);
+    KAdjLoss<<<1, 256, 0, st>>>(st_rate, round_groups, h_rate + t);
+    KEncQuant<<<DivCeil(nb, kEncQuantThreads), kEncQuantThreads, 0, st>>>(
+        P, B, coef, itable, gsf, ac16);
+    KAdjForward<<<grid, 256, 0, st>>>(P, B, K, A);
+    KAdjBackward<<<grid, 256, 0, st>>>(P, B, K, A);
+    KAdjLoss<<<1, 256, 0, st>>>(A.loss_part, tiles, h_loss + t);
+    if (t == steps) break;
+    KSearchDist<<<DivCeil(nb, 4), 256, 0, st>>>(P, B, coef, grad, st_d);
+    if (t == 0) {
+      KSearchLambda<<<1, 256, 0, st>>>(
+          st_d, S.weight, int(nb), prm->lambda,
+          prm->lambda < 0.0f ? -prm->lambda : 1.0f, st_lam);
+    }
+    KSearchUpdate<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, S, t + 1, st_d,
+                                                    st_u, st_m, st_v);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  // outputs
+  if (io->quant_field) {
+    HIP_TRY(ctx, hipMemcpy2DAsync(io->quant_field, io->quant_field_stride * 4,
+                                  qf, bx * 4, bx * 4, by,
+                                  hipMemcpyDeviceToHost, st));
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (nac) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(io->ac[c], io->ac_stride * 2,
+                                    ac16 + c * nb * 64, bx * 128, bx * 128, by,
+                                    hipMemcpyDeviceToHost, st));
+    }
+    if (ndc) {
+      HIP_TRY(ctx, hipMemcpy2DAsync(io->dc[c], io->dc_stride * 2,
+                                    dc16 + c * nb, bx * 2, bx * 2, by,
+                                    hipMemcpyDeviceToHost, st));
+    }
+  }
+  if (io->loss) {
+    HIP_TRY(ctx, hipMemcpyAsync(io->loss, h_loss, size_t(steps + 1) * 8,
+                                hipMemcpyDeviceToHost, st));
+  }
+  if (io->rate) {
+    HIP_TRY(ctx, hipMemcpyAsync(io->rate, h_rate, size_t(steps + 1) * 8,
+                                hipMemcpyDeviceToHost, st));
+  }
+  if (io->lambda_used) {
+    HIP_TRY(ctx, hipMemcpyAsync(io->lambda_used, st_lam, 4,
+                                hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return PIKGPU_OK;
 }
 
 }  // extern "C"

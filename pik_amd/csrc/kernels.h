@@ -645,6 +645,25 @@ __global__ void KAdjBackward(FrameParams P, FrameBuffers B,
                              const DeviceConsts* K, AdjArgs A);
 __global__ void KAdjLoss(const float* part, int n, double* out);
 
+// Parameters of one Adam step of the quant-field search (search.hip).
+struct SearchArgs {
+  const float* weight;  // per-block perceptual weight w_b, or null (1)
+  const float* lambda;  // rate weight per block (device scalar)
+  float lr, beta1, beta2, eps;
+  float ulo, uhi;       // bounds of the log field
+};
+__global__ void KSearchRound(FrameParams P, FrameBuffers B, const float* u,
+                             int qmax, int32_t* qf, float* qfr, float* part);
+__global__ void KSearchDist(FrameParams P, FrameBuffers B, const float* coef,
+                            const float* grad, float* dist);
+__global__ void KSearchUpdate(FrameParams P, FrameBuffers B, SearchArgs S,
+                              int t, const float* dist, float* u, float* m,
+                              float* v);
+__global__ void KSearchInit(FrameParams P, FrameBuffers B, float* u, float* m,
+                            float* v);
+__global__ void KSearchLambda(const float* dist, const float* weight, int n,
+                              float fixed, float kappa, float* out);
+
 // Generic image description for the filters.
 struct Img {
   const float* p;  // 3 planes

@@ -141,6 +141,60 @@ class Reconstructor:
             bx * 64, ctypes.cast(dp, vp), bx))
         return ac, dc
 
+    def search_quant_field(self, opsin, target, steps=20, learning_rate=0.05,
+                           beta1=0.9, beta2=0.999, eps=1e-8, lam=0.0,
+                           qf_max=256, weight=None):
+        """Gradient quant-field search on the uploaded frame
+        (pikgpu_search_quant_field). opsin: the padded XYB image the encoder
+        quantizes (3, by * 8, bx * 8); target: the linear RGB original
+        (3, ysize, xsize); weight: optional per-block weights (by, bx).
+        lam > 0 is the rate weight; lam <= 0 keeps the initial distortion
+        level times -lam (0: times 1). The context then holds the frame
+        quantized with the final field (run() decodes it). Returns a dict:
+        quant_field (by, bx) int32, ac, dc (int16), loss and rate
+        (steps + 1 float64 each), lam (the rate weight used)."""
+        f = self._frame
+        bx, by = f.xsize_blocks, f.ysize_blocks
+        img = np.ascontiguousarray(opsin, np.float32)
+        if img.shape != (3, by * 8, bx * 8):
+            raise ValueError("opsin must be (3, %d, %d)" % (by * 8, bx * 8))
+        tgt = np.ascontiguousarray(target, np.float32)
+        if tgt.shape != (3, f.ysize, f.xsize):
+            raise ValueError("target must be (3, %d, %d)" % (f.ysize, f.xsize))
+        vp = ctypes.c_void_p
+        prm = abi.SearchParamsC(int(steps), learning_rate, beta1, beta2, eps,
+                                lam, int(qf_max))
+        io = abi.SearchIoC()
+        io.opsin = (vp * 3)(*[img[c].ctypes.data for c in range(3)])
+        io.opsin_stride = bx * 8
+        io.target = (vp * 3)(*[tgt[c].ctypes.data for c in range(3)])
+        io.target_stride = f.xsize
+        if weight is not None:
+            weight = np.ascontiguousarray(weight, np.float32)
+            if weight.shape != (by, bx):
+                raise ValueError("weight must be (%d, %d)" % (by, bx))
+            io.weight = weight.ctypes.data
+            io.weight_stride = bx
+        out = {"quant_field": np.zeros((by, bx), np.int32),
+               "ac": np.zeros((3, by, bx * 64), np.int16),
+               "dc": np.zeros((3, by, bx), np.int16),
+               "loss": np.zeros(steps + 1, np.float64),
+               "rate": np.zeros(steps + 1, np.float64)}
+        io.quant_field = out["quant_field"].ctypes.data
+        io.quant_field_stride = bx
+        io.ac = (vp * 3)(*[out["ac"][c].ctypes.data for c in range(3)])
+        io.ac_stride = bx * 64
+        io.dc = (vp * 3)(*[out["dc"][c].ctypes.data for c in range(3)])
+        io.dc_stride = bx
+        io.loss = out["loss"].ctypes.data
+        io.rate = out["rate"].ctypes.data
+        lam_used = ctypes.c_float()
+        io.lambda_used = ctypes.addressof(lam_used)
+        self._check(self._lib.pikgpu_search_quant_field(
+            self._ctx, ctypes.byref(prm), ctypes.byref(io)))
+        out["lam"] = lam_used.value
+        return out
+
     def set_downsample(self, factor):
         """Downsampling factor (1, 2, 4, 8) of the following runs
         (pikgpu_set_downsample)."""

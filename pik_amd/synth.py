@@ -195,7 +195,95 @@ def forward_transform(raw, p):
         return _fwd_dct2x2(p)
     if raw == abi.ACS_IDENTITY:
         return _fwd_identity(p)
+    if raw == abi.ACS_LINES:
+        return _fwd_lines(p)
     raise ValueError("unsupported strategy %d" % raw)
+
+
+# LINES (pik/ac_strategy.cc:380-568), row-major pixel / slot numbers: the
+# diagonal lines (pixels, coefficient slots), the four mean groups and the
+# slots of their means' Hadamard.
+_LINES = [
+    ([48, 56, 57], [48, 56, 57]), ([40, 49, 58], [40, 49, 58]),
+    ([5, 14, 23], [5, 14, 23]), ([6, 7, 15], [6, 7, 15]),
+    ([32, 41, 50, 59], [32, 41, 50, 59]), ([24, 33, 51, 60], [24, 33, 42, 51]),
+    ([3, 12, 30, 39], [3, 12, 21, 30]), ([4, 13, 22, 31], [4, 13, 22, 31]),
+    ([16, 25, 34, 43, 52, 61], [16, 25, 34, 43, 52, 61]),
+    ([8, 17, 26, 44, 53, 62], [17, 26, 35, 44, 53, 62]),
+    ([1, 10, 19, 37, 46, 55], [10, 19, 28, 37, 46, 55]),
+    ([2, 11, 20, 29, 38, 47], [2, 11, 20, 29, 38, 47]),
+    ([0, 9, 18, 27, 36, 45, 54, 63], [18, 27, 36, 45, 54, 60, 39, 63])]
+_LINES_GROUPS = [
+    [0, 1, 9, 2, 10, 18, 11, 19, 27, 20, 28, 36, 29, 37, 45, 38, 46, 54, 47,
+     55, 63],
+    [3, 4, 12, 5, 13, 21, 6, 14, 22, 30, 7, 15, 23, 31, 39],
+    [24, 32, 40, 48, 56, 33, 41, 49, 57, 42, 50, 58, 51, 59, 60],
+    [8, 16, 17, 25, 26, 34, 35, 43, 44, 52, 53, 61, 62]]
+_LINES_SLOTS = [0, 8, 1, 9]
+
+
+def _lines_dct(n, v):
+    """The 1-D forward transforms of LINES (DCT3, DCT4, DCT6, DCT8,
+    pik/ac_strategy.cc:66-257) in float64 on the last axis of v."""
+    i = [v[..., k] for k in range(n)]
+    if n == 3:
+        return np.stack([(i[0] + i[1] + i[2]) / 3.0,
+                         (i[0] - i[2]) * 0.57735029,
+                         (i[0] + i[2]) / 3.0 - i[1] * (2 / 3.0)], -1)
+    if n == 4:
+        t0, t1, t2, t3 = i[0] + i[3], i[1] + i[2], i[0] - i[3], i[1] - i[2]
+        t6, t7 = t2 - t3, t3 * 1.414213562373095048
+        return np.stack([(t0 + t1) * 0.25, (t6 + t7) * (0.653281482438188264 / 2),
+                         (t0 - t1) * 0.25, (t6 - t7) * (0.270598050073098492 / 2)],
+                        -1)
+    if n == 6:
+        t0 = (i[1] - i[4]) * 0.23570227
+        return np.stack([
+            (i[0] + i[1] + i[2] + i[3] + i[4] + i[5]) / 6.0,
+            (i[0] - i[5]) * 0.32197529 + t0 + (i[2] - i[3]) * 0.08627302,
+            (i[0] - i[2] - i[3] + i[5]) * 0.28867514,
+            (i[0] - i[1] - i[2] + i[3] + i[4] - i[5]) * 0.23570227,
+            (i[0] + i[2] + i[3] + i[5]) / 6.0 - (i[1] + i[4]) / 3.0,
+            (i[0] - i[5]) * 0.08627302 - t0 + (i[2] - i[3]) * 0.32197529], -1)
+    c1, c2 = 0.707106781186548, 0.382683432365090
+    c3, c4 = 1.30656296487638, 0.541196100146197
+    t00, t01 = i[0] + i[7], i[0] - i[7]
+    t02, t03 = i[3] + i[4], i[3] - i[4]
+    t04, t05 = i[2] + i[5], i[2] - i[5]
+    t06, t07 = i[1] + i[6], i[1] - i[6]
+    t08, t09, t10, t11 = t00 + t02, t00 - t02, t06 + t04, t06 - t04
+    t12, t13, t14 = t07 + t05, t01 + t07, t05 + t03
+    t17, t18, t19 = c1 * (t11 + t09), c1 * t12, c2 * (t14 - t13)
+    t20, t21 = t01 + t18, t01 - t18
+    t22, t23 = c3 * t13 + t19, c4 * t14 + t19
+    k = 2.8284271247461903
+    return np.stack([(t08 + t10) * (0.353553390593273762 / k),
+                     (t20 + t22) * (0.254897789552079584 / k),
+                     (t09 + t17) * (0.270598050073098492 / k),
+                     (t21 - t23) * (0.30067244346752264 / k),
+                     (t08 - t10) * (0.353553390593273762 / k),
+                     (t21 + t23) * (0.449988111568207852 / k),
+                     (t09 - t17) * (0.653281482438188264 / k),
+                     (t20 - t22) * (1.28145772387075309 / k)], -1)
+
+
+def _fwd_lines(p):
+    """Forward LINES of (..., 8, 8) pixel blocks (float64)."""
+    flat = np.array(p, np.float64).reshape(p.shape[:-2] + (64,))
+    out = np.zeros_like(flat)
+    means = []
+    for g in _LINES_GROUPS:
+        m = flat[..., g].mean(axis=-1)
+        flat[..., g] -= m[..., None]
+        means.append(m)
+    m0, m1, m2, m3 = means
+    out[..., _LINES_SLOTS[0]] = (m0 + m1 + m2 + m3) * 0.25
+    out[..., _LINES_SLOTS[1]] = (m0 - m1 + m2 - m3) * 0.25
+    out[..., _LINES_SLOTS[2]] = (m0 + m1 - m2 - m3) * 0.25
+    out[..., _LINES_SLOTS[3]] = (m0 - m1 - m2 + m3) * 0.25
+    for pix, slots in _LINES:
+        out[..., slots] = _lines_dct(len(pix), flat[..., pix])
+    return out.reshape(p.shape)
 
 
 def covered(raw):
@@ -224,9 +312,12 @@ def _hash01(a, b, seed):
     return (h & np.uint64(0xFFFFFF)).astype(np.float64) / float(1 << 24)
 
 
-def choose_strategies(xyb_pad, seed):
+def choose_strategies(xyb_pad, seed, lines=0.0):
     """Per-block raw strategy map (by, bx) uint8 from local activity, with a
-    deterministic mix that covers every strategy."""
+    deterministic mix that covers every strategy the reference encoder
+    emits; `lines` is the fraction of the single blocks coded with LINES,
+    which the reference decodes but its encoder never chooses
+    (pik/ac_strategy.cc:18)."""
     _, h, w = xyb_pad.shape
     by, bx = h // 8, w // 8
     y = xyb_pad[1].reshape(by, 8, bx, 8)
@@ -253,6 +344,9 @@ def choose_strategies(xyb_pad, seed):
     single[(~hi) & (r >= 0.09) & (r < 0.13)] = abi.ACS_DCT2X2
     single[(~hi) & (r >= 0.13) & (r < 0.16)] = abi.ACS_IDENTITY
     single[(~hi) & (r >= 0.16) & (r < 0.22)] = abi.ACS_DCT4X4
+    if lines > 0:
+        r2 = _hash01(np.arange(by)[:, None], np.arange(bx)[None, :], seed + 77)
+        single[r2 < lines] = abi.ACS_LINES
     raw = np.where(free, single, raw)
     out = np.zeros((by, bx), np.uint8)
     for code in (abi.ACS_DCT32X32, abi.ACS_DCT16X16):
@@ -296,7 +390,7 @@ def set_quant_field(qf_float, quant_dc_float, global_scale=None):
 
 def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
                  lut_ids="zero", predict=True, grayscale=False,
-                 global_scale=None):
+                 global_scale=None, lines=0.0):
     """Encodes an (H, W, 3) uint8 sRGB image into a Frame. The quantizer
     scalars come from a float quant field as the reference encoder derives
     them (set_quant_field), so global_scale is in general no power of two;
@@ -317,7 +411,7 @@ def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
     xyb = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
                  mode="edge")
     table, offsets = tables.default_tables()
-    acs = choose_strategies(xyb, seed)
+    acs = choose_strategies(xyb, seed, lines)
     raw = (acs >> 4).astype(np.int32)
     first = (acs & 15) == 0
 
@@ -343,7 +437,7 @@ def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
     coef = np.zeros((3, by, bx * 64), np.float64)
     for code in (abi.ACS_DCT, abi.ACS_IDENTITY, abi.ACS_DCT2X2,
                  abi.ACS_DCT4X4, abi.ACS_DCT16X16, abi.ACS_DCT32X32,
-                 abi.ACS_DCT_NOHF, abi.ACS_DCT4X4_NOHF):
+                 abi.ACS_DCT_NOHF, abi.ACS_DCT4X4_NOHF, abi.ACS_LINES):
         s = covered(code)
         ys, xs = np.nonzero(first & (raw == code))
         if len(ys) == 0:
@@ -481,6 +575,60 @@ def make_opsin_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
     xyb = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
                  mode="edge")
     return frame, np.ascontiguousarray(xyb, np.float32)
+
+
+# The encoder's pre-sharpening before quantization when Gaborish is on
+# (GaborishInverse, pik/gaborish.cc:92-119; applied at pik/pik_frame.cc:
+# 527-529): a 5x5 symmetric kernel, normalised to unit sum, with clamped
+# borders. Quadrant weights [|dy|][|dx|] before scaling by 0.92718927264540152
+# (the centre stays 1).
+_GAB_INV = (-0.092359145662814029, -0.039253623634014627, 0.016176494530216929,
+            0.00083458437774987476, 0.004512465323949319)
+_GAB_INV_MUL = 0.92718927264540152
+
+
+def gaborish_inverse(xyb):
+    """GaborishInverse of a (3, H, W) image (float32 weights, float64
+    accumulation)."""
+    g = [np.float32(_GAB_INV_MUL * v) for v in _GAB_INV]
+    quad = np.array([[1.0, g[0], g[2]],
+                     [g[0], g[1], g[3]],
+                     [g[2], g[3], g[4]]], np.float32)
+    w = np.zeros((5, 5), np.float64)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            w[dy + 2, dx + 2] = quad[abs(dy), abs(dx)]
+    w = (w * (1.0 / w.sum())).astype(np.float32).astype(np.float64)
+    img = np.asarray(xyb, np.float64)
+    _, h, wd = img.shape
+    pad = np.pad(img, ((0, 0), (2, 2), (2, 2)), mode="edge")
+    out = np.zeros_like(img)
+    for dy in range(5):
+        for dx in range(5):
+            out += w[dy, dx] * pad[:, dy:dy + h, dx:dx + wd]
+    return out.astype(np.float32)
+
+
+def make_search_inputs(xsize, ysize, seed=0, num_rects=20, line_art=False,
+                       target=1.0):
+    """Inputs of a quant-field search at Butteraugli distance `target`: the
+    frame's side information (no LF / HF predictions, no dots; AC strategy,
+    colour correlation and sigma LUTs from this generator), the padded XYB
+    image the encoder quantizes (pre-sharpened by GaborishInverse as the
+    reference encoder does with Gaborish on), the unpadded XYB original and
+    its linear RGB. Returns (frame, opsin, opsin_orig, linear)."""
+    rgb = synthetic_srgb(xsize, ysize, seed, num_rects=num_rects,
+                         line_art=line_art)
+    frame = encode_frame(rgb, distance=target, seed=seed, predict=False)
+    linear = srgb8_to_linear255(rgb).astype(np.float32)
+    xyb = linear_to_xyb(linear).astype(np.float32)
+    bx, by = div_ceil(xsize, 8), div_ceil(ysize, 8)
+    padded = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
+                    mode="edge")
+    opsin = gaborish_inverse(padded) if frame.gaborish else padded
+    return (frame, np.ascontiguousarray(opsin, np.float32),
+            np.ascontiguousarray(xyb, np.float32),
+            np.ascontiguousarray(linear, np.float32))
 
 
 def make_extra_passes(frame, num_passes, seed=0, density=0.02, amplitude=1):

@@ -70,6 +70,16 @@ int main(void) {
   F(pikgpu_frame, num_dequant_offsets) F(pikgpu_frame, dots)
   F(pikgpu_frame, num_dots) F(pikgpu_frame, grayscale) F(pikgpu_frame, flags)
   F(pikgpu_dot, transform) F(pikgpu_dot, width) F(pikgpu_dot, pixels)
+  printf("pikgpu_adj_io size %zu\n", sizeof(pikgpu_adj_io));
+  F(pikgpu_adj_io, qf) F(pikgpu_adj_io, target) F(pikgpu_adj_io, in_stride)
+  F(pikgpu_adj_io, dl_dqf) F(pikgpu_adj_io, dl_dcoeffs) F(pikgpu_adj_io, loss)
+  printf("pikgpu_search_params size %zu\n", sizeof(pikgpu_search_params));
+  F(pikgpu_search_params, learning_rate) F(pikgpu_search_params, qf_max)
+  printf("pikgpu_search_io size %zu\n", sizeof(pikgpu_search_io));
+  F(pikgpu_search_io, target) F(pikgpu_search_io, weight)
+  F(pikgpu_search_io, quant_field) F(pikgpu_search_io, ac)
+  F(pikgpu_search_io, dc_stride) F(pikgpu_search_io, loss)
+  F(pikgpu_search_io, lambda_used)
   return 0;
 }
 """
@@ -84,7 +94,9 @@ def test_ctypes_layout_matches_header(tmp_path):
     out = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE,
                          text=True).stdout
     types = {"pikgpu_frame": abi.FrameC, "pikgpu_dot": abi.Dot,
-             "pikgpu_config": abi.ConfigC}
+             "pikgpu_config": abi.ConfigC, "pikgpu_adj_io": abi.AdjIoC,
+             "pikgpu_search_params": abi.SearchParamsC,
+             "pikgpu_search_io": abi.SearchIoC}
     for line in out.splitlines():
         t, m, v = line.split()
         if m == "size":
@@ -149,7 +161,9 @@ def test_rejects_inconsistent_strategies(frame):
     acs[0, 0] = 0x9 << 4
     _expect(frame, abi.EINVAL, "invalid AC strategy", ac_strategy=acs)
     acs[0, 0] = abi.ACS_LINES << 4
-    _expect(frame, abi.EUNSUPPORTED, "LINES", ac_strategy=acs)
+    fr = frame.copy()
+    fr.ac_strategy = acs.copy()
+    validate(fr)  # LINES decodes (pik/ac_strategy.cc:680-724)
     acs[0, 0] = (abi.ACS_DCT << 4) | 1
     _expect(frame, abi.EINVAL, "block index", ac_strategy=acs)
     # a DCT16 missing its other three blocks

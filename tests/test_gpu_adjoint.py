@@ -196,3 +196,18 @@ def test_argument_errors(rec):
     assert lib.pikgpu_fwd_adj(rec._ctx, ctypes.byref(io)) == abi.EINVAL
     io.in_stride = fr.xsize
     assert lib.pikgpu_fwd_adj(rec._ctx, ctypes.byref(io)) == abi.OK
+
+
+def test_lines_frames_are_refused(rec):
+    # the model has no adjoint of the LINES transform: such a frame is
+    # refused before any launch, and the context stays usable
+    fr = _model_frame(synth.make_frame(64, 64, seed=9, lines=0.5))
+    rec.upload(fr)
+    qf = fr.quant_field.astype(np.float32)
+    tgt = np.zeros((3, fr.ysize, fr.xsize), np.float32)
+    with pytest.raises(PikGpuError) as e:
+        rec.fwd_adj(qf, target=tgt)
+    assert e.value.code == abi.EUNSUPPORTED
+    ok = _model_frame(synth.make_frame(64, 64, seed=9))
+    rec.upload(ok)
+    rec.fwd_adj(ok.quant_field.astype(np.float32), target=tgt)

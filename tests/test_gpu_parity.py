@@ -79,9 +79,15 @@ def _cases():
     for gs in (1, 3, 1000, 12345, 32767, 32768):
         c["gs%d" % gs] = (lambda gs=gs: synth.make_frame(
             136, 104, seed=90 + gs % 7, num_dots=2, global_scale=gs))
+    # LINES (decoded by the reference, never chosen by its encoder)
+    c["lines_mixed"] = lambda: synth.make_frame(264, 200, seed=96,
+                                                num_dots=3, lines=0.3,
+                                                lut_ids="rand")
+    c["lines_nopredict"] = lambda: synth.make_frame(136, 72, seed=97,
+                                                    lines=0.5, predict=False)
     for raw in (abi.ACS_DCT, abi.ACS_IDENTITY, abi.ACS_DCT2X2,
                 abi.ACS_DCT4X4, abi.ACS_DCT16X16, abi.ACS_DCT32X32,
-                abi.ACS_DCT_NOHF, abi.ACS_DCT4X4_NOHF):
+                abi.ACS_DCT_NOHF, abi.ACS_DCT4X4_NOHF, abi.ACS_LINES):
         c["all_raw%d" % raw] = (lambda raw=raw: forced(synth.make_frame(
             128, 96, seed=30 + raw, lut_ids="rand"), raw))
     return c

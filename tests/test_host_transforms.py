@@ -24,6 +24,10 @@ HOST_LIB = os.path.join(ROOT, "pik_amd", "libpikgpu_host.so")
 STRATS = [abi.ACS_DCT, abi.ACS_IDENTITY, abi.ACS_DCT2X2, abi.ACS_DCT4X4,
           abi.ACS_DCT16X16, abi.ACS_DCT32X32, abi.ACS_DCT_NOHF,
           abi.ACS_DCT4X4_NOHF]
+# strategies whose forward transform the inverse undoes exactly (LINES
+# interpolates four pixels instead)
+INVERTIBLE = STRATS
+ALL = STRATS + [abi.ACS_LINES]
 
 
 @pytest.fixture(scope="module")
@@ -93,7 +97,7 @@ def test_dct8_dc_is_block_mean(host):
     assert abs(coef[0, 0] - pix.mean()) < 1e-6
 
 
-@pytest.mark.parametrize("raw", STRATS)
+@pytest.mark.parametrize("raw", INVERTIBLE)
 def test_round_trip(host, raw):
     n = size_of(raw)
     rng = np.random.default_rng(100 + raw)
@@ -101,6 +105,34 @@ def test_round_trip(host, raw):
         pix = rng.uniform(-2, 2, (n, n)).astype(np.float32)
         back = host_tf(host, raw, 0, host_tf(host, raw, 1, pix))
         assert np.abs(back - pix).max() < 2e-5
+
+
+# LINES' four pixels on no diagonal line (row-major index) and the
+# neighbours above and below them that they are interpolated from
+# (pik/ac_strategy.cc:711-717).
+LINES_GAPS = {21: (12, 30), 28: (19, 37), 35: (26, 44), 42: (33, 51)}
+
+
+def test_lines_round_trip_and_gaps(host):
+    rng = np.random.default_rng(7)
+    on_lines = np.ones(64, bool)
+    on_lines[list(LINES_GAPS)] = False
+    for _ in range(20):
+        pix = rng.uniform(-2, 2, (8, 8)).astype(np.float32)
+        back = host_tf(host, abi.ACS_LINES, 0,
+                       host_tf(host, abi.ACS_LINES, 1, pix)).reshape(64)
+        assert np.abs(back[on_lines] - pix.reshape(64)[on_lines]).max() < 2e-5
+        for g, (a, b) in LINES_GAPS.items():
+            assert back[g] == np.float32((back[a] + back[b]) * np.float32(0.5))
+
+
+def test_lines_constant_block(host):
+    # a flat block: every line is constant, so only the slot-0 mean of the
+    # four groups' Hadamard survives
+    pix = np.full((8, 8), 3.5, np.float32)
+    coef = host_tf(host, abi.ACS_LINES, 1, pix).reshape(64)
+    assert coef[0] == np.float32(3.5)
+    assert np.abs(np.delete(coef, 0)).max() < 1e-6
 
 
 def test_dct4x4_is_four_dct4_with_hadamard_dc(host):
@@ -155,7 +187,7 @@ def _ref():
     return ref
 
 
-@pytest.mark.parametrize("raw", STRATS)
+@pytest.mark.parametrize("raw", ALL)
 @pytest.mark.parametrize("direction", [0, 1])
 def test_bit_exact_against_reference(host, raw, direction):
     ref = _ref()

@@ -65,10 +65,16 @@ CASES = {
     "sharpened": lambda: _with(synth.make_frame(96, 64, seed=26),
                                epf_use_sharpened=1),
     "d3": lambda: synth.make_frame(128, 96, seed=27, distance=3.0),
+    # LINES blocks among the others (the reference decodes them; its encoder
+    # never chooses them), with and without predictions
+    "lines_mixed": lambda: synth.make_frame(136, 104, seed=28, num_dots=2,
+                                            lines=0.3, lut_ids="rand"),
+    "lines_nopredict": lambda: synth.make_frame(96, 72, seed=29, lines=0.4,
+                                                predict=False),
 }
 for _raw in (abi.ACS_IDENTITY, abi.ACS_DCT2X2, abi.ACS_DCT4X4,
              abi.ACS_DCT16X16, abi.ACS_DCT32X32, abi.ACS_DCT_NOHF,
-             abi.ACS_DCT4X4_NOHF):
+             abi.ACS_DCT4X4_NOHF, abi.ACS_LINES):
     CASES["all_raw%d" % _raw] = (lambda raw=_raw: forced(synth.make_frame(
         64, 64, seed=40 + raw, lut_ids="rand"), raw))
 
@@ -108,14 +114,19 @@ def test_oracle_transforms_closed_form(raw):
 
 @pytest.mark.parametrize("raw", [abi.ACS_DCT, abi.ACS_IDENTITY,
                                  abi.ACS_DCT2X2, abi.ACS_DCT4X4,
-                                 abi.ACS_DCT16X16, abi.ACS_DCT32X32])
+                                 abi.ACS_DCT16X16, abi.ACS_DCT32X32,
+                                 abi.ACS_LINES])
 def test_oracle_transform_round_trip_and_reference(raw):
     port = _port()
     n = size_of(raw)
     rng = np.random.default_rng(70 + raw)
     pix = rng.uniform(-1, 1, (n, n)).astype(np.float32)
     back = port.transform(raw, 0, port.transform(raw, 1, pix))
-    assert np.abs(back - pix).max() < 2e-5
+    keep = np.ones((n, n), bool)
+    if raw == abi.ACS_LINES:
+        # the four pixels on no line are interpolated, not reproduced
+        keep.reshape(-1)[[21, 28, 35, 42]] = False
+    assert np.abs(back - pix)[keep].max() < 2e-5
     from oracle import ref
     if ref.available("scalar"):
         for d in (0, 1):
