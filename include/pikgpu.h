@@ -482,6 +482,33 @@ PIKGPU_API int pikgpu_search_quant_field(pikgpu_ctx* ctx,
                                          const pikgpu_search_params* params,
                                          const pikgpu_search_io* io);
 
+/* ------------------------------------------------------------------------
+ * Reconstruction without device allocation. No counterpart in the
+ * reference (its decoder allocates its images per frame); this entry point
+ * is for callers that cannot allocate device memory: code running while a
+ * HIP graph is captured around it, tight serving loops, harnesses that
+ * forbid hipMalloc. It does what pikgpu_reconstruct does (upload + run +
+ * download of the linear Image3F, same bits) without a context: every
+ * device buffer of the frame lives in one fixed array of
+ * PIKGPU_NOALLOC_SCRATCH_BYTES per device, which the library holds from its
+ * first use on a device until the process ends, and the call makes no
+ * hipMalloc / hipFree. A frame that needs more scratch than that (about
+ * 2.7 megapixels; 1920x1080 fits) is refused with PIKGPU_EINVAL before the
+ * device is touched.
+ *
+ * Runs on the current device, on `stream` (a hipStream_t; NULL: the null
+ * stream), and returns when the output is in out_planes (host memory, rows
+ * of out_stride_bytes). Calls on one device share its array, which keeps
+ * the last call's contents: calls that overlap on one device race, so the
+ * caller serialises them. `msg` (may be NULL) receives the reason of a
+ * failure, truncated to msg_size. */
+#define PIKGPU_NOALLOC_SCRATCH_BYTES (128u << 20)
+PIKGPU_API int pikgpu_reconstruct_noalloc(const pikgpu_frame* frame,
+                                          float* out_planes[3],
+                                          size_t out_stride_bytes,
+                                          void* stream, char* msg,
+                                          size_t msg_size);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

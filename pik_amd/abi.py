@@ -193,6 +193,9 @@ def load():
                                         c_float_p, ctypes.c_int]
     lib.pikgpu_stream.argtypes = [vp]
     lib.pikgpu_stream.restype = vp
+    lib.pikgpu_reconstruct_noalloc.argtypes = [
+        ctypes.POINTER(FrameC), c_float_p * 3, ctypes.c_size_t, vp,
+        ctypes.c_char_p, ctypes.c_size_t]
     lib.pikgpu_search_quant_field.argtypes = [
         vp, ctypes.POINTER(SearchParamsC), ctypes.POINTER(SearchIoC)]
     lib.pikgpu_batch_create.restype = vp
@@ -236,5 +239,6 @@ EXPORTED_SYMBOLS = [
     "pikgpu_batch_last_error", "pikgpu_batch_upload", "pikgpu_batch_run",
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
-    "pikgpu_frame_add_pass", "pikgpu_search_quant_field",
+    "pikgpu_frame_add_pass", "pikgpu_reconstruct_noalloc",
+    "pikgpu_search_quant_field",
 ]

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIKGPU_ARCH", "gfx950")
 
 SOURCES = ["recon_lf.hip", "recon_blocks.hip", "filter.hip", "adjoint.hip",
-           "encode.hip", "search.hip", "api.hip",
+           "encode.hip", "search.hip", "api.hip", "noalloc.hip",
            "host_tables.cpp", "quant_tables.cpp"]
 HEADERS = ["common.h", "device.h", "kernels.h", "xform.h", "dct1d.h",
            "dct_consts.h", "host_api.h"]
