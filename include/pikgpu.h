@@ -454,6 +454,12 @@ typedef struct pikgpu_search_params {
                             times -lambda (0: that mean itself), i.e. the
                             distortion level of the start is kept */
   int32_t qf_max;        /* largest integer quant value, <= 256 */
+  float trust_low;       /* each transform's field stays within
+                            [trust_low, trust_high] times its initial value
+                            (0 < trust_low <= 1 <= trust_high), as the
+                            reference search bounds its field by the
+                            initial one's range */
+  float trust_high;
 } pikgpu_search_params;
 
 typedef struct pikgpu_search_io {

@@ -137,6 +137,8 @@ class SearchParamsC(ctypes.Structure):
         ("eps", ctypes.c_float),
         ("lambda_", ctypes.c_float),
         ("qf_max", ctypes.c_int32),
+        ("trust_low", ctypes.c_float),
+        ("trust_high", ctypes.c_float),
     ]
 
 

@@ -1582,7 +1582,9 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   if (prm->steps < 0 || prm->steps > 1000 || !(prm->learning_rate > 0.0f) ||
       !(prm->beta1 >= 0.0f && prm->beta1 < 1.0f) ||
       !(prm->beta2 >= 0.0f && prm->beta2 < 1.0f) || !(prm->eps > 0.0f) ||
-      prm->qf_max < 1 || prm->qf_max > 256 || !std::isfinite(prm->lambda)) {
+      prm->qf_max < 1 || prm->qf_max > 256 || !std::isfinite(prm->lambda) ||
+      !(prm->trust_low > 0.0f && prm->trust_low <= 1.0f) ||
+      !(prm->trust_high >= 1.0f && std::isfinite(prm->trust_high))) {
     return Fail(ctx, PIKGPU_EINVAL, "invalid search parameters");
   }
   if (!io->opsin[0] || !io->opsin[1] || !io->opsin[2] || !io->target[0] ||
@@ -1622,9 +1624,9 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   const int steps = prm->steps;
   const int round_groups = int(DivCeil(nb, 256));
   const int tiles = int(DivCeil(bx, 4) * DivCeil(by, 4));
-  // search state: u, m, v, weight, dist (nb floats each), lambda, rate
+  // search state: u, m, v, weight, dist, u0 (nb floats each), lambda, rate
   // partials (round_groups floats), loss / rate histories (steps + 1 doubles)
-  const size_t hist_off = DivCeil((5 * nb + 1 + size_t(round_groups)) * 4, 8) * 8;
+  const size_t hist_off = DivCeil((6 * nb + 1 + size_t(round_groups)) * 4, 8) * 8;
   int rc;
   if ((rc = Grow(ctx, &ctx->eopsin, 3 * plane * 4)) ||
       (rc = Grow(ctx, &ctx->ecoef, 3 * nb * 64 * 4)) ||
@@ -1652,7 +1654,8 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   float* st_v = st_m + nb;
   float* st_w = st_v + nb;
   float* st_d = st_w + nb;
-  float* st_lam = st_d + nb;
+  float* st_u0 = st_d + nb;
+  float* st_lam = st_u0 + nb;
   float* st_rate = st_lam + 1;
   double* h_loss = reinterpret_cast<double*>(
       static_cast<char*>(ctx->srch.p) + hist_off);
@@ -1688,7 +1691,8 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   EnqueueDc(P, B, K, st);
   EnqueueLf(P, B, K, st, false);  // the LLF (CopyLlf) and the LF caches
   ctx->adj_ready = true;
-  KSearchInit<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, st_u, st_m, st_v);
+  KSearchInit<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, st_u0, st_u, st_m,
+                                                  st_v);
 
   AdjArgs A;
   memset(&A, 0, sizeof(A));
@@ -1712,6 +1716,8 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   S.eps = prm->eps;
   S.ulo = 0.0f;
   S.uhi = std::log(float(qmax));
+  S.dlo = std::log(prm->trust_low);
+  S.dhi = std::log(prm->trust_high);
   const float* itable = static_cast<const float*>(ctx->itable.p);
   const int grid = XcdGrid(tiles);
   for (int t = 0; t <= steps; ++t) {
@@ -1731,7 +1737,7 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
           prm->lambda < 0.0f ? -prm->lambda : 1.0f, st_lam);
     }
     KSearchUpdate<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, S, t + 1, st_d,
-                                                    st_u, st_m, st_v);
+                                                    st_u0, st_u, st_m, st_v);
   }
   HIP_TRY(ctx, hipGetLastError());
   // outputs

@@ -651,16 +651,17 @@ struct SearchArgs {
   const float* lambda;  // rate weight per block (device scalar)
   float lr, beta1, beta2, eps;
   float ulo, uhi;       // bounds of the log field
+  float dlo, dhi;       // trust region around the initial log field
 };
 __global__ void KSearchRound(FrameParams P, FrameBuffers B, const float* u,
                              int qmax, int32_t* qf, float* qfr, float* part);
 __global__ void KSearchDist(FrameParams P, FrameBuffers B, const float* coef,
                             const float* grad, float* dist);
 __global__ void KSearchUpdate(FrameParams P, FrameBuffers B, SearchArgs S,
-                              int t, const float* dist, float* u, float* m,
-                              float* v);
-__global__ void KSearchInit(FrameParams P, FrameBuffers B, float* u, float* m,
-                            float* v);
+                              int t, const float* dist, const float* u0,
+                              float* u, float* m, float* v);
+__global__ void KSearchInit(FrameParams P, FrameBuffers B, float* u0, float* u,
+                            float* m, float* v);
 __global__ void KSearchLambda(const float* dist, const float* weight, int n,
                               float fixed, float kappa, float* out);
 

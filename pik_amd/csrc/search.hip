@@ -83,10 +83,12 @@ __global__ void __launch_bounds__(256)
 
 // One Adam step (step number t >= 1) on each transform's log field, at its
 // first block: gradient sum over the covered blocks of -2 w D + lambda;
-// u clamped to [ulo, uhi].
+// u clamped to [ulo, uhi] and to the trust region [u0 + dlo, u0 + dhi]
+// around its initial value u0.
 __global__ void __launch_bounds__(256)
     KSearchUpdate(FrameParams P, FrameBuffers B, SearchArgs S, int t,
-                  const float* dist, float* u, float* m, float* v) {
+                  const float* dist, const float* u0, float* u, float* m,
+                  float* v) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.bx * P.by) return;
   const uint8_t a = B.acs[i];
@@ -108,7 +110,9 @@ __global__ void __launch_bounds__(256)
   const float mh = mi / (1.0f - powf(S.beta1, float(t)));
   const float vh = vi / (1.0f - powf(S.beta2, float(t)));
   const float ui = u[i] - S.lr * mh / (sqrtf(vh) + S.eps);
-  u[i] = fminf(fmaxf(ui, S.ulo), S.uhi);
+  const float lo = fmaxf(S.ulo, u0[i] + S.dlo);
+  const float hi = fminf(S.uhi, u0[i] + S.dhi);
+  u[i] = fminf(fmaxf(ui, lo), hi);
 }
 
 // The rate weight: `fixed` when positive, else kappa times the mean over
@@ -132,12 +136,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Initial state: u = log of the frame's integer field, Adam moments zero.
+// Initial state: u = u0 = log of the frame's integer field, Adam moments
+// zero.
 __global__ void __launch_bounds__(256)
-    KSearchInit(FrameParams P, FrameBuffers B, float* u, float* m, float* v) {
+    KSearchInit(FrameParams P, FrameBuffers B, float* u0, float* u, float* m,
+                float* v) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.bx * P.by) return;
   u[i] = __logf(float(B.qf[i]));
+  u0[i] = u[i];
   m[i] = 0.0f;
   v[i] = 0.0f;
 }

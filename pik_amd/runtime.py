@@ -143,11 +143,12 @@ class Reconstructor:
 
     def search_quant_field(self, opsin, target, steps=20, learning_rate=0.05,
                            beta1=0.9, beta2=0.999, eps=1e-8, lam=0.0,
-                           qf_max=256, weight=None):
+                           qf_max=256, weight=None, trust=(0.0625, 16.0)):
         """Gradient quant-field search on the uploaded frame
         (pikgpu_search_quant_field). opsin: the padded XYB image the encoder
         quantizes (3, by * 8, bx * 8); target: the linear RGB original
-        (3, ysize, xsize); weight: optional per-block weights (by, bx).
+        (3, ysize, xsize); weight: optional per-block weights (by, bx);
+        trust: the field's bounds relative to its initial value.
         lam > 0 is the rate weight; lam <= 0 keeps the initial distortion
         level times -lam (0: times 1). The context then holds the frame
         quantized with the final field (run() decodes it). Returns a dict:
@@ -163,7 +164,8 @@ class Reconstructor:
             raise ValueError("target must be (3, %d, %d)" % (f.ysize, f.xsize))
         vp = ctypes.c_void_p
         prm = abi.SearchParamsC(int(steps), learning_rate, beta1, beta2, eps,
-                                lam, int(qf_max))
+                                lam, int(qf_max), float(trust[0]),
+                                float(trust[1]))
         io = abi.SearchIoC()
         io.opsin = (vp * 3)(*[img[c].ctypes.data for c in range(3)])
         io.opsin_stride = bx * 8

@@ -616,12 +616,13 @@ def make_search_inputs(xsize, ysize, seed=0, num_rects=20, line_art=False,
     colour correlation and sigma LUTs from this generator), the padded XYB
     image the encoder quantizes (pre-sharpened by GaborishInverse as the
     reference encoder does with Gaborish on), the unpadded XYB original and
-    its linear RGB. Returns (frame, opsin, opsin_orig, linear)."""
+    its linear RGB (3, H, W). Returns (frame, opsin, opsin_orig, linear)."""
     rgb = synthetic_srgb(xsize, ysize, seed, num_rects=num_rects,
                          line_art=line_art)
     frame = encode_frame(rgb, distance=target, seed=seed, predict=False)
-    linear = srgb8_to_linear255(rgb).astype(np.float32)
-    xyb = linear_to_xyb(linear).astype(np.float32)
+    lin = srgb8_to_linear255(rgb)
+    xyb = linear_to_xyb(lin).astype(np.float32)
+    linear = np.ascontiguousarray(lin.transpose(2, 0, 1), np.float32)
     bx, by = div_ceil(xsize, 8), div_ceil(ysize, 8)
     padded = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
                     mode="edge")

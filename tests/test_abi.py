@@ -75,6 +75,7 @@ int main(void) {
   F(pikgpu_adj_io, dl_dqf) F(pikgpu_adj_io, dl_dcoeffs) F(pikgpu_adj_io, loss)
   printf("pikgpu_search_params size %zu\n", sizeof(pikgpu_search_params));
   F(pikgpu_search_params, learning_rate) F(pikgpu_search_params, qf_max)
+  F(pikgpu_search_params, trust_high)
   printf("pikgpu_search_io size %zu\n", sizeof(pikgpu_search_io));
   F(pikgpu_search_io, target) F(pikgpu_search_io, weight)
   F(pikgpu_search_io, quant_field) F(pikgpu_search_io, ac)

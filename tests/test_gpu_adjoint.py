@@ -211,3 +211,43 @@ def test_lines_frames_are_refused(rec):
     ok = _model_frame(synth.make_frame(64, 64, seed=9))
     rec.upload(ok)
     rec.fwd_adj(ok.quant_field.astype(np.float32), target=tgt)
+
+
+def test_full_size_2048_with_256_field(rec):
+    """The forward + adjoint workload at its stated size: a 2048x2048 frame
+    (256x256 quant field). The forward with the integer field is the
+    decoder's output bit for bit at full size, and central finite
+    differences of the GPU's own loss (relative step 1e-3) agree with
+    dL/dqf on sampled blocks spread over the frame."""
+    print("fwd+adj 2048x2048", flush=True)
+    fr = synth.make_frame(2048, 2048, seed=19, distance=1.0, num_rects=100,
+                          num_dots=8)
+    assert fr.quant_field.shape == (256, 256)
+    target = rec.reconstruct(fr)  # the full decode (HF prediction, AR, dots)
+    want = rec.reconstruct(_model_frame(fr))
+    rec.upload(fr)
+    out = rec.fwd_adj(fr.quant_field.astype(np.float32), target=target,
+                      want_linear=True)
+    assert np.array_equal(out["linear"].view(np.uint32), want.view(np.uint32))
+    assert out["loss"] > 0
+    rng = np.random.default_rng(19)
+    qf = (fr.quant_field * rng.uniform(0.8, 1.25, fr.quant_field.shape)
+          ).astype(np.float32)
+    g = rec.fwd_adj(qf, target=target)["dl_dqf"]
+    scale = np.abs(g).max()
+    assert np.isfinite(g).all() and scale > 0
+    picks = [(0, 0), (255, 255), (0, 255), (128, 64)]
+    picks += [tuple(int(v) for v in rng.integers(0, 256, 2)) for _ in range(8)]
+    # and the blocks with the largest gradients
+    big = np.argsort(np.abs(g).ravel())[-4:]
+    picks += [divmod(int(i), 256) for i in big]
+    for (y, x) in picks:
+        h = np.float32(1e-3) * qf[y, x]
+        qp, qm = qf.copy(), qf.copy()
+        qp[y, x] += h
+        qm[y, x] -= h
+        lp = rec.fwd_adj(qp, target=target)["loss"]
+        lm = rec.fwd_adj(qm, target=target)["loss"]
+        fd = (lp - lm) / float(qp[y, x] - qm[y, x])
+        assert abs(fd - g[y, x]) <= 2e-2 * abs(g[y, x]) + 2e-3 * scale, \
+            ((y, x), fd, g[y, x])

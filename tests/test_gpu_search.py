@@ -1,0 +1,115 @@
+"""pikgpu_search_quant_field on the GPU.
+
+  * Every quantization inside the search is the reference encoder's: the
+    final integer field's AC / DC equal the reference's quantization of the
+    same image with that field, bit for bit, and the context's decode of
+    them equals the reference decoder's.
+  * The search improves the image it encodes: starting from the reference
+    encoder's heuristic field (InitialQuantField), the squared linear-RGB
+    error falls and the reference Butteraugli distance (the judge,
+    pik/butteraugli_comparator.cc) of the decoded result drops below the
+    initial field's.
+  * Several frames searched concurrently (one context and stream each) give
+    each frame's single-search result.
+  * Frames the search does not take and bad parameters are refused.
+"""
+import numpy as np
+import pytest
+
+from pik_amd import abi, synth
+from pik_amd.runtime import PikGpuError, Reconstructor
+from pik_amd.search import apply_result, search_batch, search_frame
+
+from .helpers import bit_equal_fraction
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref():
+    from oracle import ref
+    if not ref.available("scalar"):
+        pytest.skip("reference harness not built (oracle/_ref)")
+    return ref
+
+
+@pytest.fixture(scope="module")
+def rec():
+    with Reconstructor(0) as r:
+        yield r
+
+
+def test_search_quantization_and_decode_are_the_references(rec):
+    ref = _ref()
+    print("search 600x520", flush=True)
+    fr, opsin, orig, lin = synth.make_search_inputs(600, 520, seed=3)
+    res = search_frame(rec, fr, opsin, lin, steps=8)
+    assert np.isfinite(res["loss"]).all() and np.isfinite(res["rate"]).all()
+    final = apply_result(fr, res)
+    ac, dc = ref.quantize(final, opsin)
+    assert np.array_equal(ac, res["ac"]) and np.array_equal(dc, res["dc"])
+    rec.run()
+    got = rec.download()
+    want, _ = ref.recon(final, threads=8, stages=False)
+    assert bit_equal_fraction(got, want) == 1.0
+    # the field moved, within the trust region around the initial one
+    q0, q1 = fr.quant_field.astype(np.float64), res["quant_field"]
+    assert (q1 != q0).mean() > 0.1
+    assert (q1 >= np.floor(0.9 * q0) - 1).all()
+    assert (q1 <= np.ceil(1.6 * q0) + 1).all()
+
+
+def test_search_lowers_error_and_butteraugli_distance(rec):
+    ref = _ref()
+    print("search 512x512 against the reference Butteraugli", flush=True)
+    fr, opsin, orig, lin = synth.make_search_inputs(512, 512, seed=1,
+                                                    num_rects=40)
+    # the start: the reference encoder's heuristic field
+    # (InitialQuantField) through SetQuantField's rule
+    init = ref.initial_quant_field(orig, 1.0)
+    fr.global_scale, fr.quant_dc, fr.quant_field = synth.set_quant_field(
+        init, ref.initial_quant_dc(1.0))
+    ac, dc = ref.quantize(fr, opsin)
+    f0 = fr.copy()
+    f0.ac, f0.dc = ac, dc
+    d0 = ref.butteraugli(orig, rec.reconstruct(f0))
+    res = search_frame(rec, fr, opsin, lin)
+    assert res["loss"][-1] < res["loss"][0]
+    rec.run()
+    d1 = ref.butteraugli(orig, rec.download())
+    print("butteraugli: initial field %.3f, searched %.3f; loss %.4g -> %.4g"
+          % (d0, d1, res["loss"][0], res["loss"][-1]), flush=True)
+    assert d1 < d0
+
+
+def test_concurrent_searches_match_single_ones(rec):
+    print("search batch of 4", flush=True)
+    items = []
+    for i, (w, h) in enumerate([(256, 192), (320, 128), (200, 200),
+                                (136, 264)]):
+        fr, opsin, _, lin = synth.make_search_inputs(w, h, seed=20 + i)
+        items.append((fr, opsin, lin))
+    got = search_batch(items, threads=4, steps=6)
+    for (fr, opsin, lin), res in zip(items, got):
+        one = search_frame(rec, fr, opsin, lin, steps=6)
+        assert np.array_equal(one["quant_field"], res["quant_field"])
+        assert np.array_equal(one["ac"], res["ac"])
+        assert np.array_equal(one["loss"], res["loss"])
+
+
+def test_refusals(rec):
+    fr, opsin, orig, lin = synth.make_search_inputs(64, 64, seed=4)
+    bad = fr.copy()
+    bad.predict_lf = bad.predict_hf = 1
+    rec.upload(bad)
+    with pytest.raises(PikGpuError) as e:
+        rec.search_quant_field(opsin, lin)
+    assert e.value.code == abi.EUNSUPPORTED
+    rec.upload(fr)
+    for kw in (dict(learning_rate=0.0), dict(qf_max=300),
+               dict(trust=(1.5, 2.0)), dict(trust=(0.5, 0.9)),
+               dict(beta1=1.0), dict(steps=-1)):
+        with pytest.raises(PikGpuError) as e:
+            rec.search_quant_field(opsin, lin, **kw)
+        assert e.value.code == abi.EINVAL, kw
+    res = rec.search_quant_field(opsin, lin, steps=0)
+    assert np.array_equal(res["quant_field"], fr.quant_field)
