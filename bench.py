@@ -27,6 +27,19 @@ Two further figures ride on the same line:
            AC quantization; MP/s of the kernels (HIP events), and the rate
            with the host-to-device copy of the float image and the int16
            downloads included.
+  search   gradient quant-field search (pikgpu_search_quant_field) of 64
+           1024x1024 images at Butteraugli distance 1.0, 20 Adam steps each
+           from the reference encoder's initial field (committed fixture),
+           one context and stream per image, wall time for all 64; beside
+           it, on a sample of the same images, the reference's own search
+           loop (its round-trip pieces, oracle/_ref) timed on the host's
+           cores, and both results judged by the reference Butteraugli and
+           sized by the reference's entropy coder.
+
+The reconstruction figure also carries `with_copies_MPps` (the drop-in
+call: frame upload from pinned host memory, run, linear download into
+pinned memory) and, in the CPU-baseline leg, the deviation of the GPU output
+from the reference's default AVX2 build (whose EPF divides with rcpps).
 """
 import argparse
 import json
@@ -91,6 +104,12 @@ def parse_args():
                     help="frame size of the fwd_adj figure (0: skip)")
     ap.add_argument("--quant-size", type=int, default=4096,
                     help="frame size of the quantize figure (0: skip)")
+    ap.add_argument("--search-images", type=int, default=64,
+                    help="1024x1024 images of the search figure (0: skip)")
+    ap.add_argument("--search-distinct", type=int, default=16,
+                    help="distinct search images cycled (<= 16)")
+    ap.add_argument("--search-sample", type=int, default=2,
+                    help="images the reference's search runs on")
     return ap.parse_args()
 
 
@@ -119,31 +138,52 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def valu_busy_from_profiles(kernel):
-    """VALU busy fraction of `kernel` on this workload, as recorded in
-    profiles/sq_counters_recon_4096.md (rocprofv3 SQ counters); None when
-    no such record exists."""
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second
+
+
+def valu_roofline(kernel, avg_ms):
+    """VALU issue rate of `kernel` against the chip's peak: instructions per
+    wave x waves from profiles/sq_counters_recon_4096.md over the measured
+    average time; None without the record."""
     path = os.path.join(ROOT, "profiles", "sq_counters_recon_4096.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(r"^\|\s*%s\s*\|\s*4194304\s*\|.*\|\s*([0-9.]+)%%\s*\|"
-                     r"\s*[0-9.]+\s*\|\s*$" % kernel)
+    pat = re.compile(r"^\|\s*%s\s*\|\s*4194304\s*\|\s*(\d+)\s*\|\s*(\d+)\s*\|"
+                     r".*\|\s*([0-9.]+)%%\s*\|\s*[0-9.]+\s*\|\s*$" % kernel)
     with open(path) as f:
         for line in f:
             m = pat.match(line.strip())
             if m:
-                return round(float(m.group(1)) / 100.0, 3)
+                waves, per_wave = int(m.group(1)), int(m.group(2))
+                insts = waves * per_wave
+                rate = insts / (avg_ms * 1e-3)
+                return {"instructions": insts,
+                        "achieved_per_s": round(rate, 0),
+                        "peak_per_s": VALU_ISSUE_PER_S,
+                        "frac": round(rate / VALU_ISSUE_PER_S, 4),
+                        "valu_busy": round(float(m.group(3)) / 100.0, 3)}
     return None
 
 
-def cpu_baseline(frame, seconds, threads):
+def cpu_baseline(frame, seconds, threads, gpu_out=None):
     """The reference's own reconstruction (AVX2 build, its ThreadPool) on
     the same frame, repeated until `seconds` have passed (at least 2
-    frames)."""
+    frames). With the GPU's output, also how far the reference's default
+    build (AVX2 EPF dividing with rcpps) lies from it: max relative error
+    (|a - g| / max |g|) and the fraction of samples above 1e-4."""
+    import numpy as np
     from oracle import ref
     if not ref.available("avx2"):
         return None
-    ref.recon(frame, threads=threads, stages=False, kind="avx2")  # warm
+    avx2, _ = ref.recon(frame, threads=threads, stages=False, kind="avx2")
+    dev = None
+    if gpu_out is not None:
+        d = np.abs(avx2.astype(np.float64) - gpu_out.astype(np.float64))
+        rel = d / max(float(np.abs(gpu_out).max()), 1e-30)
+        dev = {"max_rel": float(rel.max()),
+               "frac_above_1e-4": float((rel > 1e-4).mean()),
+               "bit_equal_frac": float((avx2.view(np.uint32) ==
+                                        gpu_out.view(np.uint32)).mean())}
     reps, t0 = 0, time.perf_counter()
     while True:
         ref.recon(frame, threads=threads, stages=False, kind="avx2")
@@ -157,7 +197,75 @@ def cpu_baseline(frame, seconds, threads):
             "sample": "%d full %dx%d frames, reference AVX2 build "
                       "(DequantImageAC + ReconOpsinImage per group + "
                       "FinalizeFrameDecoding), %d threads, %.1f s"
-                      % (reps, frame.xsize, frame.ysize, threads, el)}
+                      % (reps, frame.xsize, frame.ysize, threads, el),
+            "avx2_vs_gpu": dev}
+
+
+def srgb8_f64(linear):
+    """8-bit sRGB of a linear image in [0, 255] by the IEC 61966-2-1
+    transfer function in float64, clamped, rounded half up (the reference's
+    CastClip01 / ToExternal, pik/external_image.cc:190-197, 666-696)."""
+    import numpy as np
+    v = np.clip(linear.astype(np.float64) / 255.0, 0.0, 1.0)
+    e = np.where(v <= 0.0031308, 12.92 * v,
+                 1.055 * np.power(v, 1.0 / 2.4) - 0.055)
+    return np.floor(np.clip(e, 0.0, 1.0) * 255.0 + 0.5).astype(np.int32)
+
+
+def srgb8_check(R, linear):
+    """The 8-bit output of the same run against the float64 transfer
+    function of its linear output: max difference and mismatching-sample
+    fraction."""
+    import numpy as np
+    R.set_output(3)
+    R.run()
+    got = R.download_srgb8().astype(np.int32).transpose(2, 0, 1)
+    R.set_output(1)
+    diff = np.abs(got - srgb8_f64(linear))
+    return {"max_diff_lsb": int(diff.max()),
+            "mismatch_frac": float((diff != 0).mean())}
+
+
+def bench_with_copies(R, frame, steps):
+    """MP/s of the drop-in call from host memory: upload (AC planes from
+    pinned host memory), run, download of the linear image into pinned
+    host memory, per frame."""
+    import numpy as np
+    from pik_amd.devmem import PinnedArray
+    pac = PinnedArray.copy_of(frame.ac)
+    pout = PinnedArray((3, frame.ysize, frame.xsize), np.float32)
+    fr = frame.copy()
+    fr.ac = pac.array
+    R.upload(fr)
+    R.run()
+    R.download(out=pout.array)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.upload(fr)
+        R.run()
+        R.download(out=pout.array)
+    el = time.perf_counter() - t0
+    return round(frame.xsize * frame.ysize / 1e6 * steps / el, 2)
+
+
+def bench_per_image(frames, reps):
+    """MP/s of the same frames decoded one after another on one context
+    (upload once each, then run + sync per frame): the per-image launch
+    sequence a batch replaces."""
+    from pik_amd.runtime import Reconstructor
+    mp = sum(f.xsize * f.ysize for f in frames) / 1e6
+    with Reconstructor(0) as R:
+        total = 0.0
+        for f in frames:
+            R.upload(f)
+            R.run()
+            R.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                R.run()
+                R.sync()
+            total += (time.perf_counter() - t0) / reps
+    return round(mp / total, 2)
 
 
 def _batch_frame(args):
@@ -214,6 +322,105 @@ def bench_batch(frames, distinct, steps, warmup):
             "megapixels": round(mp, 2), "steps": steps,
             "workload": "batch of %d frames, W,H ~ U[256,2048], d ~ "
                         "U[0.5,3.0], one graph launch per step" % n}
+
+
+SEARCH_FIELDS = os.path.join(ROOT, "tests", "golden", "search_fields_1024.npz")
+
+
+def _search_item(args):
+    """Search image `seed`: the frame started from the reference encoder's
+    initial field (the committed fixture) through SetQuantField's rule."""
+    seed, field, qdc_f = args
+    from pik_amd import synth
+    fr, opsin, orig, lin = synth.make_search_inputs(1024, 1024, seed=seed,
+                                                    num_rects=40, target=1.0)
+    start = synth.max_over_transforms(field, fr.ac_strategy)
+    fr.global_scale, fr.quant_dc, fr.quant_field = synth.set_quant_field(
+        start, qdc_f)
+    return fr, opsin, orig, lin, start
+
+
+def search_inputs(distinct):
+    """The first `distinct` search images (process pool, before the GPU is
+    touched); None without the fixture."""
+    import numpy as np
+    from multiprocessing import Pool
+    if distinct <= 0 or not os.path.exists(SEARCH_FIELDS):
+        return None
+    z = np.load(SEARCH_FIELDS)
+    fields, qdc_f = z["fields"], float(z["quant_dc"])
+    distinct = min(distinct, len(fields))
+    t = time.perf_counter()
+    with Pool(min(distinct, host_threads())) as pool:
+        items = pool.map(_search_item,
+                         [(i, fields[i], qdc_f) for i in range(distinct)])
+    print("search: %d distinct 1024x1024 images synthesised in %.1f s"
+          % (distinct, time.perf_counter() - t), file=sys.stderr, flush=True)
+    return items
+
+
+def bench_search(items, images, steps):
+    """Wall time of the GPU search of `images` images (the distinct ones
+    cycled), one context and stream each, 8 host threads."""
+    from pik_amd.search import search_batch
+    batch = [items[i % len(items)][:2] + (items[i % len(items)][3],)
+             for i in range(images)]
+    search_batch(batch[:2], threads=2, steps=2)  # warm
+    t0 = time.perf_counter()
+    res = search_batch(batch, threads=8, steps=steps)
+    el = time.perf_counter() - t0
+    mp = images * 1024 * 1024 / 1e6
+    return {"value": round(mp / el, 2), "unit": "MP/s",
+            "wall_s": round(el, 3), "ms_per_image": round(el * 1e3 / images, 2),
+            "images": images, "distinct_images": len(items), "steps": steps,
+            "workload": "quant-field search of %d 1024x1024 images at "
+                        "Butteraugli distance 1.0, %d Adam steps each from "
+                        "the reference's initial field, predictions off"
+                        % (images, steps)}, res
+
+
+def search_baseline(items, results, sample, threads):
+    """The reference's own search loop (12 Butteraugli rounds, restated over
+    its functions in oracle/_ref) on the first `sample` images, timed on the
+    host's cores; both results judged by the reference Butteraugli and
+    sized by the reference's entropy coder, beside the initial field's."""
+    from oracle import ref
+    from pik_amd.runtime import Reconstructor
+    from pik_amd.search import apply_result
+    if not ref.available("scalar"):
+        return None
+    rows = []
+    with Reconstructor(0) as R:
+        for k in range(min(sample, len(items))):
+            fr, opsin, orig, lin, start = items[k]
+            row = {}
+            f0 = fr.copy()
+            f0.ac, f0.dc = ref.quantize(fr, opsin)
+            row["initial"] = {"distance": round(ref.butteraugli(
+                orig, R.reconstruct(f0)), 4), "bytes": sum(
+                ref.encoded_bytes(f0))}
+            g = apply_result(fr, results[k])
+            row["gpu"] = {"distance": round(ref.butteraugli(
+                orig, R.reconstruct(g)), 4), "bytes": sum(
+                ref.encoded_bytes(g))}
+            fb = ref.find_best_quantization(fr, opsin, orig, start, 1.0,
+                                            threads=threads)
+            h = fr.copy()
+            h.quant_field = fb["quant_field"]
+            h.global_scale, h.quant_dc = fb["global_scale"], fb["quant_dc"]
+            h.ac, h.dc = ref.quantize(h, opsin)
+            row["reference"] = {"distance": round(fb["distance"], 4),
+                                "bytes": sum(ref.encoded_bytes(h)),
+                                "seconds": round(fb["seconds"], 3)}
+            rows.append(row)
+            print("search sample %d: %s" % (k, json.dumps(row)),
+                  file=sys.stderr, flush=True)
+    ref_s = sum(r["reference"]["seconds"] for r in rows) / len(rows)
+    return {"images": rows, "reference_seconds_per_image": round(ref_s, 3),
+            "threads": threads, "kind": "reference",
+            "sample": "%d of the 1024x1024 images: the reference's search "
+                      "loop (quantize + decode + Butteraugli x 12, oracle/"
+                      "_ref), %d threads" % (len(rows), threads)}
 
 
 # Algorithmic HBM bytes per pixel of fwd_adj: int16 AC in (6), target in
@@ -301,6 +508,7 @@ def main():
     batch = None
     if a.batch_frames > 0:
         batch = synth_batch(a.batch_frames, a.batch_distinct, a.seed)
+    search_items = search_inputs(a.search_distinct) if a.search_images else None
     from pik_amd import synth
     from pik_amd.runtime import Reconstructor
 
@@ -333,27 +541,50 @@ def main():
         {k: round(v, 4) for k, v in fam_ms.items()}), file=sys.stderr,
         flush=True)
 
-    # Dominant single-kernel family.
+    # Dominant single-kernel family. `achieved` divides the path's
+    # algorithmic bytes per pixel (int16 AC in, per-block side data, f32
+    # linear out: PATH_BYTES_PER_PX) times the frame's pixels by that
+    # kernel's average HIP-event time; the kernel's own input / output
+    # bytes (BYTES_PER_PX) ride along as `kernel_bytes`. The kernel is
+    # limited by VALU issue, not HBM: `valu` gives its issue rate from the
+    # SQ counter record of the same workload (VALU instructions per wave x
+    # waves) against the chip's peak (1024 SIMDs x 2.4 GHz, one wave64
+    # instruction per 4 cycles).
     dom = max((k for k in BYTES_PER_PX if fam_ms.get(k, 0.0) > 0),
               key=lambda k: fam_ms[k])
     pad_px = (frame.xsize_blocks * 8) * (frame.ysize_blocks * 8)
-    algo_bytes = BYTES_PER_PX[dom] * (pad_px if dom != "final_opsin"
-                                      else frame.xsize * frame.ysize)
+    algo_bytes = PATH_BYTES_PER_PX * frame.xsize * frame.ysize
+    kernel_bytes = BYTES_PER_PX[dom] * (pad_px if dom != "final_opsin"
+                                        else frame.xsize * frame.ysize)
     achieved = algo_bytes / (fam_ms[dom] * 1e-3) / 1e9
     roof = {"kernel": KERNEL_NAMES[dom], "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic_from_profiles(KERNEL_NAMES[dom]),
             "algorithmic_bytes": int(algo_bytes),
+            "kernel_bytes": int(kernel_bytes),
             "avg_ms": round(fam_ms[dom], 4),
-            # the EPF kernel is VALU-bound; its VALU busy fraction from the
-            # SQ counters of the same workload
-            "valu_busy": valu_busy_from_profiles(KERNEL_NAMES[dom])}
+            "limiter": "valu",
+            "valu": valu_roofline(KERNEL_NAMES[dom], fam_ms[dom])}
+
+    # the output, its 8-bit form, and the drop-in call with host copies
+    gpu_out = R.download()
+    srgb = srgb8_check(R, gpu_out)
+    with_copies = bench_with_copies(R, frame, max(1, a.steps // 4))
+    print("srgb8: %s; with copies %.1f MP/s" % (json.dumps(srgb),
+          with_copies), file=sys.stderr, flush=True)
 
     cpu = None
+    threads = a.cpu_threads or host_threads()
     if not a.no_cpu_baseline:
-        threads = a.cpu_threads or host_threads()
-        cpu = cpu_baseline(frame, a.cpu_seconds, threads)
+        cpu = cpu_baseline(frame, a.cpu_seconds, threads, gpu_out)
+    acs = frame.ac_strategy
+    first = (acs & 15) == 0
+    names = {0: "DCT8", 1: "IDENTITY", 2: "DCT2X2", 3: "DCT4X4", 4: "DCT16",
+             5: "DCT32", 6: "LINES", 7: "DCT8_NOHF", 8: "DCT4X4_NOHF"}
+    import numpy as np
+    hist = {names[r]: round(float(((acs >> 4) == r).mean()), 4)
+            for r in range(9) if ((acs >> 4) == r).any()}
 
     out = {
         "metric": METRIC,
@@ -374,7 +605,17 @@ def main():
                    "megapixels": round(mp, 3), "seed": a.seed,
                    "adaptive_reconstruction": a.adaptive_reconstruction,
                    "path_GBps": round(PATH_BYTES_PER_PX * value / 1e3, 2),
-                   "family_ms": {k: round(v, 4) for k, v in fam_ms.items()}},
+                   "family_ms": {k: round(v, 4) for k, v in fam_ms.items()},
+                   "with_copies_MPps": with_copies,
+                   "srgb8": srgb,
+                   "side_information": "generator's (AC strategy, quant "
+                                       "field, colour correlation)",
+                   "global_scale": int(frame.global_scale),
+                   "quant_field_p5_p50_p95": [
+                       int(v) for v in np.percentile(frame.quant_field,
+                                                     [5, 50, 95])],
+                   "block_fraction_by_strategy": hist,
+                   "transforms": int(first.sum())},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -384,8 +625,26 @@ def main():
     if batch:
         out["batch"] = bench_batch(batch[0], batch[1], max(1, a.steps // 4),
                                    a.warmup)
+        per = bench_per_image(batch[0][:batch[1]], 3)
+        out["batch"]["per_image_MPps"] = per
+        out["batch"]["over_per_image"] = round(out["batch"]["value"] / per, 3)
+        out["batch"]["over_single_4096"] = round(out["batch"]["value"] /
+                                                 value, 3)
         print("batch: " + json.dumps(out["batch"]), file=sys.stderr,
               flush=True)
+    if search_items:
+        out["search"], results = bench_search(search_items, a.search_images,
+                                              20)
+        print("search: " + json.dumps(out["search"]), file=sys.stderr,
+              flush=True)
+        if not a.no_cpu_baseline and a.search_sample > 0:
+            sb = search_baseline(search_items, results, a.search_sample,
+                                 threads)
+            if sb:
+                out["search"]["cpu_baseline"] = sb
+                out["search"]["speedup_vs_reference"] = round(
+                    sb["reference_seconds_per_image"] * a.search_images /
+                    out["search"]["wall_s"], 2)
     if a.adj_size > 0:
         out["fwd_adj"] = bench_fwd_adj(a.adj_size, a.steps, a.warmup, a.seed)
         print("fwd_adj: " + json.dumps(out["fwd_adj"]), file=sys.stderr,

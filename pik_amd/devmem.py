@@ -28,6 +28,8 @@ def hip():
         L.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
         L.hipMemset.argtypes = [vp, ctypes.c_int, sz]
         L.hipSetDevice.argtypes = [ctypes.c_int]
+        L.hipHostMalloc.argtypes = [ctypes.POINTER(vp), sz, ctypes.c_uint]
+        L.hipHostFree.argtypes = [vp]
         L.hipGetErrorString.restype = ctypes.c_char_p
         L.hipGetErrorString.argtypes = [ctypes.c_int]
         _hip = L
@@ -93,6 +95,44 @@ class DeviceArray:
             hip().hipFree(self._ptr)
         self._ptr = None
         self._owned = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Page-locked host memory (hipHostMalloc) viewed as a numpy array
+    (`.array`): host <-> device copies from it run at the link's rate
+    instead of through the runtime's staging of pageable memory."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        nbytes = max(int(np.prod(self.shape)) * self.dtype.itemsize, 4)
+        p = ctypes.c_void_p()
+        _check(hip().hipHostMalloc(ctypes.byref(p), nbytes, 0),
+               "hipHostMalloc")
+        self._ptr = p.value
+        buf = (ctypes.c_char * nbytes).from_address(self._ptr)
+        self.array = np.frombuffer(buf, self.dtype,
+                                   int(np.prod(self.shape))).reshape(
+                                       self.shape)
+
+    @classmethod
+    def copy_of(cls, a):
+        a = np.asarray(a)
+        p = cls(a.shape, a.dtype)
+        p.array[...] = a
+        return p
+
+    def free(self):
+        if self._ptr:
+            self.array = None
+            hip().hipHostFree(self._ptr)
+        self._ptr = None
 
     def __del__(self):
         try:

@@ -388,6 +388,19 @@ def set_quant_field(qf_float, quant_dc_float, global_scale=None):
     return int(global_scale), quant_dc, field
 
 
+def max_over_transforms(field, ac_strategy):
+    """A float quant field with every multi-block transform's blocks set to
+    their maximum (AdjustQuantField, pik/adaptive_quantization.cc:554-578)."""
+    out = np.array(field, np.float32, copy=True)
+    raw = ac_strategy >> 4
+    first = (ac_strategy & 15) == 0
+    for code in (abi.ACS_DCT16X16, abi.ACS_DCT32X32):
+        s = covered(code)
+        for y0, x0 in zip(*np.nonzero(first & (raw == code))):
+            out[y0:y0 + s, x0:x0 + s] = out[y0:y0 + s, x0:x0 + s].max()
+    return out
+
+
 def encode_frame(rgb, distance=1.0, seed=0, num_dots=0,
                  lut_ids="zero", predict=True, grayscale=False,
                  global_scale=None, lines=0.0):

@@ -54,8 +54,8 @@ def test_search_quantization_and_decode_are_the_references(rec):
     # the field moved, within the trust region around the initial one
     q0, q1 = fr.quant_field.astype(np.float64), res["quant_field"]
     assert (q1 != q0).mean() > 0.1
-    assert (q1 >= np.floor(0.9 * q0) - 1).all()
-    assert (q1 <= np.ceil(1.6 * q0) + 1).all()
+    assert (q1 >= np.floor(0.75 * q0) - 1).all()
+    assert (q1 <= np.ceil(1.25 * q0) + 1).all()
 
 
 def test_search_lowers_error_and_butteraugli_distance(rec):
@@ -72,7 +72,10 @@ def test_search_lowers_error_and_butteraugli_distance(rec):
     f0 = fr.copy()
     f0.ac, f0.dc = ac, dc
     d0 = ref.butteraugli(orig, rec.reconstruct(f0))
-    res = search_frame(rec, fr, opsin, lin)
+    # the rate weight at the start's distortion level (lam -1) with room to
+    # add precision: the search spends bits where the attributed error is
+    # largest
+    res = search_frame(rec, fr, opsin, lin, lam=-1.0, trust=(0.9, 1.6))
     assert res["loss"][-1] < res["loss"][0]
     rec.run()
     d1 = ref.butteraugli(orig, rec.download())
