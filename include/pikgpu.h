@@ -53,7 +53,8 @@ enum {
   PIKGPU_EHIP = -2,      /* HIP runtime error */
   PIKGPU_ENOMEM = -3,    /* device allocation failed */
   PIKGPU_ESTATE = -4,    /* call out of order (e.g. run before upload) */
-  PIKGPU_EUNSUPPORTED = -5
+  PIKGPU_EUNSUPPORTED = -5,
+  PIKGPU_EDATA = -6      /* corrupt bitstream (host entropy decoding) */
 };
 
 /* AC strategy raw types; the per-block strategy byte is (type << 4) | index
@@ -514,6 +515,44 @@ PIKGPU_API int pikgpu_reconstruct_noalloc(const pikgpu_frame* frame,
                                           size_t out_stride_bytes,
                                           void* stream, char* msg,
                                           size_t msg_size);
+
+/* ------------------------------------------------------------------------
+ * Host entropy decoding of one pass's AC groups into the frame-wide int16
+ * AC planes that pikgpu_frame.ac (and pikgpu_frame_add_pass) take.
+ *
+ * Replaces the entropy half of DecodeFromBitstream<first>
+ * (pik/compressed_image.cc:719-813): DecodeNoise, DecodeCoeffOrder x 3,
+ * DecodeHistograms, then DecodeAC per 64x64 tile with GatherCoefficients
+ * for 16x16 and 32x32 transforms (pik/noise.cc:529-539,
+ * pik/entropy_coder.cc:822-836, 889-924, 1149-1212, pik/ac_strategy.h:
+ * 254-272). The dequantization the reference does per tile runs on the GPU
+ * for the whole frame instead. Runs on the host only (no device calls).
+ *
+ * Groups are 512x512 pixels, row-major; group i's bytes start at its
+ * noise parameters (the start of its pass data after the group header,
+ * pik/pik_frame.cc:902-915). The AC strategy bytes are those of
+ * pikgpu_frame.ac_strategy. Every block of every group is written. Groups
+ * decode in parallel on `threads` host threads (<= 0: one per hardware
+ * thread). Returns PIKGPU_OK, PIKGPU_EINVAL for inconsistent arguments, or
+ * PIKGPU_EDATA when a group's bitstream is invalid (the lowest such group
+ * is named in `msg`); the AC of a failed group is unspecified. */
+typedef struct pikgpu_ac_pass {
+  uint32_t xsize, ysize;          /* frame size in pixels */
+  const uint8_t* ac_strategy;     /* ceil(ysize/8) rows of ceil(xsize/8) */
+  size_t ac_strategy_stride;
+  size_t num_groups;              /* ceil(xsize/512) * ceil(ysize/512) */
+  const uint8_t* const* group_data;
+  const size_t* group_size;       /* bytes of each group's pass data */
+  int16_t* ac[3];                 /* out: X, Y, B; rows of >= 64 * bx */
+  size_t ac_stride;
+  float* noise;                   /* out (may be NULL): alpha, gamma, beta
+                                     per group (NoiseParams) */
+  size_t* group_consumed;         /* out (may be NULL): bytes read per
+                                     group */
+} pikgpu_ac_pass;
+
+PIKGPU_API int pikgpu_decode_ac_pass(const pikgpu_ac_pass* pass, int threads,
+                                     char* msg, size_t msg_size);
 
 #ifdef __cplusplus
 }  /* extern "C" */

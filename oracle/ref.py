@@ -323,3 +323,64 @@ def find_best_quantization(frame, opsin, opsin_orig, init, target,
                            % L.pikref_last_error().decode())
     return dict(quant_field=qf, global_scale=int(sc[0]), quant_dc=int(sc[1]),
                 distance=d.value, seconds=sec.value)
+
+
+def encode_ac_groups(frame, noise=None, fast_mode=False, kind="scalar"):
+    """The frame's AC as the reference encoder writes it, one bitstream per
+    512x512 group (EncodeToBitstream). Returns a list of bytes objects."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_encode_ac_groups.argtypes = [
+        ctypes.POINTER(abi.FrameC), vp, ctypes.c_int, vp, ctypes.c_size_t,
+        vp, vp]
+    fc, keep = frame.to_c()
+    ng = (-(-frame.xsize // 512)) * (-(-frame.ysize // 512))
+    sizes = np.zeros(ng, np.uintp)
+    total = ctypes.c_size_t()
+    nz = None if noise is None else np.asarray(noise, np.float32)
+    nzp = None if nz is None else nz.ctypes.data
+    rc = L.pikref_encode_ac_groups(ctypes.byref(fc), nzp, int(fast_mode),
+                                   None, 0, sizes.ctypes.data,
+                                   ctypes.byref(total))
+    if rc != 0:
+        raise RuntimeError("pikref_encode_ac_groups: %s"
+                           % L.pikref_last_error().decode())
+    buf = np.zeros(max(1, total.value), np.uint8)
+    rc = L.pikref_encode_ac_groups(ctypes.byref(fc), nzp, int(fast_mode),
+                                   buf.ctypes.data, buf.size,
+                                   sizes.ctypes.data, ctypes.byref(total))
+    if rc != 0:
+        raise RuntimeError("pikref_encode_ac_groups: %s"
+                           % L.pikref_last_error().decode())
+    out, pos = [], 0
+    for s in sizes:
+        out.append(buf[pos:pos + int(s)].tobytes())
+        pos += int(s)
+    return out
+
+
+def decode_ac_groups(frame, groups, check_dequant=True, kind="scalar"):
+    """Decodes group bitstreams with the reference decoder's functions (and,
+    with check_dequant, checks DecodeFromBitstream<true> agrees). Returns
+    (ac int16 (3, by, bx*64), noise (groups, 3), consumed bytes per group)."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_decode_ac_groups.argtypes = [
+        ctypes.POINTER(abi.FrameC), vp, vp, vp, ctypes.c_size_t, vp, vp,
+        ctypes.c_int]
+    fc, keep = frame.to_c()
+    by, bx = frame.ysize_blocks, frame.xsize_blocks
+    data = np.frombuffer(b"".join(groups) + b"\0", np.uint8)
+    sizes = np.array([len(g) for g in groups], np.uintp)
+    ac = np.zeros((3, by, bx * 64), np.int16)
+    ptrs = (vp * 3)(*[ac[c].ctypes.data for c in range(3)])
+    noise = np.zeros((len(groups), 3), np.float32)
+    used = np.zeros(len(groups), np.uintp)
+    rc = L.pikref_decode_ac_groups(ctypes.byref(fc), data.ctypes.data,
+                                   sizes.ctypes.data, ctypes.cast(ptrs, vp),
+                                   bx * 64, noise.ctypes.data,
+                                   used.ctypes.data, int(check_dequant))
+    if rc != 0:
+        raise RuntimeError("pikref_decode_ac_groups: %s"
+                           % L.pikref_last_error().decode())
+    return ac, noise, used.astype(np.int64)

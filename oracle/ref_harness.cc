@@ -1243,3 +1243,265 @@ API int pikref_find_best_quantization(const pikgpu_frame* f,
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   return 0;
 }
+
+// --------------------------------------------------------------------------
+// AC group bitstreams: the reference encoder's EncodeToBitstream per
+// 512x512 group (pik/compressed_image.cc:457-573, as EncodePass calls it,
+// pik/pik_frame.cc:374-410) and the reference decoder's entropy half of
+// DecodeFromBitstream (pik/compressed_image.cc:719-813), for the host
+// entropy decoder's parity tests and fixtures.
+
+namespace {
+
+int ImportAcStrategy(const pikgpu_frame* f, AcStrategyImage* acs) {
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  ImageB raw(bx, by);
+  for (size_t y = 0; y < by; ++y) {
+    for (size_t x = 0; x < bx; ++x) {
+      raw.Row(y)[x] = f->ac_strategy[y * f->ac_strategy_stride + x] >> 4;
+    }
+  }
+  *acs = AcStrategyImage(bx, by);
+  acs->SetFromRaw(Rect(0, 0, bx, by), raw);
+  for (size_t y = 0; y < by; ++y) {
+    if (memcmp(acs->ConstRaw().ConstRow(y),
+               f->ac_strategy + y * f->ac_strategy_stride, bx) != 0) {
+      return Fail("inconsistent ac_strategy bytes");
+    }
+  }
+  return 0;
+}
+
+Quantizer FrameQuantizer(const pikgpu_frame* f, const DequantMatrices* m) {
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  Quantizer q = DecoderQuantizer(m, bx, by, f->quant_dc, f->global_scale);
+  ImageI qf(bx, by);
+  for (size_t y = 0; y < by; ++y) {
+    memcpy(qf.Row(y), f->quant_field + y * f->quant_field_stride, bx * 4);
+  }
+  q.SetRawQuantField(std::move(qf));
+  return q;
+}
+
+}  // namespace
+
+// Writes the groups' bitstreams back to back into `out` (may be NULL to
+// size them) and each group's length into group_size; *total receives the
+// sum. `noise` (alpha, gamma, beta; may be NULL) goes into every group's
+// noise parameters; fast_mode selects BuildAndEncodeHistogramsFast.
+API int pikref_encode_ac_groups(const pikgpu_frame* f, const float* noise,
+                                int fast_mode, uint8_t* out, size_t cap,
+                                size_t* group_size, size_t* total) {
+  if (f == nullptr) return Fail("null frame");
+  AcStrategyImage acs;
+  if (ImportAcStrategy(f, &acs) != 0) return -1;
+  DequantMatrices matrices(/*need_inv_matrices=*/false);
+  const Quantizer quantizer = FrameQuantizer(f, &matrices);
+  NoiseParams np;
+  if (noise != nullptr) {
+    np.alpha = noise[0];
+    np.gamma = noise[1];
+    np.beta = noise[2];
+  }
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  const size_t gxs = DivCeil<size_t>(f->xsize, kGroupDim);
+  const size_t gys = DivCeil<size_t>(f->ysize, kGroupDim);
+  size_t pos = 0;
+  for (size_t gy = 0; gy < gys; ++gy) {
+    for (size_t gx = 0; gx < gxs; ++gx) {
+      const Rect group(gx * kGroupDim, gy * kGroupDim, kGroupDim, kGroupDim,
+                       bx * kBlockDim, by * kBlockDim);
+      const Rect blocks(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                        kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+      EncCache cache;
+      cache.xsize_blocks = blocks.xsize();
+      cache.ysize_blocks = blocks.ysize();
+      cache.ac_strategy = acs.Copy(blocks);
+      cache.ac = Image3S(blocks.xsize() * 64, blocks.ysize());
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < blocks.ysize(); ++y) {
+          memcpy(cache.ac.PlaneRow(c, y),
+                 f->ac[c] + (blocks.y0() + y) * f->ac_stride + blocks.x0() * 64,
+                 blocks.xsize() * 64 * 2);
+        }
+      }
+      const PaddedBytes bytes = EncodeToBitstream(
+          cache, Rect(0, 0, group.xsize(), group.ysize()),
+          quantizer.Copy(blocks), np, fast_mode != 0, /*handler=*/nullptr);
+      if (group_size != nullptr) group_size[gy * gxs + gx] = bytes.size();
+      if (out != nullptr) {
+        if (pos + bytes.size() > cap) return Fail("output buffer too small");
+        memcpy(out + pos, bytes.data(), bytes.size());
+      }
+      pos += bytes.size();
+    }
+  }
+  if (total != nullptr) *total = pos;
+  return 0;
+}
+
+// Decodes the groups' bitstreams (back to back in `data`) with the
+// reference's own functions in DecodeFromBitstream's order -- DecodeNoise,
+// DecodeCoeffOrder, DecodeHistograms, one ANSSymbolReader through DecodeAC
+// of every tile, GatherCoefficients, CheckANSFinalState -- into frame-wide
+// int16 AC. With check_dequant set it also runs DecodeFromBitstream<true>
+// itself on every group and fails unless its dequantized coefficients equal
+// DequantImageAC of the int16 decoded here, bit for bit.
+API int pikref_decode_ac_groups(const pikgpu_frame* f, const uint8_t* data,
+                                const size_t* group_size,
+                                int16_t* const* ac_out, size_t ac_stride,
+                                float* noise_out, size_t* consumed,
+                                int check_dequant) {
+  if (f == nullptr) return Fail("null frame");
+  AcStrategyImage acs;
+  if (ImportAcStrategy(f, &acs) != 0) return -1;
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  const size_t gxs = DivCeil<size_t>(f->xsize, kGroupDim);
+  const size_t gys = DivCeil<size_t>(f->ysize, kGroupDim);
+
+  std::unique_ptr<DequantMatrices> matrices;
+  std::unique_ptr<Quantizer> quantizer;
+  std::unique_ptr<ColorCorrelationMap> cmap;
+  FrameDecCache fdc;
+  FrameHeader frame_header;
+  GroupHeader group_header;
+  if (check_dequant) {
+    matrices.reset(new DequantMatrices(/*need_inv_matrices=*/false));
+    quantizer.reset(new Quantizer(FrameQuantizer(f, matrices.get())));
+    cmap.reset(new ColorCorrelationMap(f->xsize, f->ysize));
+    for (size_t y = 0; y < cmap->ytox_map.ysize(); ++y) {
+      memcpy(cmap->ytox_map.Row(y), f->ytox_map + y * f->ytox_stride,
+             cmap->ytox_map.xsize() * 4);
+      memcpy(cmap->ytob_map.Row(y), f->ytob_map + y * f->ytob_stride,
+             cmap->ytob_map.xsize() * 4);
+    }
+    cmap->ytox_dc = f->ytox_dc;
+    cmap->ytob_dc = f->ytob_dc;
+    fdc.ac_strategy = acs.Copy();
+    fdc.raw_quant_field = CopyImage(quantizer->RawQuantField());
+    const size_t tx = DivCeil<size_t>(f->xsize, kColorTileDim);
+    const size_t ty = DivCeil<size_t>(f->ysize, kColorTileDim);
+    fdc.dequant_control_field = ImageB(tx, ty);
+    for (size_t y = 0; y < ty; ++y) {
+      memcpy(fdc.dequant_control_field.Row(y),
+             f->dequant_cf + y * f->dequant_cf_stride, tx);
+    }
+    memcpy(fdc.dequant_map, f->dequant_map, sizeof(fdc.dequant_map));
+  }
+
+  size_t pos = 0;
+  for (size_t gy = 0; gy < gys; ++gy) {
+    for (size_t gx = 0; gx < gxs; ++gx) {
+      const size_t gi = gy * gxs + gx;
+      const uint8_t* bytes = data + pos;
+      const size_t size = group_size[gi];
+      pos += size;
+      const Rect blocks(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                        kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+      const size_t xsb = blocks.xsize(), ysb = blocks.ysize();
+      BitReader br(bytes, size);
+      NoiseParams np;
+      if (!DecodeNoise(&br, &np)) return Fail("DecodeNoise failed");
+      int order[kOrderContexts * kDCTBlockSize];
+      for (size_t c = 0; c < kOrderContexts; ++c) {
+        DecodeCoeffOrder(&order[c * kDCTBlockSize], &br);
+      }
+      if (!br.JumpToByteBoundary()) return Fail("order padding");
+      ANSCode code;
+      std::vector<uint8_t> context_map;
+      if (!DecodeHistograms(&br, kNumContexts, 256, &code, &context_map)) {
+        return Fail("DecodeHistograms failed");
+      }
+      if (!br.JumpToByteBoundary()) return Fail("histogram padding");
+      ANSSymbolReader decoder(&code);
+      Image3S tile(kTileDimInBlocks * kDCTBlockSize, kTileDimInBlocks);
+      Image3I nz(kTileDimInBlocks, kTileDimInBlocks);
+      Image3S group_ac(xsb * kDCTBlockSize, ysb);
+      const size_t stride = tile.PixelsPerRow();
+      SIMD_ALIGN int16_t unscattered[AcStrategy::kMaxCoeffArea];
+      const size_t ntx = DivCeil(xsb, kTileDimInBlocks);
+      const size_t nty = DivCeil(ysb, kTileDimInBlocks);
+      for (size_t t = 0; t < ntx * nty; ++t) {
+        const Rect rect((t % ntx) * kTileDimInBlocks,
+                        (t / ntx) * kTileDimInBlocks, kTileDimInBlocks,
+                        kTileDimInBlocks, xsb, ysb);
+        if (!DecodeAC(context_map, order, &br, &decoder, &tile, rect, &nz)) {
+          return Fail("DecodeAC failed");
+        }
+        for (int c = 0; c < 3; ++c) {
+          for (size_t y = 0; y < rect.ysize(); ++y) {
+            for (size_t x = 0; x < rect.xsize(); ++x) {
+              const AcStrategy s =
+                  acs.ConstRow(blocks, rect.y0() + y)[rect.x0() + x];
+              if (!s.IsFirstBlock()) continue;
+              const int16_t* block = tile.ConstPlaneRow(c, y) + x * 64;
+              const size_t w = s.covered_blocks_x() * kDCTBlockSize;
+              s.GatherCoefficients(block, stride, unscattered, w);
+              for (size_t i = 0; i < s.covered_blocks_y(); ++i) {
+                memcpy(group_ac.PlaneRow(c, rect.y0() + y + i) +
+                           (rect.x0() + x) * 64,
+                       unscattered + w * i, w * sizeof(int16_t));
+              }
+            }
+          }
+        }
+      }
+      if (!decoder.CheckANSFinalState()) return Fail("ANS final state");
+      if (!br.JumpToByteBoundary()) return Fail("AC padding");
+      if (consumed != nullptr) consumed[gi] = br.Position();
+      if (noise_out != nullptr) {
+        noise_out[3 * gi + 0] = np.alpha;
+        noise_out[3 * gi + 1] = np.gamma;
+        noise_out[3 * gi + 2] = np.beta;
+      }
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < ysb; ++y) {
+          memcpy(ac_out[c] + (blocks.y0() + y) * ac_stride + blocks.x0() * 64,
+                 group_ac.ConstPlaneRow(c, y), xsb * 64 * sizeof(int16_t));
+        }
+      }
+      if (!check_dequant) continue;
+      const Rect padded_rect(blocks.x0() * kBlockDim, blocks.y0() * kBlockDim,
+                             xsb * kBlockDim, ysb * kBlockDim);
+      const Rect group_in_color_tiles(
+          blocks.x0() / kColorTileDimInBlocks,
+          blocks.y0() / kColorTileDimInBlocks,
+          DivCeil(xsb, kColorTileDimInBlocks),
+          DivCeil(ysb, kColorTileDimInBlocks));
+      const Quantizer quant = quantizer->Copy(blocks);
+      PaddedBytes compressed(size);
+      if (size != 0) memcpy(compressed.data(), bytes, size);
+      BitReader br2(compressed.data(), compressed.size());
+      GroupDecCache full;
+      NoiseParams np2;
+      if (!DecodeFromBitstream<true>(frame_header, group_header, compressed,
+                                     &br2, padded_rect, /*handler=*/nullptr,
+                                     xsb, ysb, *cmap, group_in_color_tiles,
+                                     &np2, quant, &fdc, &full, nullptr)) {
+        return Fail("DecodeFromBitstream<true> failed");
+      }
+      GroupDecCache mine;
+      mine.InitOnce(xsb, ysb);
+      DequantImageAC(quant, *cmap, group_in_color_tiles, group_ac, &fdc,
+                     &mine, padded_rect, nullptr);
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < ysb; ++y) {
+          if (memcmp(full.ac.ConstPlaneRow(c, y), mine.ac.ConstPlaneRow(c, y),
+                     xsb * 64 * sizeof(float)) != 0) {
+            return Fail("DecodeFromBitstream<true> disagrees with the "
+                        "decoded AC");
+          }
+        }
+      }
+      if (np2.alpha != np.alpha || np2.gamma != np.gamma ||
+          np2.beta != np.beta) {
+        return Fail("DecodeFromBitstream<true> noise parameters differ");
+      }
+    }
+  }
+  return 0;
+}

@@ -18,6 +18,7 @@ EHIP = -2
 ENOMEM = -3
 ESTATE = -4
 EUNSUPPORTED = -5
+EDATA = -6
 
 # AC strategy raw types (pik/ac_strategy.h:55-75).
 ACS_DCT = 0
@@ -163,6 +164,23 @@ class SearchIoC(ctypes.Structure):
     ]
 
 
+class AcPassC(ctypes.Structure):
+    """pikgpu_ac_pass."""
+    _fields_ = [
+        ("xsize", ctypes.c_uint32),
+        ("ysize", ctypes.c_uint32),
+        ("ac_strategy", ctypes.c_void_p),
+        ("ac_strategy_stride", ctypes.c_size_t),
+        ("num_groups", ctypes.c_size_t),
+        ("group_data", ctypes.c_void_p),
+        ("group_size", ctypes.c_void_p),
+        ("ac", ctypes.c_void_p * 3),
+        ("ac_stride", ctypes.c_size_t),
+        ("noise", ctypes.c_void_p),
+        ("group_consumed", ctypes.c_void_p),
+    ]
+
+
 def load():
     """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
     global _lib
@@ -227,6 +245,8 @@ def load():
     lib.pikgpu_validate_frame.argtypes = [ctypes.POINTER(FrameC),
                                           ctypes.c_char_p, ctypes.c_size_t]
     lib.pikgpu_fwd_adj.argtypes = [vp, ctypes.POINTER(AdjIoC)]
+    lib.pikgpu_decode_ac_pass.argtypes = [ctypes.POINTER(AcPassC), ctypes.c_int,
+                                          ctypes.c_char_p, ctypes.c_size_t]
     _lib = lib
     return lib
 
@@ -242,5 +262,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_batch_sync", "pikgpu_batch_last_ms", "pikgpu_batch_download",
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
     "pikgpu_frame_add_pass", "pikgpu_reconstruct_noalloc",
-    "pikgpu_search_quant_field",
+    "pikgpu_search_quant_field", "pikgpu_decode_ac_pass",
 ]

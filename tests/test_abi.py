@@ -81,6 +81,10 @@ int main(void) {
   F(pikgpu_search_io, quant_field) F(pikgpu_search_io, ac)
   F(pikgpu_search_io, dc_stride) F(pikgpu_search_io, loss)
   F(pikgpu_search_io, lambda_used)
+  printf("pikgpu_ac_pass size %zu\n", sizeof(pikgpu_ac_pass));
+  F(pikgpu_ac_pass, ac_strategy) F(pikgpu_ac_pass, num_groups)
+  F(pikgpu_ac_pass, group_size) F(pikgpu_ac_pass, ac)
+  F(pikgpu_ac_pass, ac_stride) F(pikgpu_ac_pass, group_consumed)
   return 0;
 }
 """
@@ -97,7 +101,8 @@ def test_ctypes_layout_matches_header(tmp_path):
     types = {"pikgpu_frame": abi.FrameC, "pikgpu_dot": abi.Dot,
              "pikgpu_config": abi.ConfigC, "pikgpu_adj_io": abi.AdjIoC,
              "pikgpu_search_params": abi.SearchParamsC,
-             "pikgpu_search_io": abi.SearchIoC}
+             "pikgpu_search_io": abi.SearchIoC,
+             "pikgpu_ac_pass": abi.AcPassC}
     for line in out.splitlines():
         t, m, v = line.split()
         if m == "size":
