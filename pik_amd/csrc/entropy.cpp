@@ -576,12 +576,13 @@ struct GroupJob {
 
 // Every transform lies in one tile of the group and its blocks carry its
 // type with indices 0, 1, ... in row-major order, so that every block is
-// written exactly once.
+// written exactly once: a first block checks its whole region, any other
+// block that its first block is one.
 bool CheckStrategies(const GroupJob& g, std::string* err) {
   for (int y = 0; y < g.bys; ++y) {
+    const uint8_t* row = g.acs + size_t(y) * g.acs_stride;
     for (int x = 0; x < g.bxs; ++x) {
-      const uint8_t a = g.acs[size_t(y) * g.acs_stride + x];
-      const int type = a >> 4, idx = a & 15;
+      const int type = row[x] >> 4, idx = row[x] & 15;
       if (type > PIKGPU_ACS_DCT4X4_NOHF) {
         *err = "unknown AC strategy";
         return false;
@@ -591,6 +592,7 @@ bool CheckStrategies(const GroupJob& g, std::string* err) {
         *err = "AC strategy block index out of range";
         return false;
       }
+      if (sc == 1) continue;
       const int fy = y - idx / sc, fx = x - idx % sc;
       if (fy < 0 || fx < 0 || fy + sc > g.bys || fx + sc > g.bxs ||
           fy / kTileBlocks != (fy + sc - 1) / kTileBlocks ||
@@ -598,9 +600,16 @@ bool CheckStrategies(const GroupJob& g, std::string* err) {
         *err = "AC strategy transform outside its tile";
         return false;
       }
-      for (int i = 0; i < sc * sc; ++i) {
+      if (idx != 0) {
+        if (g.acs[size_t(fy) * g.acs_stride + fx] != uint8_t(type << 4)) {
+          *err = "inconsistent AC strategy";
+          return false;
+        }
+        continue;
+      }
+      for (int i = 1; i < sc * sc; ++i) {
         const uint8_t want = uint8_t(type << 4 | i);
-        if (g.acs[size_t(fy + i / sc) * g.acs_stride + fx + i % sc] != want) {
+        if (g.acs[size_t(y + i / sc) * g.acs_stride + x + i % sc] != want) {
           *err = "inconsistent AC strategy";
           return false;
         }
@@ -610,29 +619,96 @@ bool CheckStrategies(const GroupJob& g, std::string* err) {
   return true;
 }
 
+// The histogram of every AC context under one group's context map:
+// count[c][b] for channel c's non-zero count with predicted-count bucket b
+// (0..32), coef[c][left][k] for a coefficient token with `left` non-zeros
+// still to come at scan position k (left + k <= 64).
+struct AcHistograms {
+  explicit AcHistograms(const std::vector<uint8_t>& map) {
+    memset(coef, 0, sizeof(coef));
+    for (int c = 0; c < 3; ++c) {
+      for (int b = 0; b <= kNonZeroBuckets; ++b) {
+        count[c][b] = map[kBlockContexts * b + c];
+      }
+      const int base = kBlockContexts * kNonZeroBuckets +
+                       kZeroDensityContexts * c;
+      for (int left = 1; left <= 64; ++left) {
+        for (int k = 0; left + k <= 64; ++k) {
+          coef[c][left][k] = map[base + ZeroDensityContext(left, k)];
+        }
+      }
+    }
+  }
+  uint8_t count[3][kNonZeroBuckets + 1];
+  uint8_t coef[3][65][64];
+};
+
+// Where each transmitted coefficient lands in the frame layout. A 1x1
+// transform's block is stored as is. An S x S-block transform's (8S)^2
+// coefficients are stored row-major over S rows of S * 64 (the layout
+// GatherCoefficients makes, pik/ac_strategy.h:254-272); the bitstream
+// carries them interleaved over the transform's blocks: block (dy, dx)
+// holds coefficient (y, x), y = (q / 8) * S + dy, x = (q % 8) * S + dx, at
+// its position q (pik/dct_util.h:21-55). For each
+// (S, dy, dx, q) the table holds the offset from the transform's first
+// slot in the frame: row * ac_stride + column.
+struct Placement {
+  explicit Placement(size_t stride) {
+    for (int q = 0; q < 64; ++q) one[q] = q;
+    Fill(2, stride, s2);
+    Fill(4, stride, s4);
+  }
+  static void Fill(int s, size_t stride, int32_t* t) {
+    for (int d = 0; d < s * s; ++d) {
+      const int dy = d / s, dx = d % s;
+      for (int q = 0; q < 64; ++q) {
+        const int y = (q / 8) * s + dy, x = (q % 8) * s + dx;
+        const int p = y * 8 * s + x;
+        const int row = p / (s * 64), col = p % (s * 64);
+        t[d * 64 + q] = int32_t(size_t(row) * stride) + col;
+      }
+    }
+  }
+  int32_t one[64];
+  int32_t s2[4 * 64];
+  int32_t s4[16 * 64];
+};
+
 // One tile's coefficients in scan order (DecodeAC,
-// pik/entropy_coder.cc:1149-1212) into `tile` (per channel, kTileBlocks^2
-// blocks of 64 in the transmitted layout: a multi-block transform's
-// coefficients interleaved over its blocks).
-bool DecodeTile(const std::vector<uint8_t>& map, const int32_t* order,
-                BitStream* bs, AnsReader* ans, int txs, int tys,
-                int16_t* tile, std::string* err) {
+// pik/entropy_coder.cc:1149-1212), written straight to their places in the
+// frame layout (the reference decodes a tile into a scratch buffer and
+// gathers the larger transforms afterwards, pik/compressed_image.cc:
+// 783-801).
+bool DecodeTile(const GroupJob& g, const AcHistograms& hist,
+                const Placement& place, const int32_t* order, BitStream* bs,
+                AnsReader* ans, int tx0, int ty0, int txs, int tys,
+                std::string* err) {
   int32_t nz[kTileBlocks][kTileBlocks];
   for (int c = 0; c < 3; ++c) {
-    const int zd_base = kBlockContexts * kNonZeroBuckets +
-                        kZeroDensityContexts * c;
     const int32_t* ord = order + 64 * c;
     for (int by = 0; by < tys; ++by) {
+      const uint8_t* acs = g.acs + size_t(ty0 + by) * g.acs_stride + tx0;
       for (int bx = 0; bx < txs; ++bx) {
-        int16_t* blk = tile + ((c * kTileBlocks + by) * kTileBlocks + bx) * 64;
-        memset(blk, 0, 64 * sizeof(int16_t));
+        const int type = acs[bx] >> 4, idx = acs[bx] & 15;
+        const int s = Covered(type);
+        // The transform's first block and this block's place in it.
+        const int fy = ty0 + by - idx / s, fx = tx0 + bx - idx % s;
+        int16_t* base = g.ac[c] + size_t(fy) * g.ac_stride + size_t(fx) * 64;
+        const int32_t* pos =
+            s == 1 ? place.one : (s == 2 ? place.s2 : place.s4) + idx * 64;
+        if (idx == 0) {
+          for (int i = 0; i < s; ++i) {
+            memset(base + size_t(i) * g.ac_stride, 0,
+                   size_t(s) * 64 * sizeof(int16_t));
+          }
+        }
         // Predicted count: mean of top and left, either alone at an edge,
         // 32 at the tile's corner.
         int pred;
         if (bx == 0) pred = by == 0 ? 32 : nz[by - 1][bx];
         else if (by == 0) pred = nz[by][bx - 1];
         else pred = (nz[by - 1][bx] + nz[by][bx - 1] + 1) / 2;
-        const int n = ans->Read(map[kBlockContexts * (pred >> 1) + c], bs);
+        const int n = ans->Read(hist.count[c][pred >> 1], bs);
         nz[by][bx] = n;
         if (n > 64) {
           *err = "AC block with more than 64 non-zeros";
@@ -646,8 +722,9 @@ bool DecodeTile(const std::vector<uint8_t>& map, const int32_t* order,
             *err = "AC non-zeros exceed the positions left";
             return false;
           }
-          const int ctx = zd_base + ZeroDensityContext(left, k - 1);
-          const uint8_t sb = kTokenSkipBits[ans->Read(map[ctx], bs)];
+          bs->Fill();
+          const uint8_t sb =
+              kTokenSkipBits[ans->ReadFilled(hist.coef[c][left][k - 1], bs)];
           k += sb >> 4;
           const unsigned nbits = sb & 15;
           if (nbits == 0) continue;
@@ -655,11 +732,12 @@ bool DecodeTile(const std::vector<uint8_t>& map, const int32_t* order,
             *err = "AC non-zeros run past the block";
             return false;
           }
-          const uint32_t u = (1u << nbits) + bs->Read(nbits) - 1;
+          const uint32_t u = (1u << nbits) + bs->Peek(nbits) - 1;
+          bs->Skip(nbits);
           // Even u -> u/2, odd u -> -(u+1)/2.
           const int32_t v = int32_t((u >> 1) ^ (((~u) & 1) - 1));
           --left;
-          blk[ord[k]] = int16_t(v);
+          base[pos[ord[k]]] = int16_t(v);
         }
         if (left != 0) {
           *err = "AC block ends with non-zeros left";
@@ -669,42 +747,6 @@ bool DecodeTile(const std::vector<uint8_t>& map, const int32_t* order,
     }
   }
   return true;
-}
-
-// Tile coefficients -> the frame layout: a 1x1 transform's block as is; an
-// S x S-block transform's (8S)^2 coefficients row-major over its S rows of
-// S * 64 (the transmitted layout puts coefficient (y, x) in block
-// (y mod S, x mod S) at (y / S) * 8 + x / S; pik/dct_util.h:21-55).
-void StoreTile(const GroupJob& g, const int16_t* tile, int tx0, int ty0,
-               int txs, int tys) {
-  for (int by = 0; by < tys; ++by) {
-    for (int bx = 0; bx < txs; ++bx) {
-      const uint8_t a =
-          g.acs[size_t(ty0 + by) * g.acs_stride + (tx0 + bx)];
-      if (a & 15) continue;
-      const int s = Covered(a >> 4);
-      for (int c = 0; c < 3; ++c) {
-        const int16_t* src = tile + c * kTileBlocks * kTileBlocks * 64;
-        int16_t* dst = g.ac[c] + size_t(ty0 + by) * g.ac_stride +
-                       size_t(tx0 + bx) * 64;
-        if (s == 1) {
-          memcpy(dst, src + (by * kTileBlocks + bx) * 64,
-                 64 * sizeof(int16_t));
-          continue;
-        }
-        const int w = 8 * s;
-        for (int i = 0; i < s; ++i) {
-          int16_t* row = dst + size_t(i) * g.ac_stride;
-          for (int j = 0; j < s * 64; ++j) {
-            const int p = i * s * 64 + j;
-            const int y = p / w, x = p - (p / w) * w;
-            const int sb = (by + y % s) * kTileBlocks + (bx + x % s);
-            row[j] = src[sb * 64 + (y / s) * 8 + (x / s)];
-          }
-        }
-      }
-    }
-  }
 }
 
 bool DecodeGroup(GroupJob* g, std::string* err) {
@@ -736,7 +778,8 @@ bool DecodeGroup(GroupJob* g, std::string* err) {
     return false;
   }
   AnsReader ans(&tables);
-  std::vector<int16_t> tile(3 * kTileBlocks * kTileBlocks * 64);
+  const AcHistograms hist(map);
+  const Placement place(g->ac_stride);
   const int ntx = (g->bxs + kTileBlocks - 1) / kTileBlocks;
   const int nty = (g->bys + kTileBlocks - 1) / kTileBlocks;
   for (int ty = 0; ty < nty; ++ty) {
@@ -744,10 +787,10 @@ bool DecodeGroup(GroupJob* g, std::string* err) {
       const int x0 = tx * kTileBlocks, y0 = ty * kTileBlocks;
       const int txs = std::min(kTileBlocks, g->bxs - x0);
       const int tys = std::min(kTileBlocks, g->bys - y0);
-      if (!DecodeTile(map, order, &bs, &ans, txs, tys, tile.data(), err)) {
+      if (!DecodeTile(*g, hist, place, order, &bs, &ans, x0, y0, txs, tys,
+                      err)) {
         return false;
       }
-      StoreTile(*g, tile.data(), x0, y0, txs, tys);
     }
   }
   if (!ans.FinalStateOk()) {

@@ -123,7 +123,13 @@ class AnsReader {
  public:
   explicit AnsReader(const AnsTables* t) : slot_(t->slot.data()) {}
   inline int Read(size_t histogram, BitStream* bs) {
+    bs->Fill();
+    return ReadFilled(histogram, bs);
+  }
+  // With at least 32 bits in the buffer; at least 16 remain afterwards.
+  inline int ReadFilled(size_t histogram, BitStream* bs) {
     if (left_ == 0) {
+      // The state is (re)loaded every kAnsChunk symbols.
       const uint32_t hi = bs->Read(16);
       state_ = (hi << 16) | bs->Read(16);
       bs->Fill();
@@ -134,7 +140,6 @@ class AnsReader {
     state_ = (e >> 18) * (state_ >> kAnsLogTab) + ((e >> 8) & 1023);
     --left_;
     if (state_ < (1u << 16)) {
-      bs->Fill();
       state_ = (state_ << 16) | bs->Peek(16);
       bs->Skip(16);
     }
