@@ -201,6 +201,56 @@ def cpu_baseline(frame, seconds, threads, gpu_out=None):
             "avx2_vs_gpu": dev}
 
 
+def entropy_baseline(frame, threads, seconds=3.0):
+    """Host entropy decoding of the frame's AC (pikgpu_decode_ac_pass)
+    beside the reference decoder's own functions on the same bitstreams:
+    the frame's 512x512 groups written by the reference encoder
+    (EncodeToBitstream), decoded by the reference single-threaded
+    (DecodeNoise, DecodeCoeffOrder, DecodeHistograms, DecodeAC per tile,
+    GatherCoefficients; its pass loop runs groups on a thread pool) and by
+    this library on 1 and on `threads` host threads. Output buffers are
+    allocated once; each figure is the best of repeated runs."""
+    import numpy as np
+    from oracle import ref
+    from pik_amd import bitstream
+    if not ref.available("avx2"):
+        return None
+    groups = ref.encode_ac_groups(frame, kind="avx2")
+    mp = frame.xsize * frame.ysize / 1e6
+    by, bx = frame.ac_strategy.shape
+    out = np.zeros((3, by, bx * 64), np.int16)
+    want = ref.decode_ac_groups(frame, groups, check_dequant=False,
+                                kind="avx2", out=out)[0].copy()
+
+    def best(fn):
+        t_end = time.perf_counter() + seconds
+        b = 1e30
+        while True:
+            t = time.perf_counter()
+            fn()
+            b = min(b, time.perf_counter() - t)
+            if time.perf_counter() > t_end:
+                return b
+
+    t_ref = best(lambda: ref.decode_ac_groups(frame, groups, False, "avx2",
+                                              out))
+    figures = {}
+    for n in sorted({1, threads}):
+        t = best(lambda: bitstream.decode_ac_pass(
+            frame.xsize, frame.ysize, frame.ac_strategy, groups, threads=n,
+            out=out))
+        assert np.array_equal(out, want)
+        figures["ours_%dt" % n] = round(mp / t, 1)
+    return {"unit": "MP/s", "bytes": int(sum(len(g) for g in groups)),
+            "groups": len(groups), "reference_1t": round(mp / t_ref, 1),
+            **figures, "threads": threads,
+            "over_reference_1t": round(figures["ours_1t"] * t_ref / mp, 2),
+            "sample": "AC of the %dx%d frame, one pass, reference encoder "
+                      "bitstreams (full histogram clustering); best of "
+                      "%.0f s each; int16 output bit-equal to the "
+                      "reference's" % (frame.xsize, frame.ysize, seconds)}
+
+
 def srgb8_f64(linear):
     """8-bit sRGB of a linear image in [0, 255] by the IEC 61966-2-1
     transfer function in float64, clamped, rounded half up (the reference's
@@ -576,8 +626,12 @@ def main():
 
     cpu = None
     threads = a.cpu_threads or host_threads()
+    entropy = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(frame, a.cpu_seconds, threads, gpu_out)
+        entropy = entropy_baseline(frame, threads)
+        print("host entropy: " + json.dumps(entropy), file=sys.stderr,
+              flush=True)
     acs = frame.ac_strategy
     first = (acs & 15) == 0
     names = {0: "DCT8", 1: "IDENTITY", 2: "DCT2X2", 3: "DCT4X4", 4: "DCT16",
@@ -618,6 +672,7 @@ def main():
                    "transforms": int(first.sum())},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "host_entropy": entropy,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)

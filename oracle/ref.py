@@ -359,7 +359,8 @@ def encode_ac_groups(frame, noise=None, fast_mode=False, kind="scalar"):
     return out
 
 
-def decode_ac_groups(frame, groups, check_dequant=True, kind="scalar"):
+def decode_ac_groups(frame, groups, check_dequant=True, kind="scalar",
+                     out=None):
     """Decodes group bitstreams with the reference decoder's functions (and,
     with check_dequant, checks DecodeFromBitstream<true> agrees). Returns
     (ac int16 (3, by, bx*64), noise (groups, 3), consumed bytes per group)."""
@@ -372,7 +373,8 @@ def decode_ac_groups(frame, groups, check_dequant=True, kind="scalar"):
     by, bx = frame.ysize_blocks, frame.xsize_blocks
     data = np.frombuffer(b"".join(groups) + b"\0", np.uint8)
     sizes = np.array([len(g) for g in groups], np.uintp)
-    ac = np.zeros((3, by, bx * 64), np.int16)
+    ac = np.zeros((3, by, bx * 64), np.int16) if out is None else out
+    assert ac.shape == (3, by, bx * 64) and ac.dtype == np.int16
     ptrs = (vp * 3)(*[ac[c].ctypes.data for c in range(3)])
     noise = np.zeros((len(groups), 3), np.float32)
     used = np.zeros(len(groups), np.uintp)
