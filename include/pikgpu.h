@@ -554,6 +554,43 @@ typedef struct pikgpu_ac_pass {
 PIKGPU_API int pikgpu_decode_ac_pass(const pikgpu_ac_pass* pass, int threads,
                                      char* msg, size_t msg_size);
 
+/* ------------------------------------------------------------------------
+ * Host decoding of a frame's DC groups section into the quantized DC and
+ * the per-block side information that pikgpu_frame takes (dc, ac_strategy,
+ * quant_field, sigma_lut_ids).
+ *
+ * Replaces DecodeDCGroups (pik/compressed_dc.cc:436-528) up to DequantDC:
+ * the size table of the 256x256-block DC groups (SizeCoderT<0x150F0E0C>),
+ * then per group DecodeDCGroup (:276-345) -- the DC residual image
+ * (DecodeImage, pik/entropy_coder.cc:839-939), the adaptive DC predictor
+ * undone (ExpandDC, :156-197, pik/dc_predictor.cc), and under one ANS state
+ * DecodeAcStrategy, DecodeQuantField and DecodeARParameters
+ * (pik/entropy_coder.cc:944-1147). DequantDC and the DC smoothing run on
+ * the GPU (pikgpu_run). Host only; groups decode on `threads` host threads
+ * (<= 0: one per hardware thread). The old DC coder only (the default;
+ * use_new_dc is refused upstream), no multi-pass hints, no gradient map.
+ * `data` starts at the size table (byte aligned, after the colour maps);
+ * `consumed` receives the section's length in bytes. Returns PIKGPU_OK,
+ * PIKGPU_EINVAL or PIKGPU_EDATA (the lowest failing group in `msg`). */
+typedef struct pikgpu_dc_section {
+  uint32_t xsize, ysize;            /* frame size in pixels */
+  const uint8_t* data;
+  size_t size;
+  int16_t* dc[3];                   /* out: quantized DC, X Y B */
+  size_t dc_stride;
+  uint8_t* ac_strategy;             /* out: (type << 4) | block index */
+  size_t ac_strategy_stride;
+  int32_t* quant_field;             /* out */
+  size_t quant_field_stride;
+  uint8_t* sigma_lut_ids;           /* out */
+  size_t sigma_lut_stride;
+  size_t consumed;                  /* out */
+} pikgpu_dc_section;
+
+PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* section,
+                                       int threads, char* msg,
+                                       size_t msg_size);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -386,3 +386,53 @@ def decode_ac_groups(frame, groups, check_dequant=True, kind="scalar",
         raise RuntimeError("pikref_decode_ac_groups: %s"
                            % L.pikref_last_error().decode())
     return ac, noise, used.astype(np.int64)
+
+
+def encode_dc_groups(frame, kind="scalar"):
+    """The frame's DC groups section (size table + groups) as the reference
+    encoder writes it: quantized DC, AC strategy, quant field, sigma LUT
+    ids. Returns bytes."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_encode_dc_groups.argtypes = [ctypes.POINTER(abi.FrameC), vp,
+                                          ctypes.c_size_t, vp]
+    fc, keep = frame.to_c()
+    total = ctypes.c_size_t()
+    if L.pikref_encode_dc_groups(ctypes.byref(fc), None, 0,
+                                 ctypes.byref(total)) != 0:
+        raise RuntimeError("pikref_encode_dc_groups: %s"
+                           % L.pikref_last_error().decode())
+    buf = np.zeros(max(1, total.value), np.uint8)
+    if L.pikref_encode_dc_groups(ctypes.byref(fc), buf.ctypes.data, buf.size,
+                                 ctypes.byref(total)) != 0:
+        raise RuntimeError("pikref_encode_dc_groups: %s"
+                           % L.pikref_last_error().decode())
+    return buf[:total.value].tobytes()
+
+
+def decode_dc_groups(xsize, ysize, data, kind="scalar"):
+    """The reference decoder's reading of a DC groups section: dict(dc
+    (3, by, bx) int16, ac_strategy (by, bx) uint8, quant_field int32,
+    sigma_lut_ids uint8, consumed)."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_decode_dc_groups.argtypes = [
+        ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_size_t, vp, vp, vp, vp,
+        vp]
+    by, bx = -(-ysize // 8), -(-xsize // 8)
+    buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
+    dc = np.zeros((3, by, bx), np.int16)
+    ptrs = (vp * 3)(*[dc[c].ctypes.data for c in range(3)])
+    acs = np.zeros((by, bx), np.uint8)
+    qf = np.zeros((by, bx), np.int32)
+    lut = np.zeros((by, bx), np.uint8)
+    used = ctypes.c_size_t()
+    rc = L.pikref_decode_dc_groups(xsize, ysize, buf.ctypes.data, len(data),
+                                   ctypes.cast(ptrs, vp), acs.ctypes.data,
+                                   qf.ctypes.data, lut.ctypes.data,
+                                   ctypes.byref(used))
+    if rc != 0:
+        raise RuntimeError("pikref_decode_dc_groups: %s"
+                           % L.pikref_last_error().decode())
+    return dict(dc=dc, ac_strategy=acs, quant_field=qf, sigma_lut_ids=lut,
+                consumed=used.value)

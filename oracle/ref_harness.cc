@@ -77,6 +77,7 @@
 #include "pik/compressed_image.h"
 #include "pik/data_parallel.h"
 #include "pik/entropy_coder.h"
+#include "pik/fields.h"
 #include "pik/epf.h"
 #include "pik/gaborish.h"
 #include "pik/headers.h"
@@ -88,6 +89,7 @@
 #include "pik/pik_params.h"
 #include "pik/quantizer.h"
 #include "pik/simd/targets.h"
+#include "pik/write_bits.h"
 
 #include "../include/pikgpu.h"
 
@@ -1503,5 +1505,172 @@ API int pikref_decode_ac_groups(const pikgpu_frame* f, const uint8_t* data,
       }
     }
   }
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// DC groups section: EncodeDCGroups (pik/compressed_dc.cc:352-433) with
+// CompressDCGroup (:233-273) for the default old DC coder and no multi-pass
+// hints, and the decoding side of DecodeDCGroups (:436-507) with
+// DecodeDCGroup (:276-345) up to DequantDC. CompressDCGroup and
+// DecodeDCGroup sit in an unnamed namespace there and the public drivers
+// need a MultipassManager whose implementations pull in colour management,
+// so both are restated here over the reference's own public pieces
+// (ShrinkDC / ExpandDC, EncodeImageData / DecodeImage, the control-field
+// tokenizers and decoders, BuildAndEncodeHistograms / DecodeHistograms,
+// WriteTokens / ANSSymbolReader, SizeCoderT's U32Coder).
+
+namespace {
+constexpr uint32_t kDcGroupSizeDistribution = 0x150F0E0C;
+}  // namespace
+
+// The section (size table + groups) of the frame's quantized DC, strategy,
+// quant field and sigma LUT ids into `out` (may be NULL to size it).
+API int pikref_encode_dc_groups(const pikgpu_frame* f, uint8_t* out,
+                                size_t cap, size_t* total) {
+  if (f == nullptr) return Fail("null frame");
+  AcStrategyImage acs;
+  if (ImportAcStrategy(f, &acs) != 0) return -1;
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  Image3S dc(bx, by);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(dc.PlaneRow(c, y), f->dc[c] + y * f->dc_stride, bx * 2);
+    }
+  }
+  ImageI qf(bx, by);
+  ImageB luts(bx, by);
+  for (size_t y = 0; y < by; ++y) {
+    memcpy(qf.Row(y), f->quant_field + y * f->quant_field_stride, bx * 4);
+    memcpy(luts.Row(y), f->sigma_lut_ids + y * f->sigma_lut_stride, bx);
+  }
+  const size_t gxs = DivCeil<size_t>(bx, kDcGroupDimInBlocks);
+  const size_t gys = DivCeil<size_t>(by, kDcGroupDimInBlocks);
+  std::vector<std::string> codes;
+  for (size_t gy = 0; gy < gys; ++gy) {
+    for (size_t gx = 0; gx < gxs; ++gx) {
+      const Rect rect(gx * kDcGroupDimInBlocks, gy * kDcGroupDimInBlocks,
+                      kDcGroupDimInBlocks, kDcGroupDimInBlocks, bx, by);
+      Image3S residuals(rect.xsize(), rect.ysize());
+      ShrinkDC(rect, dc, &residuals);
+      const std::string dc_code =
+          EncodeImageData(Rect(residuals), residuals, nullptr);
+      std::vector<std::vector<Token>> tokens(1);
+      TokenizeAcStrategy(rect, acs, nullptr, &tokens[0]);
+      TokenizeQuantField(rect, qf, nullptr, acs, &tokens[0]);
+      TokenizeARParameters(rect, luts, acs, &tokens[0]);
+      std::vector<uint8_t> context_map;
+      std::vector<ANSEncodingData> codes_ans;
+      const std::string histo = BuildAndEncodeHistograms(
+          kNumControlFieldContexts, tokens, &codes_ans, &context_map, nullptr);
+      codes.push_back(dc_code + histo +
+                      WriteTokens(tokens[0], codes_ans, context_map, nullptr));
+    }
+  }
+  std::vector<uint8_t> toc(
+      DivCeil<size_t>(U32Coder::MaxEncodedBits(kDcGroupSizeDistribution) *
+                          codes.size(), 8) + 8, 0);
+  size_t pos = 0;
+  for (const std::string& c : codes) {
+    if (!U32Coder::Write(kDcGroupSizeDistribution, c.size(), &pos,
+                         toc.data())) {
+      return Fail("DC group too large for the size table");
+    }
+  }
+  WriteZeroesToByteBoundary(&pos, toc.data());
+  size_t n = pos / 8;
+  for (const std::string& c : codes) n += c.size();
+  if (total != nullptr) *total = n;
+  if (out == nullptr) return 0;
+  if (n > cap) return Fail("output buffer too small");
+  memcpy(out, toc.data(), pos / 8);
+  size_t at = pos / 8;
+  for (const std::string& c : codes) {
+    memcpy(out + at, c.data(), c.size());
+    at += c.size();
+  }
+  return 0;
+}
+
+// Decodes a DC groups section for an xsize x ysize frame into the
+// quantized DC (after ExpandDC), the combined strategy bytes
+// (AcStrategyImage's layers), the quant field and the sigma LUT ids;
+// *consumed receives the section's length.
+API int pikref_decode_dc_groups(uint32_t xsize, uint32_t ysize,
+                                const uint8_t* data, size_t size,
+                                int16_t* const* dc_out, uint8_t* acs_out,
+                                int32_t* qf_out, uint8_t* lut_out,
+                                size_t* consumed) {
+  const size_t bx = DivCeil<size_t>(xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(ysize, kBlockDim);
+  PaddedBytes compressed(size);
+  if (size != 0) memcpy(compressed.data(), data, size);
+  BitReader reader(compressed.data(), compressed.size());
+  const size_t gxs = DivCeil<size_t>(bx, kDcGroupDimInBlocks);
+  const size_t gys = DivCeil<size_t>(by, kDcGroupDimInBlocks);
+  const size_t ng = gxs * gys;
+  std::vector<size_t> offsets(1, 0);
+  for (size_t i = 0; i < ng; ++i) {
+    offsets.push_back(offsets.back() +
+                      U32Coder::Read(kDcGroupSizeDistribution, &reader));
+  }
+  if (!reader.JumpToByteBoundary()) return Fail("size table padding");
+  const size_t begin = reader.Position();
+  if (begin + offsets.back() > size) return Fail("groups past the section");
+  AcStrategyImage acs(bx, by);
+  ImageI qf(bx, by);
+  ZeroFillImage(&qf);
+  ImageB luts(bx, by);
+  ZeroFillImage(&luts);
+  GroupDecCache cache;
+  for (size_t gi = 0; gi < ng; ++gi) {
+    const size_t gx = gi % gxs, gy = gi / gxs;
+    const Rect rect(gx * kDcGroupDimInBlocks, gy * kDcGroupDimInBlocks,
+                    kDcGroupDimInBlocks, kDcGroupDimInBlocks, bx, by);
+    BitReader br(compressed.data(), begin + offsets[gi + 1]);
+    br.SkipBits((begin + offsets[gi]) * kBitsPerByte);
+    cache.InitDecodeDCGroup(rect.xsize(), rect.ysize());
+    if (!DecodeImage(&br, Rect(cache.quantized_dc), &cache.quantized_dc)) {
+      return Fail("DecodeImage failed");
+    }
+    ExpandDC(Rect(cache.quantized_dc), &cache.quantized_dc, &cache.dc_y,
+             &cache.dc_xz_residuals, &cache.dc_xz_expanded);
+    if (!br.JumpToByteBoundary()) return Fail("DC padding");
+    ANSCode code;
+    std::vector<uint8_t> context_map;
+    if (!DecodeHistograms(&br, kNumControlFieldContexts, 256, &code,
+                          &context_map)) {
+      return Fail("control field histograms");
+    }
+    if (!br.JumpToByteBoundary()) return Fail("histogram padding");
+    ANSSymbolReader decoder(&code);
+    if (!DecodeAcStrategy(&br, &decoder, context_map, &cache.ac_strategy_raw,
+                          rect, &acs, nullptr)) {
+      return Fail("DecodeAcStrategy failed");
+    }
+    if (!DecodeQuantField(&br, &decoder, context_map, rect, acs, &qf,
+                          nullptr)) {
+      return Fail("DecodeQuantField failed");
+    }
+    if (!DecodeARParameters(&br, &decoder, context_map, rect, acs, &luts)) {
+      return Fail("DecodeARParameters failed");
+    }
+    if (!decoder.CheckANSFinalState()) return Fail("ANS final state");
+    if (!br.JumpToByteBoundary()) return Fail("control field padding");
+    for (int c = 0; c < 3; ++c) {
+      for (size_t y = 0; y < rect.ysize(); ++y) {
+        memcpy(dc_out[c] + (rect.y0() + y) * bx + rect.x0(),
+               cache.quantized_dc.ConstPlaneRow(c, y),
+               rect.xsize() * sizeof(int16_t));
+      }
+    }
+  }
+  for (size_t y = 0; y < by; ++y) {
+    memcpy(acs_out + y * bx, acs.ConstRaw().ConstRow(y), bx);
+    memcpy(qf_out + y * bx, qf.ConstRow(y), bx * 4);
+    memcpy(lut_out + y * bx, luts.ConstRow(y), bx);
+  }
+  if (consumed != nullptr) *consumed = begin + offsets.back();
   return 0;
 }

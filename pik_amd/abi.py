@@ -181,6 +181,25 @@ class AcPassC(ctypes.Structure):
     ]
 
 
+class DcSectionC(ctypes.Structure):
+    """pikgpu_dc_section."""
+    _fields_ = [
+        ("xsize", ctypes.c_uint32),
+        ("ysize", ctypes.c_uint32),
+        ("data", ctypes.c_void_p),
+        ("size", ctypes.c_size_t),
+        ("dc", ctypes.c_void_p * 3),
+        ("dc_stride", ctypes.c_size_t),
+        ("ac_strategy", ctypes.c_void_p),
+        ("ac_strategy_stride", ctypes.c_size_t),
+        ("quant_field", ctypes.c_void_p),
+        ("quant_field_stride", ctypes.c_size_t),
+        ("sigma_lut_ids", ctypes.c_void_p),
+        ("sigma_lut_stride", ctypes.c_size_t),
+        ("consumed", ctypes.c_size_t),
+    ]
+
+
 def load():
     """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
     global _lib
@@ -247,6 +266,9 @@ def load():
     lib.pikgpu_fwd_adj.argtypes = [vp, ctypes.POINTER(AdjIoC)]
     lib.pikgpu_decode_ac_pass.argtypes = [ctypes.POINTER(AcPassC), ctypes.c_int,
                                           ctypes.c_char_p, ctypes.c_size_t]
+    lib.pikgpu_decode_dc_groups.argtypes = [
+        ctypes.POINTER(DcSectionC), ctypes.c_int, ctypes.c_char_p,
+        ctypes.c_size_t]
     _lib = lib
     return lib
 
@@ -263,4 +285,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
     "pikgpu_frame_add_pass", "pikgpu_reconstruct_noalloc",
     "pikgpu_search_quant_field", "pikgpu_decode_ac_pass",
+    "pikgpu_decode_dc_groups",
 ]

@@ -63,3 +63,39 @@ def decode_ac_pass(xsize, ysize, ac_strategy, groups, threads=0, out=None):
     if rc != abi.OK:
         raise PikGpuError(rc, msg.value.decode())
     return dict(ac=out, noise=noise, consumed=used.astype(np.int64))
+
+
+def decode_dc_groups(xsize, ysize, data, threads=0):
+    """Decodes a frame's DC groups section (size table + 256x256-block
+    groups): dict(dc=(3, by, bx) int16 quantized DC, ac_strategy=(by, bx)
+    uint8, quant_field=(by, bx) int32, sigma_lut_ids=(by, bx) uint8,
+    consumed=bytes of the section). Raises PikGpuError (abi.EDATA for an
+    invalid section)."""
+    lib = abi.load()
+    by, bx = -(-ysize // 8), -(-xsize // 8)
+    raw = np.frombuffer(bytes(data), np.uint8) if len(data) else \
+        np.zeros(1, np.uint8)
+    dc = np.zeros((3, by, bx), np.int16)
+    acs = np.zeros((by, bx), np.uint8)
+    qf = np.zeros((by, bx), np.int32)
+    lut = np.zeros((by, bx), np.uint8)
+    s = abi.DcSectionC()
+    s.xsize, s.ysize = xsize, ysize
+    s.data = raw.ctypes.data
+    s.size = len(data)
+    for c in range(3):
+        s.dc[c] = dc[c].ctypes.data
+    s.dc_stride = bx
+    s.ac_strategy = acs.ctypes.data
+    s.ac_strategy_stride = bx
+    s.quant_field = qf.ctypes.data
+    s.quant_field_stride = bx
+    s.sigma_lut_ids = lut.ctypes.data
+    s.sigma_lut_stride = bx
+    msg = ctypes.create_string_buffer(256)
+    rc = lib.pikgpu_decode_dc_groups(ctypes.byref(s), int(threads), msg,
+                                     len(msg))
+    if rc != abi.OK:
+        raise PikGpuError(rc, msg.value.decode())
+    return dict(dc=dc, ac_strategy=acs, quant_field=qf, sigma_lut_ids=lut,
+                consumed=int(s.consumed))

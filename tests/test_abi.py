@@ -85,6 +85,10 @@ int main(void) {
   F(pikgpu_ac_pass, ac_strategy) F(pikgpu_ac_pass, num_groups)
   F(pikgpu_ac_pass, group_size) F(pikgpu_ac_pass, ac)
   F(pikgpu_ac_pass, ac_stride) F(pikgpu_ac_pass, group_consumed)
+  printf("pikgpu_dc_section size %zu\n", sizeof(pikgpu_dc_section));
+  F(pikgpu_dc_section, data) F(pikgpu_dc_section, dc)
+  F(pikgpu_dc_section, ac_strategy) F(pikgpu_dc_section, quant_field_stride)
+  F(pikgpu_dc_section, sigma_lut_ids) F(pikgpu_dc_section, consumed)
   return 0;
 }
 """
@@ -102,7 +106,8 @@ def test_ctypes_layout_matches_header(tmp_path):
              "pikgpu_config": abi.ConfigC, "pikgpu_adj_io": abi.AdjIoC,
              "pikgpu_search_params": abi.SearchParamsC,
              "pikgpu_search_io": abi.SearchIoC,
-             "pikgpu_ac_pass": abi.AcPassC}
+             "pikgpu_ac_pass": abi.AcPassC,
+             "pikgpu_dc_section": abi.DcSectionC}
     for line in out.splitlines():
         t, m, v = line.split()
         if m == "size":
