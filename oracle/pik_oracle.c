@@ -2176,13 +2176,236 @@ static float adjust_quant_bias_int(int c, int q) {
   return q - 0.145f / q;
 }
 
+/* The encoder's side of the LF / HF predictions for one group
+ * (ComputeCoefficients' "enc predictions", pik/compressed_image.cc:373-388):
+ * PredictLfForEncoder (pik/ac_predictions.cc:477-557) with
+ * ComputeDecoderBlockAnd2x2DC (:136-199) and ComputeDecoderCoefficients
+ * (:73-113), then ComputePredictionResiduals (:637-650). `coef` holds the
+ * frame's float coefficients with the LLF already copied in; `cdc` / `llf`
+ * the group's DC cache and its LLF (cw x ch, one-block border). On return
+ * the LF positions of every transform hold the residual against the LF
+ * prediction and the HF positions the residual against the HF prediction. */
+static int encoder_predictions(const Ctx* k, int x0, int y0, int gw, int gh,
+                               float* const cdc[3], float* const llf[3],
+                               float* const coef[3]) {
+  static const float kZeroBias[3] = {0.65f, 0.6f, 0.7f};
+  static const float kLfBlur[3] = {(float)0.41459272584128337,
+                                   (float)0.25489157325704559,
+                                   (float)0.046449679523692139};
+  const pikgpu_frame* f = k->f;
+  const int cw = gw + 2, ch = gh + 2, w2 = 2 * cw, h2 = 2 * ch;
+  const size_t row = (size_t)k->bx * 64;
+  const float gsf = (float)(f->global_scale * (1.0 / 65536));
+  float* mem = (float*)calloc((size_t)9 * w2 * h2, sizeof(float));
+  if (!mem) return fail(PIKGPU_ENOMEM, "out of memory");
+  float *tmp2x2[3], *lf2x2[3], *dc2x2[3];
+  for (int c = 0; c < 3; ++c) {
+    tmp2x2[c] = mem + (size_t)(3 * c) * w2 * h2;
+    lf2x2[c] = tmp2x2[c] + (size_t)w2 * h2;
+    dc2x2[c] = lf2x2[c] + (size_t)w2 * h2;
+  }
+#define ENC_BORDER(x, y) ((x) == 0 || (y) == 0 || (x) == cw - 1 || (y) == ch - 1)
+  /* PredictLf, as the decoder forms it from the same LLF. */
+  if (f->predict_lf) {
+    for (int c = 0; c < 3; ++c) {
+      for (int y = 0; y < ch; ++y) {
+        for (int x = 0; x < cw; ++x) {
+          const int border = ENC_BORDER(x, y);
+          if (!border && !acs_first_at(k, x0 + x - 1, y0 + y - 1)) continue;
+          const int raw = border ? kDCT : acs_raw_at(k, x0 + x - 1, y0 + y - 1);
+          dc2x2_from_llf(raw, llf[c] + y * cw + x, cw,
+                         tmp2x2[c] + 2 * y * w2 + 2 * x, w2);
+        }
+      }
+      symmetric3(tmp2x2[c], w2, h2, kLfBlur, lf2x2[c]);
+      for (int y = 0; y < ch; ++y) {
+        for (int x = 0; x < cw; ++x) {
+          const int border = ENC_BORDER(x, y);
+          if (!border && !acs_first_at(k, x0 + x - 1, y0 + y - 1)) continue;
+          const int raw = border ? kDCT : acs_raw_at(k, x0 + x - 1, y0 + y - 1);
+          lf_from_dc2x2(raw, lf2x2[c] + 2 * y * w2 + 2 * x, w2);
+        }
+      }
+    }
+  }
+  /* Per block of the cache, Y then X then B: remove the LF prediction,
+   * quantize the block as the encoder will (the LF positions' values
+   * depend only on the low-frequency inputs) and dequantize its LF
+   * positions, add the prediction back and form the block's 2x2 DC. */
+  for (int yc = 0; yc < ch; ++yc) {
+    for (int xc = 0; xc < cw; ++xc) {
+      const int border = ENC_BORDER(xc, yc);
+      float ydec[64 * 16];
+      if (border) {
+        for (int ci = 0; ci < 3; ++ci) {
+          const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+          float dcf[64] = {0};
+          dcf[0] = cdc[c][yc * cw + xc];
+          if (f->predict_lf) {
+            dcf[1] = lf2x2[c][2 * yc * w2 + 2 * xc + 1];
+            dcf[8] = lf2x2[c][(2 * yc + 1) * w2 + 2 * xc];
+            dcf[9] = lf2x2[c][(2 * yc + 1) * w2 + 2 * xc + 1];
+          }
+          if (f->predict_hf) {
+            dc2x2_from_lf(kDCT, dcf, dc2x2[c] + 2 * yc * w2 + 2 * xc, w2);
+          }
+        }
+        continue;
+      }
+      const int x = x0 + xc - 1, y = y0 + yc - 1;
+      if (!acs_first_at(k, x, y)) continue;
+      const int raw = acs_raw_at(k, x, y), s = covered(raw), n = 64 * s * s;
+      const int kind = quant_kind(raw);
+      const int quant = f->quant_field[y * f->quant_field_stride + x];
+      const int tx = x / 8, ty = y / 8;
+      const int cf = f->dequant_cf[ty * f->dequant_cf_stride + tx];
+      if (cf * 256 + quant >= 16 * 256) {
+        free(mem);
+        return fail(PIKGPU_EUNSUPPORTED, "quant value past the dequant map");
+      }
+      const int table = f->dequant_map[cf * 256 + quant];
+      const float qac = gsf * quant;
+      const float inv_qac = k->inv_global_scale / quant;
+      const float* lp = lf2x2[0];
+      for (int ci = 0; ci < 3; ++ci) {
+        const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+        const float fac =
+            c == 0 ? 1.0f * (f->ytox_map[ty * f->ytox_stride + tx] - 128) *
+                         (1.0f / 256)
+                   : 1.0f * (f->ytob_map[ty * f->ytob_stride + tx] - 0) *
+                         (1.0f / 128);
+        const float* dm = dequant_matrix(k, table, kind, c);
+        float inv_m[64 * 16], dcf[64 * 16];
+        int16_t q[64 * 16];
+        /* the transform's first storage row holds its 2s x 2s corner */
+        float* blk = coef[c] + (size_t)y * row + (size_t)x * 64;
+        lp = lf2x2[c] + 2 * yc * w2 + 2 * xc;
+        if (f->predict_lf) {
+          for (int i = 0; i < 2 * s; ++i) {
+            for (int j = i < s ? s : 0; j < 2 * s; ++j) {
+              blk[i * 8 * s + j] -= lp[i * w2 + j];
+            }
+          }
+        }
+        for (int r = 0; r < s; ++r) {
+          memcpy(dcf + r * 64 * s, coef[c] + (size_t)(y + r) * row +
+                                       (size_t)x * 64,
+                 sizeof(float) * 64 * s);
+        }
+        if (c != 1) {
+          for (int i = 0; i < n; ++i) dcf[i] -= ydec[i] * fac;
+        }
+        for (int i = 0; i < n; ++i) inv_m[i] = 1.0f / dm[i];
+        for (int iy = 0; iy < s; ++iy) {
+          for (int ix = 0; ix < s; ++ix) {
+            quantize_sub_block(dcf, inv_m, qac, kZeroBias[c], s, s, ix, iy, q);
+          }
+        }
+        /* QuantizeRoundtripBlockCoefficients (pik/quantizer.h:317-348) of
+         * positions 1, 8, 9 of every interleaved 8x8 sub-block */
+        for (int iy = 0; iy < s; ++iy) {
+          for (int ix = 0; ix < s; ++ix) {
+            static const int kLf[3] = {1, 8, 9};
+            for (int m = 0; m < 3; ++m) {
+              const int kx = kLf[m] % 8, ky = kLf[m] / 8;
+              const int pos = (s * ky + iy) * 8 * s + s * kx + ix;
+              dcf[pos] = adjust_quant_bias_int(c, q[pos]) * dm[pos] * inv_qac;
+            }
+          }
+        }
+        if (c != 1) {
+          for (int i = 0; i < n; ++i) dcf[i] += ydec[i] * fac;
+        } else {
+          memcpy(ydec, dcf, sizeof(float) * n);
+        }
+        if (f->predict_lf) {
+          for (int i = 0; i < 2 * s; ++i) {
+            const int start = i < s ? s : 0;
+            for (int j = 0; j < start; ++j) dcf[i * 8 * s + j] = blk[i * 8 * s + j];
+            for (int j = start; j < 2 * s; ++j) dcf[i * 8 * s + j] += lp[i * w2 + j];
+          }
+        }
+        if (f->predict_hf) {
+          dc2x2_from_lf(raw, dcf, dc2x2[c] + 2 * yc * w2 + 2 * xc, w2);
+        }
+      }
+    }
+  }
+#undef ENC_BORDER
+  /* ComputePredictionResiduals: UpSample4x4BlurDCT<false>
+   * (pik/ac_predictions.cc:263-374): the same prediction the decoder adds,
+   * subtracted outside each transform's 2s x 2s corner. */
+  if (f->predict_hf) {
+    float kern[3][4];
+    ub4_kernel(kern);
+    const int xs = 2 * gw, ys = 2 * gh;
+    float* blur_x = (float*)malloc(sizeof(float) * (size_t)xs * 4 * (ys + 2));
+    if (!blur_x) {
+      free(mem);
+      return fail(PIKGPU_ENOMEM, "out of memory");
+    }
+    for (int c = 0; c < 3; ++c) {
+      for (int y = 0; y < ys + 2; ++y) {
+        const float* r = dc2x2[c] + (y + 1) * w2;
+        float* o = blur_x + (size_t)y * xs * 4;
+        for (int x = 0; x < xs; ++x) {
+          const float v0 = r[x + 1], v1 = r[x + 2], v2 = r[x + 3];
+          for (int ix = 0; ix < 4; ++ix) {
+            o[4 * x + ix] = v0 * kern[0][ix] + v1 * kern[1][ix] + v2 * kern[2][ix];
+          }
+        }
+      }
+      for (int by = 0; by < gh; ++by) {
+        for (int bx = 0; bx < gw; ++bx) {
+          if (!acs_first_at(k, x0 + bx, y0 + by)) continue;
+          const int raw = acs_raw_at(k, x0 + bx, y0 + by);
+          if (!predicts_hf(raw)) continue;
+          const int s = covered(raw), n = 8 * s;
+          float block[32 * 32], coeffs[32 * 32];
+          for (int idy = 0; idy < s; ++idy) {
+            const float* r0 = blur_x + (size_t)2 * (by + idy) * xs * 4;
+            const float* r1 = r0 + xs * 4;
+            const float* r2 = r1 + xs * 4;
+            const float* r3 = r2 + xs * 4;
+            for (int idx = 0; idx < s; ++idx) {
+              for (int ix = 0; ix < 8; ++ix) {
+                const int col = (bx + idx) * 8 + ix;
+                const float v0 = r0[col], v1 = r1[col], v2 = r2[col],
+                            v3 = r3[col];
+                for (int iy = 0; iy < 4; ++iy) {
+                  const float va =
+                      v0 * kern[0][iy] + v1 * kern[1][iy] + v2 * kern[2][iy];
+                  const float vb =
+                      v1 * kern[0][iy] + v2 * kern[1][iy] + v3 * kern[2][iy];
+                  block[(idy * 8 + iy) * n + idx * 8 + ix] = va;
+                  block[(idy * 8 + 4 + iy) * n + idx * 8 + ix] = vb;
+                }
+              }
+            }
+          }
+          from_pixels(raw, block, n, coeffs, 64 * s);
+          float* a = coef[c] + (size_t)(y0 + by) * row + (size_t)(x0 + bx) * 64;
+          for (int i = 0; i < n * n; ++i) {
+            const int fi = i / n, fj = i % n;
+            if (fi < 2 * s && fj < 2 * s) continue;
+            a[(size_t)(i / (64 * s)) * row + i % (64 * s)] -= coeffs[i];
+          }
+        }
+      }
+    }
+    free(blur_x);
+  }
+  free(mem);
+  return 0;
+}
+
 int pikoracle_quantize(const pikgpu_frame* f, const float* const* opsin,
                        int16_t* const* ac_out, int16_t* const* dc_out) {
   init_tables();
   if (!f || !opsin) return fail(PIKGPU_EINVAL, "null argument");
-  if (f->predict_lf || f->predict_hf || f->num_dots || is_gray(f)) {
+  if (f->num_dots || is_gray(f)) {
     return fail(PIKGPU_EUNSUPPORTED,
-                "quantization of frames with predictions, dots or grayscale");
+                "quantization of frames with dots or grayscale");
   }
   pikgpu_frame g = *f;
   const int bx = (int)div_ceil(f->xsize, 8), by = (int)div_ceil(f->ysize, 8);
@@ -2279,32 +2502,50 @@ int pikoracle_quantize(const pikgpu_frame* f, const float* const* opsin,
         const int gw = bx - x0 < kGroupBlocks ? bx - x0 : kGroupBlocks;
         const int gh = by - y0 < kGroupBlocks ? by - y0 : kGroupBlocks;
         const int cw = gw + 2, ch = gh + 2;
-        float* cdc = (float*)malloc(sizeof(float) * cw * ch);
-        float llf[16];
-        if (!cdc) {
+        float* cmem = (float*)calloc((size_t)6 * cw * ch, sizeof(float));
+        if (!cmem) {
           rc = fail(PIKGPU_ENOMEM, "out of memory");
           goto done;
         }
+        float *cdc[3], *cllf[3];
         for (int c = 0; c < 3; ++c) {
+          cdc[c] = cmem + (size_t)c * cw * ch;
+          cllf[c] = cmem + (size_t)(3 + c) * cw * ch;
           for (int y = 0; y < ch; ++y) {
             const size_t sy = source_coord(y + y0, by);
             for (int x = 0; x < cw; ++x) {
-              cdc[y * cw + x] = dc[c][sy * bx + source_coord(x + x0, bx)];
+              cdc[c][y * cw + x] = dc[c][sy * bx + source_coord(x + x0, bx)];
+            }
+          }
+          /* ComputeLlf over the cache (border positions are DCT8) */
+          for (int y = 0; y < ch; ++y) {
+            for (int x = 0; x < cw; ++x) {
+              const int border = x == 0 || y == 0 || x == cw - 1 || y == ch - 1;
+              if (!border && !acs_first_at(&k, x0 + x - 1, y0 + y - 1)) continue;
+              const int raw =
+                  border ? kDCT : acs_raw_at(&k, x0 + x - 1, y0 + y - 1);
+              llf_from_dc(raw, cdc[c] + y * cw + x, cw, cllf[c] + y * cw + x,
+                          cw);
             }
           }
           for (int y = 0; y < gh; ++y) {
             for (int x = 0; x < gw; ++x) {
               if (!acs_first_at(&k, x0 + x, y0 + y)) continue;
               const int raw = acs_raw_at(&k, x0 + x, y0 + y), s = covered(raw);
-              llf_from_dc(raw, cdc + (y + 1) * cw + x + 1, cw, llf, 4);
               float* cb = coef[c] + (y0 + y) * row + (size_t)(x0 + x) * 64;
               for (int iy = 0; iy < s; ++iy) {
-                for (int ix = 0; ix < s; ++ix) cb[s * 8 * iy + ix] = llf[iy * 4 + ix];
+                for (int ix = 0; ix < s; ++ix) {
+                  cb[s * 8 * iy + ix] = cllf[c][(y + 1 + iy) * cw + x + 1 + ix];
+                }
               }
             }
           }
         }
-        free(cdc);
+        if (f->predict_lf || f->predict_hf) {
+          rc = encoder_predictions(&k, x0, y0, gw, gh, cdc, cllf, coef);
+        }
+        free(cmem);
+        if (rc) goto done;
       }
     }
     /* QuantizeRoundtripBlockAC of Y, UnapplyColorCorrelationAC, then

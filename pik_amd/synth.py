@@ -574,7 +574,7 @@ def make_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
 
 
 def make_opsin_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
-                     line_art=False, predict=True, lut_ids="zero"):
+                     line_art=False, predict=True, lut_ids="zero", lines=0.0):
     """A frame's side information together with the XYB image it was made
     from, padded to whole blocks by edge replication as PadImageToMultiple
     does (pik/image.cc:45-80): (frame, opsin (3, by * 8, bx * 8) float32).
@@ -582,7 +582,7 @@ def make_opsin_frame(xsize, ysize, seed=0, distance=1.0, num_rects=20,
     rgb = synthetic_srgb(xsize, ysize, seed, num_rects=num_rects,
                          line_art=line_art)
     frame = encode_frame(rgb, distance=distance, seed=seed, predict=predict,
-                         lut_ids=lut_ids)
+                         lut_ids=lut_ids, lines=lines)
     xyb = linear_to_xyb(srgb8_to_linear255(rgb))
     bx, by = div_ceil(xsize, 8), div_ceil(ysize, 8)
     xyb = np.pad(xyb, ((0, 0), (0, by * 8 - ysize), (0, bx * 8 - xsize)),
