@@ -284,7 +284,10 @@ PIKGPU_API int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3]
  * frame's (later passes are dropped), so a following pikgpu_run decodes the
  * round trip; ac_out / dc_out (host, may be NULL) also receive them in the
  * layouts of pikgpu_frame.ac / .dc (rows of ac_stride / dc_stride int16).
- * Frames with LF / HF prediction, dots or the grayscale flag return
+ * With the frame's predict_lf / predict_hf flags the coefficients carry the
+ * LF / HF prediction residuals as the reference encoder forms them
+ * (PredictLfForEncoder, ComputePredictionResiduals, pik/ac_predictions.cc:
+ * 477-557, 637-650). Frames with dots or the grayscale flag return
  * PIKGPU_EUNSUPPORTED, as does a quant value the encoder's dequant-map
  * lookup (indexed by the quant value itself, :417, 445) would read past the
  * map with. */

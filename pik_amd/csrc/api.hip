@@ -60,7 +60,7 @@ struct pikgpu_ctx {
   // pikgpu_fwd_adj staging (host-pointer calls) and loss partials
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
   // pikgpu_quantize: XYB input, coefficients, DC, the inverse dequant table
-  DevBuf eopsin, ecoef, edcf, itable;
+  DevBuf eopsin, ecoef, edcf, itable, ecorner;
   // pikgpu_search_quant_field: log field, Adam moments, weights, per-block
   // distortion, rate partials, the rate weight, loss / rate histories
   DevBuf srch;
@@ -133,6 +133,7 @@ void FreeAll(pikgpu_ctx* ctx) {
                    &ctx->ar_keep, &ctx->gimg,   &ctx->predlist, &ctx->adj_qf, &ctx->adj_in, &ctx->adj_lin,
                    &ctx->adj_dqf, &ctx->adj_dc, &ctx->adj_loss,
                    &ctx->eopsin, &ctx->ecoef, &ctx->edcf, &ctx->itable,
+                   &ctx->ecorner,
                    &ctx->srch};
   for (DevBuf* b : all) {
     if (b->p) (void)hipFree(b->p);
@@ -853,14 +854,15 @@ static void EnqueueLf(const FrameParams& P, const FrameBuffers& B,
   if (hf) {
     // (positions of pred2x2 no block writes were zeroed at upload)
     if (P.predict_lf) {
-      KPred2x2Border<<<gc, 256, 0, st>>>(P, B, K);
+      KPred2x2Border<true><<<gc, 256, 0, st>>>(P, B, K);
     } else {
       KDc2x2Border<<<gc, 256, 0, st>>>(P, B, K);
     }
     if (P.pred_end[3] > 0) {
       // segments as above, then one lane per unit
       const int wg1 = int(DivCeil(3 * P.pred_end[1], 256));
-      KPred2x2Interior<<<wg4 + wg2 + wg1, 256, 0, st>>>(P, B, K, wg4, wg2);
+      KPred2x2Interior<false><<<wg4 + wg2 + wg1, 256, 0, st>>>(P, B, K, wg4,
+                                                               wg2, nullptr);
     }
   }
 }
@@ -1162,9 +1164,9 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
   if (opsin_stride < size_t(P.px)) {
     return Fail(ctx, PIKGPU_EINVAL, "opsin stride below the padded width");
   }
-  if (P.predict_lf || P.predict_hf || P.num_dots != 0 || P.grayscale) {
+  if (P.num_dots != 0 || P.grayscale) {
     return Fail(ctx, PIKGPU_EUNSUPPORTED,
-                "quantization of frames with predictions, dots or grayscale");
+                "quantization of frames with dots or grayscale");
   }
   if (!ctx->enc_map_ok) {
     return Fail(ctx, PIKGPU_EUNSUPPORTED, "quant value past the dequant map");
@@ -1181,6 +1183,8 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
   if ((rc = Grow(ctx, &ctx->eopsin, 3 * plane * 4))) return rc;
   if ((rc = Grow(ctx, &ctx->ecoef, 3 * nb * 64 * 4))) return rc;
   if ((rc = Grow(ctx, &ctx->edcf, 3 * nb * 4))) return rc;
+  const bool predict = P.predict_lf || P.predict_hf;
+  if (predict && (rc = Grow(ctx, &ctx->ecorner, 3 * nb * 64 * 4))) return rc;
   float* dev_opsin = static_cast<float*>(ctx->eopsin.p);
   for (int c = 0; c < 3; ++c) {
     if ((rc = Upload2D(ctx, dev_opsin + c * plane, opsin[c], opsin_stride * 4,
@@ -1211,19 +1215,39 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
     S.scale[c] = inv0 * qdc;
   }
   KEncDc<<<DivCeil(nb, 256), 256, 0, st>>>(P, dcf, S, dc16);
-  // dc_dec and the LLF exactly as the decoder computes them
+  // dc_dec, the LLF and the LF prediction exactly as the decoder computes
+  // them (the encoder calls the same ComputeLlf and PredictLf)
   EnqueueDc(P, B, K, st);
-  {
-    const size_t ngroups = size_t(P.gx) * P.gy;
-    const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
-    dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
-    const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
-    const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
-    KComputeLlf<<<wg4 + wg2 + gc.x * gc.y * gc.z, 256, 0, st>>>(
-        P, B, K, 0, wg4, wg2, int(gc.x), int(ngroups));
+  EnqueueLf(P, B, K, st, false);
+  const float* itable = static_cast<const float*>(ctx->itable.p);
+  if (predict) {
+    float* corner = static_cast<float*>(ctx->ecorner.p);
+    KEncLf<<<DivCeil(nb, 256), 256, 0, st>>>(P, B, coef, itable, gsf,
+                                             corner);
+    if (P.predict_hf) {
+      // the encoder's pred2x2: the ring as the decoder forms it, the
+      // transforms from KEncLf's corners (positions no block writes were
+      // zeroed at upload)
+      const size_t ngroups = size_t(P.gx) * P.gy;
+      const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
+      dim3 gc(DivCeil(cache_pos, 256), ngroups, 3);
+      if (P.predict_lf) {
+        KPred2x2Border<true><<<gc, 256, 0, st>>>(P, B, K);
+      } else {
+        KPred2x2Border<false><<<gc, 256, 0, st>>>(P, B, K);
+      }
+      if (P.pred_end[3] > 0) {
+        const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
+        const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+        const int wg1 = int(DivCeil(3 * P.pred_end[1], 256));
+        KPred2x2Interior<true><<<wg4 + wg2 + wg1, 256, 0, st>>>(
+            P, B, K, wg4, wg2, corner);
+      }
+      KEncHfResidual<<<XcdGrid(tiles), 256, 0, st>>>(P, B, coef);
+    }
   }
   KEncQuant<<<DivCeil(nb, kEncQuantThreads), kEncQuantThreads, 0, st>>>(
-      P, B, coef, static_cast<const float*>(ctx->itable.p), gsf, ac16);
+      P, B, coef, itable, gsf, ac16);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->enc_ev[1], st));
   for (int c = 0; c < 3; ++c) {

@@ -2,8 +2,8 @@
 // information: the coefficient half of the reference's RoundtripImage
 // (pik/adaptive_quantization.cc:580-651), i.e. InitializeFrameEncCache
 // (pik/compressed_image.cc:210-304) and ComputeCoefficients (:351-455), for
-// frames without LF / HF prediction. Its int16 AC and DC land in the
-// context's coefficient buffers, so the decode that follows is the round
+// frames with and without LF / HF prediction. Its int16 AC and DC land in
+// the context's coefficient buffers, so the decode that follows is the round
 // trip.
 //
 //   KEncForward  per 32x32 tile: TransformFromPixels of every transform
@@ -11,8 +11,13 @@
 //                and DCFromLowestFrequencies;
 //   KEncDc       per block: QuantizeRoundtripDC of Y, the DC colour
 //                correlation removed from X and B, QuantizeCoeffsDC;
-//   (the decoder's DequantDC + AdaptiveDCReconstruction and ComputeLlf
-//    then give dc_dec and the LLF that CopyLlf puts into the coefficients)
+//   (the decoder's DequantDC + AdaptiveDCReconstruction, ComputeLlf and
+//    PredictLf then give dc_dec, the LLF that CopyLlf puts into the
+//    coefficients and the LF prediction)
+//   KEncLf       per block: LF residuals and the decoder-side LF corner
+//                (PredictLfForEncoder), whose 2x DC image KPred2x2Interior
+//                forms; KEncHfResidual per tile: the HF residuals
+//                (ComputePredictionResiduals);
 //   KEncQuant    per block: one 8x8 dither sub-block of its transform, Y
 //                quantized and dequantised (QuantizeRoundtripBlockAC), X and
 //                B with Y's colour correlation removed
@@ -256,6 +261,193 @@ __global__ void __launch_bounds__(kEncQuantThreads)
       PIKGPU_ENC_STEP(3) PIKGPU_ENC_STEP(4) PIKGPU_ENC_STEP(5)
       PIKGPU_ENC_STEP(6) PIKGPU_ENC_STEP(7)
 #undef PIKGPU_ENC_STEP
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prediction residuals (ComputeCoefficients' "enc predictions",
+// pik/compressed_image.cc:373-388), between the LLF / LF prediction the
+// decoder's kernels form from dc_dec (the same ComputeLlf and PredictLf the
+// encoder calls) and KEncQuant.
+
+// PredictLfForEncoder's per-block step (ComputeDecoderBlockAnd2x2DC,
+// pik/ac_predictions.cc:136-199, with ComputeDecoderCoefficients, :73-113)
+// for one 8x8 dither sub-block (ix, iy) of a transform, one thread per
+// storage block: channel order Y, X, B. With predict_lf the LF positions of
+// `coef` become residuals against the LF prediction. The sub-block's LF
+// positions k = 1, 8, 9 are quantized as QuantizeBlockAC will quantize them
+// (their rounding errors depend only on k = 0, 1, 8, 9, so the rest of the
+// block, still without its HF residual here, does not matter) and
+// dequantised with Y's colour correlation; with the LF prediction added
+// back (and the LLF position restored) they are the decoder-side corner
+// values that DC2x2FromLowFrequencies reads. They go to `corner` (the
+// coefficient layout), for KPred2x2Interior<true>.
+__global__ void __launch_bounds__(256)
+    KEncLf(FrameParams P, FrameBuffers B, float* coef, const float* itable,
+           float gsf, float* corner) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.bx * P.by) return;
+  const int y = i / P.bx, x = i - y * P.bx;
+  const uint8_t a = B.acs[i];
+  const int raw = a >> 4, idx = a & 15, s = AcsCovered(raw);
+  const int iy = idx / s, ix = idx - iy * s;
+  const int x0 = x - ix, y0 = y - iy;
+  const int quant = B.qf[size_t(y0) * P.bx + x0];
+  const int cf = B.dcf[(y0 >> 3) * P.tx + (x0 >> 3)];
+  const int table = B.dqmap[cf * 256 + quant];
+  const int kind = AcsQuantKind(raw);
+  const float qac = gsf * quant;
+  const float inv_qac = P.inv_global_scale / quant;
+  const size_t nb = size_t(P.bx) * P.by;
+  const size_t base = size_t(y0) * P.bx * 64 + size_t(x0) * 64;
+  const int g = (y0 / kGroupDimBlocks) * P.gx + x0 / kGroupDimBlocks;
+  const int lx0 = x0 % kGroupDimBlocks + 1, ly0 = y0 % kGroupDimBlocks + 1;
+  const int ctile = (y0 >> 3) * P.tx + (x0 >> 3);
+  constexpr float kZeroBias[3] = {0.65f, 0.6f, 0.7f};
+  // F-matrix coordinates of the sub-block's LLF (k = 0) and LF positions
+  // (k = 1, 8, 9): F[s*(k/8) + iy][s*(k%8) + ix]
+  const int fi[4] = {iy, iy, s + iy, s + iy};
+  const int fj[4] = {ix, s + ix, ix, s + ix};
+  float ydec[4];
+#pragma unroll 1
+  for (int ci = 0; ci < 3; ++ci) {
+    const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+    const uint32_t off = B.offsets[(table * kNumQuantKinds + kind) * 3 + c];
+    const float* dm = B.table + off;
+    const float* qm = itable + off;
+    const float fac =
+        c == 0 ? 1.0f * float(B.ytox[ctile] - 128) * (1.0f / 256)
+               : 1.0f * float(B.ytob[ctile] - 0) * (1.0f / 128);
+    const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane +
+                      2 * ly0 * kG2Pitch + 2 * lx0;
+    const float llf =
+        B.llf[(size_t(g) * 3 + c) * kLlfPlane + (ly0 + iy) * kLlfPitch + lx0 + ix];
+    float* cp = coef + c * nb * 64 + base;
+    float dec[4];
+    double err = 0, prev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int pos = fi[m] * 8 * s + fj[m];
+      float v = m == 0 ? llf : cp[pos];
+      if (m > 0 && P.predict_lf) {
+        v -= lf[fi[m] * kG2Pitch + fj[m]];
+        cp[pos] = v;  // the residual the block's quantization sees
+      }
+      if (c != 1) v -= ydec[m] * fac;
+      const float val = v * (qm[pos] * qac);
+      // k = 0, 1 are steps 0 and 1 of the first row, k = 8, 9 those of the
+      // second (the steps between leave err and prev[0..1] as found)
+      int16_t q;
+      if (m == 0) {
+        q = DitherStep<0>(val, kZeroBias[c], true, err, prev);
+      } else if (m == 1) {
+        q = DitherStep<1>(val, kZeroBias[c], false, err, prev);
+      } else if (m == 2) {
+        q = DitherStep<0>(val, kZeroBias[c], false, err, prev);
+      } else {
+        q = DitherStep<1>(val, kZeroBias[c], false, err, prev);
+      }
+      if (m == 0) {
+        // the LLF position keeps its value (through the colour correlation
+        // taken out and put back for X and B)
+        dec[0] = c != 1 ? v + ydec[0] * fac : v;
+      } else {
+        const float d = AdjustQuantBiasInt(c, q) * dm[pos] * inv_qac;
+        dec[m] = c != 1 ? d + ydec[m] * fac : d;
+      }
+    }
+    if (c == 1) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) ydec[m] = dec[m];
+    }
+    float* out = corner + c * nb * 64 + base;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float v = dec[m];
+      if (P.predict_lf) v = m == 0 ? llf : v + lf[fi[m] * kG2Pitch + fj[m]];
+      out[fi[m] * 8 * s + fj[m]] = v;
+    }
+  }
+}
+
+// ComputePredictionResiduals (pik/ac_predictions.cc:637-650): per 32x32
+// tile, the HF prediction the decoder adds (UpSample4x4BlurDCT, the blur of
+// the tile's pred2x2 window and its forward transform, as in KBlocks)
+// subtracted from the coefficients outside each transform's 2s x 2s
+// corner.
+__global__ void __launch_bounds__(256)
+    KEncHfResidual(FrameParams P, FrameBuffers B, float* coef) {
+  __shared__ float s_coef[3][kCoefPlane];
+  __shared__ float s_pred[3][kTilePlane];
+  __shared__ float s_tmp[3][kTilePlane];
+  __shared__ uint8_t s_acs[16];
+  __shared__ TileTasks s_fwd;
+  const int tid = threadIdx.x;
+  const int tiles_x = (P.bx + 3) / 4;
+  const int tile = XcdTile(blockIdx.x, tiles_x * ((P.by + 3) / 4));
+  if (tile < 0) return;
+  const int tbx = (tile % tiles_x) * 4, tby = (tile / tiles_x) * 4;
+  float (*s_p2)[100] = reinterpret_cast<float (*)[100]>(&s_tmp[0][0]);
+  const int gxi = tbx / kGroupDimBlocks, gyi = tby / kGroupDimBlocks;
+  const int g = gyi * P.gx + gxi;
+  const int gbx0 = gxi * kGroupDimBlocks, gby0 = gyi * kGroupDimBlocks;
+  const int p2r0 = 2 * (tby - gby0) + 1, p2c0 = 2 * (tbx - gbx0) + 1;
+  const size_t nb = size_t(P.bx) * P.by, row = size_t(P.bx) * 64;
+  if (tid < 16) {
+    const int bxa = tbx + (tid & 3), bya = tby + (tid >> 2);
+    s_acs[tid] = (bxa < P.bx && bya < P.by) ? B.acs[bya * P.bx + bxa]
+                                            : uint8_t(0xFF);
+  }
+  for (int k = tid; k < 300; k += 256) {
+    const int c = k / 100, i = (k % 100) / 10, j = k % 10;
+    s_p2[c][k % 100] = B.pred2x2[(size_t(g) * 3 + c) * kG2Plane +
+                                 (p2r0 + i) * kG2Pitch + p2c0 + j];
+  }
+  for (int k = tid; k < 3 * 16 * 64; k += 256) {
+    const int c = k >> 10, blk = (k >> 6) & 15, e = k & 63;
+    const int r = blk >> 2, b = blk & 3;
+    const int yy = tby + r, xx = tbx + b;
+    if (xx < P.bx && yy < P.by) {
+      s_coef[c][CoefAt(r, b * 64 + e)] =
+          coef[c * nb * 64 + size_t(yy) * row + size_t(xx) * 64 + e];
+    }
+  }
+  __syncthreads();
+  BuildTasks(s_acs, true, &s_fwd);
+  const int pr = tid >> 3, pc0 = (tid & 7) << 2;
+  const uint8_t pa = s_acs[(pr >> 3) * 4 + (pc0 >> 3)];
+  if (pa != 0xFF && AcsPredictHF(pa >> 4)) {
+    const int py = pr & 7, iy = py & 3;
+    const int wr = 2 * (pr >> 3) + (py < 4 ? 0 : 1);
+    const int wc = pc0 >> 2;
+    for (int c = 0; c < 3; ++c) {
+      float rw[3][3];
+      for (int t = 0; t < 3; ++t) {
+        for (int u = 0; u < 3; ++u) rw[t][u] = s_p2[c][(wr + t) * 10 + wc + u];
+      }
+      for (int j = 0; j < 4; ++j) {
+        float v[3];
+        for (int t = 0; t < 3; ++t) {
+          v[t] = rw[t][0] * P.ub4[0][j] + rw[t][1] * P.ub4[1][j] +
+                 rw[t][2] * P.ub4[2][j];
+        }
+        s_pred[c][PixAt(pr, pc0 + j)] =
+            v[0] * P.ub4[0][iy] + v[1] * P.ub4[1][iy] + v[2] * P.ub4[2][iy];
+      }
+    }
+  }
+  __syncthreads();
+  TileForward3<false, true, true>(s_fwd, &s_pred[0][0], &s_coef[0][0],
+                                  &s_tmp[0][0]);
+  __syncthreads();
+  for (int k = tid; k < 3 * 16 * 64; k += 256) {
+    const int c = k >> 10, blk = (k >> 6) & 15, e = k & 63;
+    const int r = blk >> 2, b = blk & 3;
+    const int yy = tby + r, xx = tbx + b;
+    if (xx < P.bx && yy < P.by) {
+      coef[c * nb * 64 + size_t(yy) * row + size_t(xx) * 64 + e] =
+          s_coef[c][CoefAt(r, b * 64 + e)];
     }
   }
 }

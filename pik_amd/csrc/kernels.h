@@ -302,9 +302,9 @@ __device__ inline void FwdLine1(const float* pix, float* tmp, int oby,
   for (int k = 0; k < N; ++k) tmp[PixAt(oby * 8 + x, obx * 8 + k)] = w[k];
 }
 // kAddHF: add the coefficients outside the 2s x 2s low-frequency corner
-// to coef instead of storing them (AddBlockExceptLFAndLLFTo,
-// pik/ac_predictions.cc:38-70).
-template <int N, bool kAddHF = false>
+// to coef instead of storing them, or with kSubHF subtract them
+// (AddBlockExceptLFAndLLFTo<true / false>, pik/ac_predictions.cc:38-70).
+template <int N, bool kAddHF = false, bool kSubHF = false>
 __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
                                 int obx, int k) {
   constexpr int s = N / 8;
@@ -318,7 +318,7 @@ __device__ inline void FwdLine2(const float* tmp, float* coef, int oby,
     if (!kAddHF) {
       dst = w[j] * scale;
     } else if (!(j < 2 * s && k < 2 * s)) {
-      dst = dst + w[j] * scale;
+      dst = kSubHF ? dst - w[j] * scale : dst + w[j] * scale;
     }
   }
 }
@@ -382,18 +382,18 @@ __device__ inline void InverseTask(int pass, int raw, int l, int oby, int obx,
 // corner to coef (the HF prediction; only strategies with PredictHF, so no
 // DCT2X2 or IDENTITY, and the DCT4X4 DC Hadamard, which only touches that
 // corner, is skipped).
-template <bool kAdjoint = false, bool kAddHF = false>
+template <bool kAdjoint = false, bool kAddHF = false, bool kSubHF = false>
 __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
                                    const float* pix, float* coef, float* tmp) {
   if (raw == kDCT || raw == kDCTNoHF) {
     if (pass == 0) FwdLine1<8>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<8, kAddHF>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<8, kAddHF, kSubHF>(tmp, coef, oby, obx, l);
   } else if (raw == kDCT16) {
     if (pass == 0) FwdLine1<16>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<16, kAddHF>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<16, kAddHF, kSubHF>(tmp, coef, oby, obx, l);
   } else if (raw == kDCT32) {
     if (pass == 0) FwdLine1<32>(pix, tmp, oby, obx, l);
-    else if (pass == 1) FwdLine2<32, kAddHF>(tmp, coef, oby, obx, l);
+    else if (pass == 1) FwdLine2<32, kAddHF, kSubHF>(tmp, coef, oby, obx, l);
   } else if (AcsIsDCT4(raw)) {
     const int sub = l >> 2, sy = sub >> 1, sx = sub & 1, q = l & 3;
     float v[4], w[4];
@@ -416,7 +416,7 @@ __device__ inline void ForwardTask(int pass, int raw, int l, int oby, int obx,
         if (!kAddHF) {
           dst = w[j] * (1.0f / 16);
         } else if (!(j == 0 && q == 0)) {  // outside the 2x2 corner
-          dst = dst + w[j] * (1.0f / 16);
+          dst = kSubHF ? dst - w[j] * (1.0f / 16) : dst + w[j] * (1.0f / 16);
         }
       }
     } else if (!kAddHF && l == 0) {
@@ -498,7 +498,7 @@ __device__ inline void TileInverse3(const TileTasks& T, const float* coef,
 // pix (3 x kTilePlane) -> coef (3 x kCoefPlane) for the transforms in T
 // (others keep their coefficient entries), tmp 3 x kTilePlane. All 256
 // threads must call.
-template <bool kAdjoint = false, bool kAddHF = false>
+template <bool kAdjoint = false, bool kAddHF = false, bool kSubHF = false>
 __device__ inline void TileForward3(const ThreadTasks& m, const float* pix,
                                     float* coef, float* tmp) {
   // pass 2 (the DCT4X4 DC Hadamard) only when the tile has a DCT4X4
@@ -509,20 +509,19 @@ __device__ inline void TileForward3(const ThreadTasks& m, const float* pix,
       const uint32_t w = m.Slot(i);
       if (w != kNoTask) {
         const int c = w & 3;
-        ForwardTask<kAdjoint, kAddHF>(pass, (w >> 2) & 15, (w >> 6) & 31,
-                                      (w >> 11) & 3, (w >> 13) & 3,
-                                      pix + c * kTilePlane,
-                                      coef + c * kCoefPlane,
-                                      tmp + c * kTilePlane);
+        ForwardTask<kAdjoint, kAddHF, kSubHF>(
+            pass, (w >> 2) & 15, (w >> 6) & 31, (w >> 11) & 3,
+            (w >> 13) & 3, pix + c * kTilePlane, coef + c * kCoefPlane,
+            tmp + c * kTilePlane);
       }
     }
     __syncthreads();
   }
 }
-template <bool kAdjoint = false, bool kAddHF = false>
+template <bool kAdjoint = false, bool kAddHF = false, bool kSubHF = false>
 __device__ inline void TileForward3(const TileTasks& T, const float* pix,
                                     float* coef, float* tmp) {
-  TileForward3<kAdjoint, kAddHF>(AssignTasks(T), pix, coef, tmp);
+  TileForward3<kAdjoint, kAddHF, kSubHF>(AssignTasks(T), pix, coef, tmp);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,6 +626,9 @@ __global__ void KEncDc(FrameParams P, const float* dcf, EncDcScale S,
                        int16_t* dc16);
 __global__ void KEncQuant(FrameParams P, FrameBuffers B, const float* coef,
                           const float* itable, float gsf, int16_t* ac);
+__global__ void KEncLf(FrameParams P, FrameBuffers B, float* coef,
+                       const float* itable, float gsf, float* corner);
+__global__ void KEncHfResidual(FrameParams P, FrameBuffers B, float* coef);
 __global__ void KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                             int dc2x2, int wg4, int wg2, int pos_blocks,
                             int ngroups);
@@ -635,9 +637,11 @@ __global__ void KDc2x2Border(FrameParams P, FrameBuffers B,
 __global__ void KLfBlur(FrameParams P, FrameBuffers B);
 __global__ void KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K,
                              int wg4, int wg2, int pos_blocks, int ngroups);
+template <bool kUseLf>
 __global__ void KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K);
+template <bool kEnc>
 __global__ void KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                                 int wg4, int wg2);
+                                 int wg4, int wg2, const float* corner);
 __global__ void KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KAdjForward(FrameParams P, FrameBuffers B,
                             const DeviceConsts* K, AdjArgs A);

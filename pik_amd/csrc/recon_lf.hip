@@ -448,6 +448,7 @@ __global__ void __launch_bounds__(256)
 
 // Border of pred2x2 from llf and lf2x2 as DCT8 blocks
 // (pik/compressed_image.cc:980-1039).
+template <bool kUseLf>
 __global__ void __launch_bounds__(256)
     KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
   const int pos = blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,11 +461,16 @@ __global__ void __launch_bounds__(256)
   const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
   const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
   float* o = B.pred2x2 + (size_t(g) * 3 + c) * kG2Plane + 2 * y * kG2Pitch + 2 * x;
-  // ReinterpretingIDCT<8, 2, 2> of {llf, lf(0,1), lf(1,0), lf(1,1)}.
+  // ReinterpretingIDCT<8, 2, 2> of {llf, lf(0,1), lf(1,0), lf(1,1)}; the
+  // encoder without LF prediction forms the same with zeros in place of
+  // the LF values (pik/ac_predictions.cc:187-198).
+  const float l01 = kUseLf ? lf[2 * y * kG2Pitch + 2 * x + 1] : 0.0f;
+  const float l10 = kUseLf ? lf[(2 * y + 1) * kG2Pitch + 2 * x] : 0.0f;
+  const float l11 = kUseLf ? lf[(2 * y + 1) * kG2Pitch + 2 * x + 1] : 0.0f;
   const float a00 = (llf[y * kLlfPitch + x] * K->ascale[0][0][0]) * K->tscale[2][0][0];
-  const float a01 = (lf[2 * y * kG2Pitch + 2 * x + 1] * K->ascale[0][0][1]) * K->tscale[2][0][1];
-  const float a10 = (lf[(2 * y + 1) * kG2Pitch + 2 * x] * K->ascale[0][1][0]) * K->tscale[2][1][0];
-  const float a11 = (lf[(2 * y + 1) * kG2Pitch + 2 * x + 1] * K->ascale[0][1][1]) * K->tscale[2][1][1];
+  const float a01 = (l01 * K->ascale[0][0][1]) * K->tscale[2][0][1];
+  const float a10 = (l10 * K->ascale[0][1][0]) * K->tscale[2][1][0];
+  const float a11 = (l11 * K->ascale[0][1][1]) * K->tscale[2][1][1];
   o[0] = a00 + a01 + a10 + a11;
   o[1] = a00 + a01 - a10 - a11;
   o[kG2Pitch] = a00 - a01 + a10 - a11;
@@ -476,10 +482,15 @@ __global__ void __launch_bounds__(256)
 // upsampled DC block (DC2x2FromLowFrequencies, pik/ac_strategy.cc:971-1017,
 // as called by UpdateLfForDecoder, pik/ac_predictions.cc:621-633). X and B
 // recompute the dequantised Y values their colour correlation needs.
-template <int S>
+//
+// kEnc: the encoder's side (ComputeDecoderBlockAnd2x2DC, pik/ac_predictions.
+// cc:136-199): the corner's values were formed by KEncLf and wait in
+// `corner` (coefficient layout).
+template <int S, bool kEnc = false>
 __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
                                     const DeviceConsts* K, int c, int bxa,
-                                    int bya, int bxg, int byg, int g) {
+                                    int bya, int bxg, int byg, int g,
+                                    const float* corner) {
   constexpr int N = 2 * S;  // LF corner size
   const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
   const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
@@ -494,13 +505,16 @@ __device__ inline void Pred2x2Block(const FrameParams& P, const FrameBuffers& B,
     if (c != 1) dqy[k] = DescBlockDq(P, B, 1, bya, bxa + k);
   }
   const size_t row0 = size_t(bya) * P.bx * 64 + size_t(bxa) * 64;
+  const size_t cplane = size_t(c) * P.by * P.bx * 64;
   float F[N * N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       float v;
-      if (i < S && j < S) {
+      if (kEnc) {
+        v = corner[cplane + row0 + i * 8 * S + j];
+      } else if (i < S && j < S) {
         v = llf[(byg + 1 + i) * kLlfPitch + bxg + 1 + j];
       } else {
         const int flat = i * 8 * S + j;
@@ -549,10 +563,11 @@ __device__ inline FirstBlock LocateBlock(const FrameParams& P, int blk) {
 // of the output. Each element sees the operations of the one-lane form in
 // the same order. Every thread of the workgroup calls (the barrier);
 // `active` is false for lanes past the last unit.
-template <int S>
+template <int S, bool kEnc = false>
 __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
                                     const DeviceConsts* K, bool active, int c,
-                                    int blk, int l, float* t) {
+                                    int blk, int l, float* t,
+                                    const float* corner) {
   constexpr int N = 2 * S;
   constexpr int ai = ScaleIdx<N>(), ti = ScaleIdx<8 * S>();
   FirstBlock f = {0, 0, 0, 0, 0};
@@ -562,10 +577,13 @@ __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
     const float* llf = B.llf + (size_t(f.g) * 3 + c) * kLlfPlane;
     const float* lf = B.lf2x2 + (size_t(f.g) * 3 + c) * kG2Plane;
     const size_t row0 = size_t(f.bya) * P.bx * 64 + size_t(f.bxa) * 64;
+    const size_t cplane = size_t(c) * P.by * P.bx * 64;
 #pragma unroll
     for (int y = 0; y < N; ++y) {
       float x;
-      if (y < S && l < S) {
+      if (kEnc) {
+        x = corner[cplane + row0 + y * 8 * S + l];
+      } else if (y < S && l < S) {
         x = llf[(f.byg + 1 + y) * kLlfPitch + f.bxg + 1 + l];
       } else {
         // F[y][l] is flat y * 8S + l, in storage block bxa + flat / 64
@@ -605,9 +623,10 @@ __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
 // one lane per unit for the DCT8-like and Hadamard classes. The three
 // planes of a block are adjacent units (X and B read the block's Y
 // coefficients and descriptors with the Y unit).
+template <bool kEnc>
 __global__ void __launch_bounds__(256)
     KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                     int wg4, int wg2) {
+                     int wg4, int wg2, const float* corner) {
   __shared__ float s_t[32 * 8 * 9];
   const int tid = threadIdx.x;
   const int wg = blockIdx.x;
@@ -616,8 +635,8 @@ __global__ void __launch_bounds__(256)
     const int kb = unit / 3, c = unit - 3 * kb;
     const bool active = kb < P.pred_end[3] - P.pred_end[2];
     const int blk = active ? int(B.pred_list[P.pred_end[2] + kb]) : 0;
-    Pred2x2Lanes<4>(P, B, K, active, c, blk, tid & 7,
-                    s_t + (tid >> 3) * 8 * 9);
+    Pred2x2Lanes<4, kEnc>(P, B, K, active, c, blk, tid & 7,
+                          s_t + (tid >> 3) * 8 * 9, corner);
     return;
   }
   if (wg < wg4 + wg2) {
@@ -625,8 +644,8 @@ __global__ void __launch_bounds__(256)
     const int kb = unit / 3, c = unit - 3 * kb;
     const bool active = kb < P.pred_end[2] - P.pred_end[1];
     const int blk = active ? int(B.pred_list[P.pred_end[1] + kb]) : 0;
-    Pred2x2Lanes<2>(P, B, K, active, c, blk, tid & 3,
-                    s_t + (tid >> 2) * 4 * 5);
+    Pred2x2Lanes<2, kEnc>(P, B, K, active, c, blk, tid & 3,
+                          s_t + (tid >> 2) * 4 * 5, corner);
     return;
   }
   const int lane = (wg - wg4 - wg2) * 256 + tid;
@@ -634,7 +653,8 @@ __global__ void __launch_bounds__(256)
   if (k >= P.pred_end[1]) return;
   const FirstBlock f = LocateBlock(P, int(B.pred_list[k]));
   if (k < P.pred_end[0]) {
-    Pred2x2Block<1>(P, B, K, c, f.bxa, f.bya, f.bxg, f.byg, f.g);
+    Pred2x2Block<1, kEnc>(P, B, K, c, f.bxa, f.bya, f.bxg, f.byg, f.g,
+                          corner);
     return;
   }
   // DCT2X2, DCT4X4(_NOHF), IDENTITY: 2x2 Hadamard of the LF corner.
@@ -646,10 +666,13 @@ __global__ void __launch_bounds__(256)
   const float* llf = B.llf + (size_t(g) * 3 + c) * kLlfPlane;
   const float* lf = B.lf2x2 + (size_t(g) * 3 + c) * kG2Plane;
   float F[4];
+  const size_t cplane = size_t(c) * P.by * P.bx * 64;
   for (int i = 0; i < 2; ++i) {
     for (int j = 0; j < 2; ++j) {
       float v;
-      if (i == 0 && j == 0) {
+      if (kEnc) {
+        v = corner[cplane + row0 + i * 8 + j];
+      } else if (i == 0 && j == 0) {
         v = llf[(byg + 1) * kLlfPitch + bxg + 1];
       } else {
         const int flat = i * 8 + j;
@@ -665,5 +688,16 @@ __global__ void __launch_bounds__(256)
              2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
   Idct2x2(F[0], F[1], F[2], F[3], o, kG2Pitch);
 }
+
+template __global__ void KPred2x2Border<true>(FrameParams, FrameBuffers,
+                                              const DeviceConsts*);
+template __global__ void KPred2x2Border<false>(FrameParams, FrameBuffers,
+                                               const DeviceConsts*);
+template __global__ void KPred2x2Interior<false>(FrameParams, FrameBuffers,
+                                                 const DeviceConsts*, int, int,
+                                                 const float*);
+template __global__ void KPred2x2Interior<true>(FrameParams, FrameBuffers,
+                                                const DeviceConsts*, int, int,
+                                                const float*);
 
 }  // namespace pikgpu
