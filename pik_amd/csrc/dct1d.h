@@ -20,19 +20,22 @@
 
 #include "dct_consts.h"
 
+// Everything is forced inline with fully unrolled loops and compile-time
+// angles: every array index is then a constant and the arrays live in
+// registers (a runtime index puts an array in per-lane scratch memory).
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
-#define PIKGPU_HD __host__ __device__ inline
+#define PIKGPU_HD __host__ __device__ inline __attribute__((always_inline))
 #else
-#define PIKGPU_HD inline
+#define PIKGPU_HD inline __attribute__((always_inline))
 #endif
 
 namespace pikgpu {
 namespace dct1d {
 
 // Angles are kept as integers J in units of pi / (2n) for an n-point
-// transform; the constants 2cos(J pi/2n) and 0.5/cos(J pi/2n) come from the
-// (tables of dct_consts.h).
+// transform (template arguments); the constants 2cos(J pi/2n) and
+// 0.5/cos(J pi/2n) come from the tables of dct_consts.h.
 template <int n>
 PIKGPU_HD float FwdConst(int J) {
   return n == 4 ? kFwdConst4[J]
@@ -53,24 +56,25 @@ PIKGPU_HD float InvConst(int J) {
 // b_k = u_{m/2+k} * 2cos(J pi/2n), p = a + b, q = a - b; a node of 2 inputs
 // writes y[J/2] = p, y[n - J/2] = q, larger nodes recurse on p at J/2 and on
 // q at (2n - J)/2.
-template <int n, int m>
+template <int n, int m, int J>
 struct VpOdd {
-  PIKGPU_HD static void Run(const float* u, int J, float* y) {
+  PIKGPU_HD static void Run(const float* u, float* y) {
     float p[m / 2], q[m / 2];
     const float c = FwdConst<n>(J);
+#pragma unroll
     for (int k = 0; k < m / 2; ++k) {
       const float a = u[k] - u[m - 1 - k];
       const float b = u[m / 2 + k] * c;
       p[k] = a + b;
       q[k] = a - b;
     }
-    VpOdd<n, m / 2>::Run(p, J / 2, y);
-    VpOdd<n, m / 2>::Run(q, (2 * n - J) / 2, y);
+    VpOdd<n, m / 2, J / 2>::Run(p, y);
+    VpOdd<n, m / 2, (2 * n - J) / 2>::Run(q, y);
   }
 };
-template <int n>
-struct VpOdd<n, 2> {
-  PIKGPU_HD static void Run(const float* u, int J, float* y) {
+template <int n, int J>
+struct VpOdd<n, 2, J> {
+  PIKGPU_HD static void Run(const float* u, float* y) {
     const float a = u[0] - u[1];
     const float b = u[1] * FwdConst<n>(J);
     y[J / 2] = a + b;
@@ -82,13 +86,15 @@ template <int n>
 struct VpDct {
   PIKGPU_HD static void Run(const float* x, float* y) {
     float s[n / 2], d[n / 2], e[n / 2];
+#pragma unroll
     for (int k = 0; k < n / 2; ++k) {
       s[k] = x[k] + x[n - 1 - k];
       d[k] = x[k] - x[n - 1 - k];
     }
     VpDct<n / 2>::Run(s, e);
+#pragma unroll
     for (int k = 0; k < n / 2; ++k) y[2 * k] = e[k];
-    VpOdd<n, n / 2>::Run(d, n / 2, y);
+    VpOdd<n, n / 2, n / 2>::Run(d, y);
   }
 };
 template <>
@@ -106,28 +112,30 @@ struct VpDct<2> {
 // c = 0.5/cos(J pi/2n) gives S_k = G_k + H_k, D_k = (G_k - H_k) c and the
 // group (S_k + D_{h-1-k})_k followed by (D_k)_k. Leaves merge the single
 // inputs x_j and x_{n-j}, j = J/2.
-template <int n, int size>
+template <int n, int size, int J>
 struct VpInvOdd {
-  PIKGPU_HD static void Run(const float* x, int J, float* out) {
+  PIKGPU_HD static void Run(const float* x, float* out) {
     float g[size / 2], h[size / 2];
-    VpInvOdd<n, size / 2>::Run(x, J / 2, g);
-    VpInvOdd<n, size / 2>::Run(x, (2 * n - J) / 2, h);
+    VpInvOdd<n, size / 2, J / 2>::Run(x, g);
+    VpInvOdd<n, size / 2, (2 * n - J) / 2>::Run(x, h);
     const float c = InvConst<n>(J);
     float s[size / 2], d[size / 2];
+#pragma unroll
     for (int k = 0; k < size / 2; ++k) {
       s[k] = g[k] + h[k];
       d[k] = (g[k] - h[k]) * c;
     }
+#pragma unroll
     for (int k = 0; k < size / 2; ++k) {
       out[k] = s[k] + d[size / 2 - 1 - k];
       out[size / 2 + k] = d[k];
     }
   }
 };
-template <int n>
-struct VpInvOdd<n, 2> {
-  PIKGPU_HD static void Run(const float* x, int J, float* out) {
-    const int j = J / 2;
+template <int n, int J>
+struct VpInvOdd<n, 2, J> {
+  PIKGPU_HD static void Run(const float* x, float* out) {
+    constexpr int j = J / 2;
     const float s = x[j] + x[n - j];
     const float d = (x[j] - x[n - j]) * InvConst<n>(J);
     out[0] = s + d;
@@ -139,9 +147,11 @@ template <int n>
 struct VpIdct {
   PIKGPU_HD static void Run(const float* x, float* y) {
     float ev[n / 2], e[n / 2], o[n / 2];
+#pragma unroll
     for (int k = 0; k < n / 2; ++k) ev[k] = x[2 * k];
     VpIdct<n / 2>::Run(ev, e);
-    VpInvOdd<n, n / 2>::Run(x, n / 2, o);
+    VpInvOdd<n, n / 2, n / 2>::Run(x, o);
+#pragma unroll
     for (int k = 0; k < n / 2; ++k) {
       y[k] = e[k] + o[k];
       y[n - 1 - k] = e[k] - o[k];

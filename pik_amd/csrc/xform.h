@@ -183,12 +183,16 @@ PIKGPU_HD void LinesDct(int n, float* v) {
 // pixels (row stride pstride).
 PIKGPU_HD void LinesInverse8(const float* c, float* p, int pstride) {
   float t[64];
+  #pragma unroll
   for (int i = 0; i < 64; ++i) t[i] = 0.0f;
+  #pragma unroll
   for (int l = 0; l < kLinesCount; ++l) {
     const int n = kLinesLen[l];
     float v[8];
+    #pragma unroll
     for (int j = 0; j < n; ++j) v[j] = c[kLinesCoef[l][j]];
     LinesIdct(n, v);
+    #pragma unroll
     for (int j = 0; j < n; ++j) t[kLinesPix[l][j]] = v[j];
   }
   const float h0 = c[0], h1 = c[8], h2 = c[1], h3 = c[9];
@@ -197,16 +201,21 @@ PIKGPU_HD void LinesInverse8(const float* c, float* p, int pstride) {
   dc[1] = h0 - h1 + h2 - h3;
   dc[2] = h0 + h1 - h2 - h3;
   dc[3] = h0 - h1 - h2 + h3;
+  #pragma unroll
   for (int g = 0; g < 4; ++g) {
+    #pragma unroll
     for (int j = 0; j < kLinesGroupLen[g]; ++j) {
       t[kLinesGroup[g][j]] += dc[g];
     }
   }
+  #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint8_t* q = kLinesInterp[k];
     t[q[0]] = (t[q[1]] + t[q[2]]) * 0.5f;
   }
+  #pragma unroll
   for (int y = 0; y < 8; ++y) {
+    #pragma unroll
     for (int x = 0; x < 8; ++x) p[y * pstride + x] = t[y * 8 + x];
   }
 }
@@ -214,15 +223,20 @@ PIKGPU_HD void LinesInverse8(const float* c, float* p, int pstride) {
 // Forward LINES of one block (p: pixels, row stride pstride; c: 64 slots).
 PIKGPU_HD void LinesForward8(const float* p, int pstride, float* c) {
   float t[64];
+  #pragma unroll
   for (int y = 0; y < 8; ++y) {
+    #pragma unroll
     for (int x = 0; x < 8; ++x) t[y * 8 + x] = p[y * pstride + x];
   }
   float dc[4];
+  #pragma unroll
   for (int g = 0; g < 4; ++g) {
     const int n = kLinesGroupLen[g];
     float sum = 0.0f;
+    #pragma unroll
     for (int j = 0; j < n; ++j) sum += t[kLinesGroup[g][j]];
     sum /= float(n);
+    #pragma unroll
     for (int j = 0; j < n; ++j) t[kLinesGroup[g][j]] -= sum;
     dc[g] = sum;
   }
@@ -231,11 +245,14 @@ PIKGPU_HD void LinesForward8(const float* p, int pstride, float* c) {
   c[kLinesGroupSlot[1]] = (i0 - i1 + i2 - i3) * 0.25f;
   c[kLinesGroupSlot[2]] = (i0 + i1 - i2 - i3) * 0.25f;
   c[kLinesGroupSlot[3]] = (i0 - i1 - i2 + i3) * 0.25f;
+  #pragma unroll
   for (int l = 0; l < kLinesCount; ++l) {
     const int n = kLinesLen[l];
     float v[8];
+    #pragma unroll
     for (int j = 0; j < n; ++j) v[j] = t[kLinesPix[l][j]];
     LinesDct(n, v);
+    #pragma unroll
     for (int j = 0; j < n; ++j) c[kLinesCoef[l][j]] = v[j];
   }
 }
