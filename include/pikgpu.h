@@ -594,6 +594,83 @@ PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* section,
                                        int threads, char* msg,
                                        size_t msg_size);
 
+/* ------------------------------------------------------------------------
+ * Host parsing of one whole frame, from its FrameHeader to the end of its
+ * last pass, into the pikgpu_frame the reconstruction takes.
+ *
+ * Replaces the bitstream half of PikPassToPixels (pik/pik_frame.cc:
+ * 935-1187) and PikGroupToPixels (:846-931) for the default decoder
+ * parameters (full size, every pass): ReadPassHeader (FrameHeader,
+ * pik/headers.h:182-294, through the field coders of pik/fields.h:67-300,
+ * 853-935), DequantMatrices::Decode (pik/quant_weights.cc:590-603),
+ * Quantizer::Decode (pik/quantizer.cc:63-71), DecodeColorMap for Y-to-B and
+ * Y-to-X (pik/color_correlation.cc:244-262), the DC groups section
+ * (pikgpu_decode_dc_groups), BlockDictionary::Decode (pik/block_dictionary.
+ * cc:199-307), DecodeDequantControlField / DecodeDequantControlFieldMap
+ * (pik/quant_weights.cc:942-958, 1054-1075), then per pass the group size
+ * table and per group its GroupHeader (first pass only, pik/headers.h:
+ * 106-136) and AC data (pikgpu_decode_ac_pass).
+ *
+ * `data` starts at the byte-aligned FrameHeader and runs to the end of the
+ * file (the reference's reader spans the whole file); xsize / ysize are the
+ * FileHeader's image size (the file header itself, with its colour
+ * metadata, is not parsed here). Groups decode on `threads` host threads
+ * (<= 0: one per hardware thread). On success *out receives a parsed frame
+ * that owns every array; release it with pikgpu_parsed_free.
+ *
+ * Returns PIKGPU_OK; PIKGPU_EINVAL for bad arguments; PIKGPU_EDATA for an
+ * invalid frame, wherever the reference decoder fails and where it would
+ * index outside its own tables (a dequant control field value >= 16, a
+ * dequant map entry naming a missing table, a dot outside the image, a
+ * dot transform of zero size, length or width, no passes); or
+ * PIKGPU_EUNSUPPORTED for frames outside this decoder: lossless frames,
+ * alpha, the gradient map, custom (non-library) quantization tables, more
+ * than 16 passes, or a dot dictionary of more than 2^20 patches or 2^22
+ * placements (the reference would allocate gigabytes). `msg` names the
+ * failing section. The colour maps' own failures are ignored, as the
+ * reference ignores DecodeColorMap's result (pik/pik_frame.cc:1049-1050):
+ * a map whose code does not parse keeps its default. */
+typedef struct pikgpu_parsed pikgpu_parsed;
+
+typedef struct pikgpu_parsed_info {
+  uint64_t frame_size;        /* FrameHeader::size (bytes; 0 if unset) */
+  uint32_t num_passes;
+  uint32_t num_groups;        /* 512x512 groups per pass */
+  uint32_t epf_sigma;         /* EpfParams::sigma (if not adaptive) */
+  uint32_t num_downsampling;  /* downsampling_factor_to_passes entries */
+  const uint32_t* downsampling; /* pairs {factor, passes} */
+  const float* noise;         /* num_passes x num_groups x {alpha, gamma,
+                                 beta} (NoiseParams per group and pass) */
+  size_t consumed;            /* bytes from the FrameHeader to the end of
+                                 the last pass's last group */
+} pikgpu_parsed_info;
+
+PIKGPU_API int pikgpu_parse_frame(const uint8_t* data, size_t size,
+                                  uint32_t xsize, uint32_t ysize, int threads,
+                                  pikgpu_parsed** out, char* msg,
+                                  size_t msg_size);
+/* Fills `frame` (pointers into the parsed frame's storage, valid until it
+ * is freed; ac is the first pass) and `info` (either may be NULL). */
+PIKGPU_API int pikgpu_parsed_get(const pikgpu_parsed* parsed,
+                                 pikgpu_frame* frame, pikgpu_parsed_info* info);
+/* The quantized AC of pass `pass` (0 .. num_passes - 1) in the layout of
+ * pikgpu_frame.ac: ac[c] receives plane c, *ac_stride the row stride. */
+PIKGPU_API int pikgpu_parsed_pass_ac(const pikgpu_parsed* parsed,
+                                     uint32_t pass, const int16_t* ac[3],
+                                     size_t* ac_stride);
+PIKGPU_API void pikgpu_parsed_free(pikgpu_parsed* parsed);
+
+/* The whole decode of one frame on a context: pikgpu_parse_frame, then
+ * pikgpu_frame_upload, pikgpu_frame_add_pass for every further pass and
+ * pikgpu_run -- PikPassToPixels end to end. The result is read with
+ * pikgpu_download / pikgpu_download_srgb8; *consumed (may be NULL) receives
+ * the frame's length in bytes. Returns the first failing step's code; the
+ * message is pikgpu_last_error(ctx). */
+PIKGPU_API int pikgpu_decode_frame(pikgpu_ctx* ctx, const uint8_t* data,
+                                   size_t size, uint32_t xsize,
+                                   uint32_t ysize, int threads,
+                                   size_t* consumed);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

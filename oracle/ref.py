@@ -436,3 +436,105 @@ def decode_dc_groups(xsize, ysize, data, kind="scalar"):
                            % L.pikref_last_error().decode())
     return dict(dc=dc, ac_strategy=acs, quant_field=qf, sigma_lut_ids=lut,
                 consumed=used.value)
+
+
+# Bits of encode_frame's `variant` (pikref_encode_frame).
+VAR_FRAME_INFO = 1
+VAR_GROUP_HEADERS = 2
+VAR_STATIC_DOTS = 4
+VAR_FAST_AC = 8
+VAR_DOWNSAMPLING = 16
+VAR_COPY_TABLE = 32
+VAR_EXTENSIONS = 64
+
+
+def encode_frame(frame, extra_ac=(), noise=None, variant=0, kind="scalar"):
+    """One whole frame (FrameHeader to the last pass's groups) written with
+    the reference's encoders: pass 0 carries frame.ac, pass i > 0
+    extra_ac[i - 1] ((3, by, bx*64) int16). noise: (passes, groups, 3) or
+    None. Returns bytes."""
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_encode_frame.argtypes = [ctypes.POINTER(abi.FrameC), ctypes.c_int,
+                                      vp, vp, ctypes.c_int, vp,
+                                      ctypes.c_size_t, vp]
+    fc, keep = frame.to_c()
+    extra = [np.ascontiguousarray(a, np.int16) for a in extra_ac]
+    for a in extra:
+        assert a.shape == frame.ac.shape and fc.ac_stride == a.shape[2]
+    ptrs = (vp * max(1, 3 * len(extra)))(
+        *[a[c].ctypes.data for a in extra for c in range(3)])
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float32)
+    npp = None if nz is None else nz.ctypes.data
+    total = ctypes.c_size_t()
+    args = (ctypes.byref(fc), 1 + len(extra), ctypes.cast(ptrs, vp), npp,
+            int(variant))
+    if L.pikref_encode_frame(*args, None, 0, ctypes.byref(total)) != 0:
+        raise RuntimeError("pikref_encode_frame: %s"
+                           % L.pikref_last_error().decode())
+    buf = np.zeros(max(1, total.value), np.uint8)
+    if L.pikref_encode_frame(*args, buf.ctypes.data, buf.size,
+                             ctypes.byref(total)) != 0:
+        raise RuntimeError("pikref_encode_frame: %s"
+                           % L.pikref_last_error().decode())
+    return buf[:total.value].tobytes()
+
+
+FRAME_HEADER_FIELDS = ("flags", "gaborish", "predict_lf", "predict_hf",
+                       "adaptive_reconstruction", "epf_enable_adaptive",
+                       "epf_sigma", "epf_use_sharpened", "num_passes",
+                       "global_scale", "quant_dc", "ytox_dc", "ytob_dc",
+                       "num_table_sets", "num_dots", "frame_size")
+
+
+def decode_frame(data, xsize, ysize, max_passes=16, max_dots=4096,
+                 kind="scalar"):
+    """The reference decoders' reading of one frame: dict with the header
+    fields (FRAME_HEADER_FIELDS), dc, ac_strategy, quant_field,
+    sigma_lut_ids, ytox_map, ytob_map, dequant_cf, dequant_map, dots
+    (structured, frame.DOT_DTYPE), passes (list of (3, by, bx*64) int16),
+    noise (passes, groups, 3) and consumed. Raises RuntimeError where the
+    reference fails."""
+    from pik_amd.frame import DOT_DTYPE
+    L = lib(kind)
+    vp = ctypes.c_void_p
+    L.pikref_decode_frame.argtypes = [
+        vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp,
+        vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_size_t, vp, vp]
+    by, bx = -(-ysize // 8), -(-xsize // 8)
+    ty, tx = -(-ysize // 64), -(-xsize // 64)
+    ng = (-(-xsize // 512)) * (-(-ysize // 512))
+    buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
+    hdr = np.zeros(16, np.int32)
+    dc = np.zeros((3, by, bx), np.int16)
+    dptr = (vp * 3)(*[dc[c].ctypes.data for c in range(3)])
+    acs = np.zeros((by, bx), np.uint8)
+    qf = np.zeros((by, bx), np.int32)
+    lut = np.zeros((by, bx), np.uint8)
+    ytox = np.zeros((ty, tx), np.int32)
+    ytob = np.zeros((ty, tx), np.int32)
+    dcf = np.zeros((ty, tx), np.uint8)
+    dqmap = np.zeros((16, 256), np.uint8)
+    dots = np.zeros(max_dots, DOT_DTYPE)
+    ac = np.zeros((max_passes, 3, by, bx * 64), np.int16)
+    noise = np.zeros((max_passes, ng, 3), np.float32)
+    used = ctypes.c_size_t()
+    rc = L.pikref_decode_frame(
+        buf.ctypes.data, len(data), xsize, ysize, hdr.ctypes.data,
+        ctypes.cast(dptr, vp), acs.ctypes.data, qf.ctypes.data,
+        lut.ctypes.data, ytox.ctypes.data, ytob.ctypes.data, dcf.ctypes.data,
+        dqmap.ctypes.data, dots.ctypes.data, max_dots, ac.ctypes.data,
+        max_passes, noise.ctypes.data, ctypes.byref(used))
+    if rc != 0:
+        raise RuntimeError("pikref_decode_frame: %s"
+                           % L.pikref_last_error().decode())
+    out = dict(zip(FRAME_HEADER_FIELDS, (int(v) for v in hdr)))
+    n = out["num_passes"]
+    if out["num_dots"] > max_dots:
+        raise RuntimeError("more dots than max_dots")
+    out.update(dc=dc, ac_strategy=acs, quant_field=qf, sigma_lut_ids=lut,
+               ytox_map=ytox, ytob_map=ytob, dequant_cf=dcf,
+               dequant_map=dqmap, dots=dots[:out["num_dots"]].copy(),
+               passes=[ac[i] for i in range(n)], noise=noise[:n],
+               consumed=int(used.value))
+    return out

@@ -1345,6 +1345,67 @@ API int pikref_encode_ac_groups(const pikgpu_frame* f, const float* noise,
   return 0;
 }
 
+namespace {
+
+// One group's pass data as DecodeFromBitstream reads it (DecodeNoise,
+// DecodeCoeffOrder x 3, DecodeHistograms, DecodeAC per tile with
+// GatherCoefficients, CheckANSFinalState, zero bits to the byte boundary)
+// into the group's int16 AC (xsb * 64 x ysb).
+int RefDecodeAcGroup(BitReader* br, const AcStrategyImage& acs,
+                     const Rect& blocks, Image3S* out, NoiseParams* np) {
+  const size_t xsb = blocks.xsize(), ysb = blocks.ysize();
+  if (!DecodeNoise(br, np)) return Fail("DecodeNoise failed");
+  int order[kOrderContexts * kDCTBlockSize];
+  for (size_t c = 0; c < kOrderContexts; ++c) {
+    DecodeCoeffOrder(&order[c * kDCTBlockSize], br);
+  }
+  if (!br->JumpToByteBoundary()) return Fail("order padding");
+  ANSCode code;
+  std::vector<uint8_t> context_map;
+  if (!DecodeHistograms(br, kNumContexts, 256, &code, &context_map)) {
+    return Fail("DecodeHistograms failed");
+  }
+  if (!br->JumpToByteBoundary()) return Fail("histogram padding");
+  ANSSymbolReader decoder(&code);
+  Image3S tile(kTileDimInBlocks * kDCTBlockSize, kTileDimInBlocks);
+  Image3I nz(kTileDimInBlocks, kTileDimInBlocks);
+  Image3S& group_ac = *out;
+  const size_t stride = tile.PixelsPerRow();
+  SIMD_ALIGN int16_t unscattered[AcStrategy::kMaxCoeffArea];
+  const size_t ntx = DivCeil(xsb, kTileDimInBlocks);
+  const size_t nty = DivCeil(ysb, kTileDimInBlocks);
+  for (size_t t = 0; t < ntx * nty; ++t) {
+    const Rect rect((t % ntx) * kTileDimInBlocks,
+                    (t / ntx) * kTileDimInBlocks, kTileDimInBlocks,
+                    kTileDimInBlocks, xsb, ysb);
+    if (!DecodeAC(context_map, order, br, &decoder, &tile, rect, &nz)) {
+      return Fail("DecodeAC failed");
+    }
+    for (int c = 0; c < 3; ++c) {
+      for (size_t y = 0; y < rect.ysize(); ++y) {
+        for (size_t x = 0; x < rect.xsize(); ++x) {
+          const AcStrategy s =
+              acs.ConstRow(blocks, rect.y0() + y)[rect.x0() + x];
+          if (!s.IsFirstBlock()) continue;
+          const int16_t* block = tile.ConstPlaneRow(c, y) + x * 64;
+          const size_t w = s.covered_blocks_x() * kDCTBlockSize;
+          s.GatherCoefficients(block, stride, unscattered, w);
+          for (size_t i = 0; i < s.covered_blocks_y(); ++i) {
+            memcpy(group_ac.PlaneRow(c, rect.y0() + y + i) +
+                       (rect.x0() + x) * 64,
+                   unscattered + w * i, w * sizeof(int16_t));
+          }
+        }
+      }
+    }
+  }
+  if (!decoder.CheckANSFinalState()) return Fail("ANS final state");
+  if (!br->JumpToByteBoundary()) return Fail("AC padding");
+  return 0;
+}
+
+}  // namespace
+
 // Decodes the groups' bitstreams (back to back in `data`) with the
 // reference's own functions in DecodeFromBitstream's order -- DecodeNoise,
 // DecodeCoeffOrder, DecodeHistograms, one ANSSymbolReader through DecodeAC
@@ -1407,53 +1468,8 @@ API int pikref_decode_ac_groups(const pikgpu_frame* f, const uint8_t* data,
       const size_t xsb = blocks.xsize(), ysb = blocks.ysize();
       BitReader br(bytes, size);
       NoiseParams np;
-      if (!DecodeNoise(&br, &np)) return Fail("DecodeNoise failed");
-      int order[kOrderContexts * kDCTBlockSize];
-      for (size_t c = 0; c < kOrderContexts; ++c) {
-        DecodeCoeffOrder(&order[c * kDCTBlockSize], &br);
-      }
-      if (!br.JumpToByteBoundary()) return Fail("order padding");
-      ANSCode code;
-      std::vector<uint8_t> context_map;
-      if (!DecodeHistograms(&br, kNumContexts, 256, &code, &context_map)) {
-        return Fail("DecodeHistograms failed");
-      }
-      if (!br.JumpToByteBoundary()) return Fail("histogram padding");
-      ANSSymbolReader decoder(&code);
-      Image3S tile(kTileDimInBlocks * kDCTBlockSize, kTileDimInBlocks);
-      Image3I nz(kTileDimInBlocks, kTileDimInBlocks);
       Image3S group_ac(xsb * kDCTBlockSize, ysb);
-      const size_t stride = tile.PixelsPerRow();
-      SIMD_ALIGN int16_t unscattered[AcStrategy::kMaxCoeffArea];
-      const size_t ntx = DivCeil(xsb, kTileDimInBlocks);
-      const size_t nty = DivCeil(ysb, kTileDimInBlocks);
-      for (size_t t = 0; t < ntx * nty; ++t) {
-        const Rect rect((t % ntx) * kTileDimInBlocks,
-                        (t / ntx) * kTileDimInBlocks, kTileDimInBlocks,
-                        kTileDimInBlocks, xsb, ysb);
-        if (!DecodeAC(context_map, order, &br, &decoder, &tile, rect, &nz)) {
-          return Fail("DecodeAC failed");
-        }
-        for (int c = 0; c < 3; ++c) {
-          for (size_t y = 0; y < rect.ysize(); ++y) {
-            for (size_t x = 0; x < rect.xsize(); ++x) {
-              const AcStrategy s =
-                  acs.ConstRow(blocks, rect.y0() + y)[rect.x0() + x];
-              if (!s.IsFirstBlock()) continue;
-              const int16_t* block = tile.ConstPlaneRow(c, y) + x * 64;
-              const size_t w = s.covered_blocks_x() * kDCTBlockSize;
-              s.GatherCoefficients(block, stride, unscattered, w);
-              for (size_t i = 0; i < s.covered_blocks_y(); ++i) {
-                memcpy(group_ac.PlaneRow(c, rect.y0() + y + i) +
-                           (rect.x0() + x) * 64,
-                       unscattered + w * i, w * sizeof(int16_t));
-              }
-            }
-          }
-        }
-      }
-      if (!decoder.CheckANSFinalState()) return Fail("ANS final state");
-      if (!br.JumpToByteBoundary()) return Fail("AC padding");
+      if (RefDecodeAcGroup(&br, acs, blocks, &group_ac, &np) != 0) return -1;
       if (consumed != nullptr) consumed[gi] = br.Position();
       if (noise_out != nullptr) {
         noise_out[3 * gi + 0] = np.alpha;
@@ -1524,11 +1540,11 @@ namespace {
 constexpr uint32_t kDcGroupSizeDistribution = 0x150F0E0C;
 }  // namespace
 
+namespace {
+
 // The section (size table + groups) of the frame's quantized DC, strategy,
-// quant field and sigma LUT ids into `out` (may be NULL to size it).
-API int pikref_encode_dc_groups(const pikgpu_frame* f, uint8_t* out,
-                                size_t cap, size_t* total) {
-  if (f == nullptr) return Fail("null frame");
+// quant field and sigma LUT ids.
+int RefEncodeDcSection(const pikgpu_frame* f, std::string* out) {
   AcStrategyImage acs;
   if (ImportAcStrategy(f, &acs) != 0) return -1;
   const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
@@ -1579,56 +1595,42 @@ API int pikref_encode_dc_groups(const pikgpu_frame* f, uint8_t* out,
     }
   }
   WriteZeroesToByteBoundary(&pos, toc.data());
-  size_t n = pos / 8;
-  for (const std::string& c : codes) n += c.size();
-  if (total != nullptr) *total = n;
-  if (out == nullptr) return 0;
-  if (n > cap) return Fail("output buffer too small");
-  memcpy(out, toc.data(), pos / 8);
-  size_t at = pos / 8;
-  for (const std::string& c : codes) {
-    memcpy(out + at, c.data(), c.size());
-    at += c.size();
-  }
+  out->assign(reinterpret_cast<const char*>(toc.data()), pos / 8);
+  for (const std::string& c : codes) *out += c;
   return 0;
 }
 
-// Decodes a DC groups section for an xsize x ysize frame into the
-// quantized DC (after ExpandDC), the combined strategy bytes
-// (AcStrategyImage's layers), the quant field and the sigma LUT ids;
-// *consumed receives the section's length.
-API int pikref_decode_dc_groups(uint32_t xsize, uint32_t ysize,
-                                const uint8_t* data, size_t size,
-                                int16_t* const* dc_out, uint8_t* acs_out,
-                                int32_t* qf_out, uint8_t* lut_out,
-                                size_t* consumed) {
-  const size_t bx = DivCeil<size_t>(xsize, kBlockDim);
-  const size_t by = DivCeil<size_t>(ysize, kBlockDim);
-  PaddedBytes compressed(size);
-  if (size != 0) memcpy(compressed.data(), data, size);
-  BitReader reader(compressed.data(), compressed.size());
+// DecodeDCGroups' reading up to DequantDC from `reader` (positioned at the
+// size table, as the frame decoder leaves it) over `data` (the whole
+// buffer, which the group readers index absolutely); the reader ends past
+// the groups.
+int RefDecodeDcSection(BitReader* reader, const uint8_t* data, size_t size,
+                       size_t bx, size_t by, AcStrategyImage* acs, ImageI* qf,
+                       ImageB* luts, Image3S* dc) {
   const size_t gxs = DivCeil<size_t>(bx, kDcGroupDimInBlocks);
   const size_t gys = DivCeil<size_t>(by, kDcGroupDimInBlocks);
   const size_t ng = gxs * gys;
   std::vector<size_t> offsets(1, 0);
   for (size_t i = 0; i < ng; ++i) {
     offsets.push_back(offsets.back() +
-                      U32Coder::Read(kDcGroupSizeDistribution, &reader));
+                      U32Coder::Read(kDcGroupSizeDistribution, reader));
   }
-  if (!reader.JumpToByteBoundary()) return Fail("size table padding");
-  const size_t begin = reader.Position();
-  if (begin + offsets.back() > size) return Fail("groups past the section");
-  AcStrategyImage acs(bx, by);
-  ImageI qf(bx, by);
-  ZeroFillImage(&qf);
-  ImageB luts(bx, by);
-  ZeroFillImage(&luts);
+  if (!reader->JumpToByteBoundary()) return Fail("size table padding");
+  const size_t begin = reader->Position();
+  reader->SkipBits(offsets.back() * kBitsPerByte);
+  if (reader->Position() > size) return Fail("groups past the section");
+  *acs = AcStrategyImage(bx, by);
+  *qf = ImageI(bx, by);
+  ZeroFillImage(qf);
+  *luts = ImageB(bx, by);
+  ZeroFillImage(luts);
+  *dc = Image3S(bx, by);
   GroupDecCache cache;
   for (size_t gi = 0; gi < ng; ++gi) {
     const size_t gx = gi % gxs, gy = gi / gxs;
     const Rect rect(gx * kDcGroupDimInBlocks, gy * kDcGroupDimInBlocks,
                     kDcGroupDimInBlocks, kDcGroupDimInBlocks, bx, by);
-    BitReader br(compressed.data(), begin + offsets[gi + 1]);
+    BitReader br(data, begin + offsets[gi + 1]);
     br.SkipBits((begin + offsets[gi]) * kBitsPerByte);
     cache.InitDecodeDCGroup(rect.xsize(), rect.ysize());
     if (!DecodeImage(&br, Rect(cache.quantized_dc), &cache.quantized_dc)) {
@@ -1646,24 +1648,70 @@ API int pikref_decode_dc_groups(uint32_t xsize, uint32_t ysize,
     if (!br.JumpToByteBoundary()) return Fail("histogram padding");
     ANSSymbolReader decoder(&code);
     if (!DecodeAcStrategy(&br, &decoder, context_map, &cache.ac_strategy_raw,
-                          rect, &acs, nullptr)) {
+                          rect, acs, nullptr)) {
       return Fail("DecodeAcStrategy failed");
     }
-    if (!DecodeQuantField(&br, &decoder, context_map, rect, acs, &qf,
+    if (!DecodeQuantField(&br, &decoder, context_map, rect, *acs, qf,
                           nullptr)) {
       return Fail("DecodeQuantField failed");
     }
-    if (!DecodeARParameters(&br, &decoder, context_map, rect, acs, &luts)) {
+    if (!DecodeARParameters(&br, &decoder, context_map, rect, *acs, luts)) {
       return Fail("DecodeARParameters failed");
     }
     if (!decoder.CheckANSFinalState()) return Fail("ANS final state");
     if (!br.JumpToByteBoundary()) return Fail("control field padding");
     for (int c = 0; c < 3; ++c) {
       for (size_t y = 0; y < rect.ysize(); ++y) {
-        memcpy(dc_out[c] + (rect.y0() + y) * bx + rect.x0(),
+        memcpy(dc->PlaneRow(c, rect.y0() + y) + rect.x0(),
                cache.quantized_dc.ConstPlaneRow(c, y),
                rect.xsize() * sizeof(int16_t));
       }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// The section (size table + groups) of the frame's quantized DC, strategy,
+// quant field and sigma LUT ids into `out` (may be NULL to size it).
+API int pikref_encode_dc_groups(const pikgpu_frame* f, uint8_t* out,
+                                size_t cap, size_t* total) {
+  if (f == nullptr) return Fail("null frame");
+  std::string sec;
+  if (RefEncodeDcSection(f, &sec) != 0) return -1;
+  if (total != nullptr) *total = sec.size();
+  if (out == nullptr) return 0;
+  if (sec.size() > cap) return Fail("output buffer too small");
+  memcpy(out, sec.data(), sec.size());
+  return 0;
+}
+
+// Decodes a DC groups section for an xsize x ysize frame into the
+// quantized DC (after ExpandDC), the combined strategy bytes
+// (AcStrategyImage's layers), the quant field and the sigma LUT ids;
+// *consumed receives the section's length.
+API int pikref_decode_dc_groups(uint32_t xsize, uint32_t ysize,
+                                const uint8_t* data, size_t size,
+                                int16_t* const* dc_out, uint8_t* acs_out,
+                                int32_t* qf_out, uint8_t* lut_out,
+                                size_t* consumed) {
+  const size_t bx = DivCeil<size_t>(xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(ysize, kBlockDim);
+  PaddedBytes compressed(size);
+  if (size != 0) memcpy(compressed.data(), data, size);
+  BitReader reader(compressed.data(), compressed.size());
+  AcStrategyImage acs;
+  ImageI qf;
+  ImageB luts;
+  Image3S dc;
+  if (RefDecodeDcSection(&reader, compressed.data(), size, bx, by, &acs, &qf,
+                         &luts, &dc) != 0) {
+    return -1;
+  }
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(dc_out[c] + y * bx, dc.ConstPlaneRow(c, y), bx * 2);
     }
   }
   for (size_t y = 0; y < by; ++y) {
@@ -1671,6 +1719,439 @@ API int pikref_decode_dc_groups(uint32_t xsize, uint32_t ysize,
     memcpy(qf_out + y * bx, qf.ConstRow(y), bx * 4);
     memcpy(lut_out + y * bx, luts.ConstRow(y), bx);
   }
-  if (consumed != nullptr) *consumed = begin + offsets.back();
+  if (consumed != nullptr) *consumed = reader.Position();
+  return 0;
+}
+
+// --------------------------------------------------------------------------
+// Whole frames: PixelsToPikPass's bitstream assembly (pik/pik_frame.cc:
+// 555-681) over the given frame with the reference's own encoders, and the
+// decoding half of PikPassToPixels / PikGroupToPixels (pik/pik_frame.cc:
+// 846-931, 935-1110) with its own decoders, restated over public pieces
+// because the drivers link colour management and the lossless codecs.
+
+namespace {
+
+constexpr uint32_t kGroupSizeDistribution = 0x150F0E0C;
+
+// Bits of pikref_encode_frame's `variant`.
+enum {
+  kVarFrameInfo = 1,      // a non-default FrameInfo (duration, timecode)
+  kVarGroupHeaders = 2,   // odd groups: a tile header with a projective
+                          // transform, group extensions
+  kVarStaticDots = 4,     // a placement of the static patch
+  kVarFastAc = 8,         // BuildAndEncodeHistogramsFast for the AC
+  kVarDownsampling = 16,  // two downsampling_factor_to_passes entries
+  kVarCopyTable = 32,     // eight encoded tables (the last a copy): two
+                          // full table sets
+  kVarExtensions = 64     // FrameHeader extensions mask set
+};
+
+std::string BitsToString(const std::vector<uint8_t>& buf, size_t bits) {
+  return std::string(reinterpret_cast<const char*>(buf.data()), bits / 8);
+}
+
+// GroupHeader (pik/headers.h:106-136) for frames without alpha, through
+// the reference's own field visitors: the real bundle's Alpha member
+// instantiates the Brotli byte coder, which is not in the checkout. With
+// nonserialized_have_alpha false the real bundle visits exactly these
+// fields.
+struct RefGroupHeader {
+  RefGroupHeader() { Bundle::Init(this); }
+  static const char* Name() { return "GroupHeader"; }
+  template <class Visitor>
+  Status VisitFields(Visitor* PIK_RESTRICT visitor) {
+    if (visitor->AllDefault(*this, &all_default)) return true;
+    for (size_t i = 0; i < kNumTilesPerGroup; ++i) {
+      PIK_RETURN_IF_ERROR(visitor->VisitNested(&tile_headers[i]));
+    }
+    visitor->BeginExtensions(&extensions);
+    return visitor->EndExtensions();
+  }
+  bool all_default;
+  TileHeader tile_headers[kNumTilesPerGroup];
+  uint64_t extensions;
+};
+
+}  // namespace
+
+// Writes one frame: FrameHeader, quantization tables, quantizer, colour
+// maps, DC groups, dot dictionary, dequant control field and map, then per
+// pass the group size table and the groups (GroupHeader in the first
+// pass). Pass 0's AC is f->ac; pass i > 0 takes extra_ac[3 (i - 1) + c]
+// (rows of f->ac_stride). noise (may be NULL): num_passes x groups x
+// {alpha, gamma, beta}.
+API int pikref_encode_frame(const pikgpu_frame* f, int num_passes,
+                            const int16_t* const* extra_ac, const float* noise,
+                            int variant, uint8_t* out, size_t cap,
+                            size_t* total) {
+  if (f == nullptr || num_passes < 1) return Fail("bad arguments");
+  AcStrategyImage acs;
+  if (ImportAcStrategy(f, &acs) != 0) return -1;
+  const size_t bx = DivCeil<size_t>(f->xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(f->ysize, kBlockDim);
+  const size_t tx = DivCeil<size_t>(f->xsize, kColorTileDim);
+  const size_t ty = DivCeil<size_t>(f->ysize, kColorTileDim);
+  const size_t gxs = DivCeil<size_t>(f->xsize, kGroupDim);
+  const size_t gys = DivCeil<size_t>(f->ysize, kGroupDim);
+  const size_t ng = gxs * gys;
+
+  FrameHeader fh;
+  if (variant & kVarFrameInfo) {
+    fh.frame.duration = 7;
+    fh.frame.have_timecode = true;
+    fh.frame.timecode = 0x01020304;
+    fh.frame.is_keyframe = true;
+  }
+  fh.encoding = ImageEncoding::kPasses;
+  fh.flags = f->flags;
+  fh.gaborish = static_cast<GaborishStrength>(f->gaborish);
+  fh.predict_lf = f->predict_lf != 0;
+  fh.predict_hf = f->predict_hf != 0;
+  fh.have_adaptive_reconstruction = f->adaptive_reconstruction != 0;
+  fh.epf_params.enable_adaptive = f->epf_enable_adaptive != 0;
+  fh.epf_params.use_sharpened = f->epf_use_sharpened != 0;
+  fh.num_passes = num_passes;
+  if (variant & kVarDownsampling) {
+    fh.downsampling_factor_to_passes = {{2, 0}, {8, 5}};
+  }
+  if (variant & kVarExtensions) fh.extensions = 1;
+  size_t ext_bits = 0, total_bits = 0;
+  if (!CanEncode(fh, &ext_bits, &total_bits)) return Fail("frame header");
+  std::vector<uint8_t> hdr(DivCeil<size_t>(total_bits, 8) + 8, 0);
+  size_t pos = 0;
+  if (!WritePassHeader(fh, ext_bits, &pos, hdr.data())) {
+    return Fail("WritePassHeader failed");
+  }
+  WriteZeroesToByteBoundary(&pos, hdr.data());
+  std::string bytes = BitsToString(hdr, pos);
+
+  DequantMatrices matrices(/*need_inv_matrices=*/false);
+  if (variant & kVarCopyTable) {
+    matrices.encodings_.assign(8, QuantEncoding::Library(0));
+    matrices.encodings_[7] = QuantEncoding::Copy(3);
+  }
+  bytes += matrices.Encode(nullptr);
+  const Quantizer quantizer = FrameQuantizer(f, &matrices);
+  bytes += quantizer.Encode(nullptr);
+  ImageI ytob(tx, ty), ytox(tx, ty);
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(ytob.Row(y), f->ytob_map + y * f->ytob_stride, tx * 4);
+    memcpy(ytox.Row(y), f->ytox_map + y * f->ytox_stride, tx * 4);
+  }
+  bytes += EncodeColorMap(ytob, Rect(ytob), f->ytob_dc, nullptr);
+  bytes += EncodeColorMap(ytox, Rect(ytox), f->ytox_dc, nullptr);
+  std::string dc_section;
+  if (RefEncodeDcSection(f, &dc_section) != 0) return -1;
+  bytes += dc_section;
+
+  BlockDictionary dots = ImportDots(*f);
+  if (variant & kVarStaticDots) {
+    // BlockDictionary::Encode restarts its scan of the sorted placements
+    // at the first one for the transformed pass (pik/block_dictionary.cc:
+    // 148-156), so it writes transformed placements only when there are
+    // no plain ones: the static patch is placed plain, or transformed in a
+    // frame with only transformed dots.
+    const size_t id = dots.dictionary_.size();
+    bool plain = false;
+    for (const BlockPosition& p : dots.positions_) plain |= !p.transform;
+    std::vector<BlockPosition> positions = dots.positions_;
+    if (plain || positions.empty()) {
+      positions.push_back(BlockPosition(8, 16, id));
+    } else {
+      BlockPosition p(20, 24, id, 5, 3, 4);
+      p.width = 4;
+      positions.push_back(p);
+    }
+    dots = BlockDictionary(dots.dictionary_, positions);
+  }
+  bytes += dots.Encode(nullptr);
+  ImageB dcf(tx, ty);
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(dcf.Row(y), f->dequant_cf + y * f->dequant_cf_stride, tx);
+  }
+  uint8_t table_map[kMaxQuantControlFieldValue][256];
+  memcpy(table_map, f->dequant_map, sizeof(table_map));
+  bytes += EncodeDequantControlField(dcf, nullptr);
+  bytes += EncodeDequantControlFieldMap(quantizer.RawQuantField(), dcf,
+                                        table_map, nullptr);
+
+  for (int i = 0; i < num_passes; ++i) {
+    const int16_t* const* planes = i == 0 ? f->ac : extra_ac + 3 * (i - 1);
+    std::vector<std::string> codes(ng);
+    for (size_t gi = 0; gi < ng; ++gi) {
+      const size_t gx = gi % gxs, gy = gi / gxs;
+      const Rect group(gx * kGroupDim, gy * kGroupDim, kGroupDim, kGroupDim,
+                       bx * kBlockDim, by * kBlockDim);
+      const Rect blocks(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                        kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+      if (i == 0) {
+        RefGroupHeader gh;
+        if ((variant & kVarGroupHeaders) && gi % 2 == 1) {
+          gh.tile_headers[3].have_projective_transform = true;
+          for (size_t k = 0; k < kNumProjectiveTransformParams; ++k) {
+            gh.tile_headers[3].projective_transform_params.corner_coords[k] =
+                uint32_t(k + 1);
+          }
+          // (the reference's field writer supports extensions in the
+          // outermost bundle only)
+          gh.extensions = 2;
+        }
+        size_t eb = 0, tb = 0;
+        if (!Bundle::CanEncode(gh, &eb, &tb)) return Fail("group header");
+        std::vector<uint8_t> buf(DivCeil<size_t>(tb, 8) + 8, 0);
+        size_t gpos = 0;
+        if (!Bundle::Write(gh, eb, &gpos, buf.data())) {
+          return Fail("WriteGroupHeader failed");
+        }
+        WriteZeroesToByteBoundary(&gpos, buf.data());
+        codes[gi] = BitsToString(buf, gpos);
+      }
+      EncCache cache;
+      cache.xsize_blocks = blocks.xsize();
+      cache.ysize_blocks = blocks.ysize();
+      cache.ac_strategy = acs.Copy(blocks);
+      cache.ac = Image3S(blocks.xsize() * 64, blocks.ysize());
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < blocks.ysize(); ++y) {
+          memcpy(cache.ac.PlaneRow(c, y),
+                 planes[c] + (blocks.y0() + y) * f->ac_stride +
+                     blocks.x0() * 64,
+                 blocks.xsize() * 64 * 2);
+        }
+      }
+      NoiseParams np;
+      if (noise != nullptr) {
+        const float* n = noise + (size_t(i) * ng + gi) * 3;
+        np.alpha = n[0];
+        np.gamma = n[1];
+        np.beta = n[2];
+      }
+      const PaddedBytes ac = EncodeToBitstream(
+          cache, Rect(0, 0, group.xsize(), group.ysize()),
+          quantizer.Copy(blocks), np, (variant & kVarFastAc) != 0, nullptr);
+      codes[gi].append(reinterpret_cast<const char*>(ac.data()), ac.size());
+    }
+    std::vector<uint8_t> toc(
+        DivCeil<size_t>(U32Coder::MaxEncodedBits(kGroupSizeDistribution) * ng,
+                        8) + 8, 0);
+    size_t tpos = 0;
+    for (const std::string& c : codes) {
+      if (!U32Coder::Write(kGroupSizeDistribution, c.size(), &tpos,
+                           toc.data())) {
+        return Fail("group too large for the size table");
+      }
+    }
+    WriteZeroesToByteBoundary(&tpos, toc.data());
+    bytes += BitsToString(toc, tpos);
+    for (const std::string& c : codes) bytes += c;
+  }
+  if (total != nullptr) *total = bytes.size();
+  if (out == nullptr) return 0;
+  if (bytes.size() > cap) return Fail("output buffer too small");
+  memcpy(out, bytes.data(), bytes.size());
+  return 0;
+}
+
+// Decodes one frame with the reference's decoders in PikPassToPixels'
+// order. hdr[16] receives flags, gaborish, predict_lf, predict_hf,
+// adaptive reconstruction, epf enable_adaptive, epf sigma, use_sharpened,
+// num_passes, global_scale, quant_dc, ytox_dc, ytob_dc, the number of full
+// table sets, the number of dots and FrameHeader::size (low 31 bits). The
+// per-block and per-tile arrays are row-major without padding; ac holds
+// up to max_passes passes of 3 planes; dots (up to dots_cap) are the
+// placements in decoding order with their patches (placements of an
+// empty patch without transform, which change nothing, are left out).
+// Where the reference would index outside its own tables (a dequant
+// control field value >= 16, a quant value outside [1, 256], a dequant map
+// entry naming a missing table, a dot position that wrapped below zero, a
+// degenerate dot transform, no passes) this fails instead.
+API int pikref_decode_frame(const uint8_t* data, size_t size, uint32_t xsize,
+                            uint32_t ysize, int32_t* hdr, int16_t* const* dc,
+                            uint8_t* acs_out, int32_t* qf_out,
+                            uint8_t* lut_out, int32_t* ytox_out,
+                            int32_t* ytob_out, uint8_t* dcf_out,
+                            uint8_t* dqmap_out, pikgpu_dot* dots,
+                            size_t dots_cap, int16_t* ac, size_t max_passes,
+                            float* noise, size_t* consumed) {
+  const size_t bx = DivCeil<size_t>(xsize, kBlockDim);
+  const size_t by = DivCeil<size_t>(ysize, kBlockDim);
+  const size_t tx = DivCeil<size_t>(xsize, kColorTileDim);
+  const size_t ty = DivCeil<size_t>(ysize, kColorTileDim);
+  const size_t gxs = DivCeil<size_t>(xsize, kGroupDim);
+  const size_t gys = DivCeil<size_t>(ysize, kGroupDim);
+  const size_t ng = gxs * gys, nb = bx * by;
+  PaddedBytes compressed(size);
+  if (size != 0) memcpy(compressed.data(), data, size);
+  BitReader reader(compressed.data(), size);
+  FrameHeader fh;
+  if (!ReadPassHeader(&reader, &fh)) return Fail("ReadPassHeader failed");
+  if (!reader.JumpToByteBoundary()) return Fail("frame header padding");
+  if (fh.encoding != ImageEncoding::kPasses) return Fail("not kPasses");
+  if (fh.has_alpha) return Fail("alpha");
+  if (fh.num_passes == 0) return Fail("no passes");
+  if (fh.num_passes > max_passes) return Fail("more passes than buffers");
+  if (fh.flags & FrameHeader::kGradientMap) return Fail("gradient map");
+  DequantMatrices matrices(/*need_inv_matrices=*/false);
+  if (!matrices.Decode(&reader)) return Fail("DequantMatrices::Decode");
+  Quantizer quantizer(&matrices, 0, 0);
+  quantizer.Decode(&reader);
+  if (!reader.JumpToByteBoundary()) return Fail("quantizer padding");
+  ColorCorrelationMap cmap(xsize, ysize);
+  DecodeColorMap(&reader, &cmap.ytob_map, &cmap.ytob_dc);
+  DecodeColorMap(&reader, &cmap.ytox_map, &cmap.ytox_dc);
+  AcStrategyImage acs;
+  ImageI qf;
+  ImageB luts;
+  Image3S dcq;
+  if (RefDecodeDcSection(&reader, compressed.data(), size, bx, by, &acs, &qf,
+                         &luts, &dcq) != 0) {
+    return -1;
+  }
+  BlockDictionary dict;
+  if (!dict.Decode(&reader, bx * kBlockDim, by * kBlockDim)) {
+    return Fail("BlockDictionary::Decode failed");
+  }
+  ImageB dcf(tx, ty);
+  if (!DecodeDequantControlField(&reader, &dcf)) {
+    return Fail("DecodeDequantControlField failed");
+  }
+  for (size_t y = 0; y < ty; ++y) {
+    for (size_t x = 0; x < tx; ++x) {
+      if (dcf.Row(y)[x] >= kMaxQuantControlFieldValue) {
+        return Fail("dequant control field value >= 16");
+      }
+    }
+  }
+  for (size_t y = 0; y < by; ++y) {
+    for (size_t x = 0; x < bx; ++x) {
+      const int q = qf.Row(y)[x];
+      if (q < 1 || q > 256) return Fail("quant value outside [1, 256]");
+    }
+  }
+  uint8_t table_map[kMaxQuantControlFieldValue][256];
+  if (!DecodeDequantControlFieldMap(&reader, qf, dcf, table_map)) {
+    return Fail("DecodeDequantControlFieldMap failed");
+  }
+  const size_t num_sets = matrices.encodings_.size() / kNumQuantKinds;
+  for (size_t i = 0; i < kMaxQuantControlFieldValue; ++i) {
+    for (size_t j = 0; j < 256; ++j) {
+      if (table_map[i][j] >= num_sets) return Fail("missing table");
+    }
+  }
+  size_t num_dots = 0;
+  for (const BlockPosition& p : dict.positions_) {
+    const QuantizedBlock& b =
+        p.id < dict.dictionary_.size()
+            ? dict.dictionary_[p.id]
+            : kStaticBlocks[p.id - dict.dictionary_.size()];
+    if (p.x >= bx * kBlockDim || p.y >= by * kBlockDim) {
+      return Fail("dot position wrapped");
+    }
+    if (p.transform && (b.xsize == 0 || b.ysize == 0 ||
+                        (p.dx == 0 && p.dy == 0) || p.width <= 0)) {
+      return Fail("degenerate dot transform");
+    }
+    if (!p.transform && (b.xsize == 0 || b.ysize == 0)) continue;
+    if (num_dots < dots_cap) {
+      pikgpu_dot& d = dots[num_dots];
+      memset(&d, 0, sizeof(d));
+      d.x = uint32_t(p.x);
+      d.y = uint32_t(p.y);
+      d.xsize = uint32_t(b.xsize);
+      d.ysize = uint32_t(b.ysize);
+      d.transform = p.transform;
+      d.dx = int32_t(p.dx);
+      d.dy = int32_t(p.dy);
+      d.width = int32_t(p.width);
+      for (int c = 0; c < 3; ++c) {
+        memcpy(d.pixels[c], b.pixels[c], b.xsize * b.ysize);
+      }
+    }
+    ++num_dots;
+  }
+
+  std::vector<size_t> begin(fh.num_passes);
+  std::vector<std::vector<size_t>> offsets(fh.num_passes);
+  for (size_t i = 0; i < fh.num_passes; ++i) {
+    offsets[i].push_back(0);
+    for (size_t g = 0; g < ng; ++g) {
+      offsets[i].push_back(offsets[i].back() +
+                           U32Coder::Read(kGroupSizeDistribution, &reader));
+    }
+    if (!reader.JumpToByteBoundary()) return Fail("group size padding");
+    begin[i] = reader.Position();
+    reader.SkipBits(offsets[i].back() * kBitsPerByte);
+    if (reader.Position() > size) return Fail("groups past the end");
+  }
+  for (size_t g = 0; g < ng; ++g) {
+    const size_t gx = g % gxs, gy = g / gxs;
+    const Rect blocks(gx * kGroupDimInBlocks, gy * kGroupDimInBlocks,
+                      kGroupDimInBlocks, kGroupDimInBlocks, bx, by);
+    for (size_t i = 0; i < fh.num_passes; ++i) {
+      BitReader br(compressed.data(), begin[i] + offsets[i][g + 1]);
+      br.SkipBits((begin[i] + offsets[i][g]) * kBitsPerByte);
+      if (i == 0) {
+        RefGroupHeader gh;
+        if (!Bundle::Read(&br, &gh)) return Fail("ReadGroupHeader failed");
+        if (!br.JumpToByteBoundary()) return Fail("group header padding");
+      }
+      Image3S group_ac(blocks.xsize() * kDCTBlockSize, blocks.ysize());
+      NoiseParams np;
+      if (RefDecodeAcGroup(&br, acs, blocks, &group_ac, &np) != 0) return -1;
+      if (!br.JumpToByteBoundary()) return Fail("pass padding");
+      int16_t* pass_ac = ac + i * 3 * nb * 64;
+      for (int c = 0; c < 3; ++c) {
+        for (size_t y = 0; y < blocks.ysize(); ++y) {
+          memcpy(pass_ac + c * nb * 64 + (blocks.y0() + y) * bx * 64 +
+                     blocks.x0() * 64,
+                 group_ac.ConstPlaneRow(c, y),
+                 blocks.xsize() * 64 * sizeof(int16_t));
+        }
+      }
+      float* n = noise + (i * ng + g) * 3;
+      n[0] = np.alpha;
+      n[1] = np.gamma;
+      n[2] = np.beta;
+    }
+  }
+
+  hdr[0] = int32_t(fh.flags);
+  hdr[1] = int32_t(fh.gaborish);
+  hdr[2] = fh.predict_lf;
+  hdr[3] = fh.predict_hf;
+  hdr[4] = fh.have_adaptive_reconstruction;
+  hdr[5] = fh.epf_params.enable_adaptive;
+  hdr[6] = int32_t(fh.epf_params.sigma);
+  hdr[7] = fh.epf_params.use_sharpened;
+  hdr[8] = int32_t(fh.num_passes);
+  {
+    // global_scale has no accessor: read it back from the encoding
+    const std::string q = quantizer.Encode(nullptr);
+    hdr[9] = ((uint8_t(q[0]) << 8) | uint8_t(q[1])) + 1;
+  }
+  hdr[10] = quantizer.QuantDC();
+  hdr[11] = cmap.ytox_dc;
+  hdr[12] = cmap.ytob_dc;
+  hdr[13] = int32_t(num_sets);
+  hdr[14] = int32_t(num_dots);
+  hdr[15] = int32_t(fh.size & 0x7FFFFFFF);
+  for (int c = 0; c < 3; ++c) {
+    for (size_t y = 0; y < by; ++y) {
+      memcpy(dc[c] + y * bx, dcq.ConstPlaneRow(c, y), bx * 2);
+    }
+  }
+  for (size_t y = 0; y < by; ++y) {
+    memcpy(acs_out + y * bx, acs.ConstRaw().ConstRow(y), bx);
+    memcpy(qf_out + y * bx, qf.ConstRow(y), bx * 4);
+    memcpy(lut_out + y * bx, luts.ConstRow(y), bx);
+  }
+  for (size_t y = 0; y < ty; ++y) {
+    memcpy(ytox_out + y * tx, cmap.ytox_map.ConstRow(y), tx * 4);
+    memcpy(ytob_out + y * tx, cmap.ytob_map.ConstRow(y), tx * 4);
+    memcpy(dcf_out + y * tx, dcf.ConstRow(y), tx);
+  }
+  memcpy(dqmap_out, table_map, sizeof(table_map));
+  if (consumed != nullptr) *consumed = reader.Position();
   return 0;
 }

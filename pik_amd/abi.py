@@ -200,6 +200,20 @@ class DcSectionC(ctypes.Structure):
     ]
 
 
+class ParsedInfoC(ctypes.Structure):
+    """pikgpu_parsed_info."""
+    _fields_ = [
+        ("frame_size", ctypes.c_uint64),
+        ("num_passes", ctypes.c_uint32),
+        ("num_groups", ctypes.c_uint32),
+        ("epf_sigma", ctypes.c_uint32),
+        ("num_downsampling", ctypes.c_uint32),
+        ("downsampling", ctypes.c_void_p),
+        ("noise", ctypes.c_void_p),
+        ("consumed", ctypes.c_size_t),
+    ]
+
+
 def load():
     """Loads libpikgpu.so (the HIP path). Raises if it is missing."""
     global _lib
@@ -269,6 +283,18 @@ def load():
     lib.pikgpu_decode_dc_groups.argtypes = [
         ctypes.POINTER(DcSectionC), ctypes.c_int, ctypes.c_char_p,
         ctypes.c_size_t]
+    lib.pikgpu_parse_frame.argtypes = [
+        vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+        ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+    lib.pikgpu_parsed_get.argtypes = [vp, ctypes.POINTER(FrameC),
+                                      ctypes.POINTER(ParsedInfoC)]
+    lib.pikgpu_parsed_pass_ac.argtypes = [vp, ctypes.c_uint32, vp * 3,
+                                          ctypes.POINTER(ctypes.c_size_t)]
+    lib.pikgpu_parsed_free.argtypes = [vp]
+    lib.pikgpu_parsed_free.restype = None
+    lib.pikgpu_decode_frame.argtypes = [
+        vp, vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+        ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
     _lib = lib
     return lib
 
@@ -285,5 +311,6 @@ EXPORTED_SYMBOLS = [
     "pikgpu_reconstruct_batch", "pikgpu_fwd_adj", "pikgpu_set_downsample", "pikgpu_quantize",
     "pikgpu_frame_add_pass", "pikgpu_reconstruct_noalloc",
     "pikgpu_search_quant_field", "pikgpu_decode_ac_pass",
-    "pikgpu_decode_dc_groups",
+    "pikgpu_decode_dc_groups", "pikgpu_parse_frame", "pikgpu_parsed_get",
+    "pikgpu_parsed_pass_ac", "pikgpu_parsed_free", "pikgpu_decode_frame",
 ]

@@ -99,3 +99,86 @@ def decode_dc_groups(xsize, ysize, data, threads=0):
         raise PikGpuError(rc, msg.value.decode())
     return dict(dc=dc, ac_strategy=acs, quant_field=qf, sigma_lut_ids=lut,
                 consumed=int(s.consumed))
+
+
+def parse_frame(data, xsize, ysize, threads=0):
+    """Parses one whole frame (FrameHeader to the end of its last pass;
+    `data` may run on past it) of an xsize x ysize image. Returns
+    (frame, info): frame a Frame holding pass 0's AC, info a dict with
+    passes (list of (3, by, bx*64) int16, pass 0 first), noise
+    (passes, groups, 3), frame_size, epf_sigma, downsampling (list of
+    (factor, passes)) and consumed (bytes of the frame). Raises PikGpuError
+    (abi.EDATA for an invalid frame, abi.EUNSUPPORTED for one outside this
+    decoder)."""
+    from .frame import DOT_DTYPE, Frame
+    lib = abi.load()
+    raw = np.frombuffer(bytes(data), np.uint8) if len(data) else \
+        np.zeros(1, np.uint8)
+    handle = ctypes.c_void_p()
+    msg = ctypes.create_string_buffer(256)
+    rc = lib.pikgpu_parse_frame(raw.ctypes.data, len(data), xsize, ysize,
+                                int(threads), ctypes.byref(handle), msg,
+                                len(msg))
+    if rc != abi.OK:
+        raise PikGpuError(rc, msg.value.decode())
+    try:
+        fc = abi.FrameC()
+        info = abi.ParsedInfoC()
+        lib.pikgpu_parsed_get(handle, ctypes.byref(fc), ctypes.byref(info))
+        by, bx = -(-ysize // 8), -(-xsize // 8)
+        ty, tx = -(-ysize // 64), -(-xsize // 64)
+
+        def arr(ptr, dtype, shape):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.zeros(shape, dtype)
+            ct = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)),
+                                      (n,))
+            return a.reshape(shape).copy()
+
+        passes = []
+        for i in range(info.num_passes):
+            planes = (ctypes.c_void_p * 3)()
+            stride = ctypes.c_size_t()
+            lib.pikgpu_parsed_pass_ac(handle, i, planes, ctypes.byref(stride))
+            passes.append(np.stack([arr(planes[c], np.int16, (by, bx * 64))
+                                    for c in range(3)]))
+        dots = np.zeros(fc.num_dots, DOT_DTYPE)
+        if fc.num_dots:
+            ctypes.memmove(dots.ctypes.data, fc.dots,
+                           fc.num_dots * DOT_DTYPE.itemsize)
+        frame = Frame(
+            xsize=xsize, ysize=ysize, ac=passes[0],
+            dc=np.stack([arr(fc.dc[c], np.int16, (by, bx)) for c in range(3)]),
+            quant_field=arr(fc.quant_field, np.int32, (by, bx)),
+            ac_strategy=arr(fc.ac_strategy, np.uint8, (by, bx)),
+            sigma_lut_ids=arr(fc.sigma_lut_ids, np.uint8, (by, bx)),
+            dequant_cf=arr(fc.dequant_cf, np.uint8, (ty, tx)),
+            ytox_map=arr(fc.ytox_map, np.int32, (ty, tx)),
+            ytob_map=arr(fc.ytob_map, np.int32, (ty, tx)),
+            dequant_map=arr(fc.dequant_map, np.uint8, (16, 256)),
+            dequant_table=arr(fc.dequant_table, np.float32,
+                              (fc.dequant_table_size,)),
+            dequant_offsets=arr(fc.dequant_offsets, np.uint32,
+                                (fc.num_dequant_offsets,)),
+            dots=dots, global_scale=fc.global_scale, quant_dc=fc.quant_dc,
+            ytox_dc=fc.ytox_dc, ytob_dc=fc.ytob_dc, gaborish=fc.gaborish,
+            predict_lf=fc.predict_lf, predict_hf=fc.predict_hf,
+            adaptive_reconstruction=fc.adaptive_reconstruction,
+            epf_enable_adaptive=fc.epf_enable_adaptive,
+            epf_use_sharpened=fc.epf_use_sharpened, grayscale=fc.grayscale,
+            flags=fc.flags)
+        ng = info.num_groups
+        ds = arr(info.downsampling, np.uint32, (2 * info.num_downsampling,))
+        out = dict(passes=passes,
+                   noise=arr(info.noise, np.float32,
+                             (info.num_passes, ng, 3)),
+                   frame_size=int(info.frame_size),
+                   epf_sigma=int(info.epf_sigma),
+                   downsampling=[(int(ds[2 * i]), int(ds[2 * i + 1]))
+                                 for i in range(info.num_downsampling)],
+                   consumed=int(info.consumed))
+        return frame, out
+    finally:
+        lib.pikgpu_parsed_free(handle)

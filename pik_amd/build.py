@@ -20,7 +20,8 @@ ARCH = os.environ.get("PIKGPU_ARCH", "gfx950")
 
 SOURCES = ["recon_lf.hip", "recon_blocks.hip", "filter.hip", "adjoint.hip",
            "encode.hip", "search.hip", "api.hip", "noalloc.hip",
-           "host_tables.cpp", "quant_tables.cpp", "entropy.cpp", "dc_groups.cpp"]
+           "host_tables.cpp", "quant_tables.cpp", "entropy.cpp", "dc_groups.cpp",
+           "frame_parse.cpp"]
 HEADERS = ["common.h", "device.h", "kernels.h", "xform.h", "dct1d.h",
            "dct_consts.h", "host_api.h", "entropy.h"]
 HOST_OUT = os.path.join(HERE, "libpikgpu_host.so")

@@ -1149,6 +1149,31 @@ int pikgpu_frame_add_pass(pikgpu_ctx* ctx, const int16_t* const ac[3],
   return PIKGPU_OK;
 }
 
+int pikgpu_decode_frame(pikgpu_ctx* ctx, const uint8_t* data, size_t size,
+                        uint32_t xsize, uint32_t ysize, int threads,
+                        size_t* consumed) {
+  if (ctx == nullptr) return PIKGPU_EINVAL;
+  char msg[256];
+  pikgpu_parsed* parsed = nullptr;
+  int rc = pikgpu_parse_frame(data, size, xsize, ysize, threads, &parsed, msg,
+                              sizeof(msg));
+  if (rc != PIKGPU_OK) return Fail(ctx, rc, msg);
+  pikgpu_frame f;
+  pikgpu_parsed_info info;
+  pikgpu_parsed_get(parsed, &f, &info);
+  rc = pikgpu_frame_upload(ctx, &f);
+  for (uint32_t i = 1; rc == PIKGPU_OK && i < info.num_passes; ++i) {
+    const int16_t* ac[3];
+    size_t stride = 0;
+    pikgpu_parsed_pass_ac(parsed, i, ac, &stride);
+    rc = pikgpu_frame_add_pass(ctx, ac, stride);
+  }
+  if (rc == PIKGPU_OK) rc = pikgpu_run(ctx, 0);
+  if (rc == PIKGPU_OK && consumed != nullptr) *consumed = info.consumed;
+  pikgpu_parsed_free(parsed);
+  return rc;
+}
+
 // Encoder-side quantization of an XYB image under the uploaded frame's side
 // information (see pikgpu.h). The quantized AC and DC replace the frame's,
 // so pikgpu_run then decodes the round trip.

@@ -393,10 +393,11 @@ void SetMsg(char* msg, size_t n, const char* fmt, ...) {
 }  // namespace ent
 }  // namespace pikgpu
 
-extern "C" PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* s,
-                                                  int threads, char* msg,
-                                                  size_t msg_size) {
-  using namespace pikgpu::ent;
+namespace pikgpu {
+namespace ent {
+
+int DecodeDcSection(pikgpu_dc_section* s, size_t skip_bits, int threads,
+                    char* msg, size_t msg_size) {
   if (msg != nullptr && msg_size > 0) msg[0] = '\0';
   if (s == nullptr || (s->data == nullptr && s->size != 0) ||
       s->dc[0] == nullptr || s->dc[1] == nullptr || s->dc[2] == nullptr ||
@@ -423,6 +424,7 @@ extern "C" PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* s,
   // (SizeCoderT<0x150F0E0C>, pik/size_coder.h:17-33, U32Coder,
   // pik/fields.h:100-115).
   BitStream bs(s->data, s->size);
+  bs.SkipBits(skip_bits);
   static const unsigned kSizeBits[4] = {12, 14, 15, 21};
   std::vector<size_t> off(ng + 1, 0);
   for (size_t i = 0; i < ng; ++i) {
@@ -489,4 +491,13 @@ extern "C" PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* s,
   }
   s->consumed = begin + off[ng];
   return PIKGPU_OK;
+}
+
+}  // namespace ent
+}  // namespace pikgpu
+
+extern "C" PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* s,
+                                                  int threads, char* msg,
+                                                  size_t msg_size) {
+  return pikgpu::ent::DecodeDcSection(s, 0, threads, msg, msg_size);
 }

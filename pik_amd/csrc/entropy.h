@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -74,6 +75,21 @@ class BitStream {
   bool JumpToByteBoundary() {
     const unsigned r = unsigned(BitsRead() & 7);
     return r == 0 || Read(8 - r) == 0;
+  }
+  // Skips n bits. A skip past the end lands 8 bytes beyond it (everything
+  // there reads as zeros, and Position() reports more than the size).
+  void SkipBits(uint64_t n) {
+    if (n <= avail_) {
+      Skip(unsigned(n));
+      return;
+    }
+    const uint64_t limit = uint64_t(size_) * 8 + 64;
+    const uint64_t at = BitsRead();
+    const uint64_t to = n > limit - std::min(at, limit) ? limit : at + n;
+    next_ = size_t(to >> 3);
+    buf_ = 0;
+    avail_ = 0;
+    if (to & 7) Read(unsigned(to & 7));
   }
 
  private:
@@ -171,6 +187,20 @@ void ReadCoeffOrder(BitStream* bs, int32_t order[64]);
 
 // The zig-zag scan of an 8x8 block (row-major positions).
 const int32_t* ZigZag8();
+
+}  // namespace ent
+}  // namespace pikgpu
+
+struct pikgpu_dc_section;
+
+namespace pikgpu {
+namespace ent {
+
+// pikgpu_decode_dc_groups with the size table starting skip_bits into
+// s->data (the frame parser reads it from wherever the colour maps left
+// the reader); s->consumed counts from s->data.
+int DecodeDcSection(pikgpu_dc_section* s, size_t skip_bits, int threads,
+                    char* msg, size_t msg_size);
 
 }  // namespace ent
 }  // namespace pikgpu

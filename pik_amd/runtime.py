@@ -8,6 +8,7 @@ Errors raise PikGpuError with the library's message (the reference returns a
 false Status).
 """
 import ctypes
+import types
 
 import numpy as np
 
@@ -74,6 +75,24 @@ class Reconstructor:
         self._check(self._lib.pikgpu_frame_upload(self._ctx, ctypes.byref(fc)))
         self._frame = frame
         self._keep = (fc, keep)
+
+    def decode_frame(self, data, xsize, ysize, threads=0):
+        """Parses one whole frame of an xsize x ysize image on the host and
+        reconstructs it (pikgpu_decode_frame: parse, upload, every further
+        pass, run); download() then returns the image. Returns the frame's
+        length in bytes."""
+        raw = np.frombuffer(bytes(data), np.uint8) if len(data) else \
+            np.zeros(1, np.uint8)
+        used = ctypes.c_size_t()
+        self._frame = None
+        self._check(self._lib.pikgpu_decode_frame(
+            self._ctx, raw.ctypes.data, len(data), xsize, ysize,
+            int(threads), ctypes.byref(used)))
+        self._frame = types.SimpleNamespace(
+            xsize=xsize, ysize=ysize, xsize_blocks=-(-xsize // 8),
+            ysize_blocks=-(-ysize // 8))
+        self._keep = None
+        return int(used.value)
 
     def run(self, keep_stages=False):
         self._check(self._lib.pikgpu_run(self._ctx, int(bool(keep_stages))))
