@@ -23,10 +23,11 @@ Two further figures ride on the same line:
            respect to the 256x256 float quant field and the dequantised
            coefficients; MP/s.
   quantize encoder-side quantization (pikgpu_quantize) of a 4096x4096 XYB
-           image without predictions: forward transforms, DC and dithered
-           AC quantization; MP/s of the kernels (HIP events), and the rate
-           with the host-to-device copy of the float image and the int16
-           downloads included.
+           image with the LF and HF prediction residuals (the reference
+           encoder's default; a figure without them beside it): forward
+           transforms, DC, residuals and dithered AC quantization; MP/s of
+           the kernels (HIP events), and the rate with the host-to-device
+           copy of the float image and the int16 downloads included.
   search   gradient quant-field search (pikgpu_search_quant_field) of 64
            1024x1024 images at Butteraugli distance 1.0, 20 Adam steps each
            from the reference encoder's initial field (committed fixture),
@@ -39,7 +40,11 @@ Two further figures ride on the same line:
 The reconstruction figure also carries `with_copies_MPps` (the drop-in
 call: frame upload from pinned host memory, run, linear download into
 pinned memory) and, in the CPU-baseline leg, the deviation of the GPU output
-from the reference's default AVX2 build (whose EPF divides with rcpps).
+from the reference's default AVX2 build (whose EPF divides with rcpps), the
+host entropy decoding of the frame's AC groups and the parse of the whole
+frame written as a bitstream by the reference encoder, each beside the
+reference decoder on the same bytes, and the bitstream-to-image rate
+(parse, upload, reconstruction).
 """
 import argparse
 import json
@@ -249,6 +254,84 @@ def entropy_baseline(frame, threads, seconds=3.0):
                       "bitstreams (full histogram clustering); best of "
                       "%.0f s each; int16 output bit-equal to the "
                       "reference's" % (frame.xsize, frame.ysize, seconds)}
+
+
+def frame_baseline(R, frame, threads, seconds=3.0):
+    """The whole frame as a bitstream: written by the reference encoder's
+    pieces (pass header, tables, quantizer, colour maps, DC groups, dots,
+    dequant control field and map, group sizes, groups), parsed by the
+    reference decoder's pieces single-threaded (the reference parses the
+    pass-global sections on one thread and the groups on a pool) and by
+    pikgpu_parse_frame on 1 and `threads` host threads, and decoded end to
+    end on the GPU (pikgpu_decode_frame: parse, upload from pageable host
+    memory, reconstruction; no download). The parsed frame is checked
+    against the reference's parse. Best of repeated runs each."""
+    import ctypes
+    import numpy as np
+    from oracle import ref
+    from pik_amd import abi, bitstream
+    if not ref.available("avx2"):
+        return None
+    data = ref.encode_frame(frame, kind="avx2")
+    mp = frame.xsize * frame.ysize / 1e6
+    bufs = {}
+    want = ref.decode_frame(data, frame.xsize, frame.ysize, max_passes=1,
+                            kind="avx2", reuse=bufs)
+    got, info = bitstream.parse_frame(data, frame.xsize, frame.ysize,
+                                      threads=threads)
+    for k in ("dc", "ac_strategy", "quant_field", "sigma_lut_ids",
+              "ytox_map", "ytob_map", "dequant_cf", "dequant_map"):
+        assert np.array_equal(getattr(got, k), want[k]), k
+    assert np.array_equal(got.ac, want["passes"][0])
+    assert np.array_equal(got.dots, want["dots"])
+    assert info["consumed"] == len(data) == want["consumed"]
+    del got, info, want
+
+    def best(fn):
+        t_end = time.perf_counter() + seconds
+        b = 1e30
+        while True:
+            t = time.perf_counter()
+            fn()
+            b = min(b, time.perf_counter() - t)
+            if time.perf_counter() > t_end:
+                return b
+
+    t_ref = best(lambda: ref.decode_frame(data, frame.xsize, frame.ysize,
+                                          max_passes=1, kind="avx2",
+                                          reuse=bufs))
+    del bufs
+    lib = abi.load()
+    raw = np.frombuffer(data, np.uint8)
+
+    def parse(n):
+        h = ctypes.c_void_p()
+        rc = lib.pikgpu_parse_frame(raw.ctypes.data, len(data), frame.xsize,
+                                    frame.ysize, n, ctypes.byref(h), None, 0)
+        assert rc == abi.OK
+        lib.pikgpu_parsed_free(h)
+
+    figures = {}
+    for n in sorted({1, threads}):
+        figures["ours_%dt" % n] = round(mp / best(lambda: parse(n)), 1)
+
+    def decode():
+        R.decode_frame(data, frame.xsize, frame.ysize, threads=threads)
+        R.sync()
+
+    t_e2e = best(decode)
+    return {"unit": "MP/s", "bytes": len(data),
+            "reference_1t": round(mp / t_ref, 1), **figures,
+            "threads": threads,
+            "over_reference_1t": round(figures["ours_1t"] * t_ref / mp, 2),
+            "bitstream_to_image_MPps": round(mp / t_e2e, 1),
+            "sample": "the %dx%d frame written whole by the reference "
+                      "encoder's pieces (%d bytes); host parse of every "
+                      "section into the frame the GPU takes, bit-equal to "
+                      "the reference's; bitstream_to_image: parse on %d "
+                      "threads + upload + reconstruction; best of %.0f s "
+                      "each" % (frame.xsize, frame.ysize, len(data), threads,
+                                seconds)}
 
 
 def srgb8_f64(linear):
@@ -636,12 +719,15 @@ def main():
 
     cpu = None
     threads = a.cpu_threads or host_threads()
-    entropy = None
+    entropy = frame_parse = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(frame, a.cpu_seconds, threads, gpu_out)
         entropy = entropy_baseline(frame, threads)
         print("host entropy: " + json.dumps(entropy), file=sys.stderr,
               flush=True)
+        frame_parse = frame_baseline(R, frame, threads)
+        print("host frame parse: " + json.dumps(frame_parse),
+              file=sys.stderr, flush=True)
     acs = frame.ac_strategy
     first = (acs & 15) == 0
     names = {0: "DCT8", 1: "IDENTITY", 2: "DCT2X2", 3: "DCT4X4", 4: "DCT16",
@@ -683,6 +769,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "host_entropy": entropy,
+        "host_frame_parse": frame_parse,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)

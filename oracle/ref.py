@@ -488,13 +488,14 @@ FRAME_HEADER_FIELDS = ("flags", "gaborish", "predict_lf", "predict_hf",
 
 
 def decode_frame(data, xsize, ysize, max_passes=16, max_dots=4096,
-                 kind="scalar"):
+                 kind="scalar", reuse=None):
     """The reference decoders' reading of one frame: dict with the header
     fields (FRAME_HEADER_FIELDS), dc, ac_strategy, quant_field,
     sigma_lut_ids, ytox_map, ytob_map, dequant_cf, dequant_map, dots
     (structured, frame.DOT_DTYPE), passes (list of (3, by, bx*64) int16),
     noise (passes, groups, 3) and consumed. Raises RuntimeError where the
-    reference fails."""
+    reference fails. `reuse` (a dict, may be empty) keeps the output
+    buffers between calls of the same sizes; the result then views them."""
     from pik_amd.frame import DOT_DTYPE
     L = lib(kind)
     vp = ctypes.c_void_p
@@ -505,19 +506,29 @@ def decode_frame(data, xsize, ysize, max_passes=16, max_dots=4096,
     ty, tx = -(-ysize // 64), -(-xsize // 64)
     ng = (-(-xsize // 512)) * (-(-ysize // 512))
     buf = np.frombuffer(bytes(data) + b"\0", np.uint8)
-    hdr = np.zeros(16, np.int32)
-    dc = np.zeros((3, by, bx), np.int16)
+    key = (xsize, ysize, max_passes, max_dots)
+    if reuse is None or reuse.get("key") != key:
+        b = dict(key=key, hdr=np.zeros(16, np.int32),
+                 dc=np.zeros((3, by, bx), np.int16),
+                 acs=np.zeros((by, bx), np.uint8),
+                 qf=np.zeros((by, bx), np.int32),
+                 lut=np.zeros((by, bx), np.uint8),
+                 ytox=np.zeros((ty, tx), np.int32),
+                 ytob=np.zeros((ty, tx), np.int32),
+                 dcf=np.zeros((ty, tx), np.uint8),
+                 dqmap=np.zeros((16, 256), np.uint8),
+                 dots=np.zeros(max_dots, DOT_DTYPE),
+                 ac=np.zeros((max_passes, 3, by, bx * 64), np.int16),
+                 noise=np.zeros((max_passes, ng, 3), np.float32))
+        if reuse is not None:
+            reuse.clear()
+            reuse.update(b)
+    else:
+        b = reuse
+    hdr, dc, acs, qf, lut = b["hdr"], b["dc"], b["acs"], b["qf"], b["lut"]
+    ytox, ytob, dcf, dqmap = b["ytox"], b["ytob"], b["dcf"], b["dqmap"]
+    dots, ac, noise = b["dots"], b["ac"], b["noise"]
     dptr = (vp * 3)(*[dc[c].ctypes.data for c in range(3)])
-    acs = np.zeros((by, bx), np.uint8)
-    qf = np.zeros((by, bx), np.int32)
-    lut = np.zeros((by, bx), np.uint8)
-    ytox = np.zeros((ty, tx), np.int32)
-    ytob = np.zeros((ty, tx), np.int32)
-    dcf = np.zeros((ty, tx), np.uint8)
-    dqmap = np.zeros((16, 256), np.uint8)
-    dots = np.zeros(max_dots, DOT_DTYPE)
-    ac = np.zeros((max_passes, 3, by, bx * 64), np.int16)
-    noise = np.zeros((max_passes, ng, 3), np.float32)
     used = ctypes.c_size_t()
     rc = L.pikref_decode_frame(
         buf.ctypes.data, len(data), xsize, ysize, hdr.ctypes.data,

@@ -18,6 +18,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -29,7 +30,10 @@ struct pikgpu_parsed {
   pikgpu_frame frame;
   pikgpu_parsed_info info;
   uint32_t bx = 0, by = 0;
-  std::vector<int16_t> dc, ac;  // ac: passes x 3 planes
+  std::vector<int16_t> dc;
+  // passes x 3 planes; every entry is written by the AC decoding, so the
+  // array is not cleared first
+  std::unique_ptr<int16_t[]> ac;
   std::vector<int32_t> qf, ytox, ytob;
   std::vector<uint8_t> acs, lut, dcf, dqmap;
   std::vector<float> table, noise;
@@ -580,7 +584,7 @@ int Parse(const uint8_t* data, size_t size, uint32_t xsize, uint32_t ysize,
   }
   const size_t consumed = pos + rest.Position();
 
-  p->ac.resize(size_t(h.num_passes) * 3 * nb * 64);
+  p->ac.reset(new int16_t[size_t(h.num_passes) * 3 * nb * 64]);
   p->noise.assign(size_t(h.num_passes) * ng * 3, 0.0f);
   std::vector<const uint8_t*> gdata(ng);
   std::vector<size_t> gsize(ng);
@@ -616,7 +620,7 @@ int Parse(const uint8_t* data, size_t size, uint32_t xsize, uint32_t ysize,
     a.num_groups = ng;
     a.group_data = gdata.data();
     a.group_size = gsize.data();
-    int16_t* base = p->ac.data() + size_t(i) * 3 * nb * 64;
+    int16_t* base = p->ac.get() + size_t(i) * 3 * nb * 64;
     for (int c = 0; c < 3; ++c) a.ac[c] = base + c * nb * 64;
     a.ac_stride = size_t(bx) * 64;
     a.noise = p->noise.data() + size_t(i) * ng * 3;
@@ -642,7 +646,7 @@ int Parse(const uint8_t* data, size_t size, uint32_t xsize, uint32_t ysize,
   f.global_scale = global_scale;
   f.quant_dc = quant_dc;
   for (int c = 0; c < 3; ++c) {
-    f.ac[c] = p->ac.data() + c * nb * 64;
+    f.ac[c] = p->ac.get() + c * nb * 64;
     f.dc[c] = p->dc.data() + c * nb;
   }
   f.ac_stride = size_t(bx) * 64;
@@ -732,7 +736,7 @@ extern "C" PIKGPU_API int pikgpu_parsed_pass_ac(const pikgpu_parsed* p,
     return PIKGPU_EINVAL;
   }
   const size_t nb = size_t(p->bx) * p->by;
-  const int16_t* base = p->ac.data() + size_t(pass) * 3 * nb * 64;
+  const int16_t* base = p->ac.get() + size_t(pass) * 3 * nb * 64;
   for (int c = 0; c < 3; ++c) ac[c] = base + c * nb * 64;
   if (ac_stride != nullptr) *ac_stride = size_t(p->bx) * 64;
   return PIKGPU_OK;
