@@ -17,7 +17,9 @@ same frame on this host's cores.
 Two further figures ride on the same line:
   batch    256 frames of seeded uneven sizes (W, H ~ U[256, 2048], distance
            ~ U[0.5, 3.0]; 32 distinct synthetic frames cycled) reconstructed
-           by one graph launch per step (pikgpu_batch_run); MP/s.
+           by one graph launch per step (pikgpu_batch_run: one launch per
+           phase over all frames, in the schedule pik_amd/autotune.py
+           chose); MP/s.
   fwd_adj  forward model + adjoint (pikgpu_fwd_adj, device pointers) of a
            2048x2048 frame, loss sum (linear - target)^2, gradient with
            respect to the 256x256 float quant field and the dequantised
@@ -304,16 +306,19 @@ def frame_baseline(R, frame, threads, seconds=3.0):
     lib = abi.load()
     raw = np.frombuffer(data, np.uint8)
 
+    # the parsed frame is reused from call to call, as the reference's
+    # buffers are
+    h = ctypes.c_void_p()
+
     def parse(n):
-        h = ctypes.c_void_p()
         rc = lib.pikgpu_parse_frame(raw.ctypes.data, len(data), frame.xsize,
                                     frame.ysize, n, ctypes.byref(h), None, 0)
         assert rc == abi.OK
-        lib.pikgpu_parsed_free(h)
 
     figures = {}
     for n in sorted({1, threads}):
         figures["ours_%dt" % n] = round(mp / best(lambda: parse(n)), 1)
+    lib.pikgpu_parsed_free(h)
 
     def decode():
         R.decode_frame(data, frame.xsize, frame.ysize, threads=threads)
@@ -402,16 +407,10 @@ def bench_per_image(frames, reps):
 
 
 def _batch_frame(args):
-    """Synthetic frame i of the batch: size and distance drawn from the
-    seeded generator."""
-    i, seed = args
-    from pik_amd import synth
-    import numpy as np
-    rng = np.random.default_rng(seed * 1000003 + i)
-    w, h = (int(v) for v in rng.integers(256, 2049, size=2))
-    d = float(rng.uniform(0.5, 3.0))
-    return synth.make_frame(w, h, seed=seed * 1000 + i, distance=d,
-                            num_rects=40, num_dots=4)
+    """Synthetic frame i of the batch (pik_amd.autotune's workload: size
+    and distance drawn from the seeded generator)."""
+    from pik_amd import autotune
+    return autotune.workload_frame(*args)
 
 
 def synth_batch(n, distinct, seed):
@@ -437,6 +436,8 @@ def bench_batch(frames, distinct, steps, warmup):
     n = len(frames)
     mp = sum(f.xsize * f.ysize for f in frames) / 1e6
     with BatchReconstructor(0) as B:
+        tuned = B.load_schedule()
+        sched = B.schedule()
         B.upload(frames)
         for _ in range(max(1, warmup)):
             B.run()
@@ -453,6 +454,12 @@ def bench_batch(frames, distinct, steps, warmup):
             "device_ms_per_step": round(dev_ms / steps, 3),
             "frames": n, "distinct_frames": distinct,
             "megapixels": round(mp, 2), "steps": steps,
+            "schedule": {"mode": ("queue", "branches")[sched["mode"]],
+                         "order": ("upload", "largest first",
+                                   "smallest first")[sched["order"]],
+                         "persist": sched["persist"],
+                         "source": "pik_amd/batch_schedule.json (autotuner)"
+                                   if tuned else "library default"},
             "workload": "batch of %d frames, W,H ~ U[256,2048], d ~ "
                         "U[0.5,3.0], one graph launch per step" % n}
 

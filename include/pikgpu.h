@@ -352,7 +352,9 @@ PIKGPU_API void* pikgpu_stream(pikgpu_ctx* ctx);
  * reference decodes one frame at a time (PikPassToPixels per frame,
  * pik/pik_frame.cc:1106-1181); a batch replaces a loop of those calls. Each
  * frame keeps its own device buffers; pikgpu_batch_run launches one hipGraph
- * holding every frame's kernels in parallel branches. */
+ * holding, per phase of the reconstruction, one launch over the work of
+ * every frame (see pikgpu_batch_schedule). Frames run at full size, linear
+ * output, single pass. */
 typedef struct pikgpu_batch pikgpu_batch;
 
 PIKGPU_API pikgpu_batch* pikgpu_batch_create(int device,
@@ -373,6 +375,33 @@ PIKGPU_API float pikgpu_batch_last_ms(const pikgpu_batch* batch);
 PIKGPU_API int pikgpu_batch_download(pikgpu_batch* batch, int i,
                                      float* out_planes[3],
                                      size_t out_stride_bytes);
+/* How a batch runs (pikgpu_batch_set_schedule). PIKGPU_BATCH_QUEUE: one
+ * launch per phase of the reconstruction over every frame of the batch,
+ * each frame's work items (its plain launch's workgroups) placed one after
+ * the other in the launch in `order`; a phase with persist[p] = k > 0 runs
+ * k workgroups per CU, each looping over the work items (the phases whose
+ * registers allow it; the others run one workgroup per item).
+ * PIKGPU_BATCH_BRANCHES: every frame's own launch sequence, the frames
+ * spread over parallel branches of the graph. The schedule is chosen by
+ * measurement (pik_amd/autotune.py); the default is the queue with the
+ * largest frames first and one workgroup per work item. */
+#define PIKGPU_BATCH_PHASES 19
+enum { PIKGPU_BATCH_QUEUE = 0, PIKGPU_BATCH_BRANCHES = 1 };
+typedef struct pikgpu_batch_schedule {
+  int32_t mode;   /* PIKGPU_BATCH_QUEUE or PIKGPU_BATCH_BRANCHES */
+  int32_t order;  /* 0: upload order; 1: largest frame first; 2: smallest
+                     first */
+  int32_t persist[PIKGPU_BATCH_PHASES]; /* per phase, in the order of
+                     pikgpu_batch_phase_name; 0 .. 16 */
+} pikgpu_batch_schedule;
+PIKGPU_API int pikgpu_batch_set_schedule(pikgpu_batch* batch,
+                                         const pikgpu_batch_schedule* s);
+PIKGPU_API int pikgpu_batch_get_schedule(const pikgpu_batch* batch,
+                                         pikgpu_batch_schedule* s);
+/* Name of phase p and whether it can run persistent (returns NULL past the
+ * last phase). */
+PIKGPU_API const char* pikgpu_batch_phase_name(int p, int* can_persist);
+
 /* Upload + run + download: out_planes holds 3 plane pointers per frame,
  * out_stride_bytes one row stride per frame. */
 PIKGPU_API int pikgpu_reconstruct_batch(pikgpu_batch* batch,
@@ -615,8 +644,11 @@ PIKGPU_API int pikgpu_decode_dc_groups(pikgpu_dc_section* section,
  * file (the reference's reader spans the whole file); xsize / ysize are the
  * FileHeader's image size (the file header itself, with its colour
  * metadata, is not parsed here). Groups decode on `threads` host threads
- * (<= 0: one per hardware thread). On success *out receives a parsed frame
- * that owns every array; release it with pikgpu_parsed_free.
+ * (<= 0: one per hardware thread). *out is NULL or a parsed frame of an
+ * earlier call, whose storage is then reused (its contents are unspecified
+ * after a failure, when *out is left as it was); on success *out receives
+ * the parsed frame, which owns every array; release it with
+ * pikgpu_parsed_free.
  *
  * Returns PIKGPU_OK; PIKGPU_EINVAL for bad arguments; PIKGPU_EDATA for an
  * invalid frame, wherever the reference decoder fails and where it would

@@ -200,6 +200,19 @@ class DcSectionC(ctypes.Structure):
     ]
 
 
+BATCH_PHASES = 19
+BATCH_QUEUE, BATCH_BRANCHES = 0, 1
+
+
+class ScheduleC(ctypes.Structure):
+    """pikgpu_batch_schedule."""
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("persist", ctypes.c_int32 * BATCH_PHASES),
+    ]
+
+
 class ParsedInfoC(ctypes.Structure):
     """pikgpu_parsed_info."""
     _fields_ = [
@@ -283,6 +296,11 @@ def load():
     lib.pikgpu_decode_dc_groups.argtypes = [
         ctypes.POINTER(DcSectionC), ctypes.c_int, ctypes.c_char_p,
         ctypes.c_size_t]
+    lib.pikgpu_batch_set_schedule.argtypes = [vp, ctypes.POINTER(ScheduleC)]
+    lib.pikgpu_batch_get_schedule.argtypes = [vp, ctypes.POINTER(ScheduleC)]
+    lib.pikgpu_batch_phase_name.argtypes = [ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int)]
+    lib.pikgpu_batch_phase_name.restype = ctypes.c_char_p
     lib.pikgpu_parse_frame.argtypes = [
         vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
         ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
@@ -313,4 +331,6 @@ EXPORTED_SYMBOLS = [
     "pikgpu_search_quant_field", "pikgpu_decode_ac_pass",
     "pikgpu_decode_dc_groups", "pikgpu_parse_frame", "pikgpu_parsed_get",
     "pikgpu_parsed_pass_ac", "pikgpu_parsed_free", "pikgpu_decode_frame",
+    "pikgpu_batch_set_schedule", "pikgpu_batch_get_schedule",
+    "pikgpu_batch_phase_name",
 ]

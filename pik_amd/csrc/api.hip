@@ -61,6 +61,8 @@ struct pikgpu_ctx {
   DevBuf adj_qf, adj_in, adj_lin, adj_dqf, adj_dc, adj_loss;
   // pikgpu_quantize: XYB input, coefficients, DC, the inverse dequant table
   DevBuf eopsin, ecoef, edcf, itable, ecorner;
+  // host storage of pikgpu_decode_frame's parse, reused frame to frame
+  pikgpu_parsed* parsed = nullptr;
   // pikgpu_search_quant_field: log field, Adam moments, weights, per-block
   // distortion, rate partials, the rate weight, loss / rate histories
   DevBuf srch;
@@ -242,6 +244,7 @@ void pikgpu_destroy(pikgpu_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   FreeAll(ctx);
+  pikgpu_parsed_free(ctx->parsed);
   if (ctx->consts) (void)hipFree(ctx->consts);
   for (int i = 0; i <= kNumRunTimers; ++i) {
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -1154,10 +1157,12 @@ int pikgpu_decode_frame(pikgpu_ctx* ctx, const uint8_t* data, size_t size,
                         size_t* consumed) {
   if (ctx == nullptr) return PIKGPU_EINVAL;
   char msg[256];
-  pikgpu_parsed* parsed = nullptr;
-  int rc = pikgpu_parse_frame(data, size, xsize, ysize, threads, &parsed, msg,
-                              sizeof(msg));
+  // the context keeps its parsed frame, so the next frame's host arrays
+  // reuse its storage
+  int rc = pikgpu_parse_frame(data, size, xsize, ysize, threads, &ctx->parsed,
+                              msg, sizeof(msg));
   if (rc != PIKGPU_OK) return Fail(ctx, rc, msg);
+  const pikgpu_parsed* parsed = ctx->parsed;
   pikgpu_frame f;
   pikgpu_parsed_info info;
   pikgpu_parsed_get(parsed, &f, &info);
@@ -1170,7 +1175,6 @@ int pikgpu_decode_frame(pikgpu_ctx* ctx, const uint8_t* data, size_t size,
   }
   if (rc == PIKGPU_OK) rc = pikgpu_run(ctx, 0);
   if (rc == PIKGPU_OK && consumed != nullptr) *consumed = info.consumed;
-  pikgpu_parsed_free(parsed);
   return rc;
 }
 
@@ -1827,13 +1831,47 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
 
 // ---------------------------------------------------------------------------
 // Batches: several frames of any sizes reconstructed by one graph launch.
-// Each frame keeps its own buffers (a context per frame); the launch
-// sequences of all frames are captured once into a hipGraph whose kBranches
-// parallel branches each run a share of the frames one after the other, so
-// kernels of different frames overlap and the whole batch costs one launch.
+// Each frame keeps its own buffers (a context per frame). In the queue
+// schedule the graph holds one launch per phase of the reconstruction over
+// all frames: the batched kernels (kernels.h, BatchSpan) find their frame
+// and its workgroup from a span table, so every phase fills the GPU however
+// small the frames are. In the branch schedule the launch sequences of all
+// frames are captured into kBranches parallel branches of the graph.
 
 namespace {
 constexpr int kBranches = 32;
+
+typedef void (*BatchKernel)(const BatchFrame*, const BatchSpan*, int,
+                            uint32_t, const DeviceConsts*);
+
+// The phases in launch order: name, kernel, whether it can loop
+// (persistent), and its minimum workgroups per CU for the loop's grid.
+struct BatchPhaseDef {
+  const char* name;
+  BatchKernel kernel;
+  bool loop;
+};
+const BatchPhaseDef kBatchPhases[PIKGPU_BATCH_PHASES] = {
+    {"dc_dequant", KDcDequantBatch, true},
+    {"dc_minmax", KMinMaxPartialDcBatch, true},
+    {"dc_minmax_final", KMinMaxFinalBatch, true},
+    {"dc_epf", KEpfDcBatch, false},
+    {"llf", KComputeLlfBatch, true},
+    {"lf_blur", KLfBlurBatch, true},
+    {"lf_from_dc2x2", KLfFromDc2x2Batch, true},
+    {"pred2x2_border", KPred2x2BorderBatch, true},
+    {"dc2x2_border", KDc2x2BorderBatch, true},
+    {"pred2x2_interior", KPred2x2InteriorBatch, false},
+    {"blocks", KBlocksBatch, false},
+    {"minmax", KMinMaxPartial4Batch, true},
+    {"minmax_final", KMinMaxFinalBatch, true},
+    {"guide", KGuideBatch, false},
+    {"guide_ring", KGuideRingBatch, true},
+    {"epf", KEpfBatch, false},
+    {"copy_planes", KCopyPlanesBatch, true},
+    {"dots", KDotsBatch, true},
+    {"final", KFinalBatch, true},
+};
 }  // namespace
 
 struct pikgpu_batch {
@@ -1849,6 +1887,10 @@ struct pikgpu_batch {
   hipGraphExec_t exec = nullptr;
   bool ran = false;
   float last_ms = 0.0f;
+  pikgpu_batch_schedule sched = {PIKGPU_BATCH_QUEUE, 1, {}};
+  int num_cus = 256;
+  // queue schedule: the frames' parameters and the phases' span tables
+  DevBuf frames, spans;
 };
 
 namespace {
@@ -1867,6 +1909,16 @@ int BFail(pikgpu_batch* b, int code, const std::string& msg) {
     }                                                                       \
   } while (0)
 
+int GrowRaw(DevBuf* b, size_t bytes) {
+  if (b->cap >= bytes) return PIKGPU_OK;
+  if (b->p) (void)hipFree(b->p);
+  b->p = nullptr;
+  b->cap = 0;
+  if (hipMalloc(&b->p, bytes) != hipSuccess) return PIKGPU_ENOMEM;
+  b->cap = bytes;
+  return PIKGPU_OK;
+}
+
 void DropGraph(pikgpu_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1874,8 +1926,222 @@ void DropGraph(pikgpu_batch* b) {
   b->graph = nullptr;
 }
 
+// One frame's grid and arguments of phase p, as its plain launch in
+// Enqueue has them; false when the phase does not run for the frame.
+bool PhaseGrid(int p, const FrameParams& P, const FrameBuffers& B, int* nx,
+               int* ny, int* nz, int* arg) {
+  *nx = *ny = *nz = 1;
+  const int nblk = P.bx * P.by;
+  const size_t ngroups = size_t(P.gx) * P.gy;
+  const int cache_pos = (kGroupDimBlocks + 2) * (kGroupDimBlocks + 2);
+  const int gcx = int(DivCeil(cache_pos, 256));
+  const int g2x = int(DivCeil(4 * cache_pos, 256));
+  const int wg4 = int(DivCeil(3 * (P.pred_end[3] - P.pred_end[2]), 32));
+  const int wg2 = int(DivCeil(3 * (P.pred_end[2] - P.pred_end[1]), 64));
+  const Img dci = DcImage(P, B);
+  const Img ii = IdctImage(P, B);
+  const bool hf = P.predict_hf != 0;
+  switch (p) {
+    case 0:  // dc_dequant
+      *nx = int(DivCeil(nblk, 256));
+      return true;
+    case 1: {  // dc_minmax
+      const dim3 g = MinMaxGrid(dci.W, dci.H);
+      *nx = int(g.x);
+      *ny = int(g.y);
+      return true;
+    }
+    case 2: {  // dc_minmax_final
+      const dim3 g = MinMaxGrid(dci.W, dci.H);
+      arg[0] = int(g.x * g.y);
+      return true;
+    }
+    case 3:  // dc_epf
+      *nx = XcdGrid(int(DivCeil(dci.W, 32) * DivCeil(dci.H, 32)));
+      return true;
+    case 4:  // llf
+      *nx = wg4 + wg2 + gcx * int(ngroups) * 3;
+      arg[0] = P.predict_lf;
+      arg[1] = wg4;
+      arg[2] = wg2;
+      arg[3] = gcx;
+      arg[4] = int(ngroups);
+      return true;
+    case 5:  // lf_blur
+      *nx = g2x;
+      *ny = int(ngroups);
+      *nz = 3;
+      return P.predict_lf != 0;
+    case 6:  // lf_from_dc2x2
+      *nx = wg4 + wg2 + gcx * int(ngroups) * 3;
+      arg[0] = wg4;
+      arg[1] = wg2;
+      arg[2] = gcx;
+      arg[3] = int(ngroups);
+      return P.predict_lf != 0;
+    case 7:  // pred2x2_border
+    case 8:  // dc2x2_border
+      *nx = gcx;
+      *ny = int(ngroups);
+      *nz = 3;
+      return hf && (P.predict_lf != 0) == (p == 7);
+    case 9: {  // pred2x2_interior
+      const int wg1 = int(DivCeil(3 * P.pred_end[1], 256));
+      *nx = wg4 + wg2 + wg1;
+      arg[0] = wg4;
+      arg[1] = wg2;
+      return hf && P.pred_end[3] > 0;
+    }
+    case 10:  // blocks
+      *nx = XcdGrid(int(DivCeil(P.bx, 4) * DivCeil(P.by, 4)));
+      return true;
+    case 11: {  // minmax
+      const dim3 g = MinMaxGrid4(ii.W, ii.H);
+      *nx = int(g.x);
+      *ny = int(g.y);
+      arg[0] = P.gaborish && !P.use_sharpened;
+      return P.ar != 0;
+    }
+    case 12: {  // minmax_final
+      const dim3 g = MinMaxGrid4(ii.W, ii.H);
+      arg[0] = int(g.x * g.y);
+      return P.ar != 0;
+    }
+    case 13:  // guide
+      *nx = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
+      arg[0] = P.gaborish && !P.use_sharpened;
+      return P.ar != 0;
+    case 14:  // guide_ring
+      *nx = int(DivCeil(3 * (2 * 6 * (P.px + 12) + 2 * 6 * P.py), 256));
+      return P.ar != 0;
+    case 15:  // epf
+      *nx = XcdGrid(int(DivCeil(P.px, 32) * DivCeil(P.py, 32)));
+      return P.ar != 0;
+    case 16:  // copy_planes
+      *nx = int(DivCeil(3 * size_t(B.ipitch) * P.py, 256));
+      return P.ar == 0;
+    case 17:  // dots
+      *nx = P.dot_tiles;
+      return P.dot_tiles > 0;
+    case 18:  // final
+      *nx = XcdGrid(int(DivCeil(P.px, 256) * DivCeil(P.py, kFinalRows)));
+      return true;
+  }
+  return false;
+}
+
+// The queue schedule's graph: the frames' parameters and span tables in
+// device memory, then one launch per phase.
+int BuildQueueGraph(pikgpu_batch* b) {
+  std::vector<BatchFrame> frames(b->n);
+  std::vector<int> order(b->n);
+  for (int i = 0; i < b->n; ++i) {
+    pikgpu_ctx* c = b->items[i];
+    frames[i].P = c->P;
+    frames[i].P.out_flags = kOutLinear;
+    frames[i].P.downsample = 1;
+    frames[i].B = c->B;
+    frames[i].B.keep = 0;
+    order[i] = i;
+  }
+  if (b->sched.order != 0) {
+    const bool big_first = b->sched.order == 1;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const long ax = long(frames[x].P.px) * frames[x].P.py;
+      const long ay = long(frames[y].P.px) * frames[y].P.py;
+      return big_first ? ax > ay : ax < ay;
+    });
+  }
+  struct Launch {
+    int phase;
+    size_t first_span;
+    int ns;
+    uint32_t total;
+  };
+  std::vector<BatchSpan> spans;
+  std::vector<Launch> launches;
+  for (int p = 0; p < PIKGPU_BATCH_PHASES; ++p) {
+    Launch L = {p, spans.size(), 0, 0};
+    uint32_t at = 0;
+    for (int i : order) {
+      BatchSpan sp;
+      memset(&sp, 0, sizeof(sp));
+      if (!PhaseGrid(p, frames[i].P, frames[i].B, &sp.nx, &sp.ny, &sp.nz,
+                     sp.arg)) {
+        continue;
+      }
+      sp.frame = uint32_t(i);
+      sp.first = at;
+      sp.count = uint32_t(sp.nx) * sp.ny * sp.nz;
+      if (sp.count == 0) continue;
+      at = uint32_t(XcdGrid(int(at + sp.count)));
+      spans.push_back(sp);
+      L.ns += 1;
+    }
+    L.total = at;
+    if (L.ns > 0) launches.push_back(L);
+  }
+  int rc;
+  if ((rc = GrowRaw(&b->frames, frames.size() * sizeof(BatchFrame))) ||
+      (rc = GrowRaw(&b->spans, std::max<size_t>(1, spans.size()) *
+                                   sizeof(BatchSpan)))) {
+    return BFail(b, rc, "batch tables: out of device memory");
+  }
+  BHIP_TRY(b, hipMemcpy(b->frames.p, frames.data(),
+                        frames.size() * sizeof(BatchFrame),
+                        hipMemcpyHostToDevice));
+  if (!spans.empty()) {
+    BHIP_TRY(b, hipMemcpy(b->spans.p, spans.data(),
+                          spans.size() * sizeof(BatchSpan),
+                          hipMemcpyHostToDevice));
+  }
+  const BatchFrame* F = static_cast<const BatchFrame*>(b->frames.p);
+  const BatchSpan* S0 = static_cast<const BatchSpan*>(b->spans.p);
+  const DeviceConsts* K = b->items[0]->consts;
+  BHIP_TRY(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (const Launch& L : launches) {
+    if (e != hipSuccess) break;
+    const BatchPhaseDef& d = kBatchPhases[L.phase];
+    uint32_t grid = L.total;
+    const int k = b->sched.persist[L.phase];
+    if (d.loop && k > 0) {
+      grid = std::min<uint32_t>(grid, uint32_t(XcdGrid(k * b->num_cus)));
+    }
+    const BatchSpan* S = S0 + L.first_span;
+    int ns = L.ns;
+    uint32_t total = L.total;
+    void* args[] = {&F, &S, &ns, &total, &K};
+    e = hipLaunchKernel(reinterpret_cast<const void*>(d.kernel), dim3(grid),
+                        dim3(256), args, 0, b->main);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e_end = hipStreamEndCapture(b->main, &g);
+  if (e != hipSuccess || e_end != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return BFail(b, PIKGPU_EHIP,
+                 std::string("graph capture: ") +
+                     hipGetErrorString(e != hipSuccess ? e : e_end));
+  }
+  b->graph = g;
+  BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  // what pikgpu_download reads of each frame's run
+  for (int i = 0; i < b->n; ++i) {
+    pikgpu_ctx* c = b->items[i];
+    c->ran = true;
+    c->kept = false;
+    c->ran_flags = kOutLinear;
+    c->out_xsize = int(c->P.xsize);
+    c->out_ysize = int(c->P.ysize);
+    c->ran_downsampled = false;
+    c->timed = false;
+  }
+  return PIKGPU_OK;
+}
+
 int BuildGraph(pikgpu_batch* b) {
   DropGraph(b);
+  if (b->sched.mode == PIKGPU_BATCH_QUEUE) return BuildQueueGraph(b);
   BHIP_TRY(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
   int rc = PIKGPU_OK;
   std::string err;
@@ -1927,6 +2193,11 @@ pikgpu_batch* pikgpu_batch_create(int device, const pikgpu_config* config) {
     pikgpu_batch_destroy(b);
     return nullptr;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            device) == hipSuccess && cus > 0) {
+    b->num_cus = cus;
+  }
   return b;
 }
 
@@ -1940,6 +2211,8 @@ void pikgpu_batch_destroy(pikgpu_batch* b) {
     if (b->branch[k]) (void)hipStreamDestroy(b->branch[k]);
     if (b->join[k]) (void)hipEventDestroy(b->join[k]);
   }
+  if (b->frames.p) (void)hipFree(b->frames.p);
+  if (b->spans.p) (void)hipFree(b->spans.p);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->t0) (void)hipEventDestroy(b->t0);
   if (b->t1) (void)hipEventDestroy(b->t1);
@@ -1973,6 +2246,38 @@ int pikgpu_batch_upload(pikgpu_batch* b, const pikgpu_frame* frames, int n) {
   }
   b->n = n;
   return PIKGPU_OK;
+}
+
+int pikgpu_batch_set_schedule(pikgpu_batch* b,
+                              const pikgpu_batch_schedule* s) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  if (s == nullptr || (s->mode != PIKGPU_BATCH_QUEUE &&
+                       s->mode != PIKGPU_BATCH_BRANCHES) ||
+      s->order < 0 || s->order > 2) {
+    return BFail(b, PIKGPU_EINVAL, "invalid schedule");
+  }
+  for (int p = 0; p < PIKGPU_BATCH_PHASES; ++p) {
+    if (s->persist[p] < 0 || s->persist[p] > 16) {
+      return BFail(b, PIKGPU_EINVAL, "persist outside 0 .. 16");
+    }
+  }
+  BHIP_TRY(b, hipSetDevice(b->device));
+  BHIP_TRY(b, hipStreamSynchronize(b->main));
+  DropGraph(b);
+  b->sched = *s;
+  return PIKGPU_OK;
+}
+
+int pikgpu_batch_get_schedule(const pikgpu_batch* b, pikgpu_batch_schedule* s) {
+  if (b == nullptr || s == nullptr) return PIKGPU_EINVAL;
+  *s = b->sched;
+  return PIKGPU_OK;
+}
+
+const char* pikgpu_batch_phase_name(int p, int* can_persist) {
+  if (p < 0 || p >= PIKGPU_BATCH_PHASES) return nullptr;
+  if (can_persist) *can_persist = kBatchPhases[p].loop ? 1 : 0;
+  return kBatchPhases[p].name;
 }
 
 int pikgpu_batch_run(pikgpu_batch* b) {

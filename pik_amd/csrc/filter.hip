@@ -119,13 +119,13 @@ __device__ inline uint32_t LdsDword(const uint8_t* p) {
 // pik/epf_target.cc:975-1004); one entry of 6 floats per workgroup. Thread
 // x walks column x of its row range with the 3x3 window of the Gaborish in
 // registers (Symmetric3's operation order, pik/convolve.h:605-621).
-__global__ void __launch_bounds__(256)
-    KMinMaxPartial(Img in, int gaborish, FrameParams P, float* partial) {
+__device__ __forceinline__ void KMinMaxPartialBody(
+    Img in, int gaborish, FrameParams P, float* partial, const VBlock& vb) {
   __shared__ float s_mm[6][256];
   const int tid = threadIdx.x;
-  const int x = blockIdx.x * 256 + tid;
-  const int rows = (in.H + int(gridDim.y) - 1) / int(gridDim.y);
-  const int ya = int(blockIdx.y) * rows, yb = min(in.H, ya + rows);
+  const int x = vb.x * 256 + tid;
+  const int rows = (in.H + vb.ny - 1) / vb.ny;
+  const int ya = vb.y * rows, yb = min(in.H, ya + rows);
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
   float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   if (x < in.W) {
@@ -222,21 +222,25 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (tid < 6) {
-    partial[(blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = s_mm[tid][0];
+    partial[(vb.y * vb.nx + vb.x) * 6 + tid] = s_mm[tid][0];
   }
+}
+__global__ void __launch_bounds__(256)
+    KMinMaxPartial(Img in, int gaborish, FrameParams P, float* partial) {
+  KMinMaxPartialBody(in, gaborish, P, partial, PlainBlock());
 }
 
 // KMinMaxPartial for an image whose rows are float4-aligned and whose width
 // is a multiple of 4 with no replicated margin (vw == W, vh == H): thread t
 // walks columns 4t .. 4t + 3 with one float4 and the two mirrored edge
 // columns per row and channel, the next row loaded one iteration ahead.
-__global__ void __launch_bounds__(256)
-    KMinMaxPartial4(Img in, int gaborish, FrameParams P, float* partial) {
+__device__ __forceinline__ void KMinMaxPartial4Body(
+    Img in, int gaborish, FrameParams P, float* partial, const VBlock& vb) {
   __shared__ float s_mm[6][256];
   const int tid = threadIdx.x;
-  const int x = (blockIdx.x * 256 + tid) * 4;
-  const int rows = (in.H + int(gridDim.y) - 1) / int(gridDim.y);
-  const int ya = int(blockIdx.y) * rows, yb = min(in.H, ya + rows);
+  const int x = (vb.x * 256 + tid) * 4;
+  const int rows = (in.H + vb.ny - 1) / vb.ny;
+  const int ya = vb.y * rows, yb = min(in.H, ya + rows);
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
   float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   if (x < in.W && ya < yb) {
@@ -317,15 +321,19 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (tid < 6) {
-    partial[(blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] = s_mm[tid][0];
+    partial[(vb.y * vb.nx + vb.x) * 6 + tid] = s_mm[tid][0];
   }
+}
+__global__ void __launch_bounds__(256)
+    KMinMaxPartial4(Img in, int gaborish, FrameParams P, float* partial) {
+  KMinMaxPartial4Body(in, gaborish, P, partial, PlainBlock());
 }
 
 // Reduces the partials to mm[0..5] = min[3], max[3]; mm[6] = 255 / range
 // (the guide multiplier and the sigma stretch, both 1 for a flat image),
 // mm[7] = the minimum over channels.
-__global__ void __launch_bounds__(256)
-    KMinMaxFinal(const float* partial, int n, float* mm) {
+__device__ __forceinline__ void KMinMaxFinalBody(
+    const float* partial, int n, float* mm, const VBlock& vb) {
   __shared__ float s_mm[6][256];
   const int tid = threadIdx.x;
   float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -359,6 +367,10 @@ __global__ void __launch_bounds__(256)
     mm[7] = all_min;
   }
 }
+__global__ void __launch_bounds__(256)
+    KMinMaxFinal(const float* partial, int n, float* mm) {
+  KMinMaxFinalBody(partial, n, mm, PlainBlock());
+}
 
 // ---------------------------------------------------------------------------
 // u8 EPF guide of the IDCT image (the guide AdaptiveFilter builds,
@@ -367,14 +379,14 @@ __global__ void __launch_bounds__(256)
 // the guide stretch, truncated to bytes. One 32x32 tile per workgroup,
 // written at (x + 6, y + 6) of the guide image; KGuideRing fills its
 // 6-pixel border.
-__global__ void __launch_bounds__(256)
-    KGuide(Img in, int gaborish, FrameParams P, FrameBuffers B,
-           const float* mm) {
+__device__ __forceinline__ void KGuideBody(
+    Img in, int gaborish, FrameParams P, FrameBuffers B, const float* mm,
+    const VBlock& vb) {
   constexpr int kWP = 35;  // pitch of the 34 x 34 window rows
   __shared__ float s_w[3][34 * kWP];
   const int tid = threadIdx.x;
   const int tiles_x = (in.W + 31) / 32;
-  const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
+  const int tile = XcdTile(vb.x, tiles_x * ((in.H + 31) / 32));
   if (tile < 0) return;
   const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
   // The tile with a 1-pixel mirrored ring (window row / column 0 is pixel
@@ -499,17 +511,23 @@ __global__ void __launch_bounds__(256)
     }
   }
 }
+__global__ void __launch_bounds__(256)
+    KGuide(Img in, int gaborish, FrameParams P, FrameBuffers B,
+           const float* mm) {
+  KGuideBody(in, gaborish, P, B, mm, PlainBlock());
+}
 
 // The 6-pixel border of the guide image: each position takes the byte of
 // the pixel it mirrors. (The guide of a mirrored position equals its
 // source pixel's bit for bit: the 3x3 sums of the mirrored window only
 // exchange the operands of commutative additions.)
-__global__ void __launch_bounds__(256) KGuideRing(FrameParams P, FrameBuffers B) {
+__device__ __forceinline__ void KGuideRingBody(
+    FrameParams P, FrameBuffers B, const VBlock& vb) {
   const int W = P.px, H = P.py;
   const int pw = W + 12;
   const int n_tb = 6 * pw, n_lr = 6 * H;
   const int per_c = 2 * n_tb + 2 * n_lr;
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int k = vb.x * 256 + threadIdx.x;
   if (k >= 3 * per_c) return;
   const int c = k / per_c;
   int r = k - c * per_c;
@@ -532,6 +550,10 @@ __global__ void __launch_bounds__(256) KGuideRing(FrameParams P, FrameBuffers B)
   uint8_t* g = B.gimg + size_t(c) * B.grows * B.gpitch;
   g[size_t(Y) * B.gpitch + X] = g[size_t(sy + 6) * B.gpitch + sx + 6];
 }
+__global__ void __launch_bounds__(256)
+    KGuideRing(FrameParams P, FrameBuffers B) {
+  KGuideRingBody(P, B, PlainBlock());
+}
 
 // ---------------------------------------------------------------------------
 // EPF (+ AR clamp) on one 32x32 tile. dc_mode: the image is the 8-padded DC
@@ -546,9 +568,10 @@ __global__ void __launch_bounds__(256) KGuideRing(FrameParams P, FrameBuffers B)
 // Image mode (!kDc) reads the u8 guide from the guide image KGuide wrote;
 // the DC mode quantises its input itself (no Gaborish on the DC image).
 template <bool kDc>
-__global__ void __launch_bounds__(256, kDc ? 3 : 4)
-    KEpf(Img in, FrameParams P, FrameBuffers B, const DeviceConsts* K,
-         const float* mm, float* out, int out_pitch, size_t out_plane) {
+__device__ __forceinline__ void KEpfBody(
+    Img in, FrameParams P, FrameBuffers B, const DeviceConsts* K,
+    const float* mm, float* out, int out_pitch, size_t out_plane,
+    const VBlock& vb) {
   constexpr int dc_mode = kDc;
   // Input window: the DC mode keeps the tile with a 7-pixel border (its
   // guide is computed from it); the image mode only the filter's window,
@@ -577,7 +600,7 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
 
   const int tid = threadIdx.x;
   const int tiles_x = (in.W + 31) / 32;
-  const int tile = XcdTile(blockIdx.x, tiles_x * ((in.H + 31) / 32));
+  const int tile = XcdTile(vb.x, tiles_x * ((in.H + 31) / 32));
   if (tile < 0) return;
   const int x0 = (tile % tiles_x) * 32, y0 = (tile / tiles_x) * 32;
   const float all_min = mm[7];
@@ -998,6 +1021,12 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
     }
   }
 }
+template <bool kDc>
+__global__ void __launch_bounds__(256, kDc ? 3 : 4)
+    KEpf(Img in, FrameParams P, FrameBuffers B, const DeviceConsts* K,
+         const float* mm, float* out, int out_pitch, size_t out_plane) {
+  KEpfBody<kDc>(in, P, B, K, mm, out, out_pitch, out_plane, PlainBlock());
+}
 
 // ---------------------------------------------------------------------------
 // Dot patches (BlockDictionary::Apply<true>, pik/block_dictionary.cc:
@@ -1005,14 +1034,15 @@ __global__ void __launch_bounds__(256, kDc ? 3 : 4)
 // dot_tile_ids[t]; each thread owns four pixels of it and adds the tile's
 // dots (listed in the reference's application order) to them one dot after
 // the other, so every pixel receives its additions in the reference's order.
-__global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
+__device__ __forceinline__ void KDotsBody(
+    FrameParams P, FrameBuffers B, const VBlock& vb) {
   __shared__ float s_blk[3][21 * 21];
   const int tid = threadIdx.x;
-  const int tile = B.dot_tile_ids[blockIdx.x];
+  const int tile = B.dot_tile_ids[vb.x];
   const int tx0 = (tile % P.dot_tiles_x) * 32, ty0 = (tile / P.dot_tiles_x) * 32;
   const int px = tx0 + (tid & 31);
-  const int begin = B.dot_tile_start[blockIdx.x];
-  const int end = B.dot_tile_start[blockIdx.x + 1];
+  const int begin = B.dot_tile_start[vb.x];
+  const int end = B.dot_tile_start[vb.x + 1];
   const float kBlockInvScale = 1.0f / 3.5f;
   const float kXybRadius[3] = {0.0301006f, 0.4512295f, 0.4716444f};
   const size_t plane = size_t(P.py) * B.ipitch;
@@ -1126,6 +1156,10 @@ __global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
     }
   }
 }
+__global__ void __launch_bounds__(256)
+    KDots(FrameParams P, FrameBuffers B) {
+  KDotsBody(P, B, PlainBlock());
+}
 
 // ---------------------------------------------------------------------------
 // Final Gaborish (on the 8-padded image) and XYB -> linear RGB of the
@@ -1133,11 +1167,11 @@ __global__ void __launch_bounds__(256) KDots(FrameParams P, FrameBuffers B) {
 // workgroup walks kFinalRows rows of column x of its 256-column strip with
 // the 3x3 window of each channel in registers, so every input value is
 // loaded three times (left, centre, right) instead of nine.
-__global__ void __launch_bounds__(256)
-    KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+__device__ __forceinline__ void KFinalBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, const VBlock& vb) {
   const int strips = (P.px + 255) / 256;
   const int tile =
-      XcdTile(blockIdx.x, strips * ((P.py + kFinalRows - 1) / kFinalRows));
+      XcdTile(vb.x, strips * ((P.py + kFinalRows - 1) / kFinalRows));
   if (tile < 0) return;
   const int x = (tile % strips) * 256 + threadIdx.x;
   const int ya = (tile / strips) * kFinalRows;
@@ -1262,11 +1296,19 @@ __global__ void __launch_bounds__(256)
     }
   }
 }
+__global__ void __launch_bounds__(256)
+    KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  KFinalBody(P, B, K, PlainBlock());
+}
 
+__device__ __forceinline__ void KCopyPlanesBody(
+    const float* src, float* dst, size_t n, const VBlock& vb) {
+  const size_t i = size_t(vb.x) * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
 __global__ void __launch_bounds__(256)
     KCopyPlanes(const float* src, float* dst, size_t n) {
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) dst[i] = src[i];
+  KCopyPlanesBody(src, dst, n, PlainBlock());
 }
 
 template __global__ void KEpf<true>(Img, FrameParams, FrameBuffers,
@@ -1275,5 +1317,45 @@ template __global__ void KEpf<true>(Img, FrameParams, FrameBuffers,
 template __global__ void KEpf<false>(Img, FrameParams, FrameBuffers,
                                      const DeviceConsts*, const float*, float*,
                                      int, size_t);
+
+// ---------------------------------------------------------------------------
+// Batched forms: one launch over every frame of a batch (BatchSpan), each
+// frame's images and buffers as the plain launches of pikgpu_run take them.
+__global__ void __launch_bounds__(256)
+    KMinMaxPartialDcBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KMinMaxPartialBody(DcImage(P, B), 0, P, B.mm_partial, vb));
+}
+__global__ void __launch_bounds__(256)
+    KMinMaxPartial4Batch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KMinMaxPartial4Body(IdctImage(P, B), s.arg[0], P,
+                                        B.mm_partial, vb));
+}
+__global__ void __launch_bounds__(256) KMinMaxFinalBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KMinMaxFinalBody(B.mm_partial, s.arg[0], B.mm, vb));
+}
+__global__ void __launch_bounds__(256, 3) KEpfDcBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_ONE(KEpfBody<true>(DcImage(P, B), P, B, K, B.mm, B.dc, P.bx,
+                                   size_t(P.bx) * P.by, vb));
+}
+__global__ void __launch_bounds__(256) KGuideBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_ONE(KGuideBody(IdctImage(P, B), s.arg[0], P, B, B.mm, vb));
+}
+__global__ void __launch_bounds__(256) KGuideRingBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KGuideRingBody(P, B, vb));
+}
+__global__ void __launch_bounds__(256, 4) KEpfBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_ONE(KEpfBody<false>(IdctImage(P, B), P, B, K, B.mm, B.ar,
+                                    B.ipitch, size_t(B.ipitch) * P.py, vb));
+}
+__global__ void __launch_bounds__(256) KCopyPlanesBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KCopyPlanesBody(B.idct, B.ar, 3 * size_t(B.ipitch) * P.py,
+                                    vb));
+}
+__global__ void __launch_bounds__(256) KDotsBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KDotsBody(P, B, vb));
+}
+__global__ void __launch_bounds__(256) KFinalBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KFinalBody(P, B, K, vb));
+}
 
 }  // namespace pikgpu

@@ -32,8 +32,9 @@ struct pikgpu_parsed {
   uint32_t bx = 0, by = 0;
   std::vector<int16_t> dc;
   // passes x 3 planes; every entry is written by the AC decoding, so the
-  // array is not cleared first
+  // array is not cleared first (and is kept when a parse reuses the object)
   std::unique_ptr<int16_t[]> ac;
+  size_t ac_cap = 0;
   std::vector<int32_t> qf, ytox, ytob;
   std::vector<uint8_t> acs, lut, dcf, dqmap;
   std::vector<float> table, noise;
@@ -493,6 +494,7 @@ int Parse(const uint8_t* data, size_t size, uint32_t xsize, uint32_t ysize,
   }
   // The rest of the pass-global data continues at the end of the DC groups.
   BitStream rest(data + std::min(pos, size), size - std::min(pos, size));
+  p->dots.clear();
   int rc = ReadDots(&rest, size_t(bx) * 8, size_t(by) * 8, &p->dots, &err);
   if (rc != PIKGPU_OK) {
     SetMsg(msg, msg_size, "%s", err.c_str());
@@ -584,7 +586,11 @@ int Parse(const uint8_t* data, size_t size, uint32_t xsize, uint32_t ysize,
   }
   const size_t consumed = pos + rest.Position();
 
-  p->ac.reset(new int16_t[size_t(h.num_passes) * 3 * nb * 64]);
+  const size_t ac_need = size_t(h.num_passes) * 3 * nb * 64;
+  if (p->ac_cap < ac_need) {
+    p->ac.reset(new int16_t[ac_need]);
+    p->ac_cap = ac_need;
+  }
   p->noise.assign(size_t(h.num_passes) * ng * 3, 0.0f);
   std::vector<const uint8_t*> gdata(ng);
   std::vector<size_t> gsize(ng);
@@ -702,17 +708,18 @@ extern "C" PIKGPU_API int pikgpu_parse_frame(const uint8_t* data, size_t size,
     SetMsg(msg, msg_size, "null argument");
     return PIKGPU_EINVAL;
   }
-  *out = nullptr;
   if (xsize == 0 || ysize == 0) {
     SetMsg(msg, msg_size, "empty image");
     return PIKGPU_EINVAL;
   }
   static const uint8_t kNone[1] = {0};
-  pikgpu_parsed* p = new pikgpu_parsed;
+  // A parsed frame passed in is reused: its storage (above all the AC
+  // planes, which need not be cleared) is kept where it is large enough.
+  pikgpu_parsed* p = *out != nullptr ? *out : new pikgpu_parsed;
   const int rc = Parse(data != nullptr ? data : kNone, size, xsize, ysize,
                        threads, p, msg, msg_size);
   if (rc != PIKGPU_OK) {
-    delete p;
+    if (*out == nullptr) delete p;
     return rc;
   }
   *out = p;

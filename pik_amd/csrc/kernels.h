@@ -545,6 +545,113 @@ __device__ inline int XcdTile(int wg, int num_tiles) {
 }
 
 // ---------------------------------------------------------------------------
+// The workgroup a kernel body works on: blockIdx / gridDim of a plain
+// launch, or, in a batched launch, the frame's own grid inside one launch
+// over every frame of the batch (BatchSpan).
+struct VBlock {
+  int x, y, z;     // workgroup index
+  int nx, ny, nz;  // grid size in workgroups
+};
+__device__ inline VBlock PlainBlock() {
+  VBlock v;
+  v.x = int(blockIdx.x);
+  v.y = int(blockIdx.y);
+  v.z = int(blockIdx.z);
+  v.nx = int(gridDim.x);
+  v.ny = int(gridDim.y);
+  v.nz = int(gridDim.z);
+  return v;
+}
+
+// A frame of a batch in device memory: its parameters and buffers as a
+// plain launch passes them.
+struct BatchFrame {
+  FrameParams P;
+  FrameBuffers B;
+};
+// One frame's share of a batched launch of one phase: launch workgroups
+// [first, first + count) run the frame's nx x ny x nz grid (count =
+// nx * ny * nz), with the phase's per-frame integer arguments. `first` is
+// a multiple of kNumXcds, so a frame's local workgroup index keeps the XCD
+// of the launch's (XcdTile).
+constexpr int kBatchArgs = 5;
+struct BatchSpan {
+  uint32_t frame, first, count;
+  int nx, ny, nz;
+  int arg[kBatchArgs];
+};
+
+// The span holding launch workgroup w (spans sorted by `first`, the first
+// at 0); false for the idle workgroups between spans.
+typedef const BatchSpan ConstSpan;
+
+__device__ inline bool FindSpan(ConstSpan* S, int ns, uint32_t w, int* si) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S[mid].first <= w) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  *si = lo;
+  return w - S[lo].first < S[lo].count;
+}
+__device__ inline VBlock SpanBlock(ConstSpan& s, uint32_t w) {
+  const int l = int(w - s.first);
+  VBlock v;
+  v.nx = s.nx;
+  v.ny = s.ny;
+  v.nz = s.nz;
+  v.x = l % s.nx;
+  v.y = (l / s.nx) % s.ny;
+  v.z = l / (s.nx * s.ny);
+  return v;
+}
+
+// The loop of a batched kernel: workgroup blockIdx.x takes launch
+// workgroups blockIdx.x, + gridDim.x, ... below `total` (one of them when
+// the launch has `total` workgroups: a persistent launch has fewer), runs
+// `body` with s (the BatchSpan), vb (the frame's workgroup), P and B (the
+// frame's), and waits for the workgroup before the next one reuses LDS.
+#define PIKGPU_BATCH_LOOP(body)                                       \
+  for (uint32_t w_ = blockIdx.x; w_ < total; w_ += gridDim.x) {       \
+    int si_ = 0;                                                      \
+    if (FindSpan(S, ns, w_, &si_)) {                                  \
+      ConstSpan& s = S[si_];                                          \
+      const VBlock vb = SpanBlock(s, w_);                             \
+      const FrameParams P = F[s.frame].P;                             \
+      const FrameBuffers B = F[s.frame].B;                            \
+      body;                                                           \
+    }                                                                 \
+    __syncthreads();                                                  \
+  }
+
+// One launch workgroup per frame workgroup (a launch of `total`
+// workgroups), for the kernels whose registers a loop around them would
+// overflow (the compiler keeps per-thread values of the body live across
+// the iterations).
+#define PIKGPU_BATCH_ONE(body)                                        \
+  {                                                                   \
+    const uint32_t w_ = blockIdx.x;                                   \
+    int si_ = 0;                                                      \
+    if (w_ < total && FindSpan(S, ns, w_, &si_)) {                    \
+      ConstSpan& s = S[si_];                                          \
+      const VBlock vb = SpanBlock(s, w_);                             \
+      const FrameParams P = F[s.frame].P;                             \
+      const FrameBuffers B = F[s.frame].B;                            \
+      body;                                                           \
+    }                                                                 \
+  }
+
+// The batched kernels' common parameters.
+#define PIKGPU_BATCH_PARAMS                                             \
+  const BatchFrame *__restrict__ F, const BatchSpan *__restrict__ S,    \
+      int ns, uint32_t total, const DeviceConsts *K
+
+
+// ---------------------------------------------------------------------------
 // Small reinterpreting transforms with compile-time sizes: every array index
 // is static after unrolling, so the blocks stay in registers.
 
@@ -678,6 +785,34 @@ struct Img {
   int vw, vh;      // valid source region (replicate beyond, <= W, H)
 };
 
+// Images of a frame as the filters take them: the IDCT image (full size,
+// width a multiple of 8, rows 32-float aligned) and the 8-padded DC image
+// (valid bx x by).
+__host__ __device__ inline Img IdctImage(const FrameParams& P,
+                                         const FrameBuffers& B) {
+  Img ii;
+  ii.p = B.idct;
+  ii.plane = size_t(B.ipitch) * P.py;
+  ii.pitch = B.ipitch;
+  ii.W = P.px;
+  ii.H = P.py;
+  ii.vw = P.px;
+  ii.vh = P.py;
+  return ii;
+}
+__host__ __device__ inline Img DcImage(const FrameParams& P,
+                                       const FrameBuffers& B) {
+  Img dci;
+  dci.p = B.dcq;
+  dci.plane = size_t(P.bx) * P.by;
+  dci.pitch = P.bx;
+  dci.W = (P.bx + 7) / 8 * 8;
+  dci.H = (P.by + 7) / 8 * 8;
+  dci.vw = P.bx;
+  dci.vh = P.by;
+  return dci;
+}
+
 __global__ void KMinMaxPartial(Img in, int gaborish, FrameParams P,
                                float* partial);
 __global__ void KMinMaxPartial4(Img in, int gaborish, FrameParams P,
@@ -695,5 +830,26 @@ __global__ void KDots(FrameParams P, FrameBuffers B);
 constexpr int kFinalRows = 16;
 __global__ void KFinal(FrameParams P, FrameBuffers B, const DeviceConsts* K);
 __global__ void KCopyPlanes(const float* src, float* dst, size_t n);
+
+// Batched forms of the reconstruction's kernels (pikgpu_batch): one launch
+// per phase over every frame of a batch; see BatchSpan.
+__global__ void KDcDequantBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KMinMaxPartialDcBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KMinMaxFinalBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KEpfDcBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KComputeLlfBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KLfBlurBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KLfFromDc2x2Batch(PIKGPU_BATCH_PARAMS);
+__global__ void KPred2x2BorderBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KDc2x2BorderBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KPred2x2InteriorBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KBlocksBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KMinMaxPartial4Batch(PIKGPU_BATCH_PARAMS);
+__global__ void KGuideBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KGuideRingBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KEpfBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KCopyPlanesBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KDotsBatch(PIKGPU_BATCH_PARAMS);
+__global__ void KFinalBatch(PIKGPU_BATCH_PARAMS);
 
 }  // namespace pikgpu

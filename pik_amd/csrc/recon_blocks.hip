@@ -14,8 +14,8 @@
 
 namespace pikgpu {
 
-__global__ void __launch_bounds__(256, 4)
-    KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+__device__ __forceinline__ void KBlocksBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, const VBlock& vb) {
   // ~39 KB: four workgroups per CU. The IDCT's pixels reuse the HF
   // prediction's pixel buffer (dead once its transform is added).
   __shared__ float s_coef[3][kCoefPlane];
@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(256, 4)
 
   const int tid = threadIdx.x;
   const int tiles_x = (P.bx + 3) / 4;
-  const int tile = XcdTile(blockIdx.x, tiles_x * ((P.by + 3) / 4));
+  const int tile = XcdTile(vb.x, tiles_x * ((P.by + 3) / 4));
   if (tile < 0) return;
   const int tbx = (tile % tiles_x) * 4, tby = (tile / tiles_x) * 4;  // blocks
   // pred2x2 window of the tile's HF prediction, 10 x 10 values per channel
@@ -271,6 +271,15 @@ __global__ void __launch_bounds__(256, 4)
     B.idct[c * oplane + size_t(oy0 + oy) * B.ipitch + ox0 + ox] =
         sum * mean_mul;
   }
+}
+__global__ void __launch_bounds__(256, 4)
+    KBlocks(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  KBlocksBody(P, B, K, PlainBlock());
+}
+
+// One launch over every frame of a batch (BatchSpan).
+__global__ void __launch_bounds__(256, 4) KBlocksBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_ONE(KBlocksBody(P, B, K, vb));
 }
 
 }  // namespace pikgpu

@@ -20,9 +20,9 @@ namespace pikgpu {
 // block kernels reach the dequant matrices with one dependent load instead
 // of the chain strategy -> quant field -> control field -> table index ->
 // table offset.
-__global__ void __launch_bounds__(256)
-    KDcDequant(FrameParams P, FrameBuffers B) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void KDcDequantBody(
+    FrameParams P, FrameBuffers B, const VBlock& vb) {
+  const int i = vb.x * blockDim.x + threadIdx.x;
   const int n = P.bx * P.by;
   if (i >= n) return;
   {
@@ -56,6 +56,10 @@ __global__ void __launch_bounds__(256)
   B.dcq[i] = x;
   B.dcq[n + i] = y;
   B.dcq[2 * n + i] = b;
+}
+__global__ void __launch_bounds__(256)
+    KDcDequant(FrameParams P, FrameBuffers B) {
+  KDcDequantBody(P, B, PlainBlock());
 }
 
 // Group geometry helpers.
@@ -268,12 +272,12 @@ __device__ inline void LlfDc2x2Lanes(const FrameParams& P,
 // wg2) the DCT16 ones (64 units of 4 lanes), the rest one lane per cache
 // position of (position block, group, plane); there the thread turns its
 // block's LLF values into the 2x image reading back what it wrote.
-__global__ void __launch_bounds__(256)
-    KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                int dc2x2, int wg4, int wg2, int pos_blocks, int ngroups) {
+__device__ __forceinline__ void KComputeLlfBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, int dc2x2, int wg4,
+    int wg2, int pos_blocks, int ngroups, const VBlock& vb) {
   __shared__ float s_t[32 * 8 * 9];
   const int tid = threadIdx.x;
-  const int wg = blockIdx.x;
+  const int wg = vb.x;
   if (wg < wg4 + wg2) {
     const bool big = wg < wg4;
     const int unit = big ? wg * 32 + (tid >> 3) : (wg - wg4) * 64 + (tid >> 2);
@@ -307,13 +311,18 @@ __global__ void __launch_bounds__(256)
   LlfAt(P, B, K, G, g, c, x, y, raw);
   if (dc2x2) Dc2x2At(P, B, K, g, c, x, y, raw, false);
 }
+__global__ void __launch_bounds__(256)
+    KComputeLlf(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                int dc2x2, int wg4, int wg2, int pos_blocks, int ngroups) {
+  KComputeLlfBody(P, B, K, dc2x2, wg4, wg2, pos_blocks, ngroups, PlainBlock());
+}
 
 // The border of pred2x2 from the LLF cache without LF prediction
 // (pik/compressed_image.cc:980-1039; the interior is KPred2x2Interior's).
-__global__ void __launch_bounds__(256)
-    KDc2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
+__device__ __forceinline__ void KDc2x2BorderBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, const VBlock& vb) {
+  const int pos = vb.x * blockDim.x + threadIdx.x;
+  const int g = vb.y, c = vb.z;
   const GroupGeom G = Geom(P, g);
   const int W = G.gw + 2, H = G.gh + 2;
   if (pos >= W * H) return;
@@ -323,13 +332,17 @@ __global__ void __launch_bounds__(256)
   if ((a & 15) != 0) return;
   Dc2x2At(P, B, K, g, c, x, y, a >> 4, true);
 }
+__global__ void __launch_bounds__(256)
+    KDc2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  KDc2x2BorderBody(P, B, K, PlainBlock());
+}
 
 // 3x3 LF prediction blur with mirrored borders (Symmetric3,
 // pik/convolve.h:502-640).
-__global__ void __launch_bounds__(256)
-    KLfBlur(FrameParams P, FrameBuffers B) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
+__device__ __forceinline__ void KLfBlurBody(
+    FrameParams P, FrameBuffers B, const VBlock& vb) {
+  const int pos = vb.x * blockDim.x + threadIdx.x;
+  const int g = vb.y, c = vb.z;
   const GroupGeom G = Geom(P, g);
   const int W = 2 * (G.gw + 2), H = 2 * (G.gh + 2);
   if (pos >= W * H) return;
@@ -339,6 +352,10 @@ __global__ void __launch_bounds__(256)
   out[y * kG2Pitch + x] =
       Symmetric3(in, kG2Pitch, W, H, x, y, P.lfblur_w[0], P.lfblur_w[1],
                  P.lfblur_w[2]);
+}
+__global__ void __launch_bounds__(256)
+    KLfBlur(FrameParams P, FrameBuffers B) {
+  KLfBlurBody(P, B, PlainBlock());
 }
 
 // ReinterpDct<8S, 2S> in place on the 2S x 2S block at p (pitch kG2Pitch)
@@ -379,12 +396,12 @@ __device__ inline void ReinterpDctLanes(const DeviceConsts* K, bool active,
 // (32 units of block x plane on 8 lanes each), [wg4, wg4 + wg2) the DCT16
 // ones (64 units of 4 lanes), the rest one lane per cache position of
 // (position block, group, plane) for the other strategies.
-__global__ void __launch_bounds__(256)
-    KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                 int wg4, int wg2, int pos_blocks, int ngroups) {
+__device__ __forceinline__ void KLfFromDc2x2Body(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, int wg4, int wg2,
+    int pos_blocks, int ngroups, const VBlock& vb) {
   __shared__ float s_t[32 * 8 * 9];
   const int tid = threadIdx.x;
-  const int wg = blockIdx.x;
+  const int wg = vb.x;
   if (wg < wg4 + wg2) {
     const bool big = wg < wg4;
     const int lanes = big ? 8 : 4;
@@ -445,14 +462,19 @@ __global__ void __launch_bounds__(256)
     p[kG2Pitch + 1] = d[3];
   }
 }
+__global__ void __launch_bounds__(256)
+    KLfFromDc2x2(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                 int wg4, int wg2, int pos_blocks, int ngroups) {
+  KLfFromDc2x2Body(P, B, K, wg4, wg2, pos_blocks, ngroups, PlainBlock());
+}
 
 // Border of pred2x2 from llf and lf2x2 as DCT8 blocks
 // (pik/compressed_image.cc:980-1039).
 template <bool kUseLf>
-__global__ void __launch_bounds__(256)
-    KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
-  const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y, c = blockIdx.z;
+__device__ __forceinline__ void KPred2x2BorderBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, const VBlock& vb) {
+  const int pos = vb.x * blockDim.x + threadIdx.x;
+  const int g = vb.y, c = vb.z;
   const GroupGeom G = Geom(P, g);
   const int W = G.gw + 2, H = G.gh + 2;
   if (pos >= W * H) return;
@@ -475,6 +497,11 @@ __global__ void __launch_bounds__(256)
   o[1] = a00 + a01 - a10 - a11;
   o[kG2Pitch] = a00 - a01 + a10 - a11;
   o[kG2Pitch + 1] = a00 - a01 - a10 + a11;
+}
+template <bool kUseLf>
+__global__ void __launch_bounds__(256)
+    KPred2x2Border(FrameParams P, FrameBuffers B, const DeviceConsts* K) {
+  KPred2x2BorderBody<kUseLf>(P, B, K, PlainBlock());
 }
 
 // Interior of pred2x2: per first block and plane, the decoder-side LF
@@ -624,12 +651,12 @@ __device__ inline void Pred2x2Lanes(const FrameParams& P, const FrameBuffers& B,
 // planes of a block are adjacent units (X and B read the block's Y
 // coefficients and descriptors with the Y unit).
 template <bool kEnc>
-__global__ void __launch_bounds__(256)
-    KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
-                     int wg4, int wg2, const float* corner) {
+__device__ __forceinline__ void KPred2x2InteriorBody(
+    FrameParams P, FrameBuffers B, const DeviceConsts* K, int wg4, int wg2,
+    const float* corner, const VBlock& vb) {
   __shared__ float s_t[32 * 8 * 9];
   const int tid = threadIdx.x;
-  const int wg = blockIdx.x;
+  const int wg = vb.x;
   if (wg < wg4) {
     const int unit = wg * 32 + (tid >> 3);
     const int kb = unit / 3, c = unit - 3 * kb;
@@ -688,6 +715,12 @@ __global__ void __launch_bounds__(256)
              2 * (byg + 1) * kG2Pitch + 2 * (bxg + 1);
   Idct2x2(F[0], F[1], F[2], F[3], o, kG2Pitch);
 }
+template <bool kEnc>
+__global__ void __launch_bounds__(256)
+    KPred2x2Interior(FrameParams P, FrameBuffers B, const DeviceConsts* K,
+                     int wg4, int wg2, const float* corner) {
+  KPred2x2InteriorBody<kEnc>(P, B, K, wg4, wg2, corner, PlainBlock());
+}
 
 template __global__ void KPred2x2Border<true>(FrameParams, FrameBuffers,
                                               const DeviceConsts*);
@@ -699,5 +732,36 @@ template __global__ void KPred2x2Interior<false>(FrameParams, FrameBuffers,
 template __global__ void KPred2x2Interior<true>(FrameParams, FrameBuffers,
                                                 const DeviceConsts*, int, int,
                                                 const float*);
+
+// ---------------------------------------------------------------------------
+// Batched forms: one launch over every frame of a batch, each frame's part
+// of the launch running the plain kernel's grid (BatchSpan, with the
+// plain launch's integer arguments in arg[]).
+__global__ void __launch_bounds__(256) KDcDequantBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KDcDequantBody(P, B, vb));
+}
+__global__ void __launch_bounds__(256) KComputeLlfBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KComputeLlfBody(P, B, K, s.arg[0], s.arg[1], s.arg[2],
+                                    s.arg[3], s.arg[4], vb));
+}
+__global__ void __launch_bounds__(256) KDc2x2BorderBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KDc2x2BorderBody(P, B, K, vb));
+}
+__global__ void __launch_bounds__(256) KLfBlurBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KLfBlurBody(P, B, vb));
+}
+__global__ void __launch_bounds__(256) KLfFromDc2x2Batch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KLfFromDc2x2Body(P, B, K, s.arg[0], s.arg[1], s.arg[2],
+                                     s.arg[3], vb));
+}
+__global__ void __launch_bounds__(256)
+    KPred2x2BorderBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_LOOP(KPred2x2BorderBody<true>(P, B, K, vb));
+}
+__global__ void __launch_bounds__(256)
+    KPred2x2InteriorBatch(PIKGPU_BATCH_PARAMS) {
+  PIKGPU_BATCH_ONE(KPred2x2InteriorBody<false>(P, B, K, s.arg[0], s.arg[1],
+                                                nullptr, vb));
+}
 
 }  // namespace pikgpu

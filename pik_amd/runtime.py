@@ -342,6 +342,25 @@ class Reconstructor:
         return {names[i].decode(): float(ms[i]) for i in range(n)}
 
 
+def batch_phases():
+    """Names of the batched reconstruction's phases, in launch order."""
+    lib = abi.load()
+    out = []
+    for p in range(abi.BATCH_PHASES):
+        out.append(lib.pikgpu_batch_phase_name(p, None).decode())
+    return out
+
+
+def batch_phase_can_persist():
+    lib = abi.load()
+    out = []
+    for p in range(abi.BATCH_PHASES):
+        c = ctypes.c_int()
+        lib.pikgpu_batch_phase_name(p, ctypes.byref(c))
+        out.append(bool(c.value))
+    return out
+
+
 class BatchReconstructor:
     """Many frames of uneven sizes reconstructed by one graph launch
     (pikgpu_batch_*): upload once, run, download frame by frame."""
@@ -377,6 +396,44 @@ class BatchReconstructor:
         if rc != abi.OK:
             msg = self._lib.pikgpu_batch_last_error(self._b)
             raise PikGpuError(rc, msg.decode() if msg else "")
+
+    def schedule(self):
+        """The batch's schedule: dict(mode, order, persist (per phase))."""
+        sc = abi.ScheduleC()
+        self._check(self._lib.pikgpu_batch_get_schedule(self._b,
+                                                        ctypes.byref(sc)))
+        return {"mode": sc.mode, "order": sc.order,
+                "persist": [int(v) for v in sc.persist]}
+
+    def set_schedule(self, mode=abi.BATCH_QUEUE, order=1, persist=None):
+        """pikgpu_batch_set_schedule. persist: per-phase workgroups per CU
+        of the phases that can loop (a list in phase order, or a dict
+        phase name -> value; 0 = one workgroup per work item)."""
+        sc = abi.ScheduleC()
+        sc.mode, sc.order = int(mode), int(order)
+        if isinstance(persist, dict):
+            names = batch_phases()
+            for k, v in persist.items():
+                sc.persist[names.index(k)] = int(v)
+        elif persist is not None:
+            for i, v in enumerate(persist):
+                sc.persist[i] = int(v)
+        self._check(self._lib.pikgpu_batch_set_schedule(self._b,
+                                                        ctypes.byref(sc)))
+
+    def load_schedule(self, path=None):
+        """Applies the schedule the autotuner chose (pik_amd/
+        batch_schedule.json, or `path`); returns whether one was found."""
+        import json
+        import os
+        path = path or os.path.join(os.path.dirname(os.path.abspath(
+            __file__)), "batch_schedule.json")
+        if not os.path.exists(path):
+            return False
+        with open(path) as f:
+            d = json.load(f)
+        self.set_schedule(d["mode"], d["order"], d.get("persist"))
+        return True
 
     def upload(self, frames):
         frames = list(frames)

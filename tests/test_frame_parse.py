@@ -427,3 +427,37 @@ def test_corrupt_frames_match_reference(name):
             if rs == "ok":
                 _compare(frame, info, rv)
     assert ("fail", "fail") in seen
+
+
+def test_reused_parse_storage():
+    # A parsed frame handed back in is refilled: parsing a different frame
+    # into it gives what a fresh parse gives, and a failed parse leaves the
+    # handle usable.
+    z, _ = _cases()
+    lib = abi.load()
+    h = ctypes.c_void_p()
+    msg = ctypes.create_string_buffer(128)
+    for name in ("passes", "dots", "tiny", "plain", "dots"):
+        r = json.loads(str(z[name + "/recipe"]))
+        raw = np.frombuffer(z[name + "/data"].tobytes(), np.uint8)
+        assert lib.pikgpu_parse_frame(raw.ctypes.data, raw.size, r["w"],
+                                      r["h"], 2, ctypes.byref(h), msg,
+                                      128) == abi.OK
+        fc = abi.FrameC()
+        info = abi.ParsedInfoC()
+        lib.pikgpu_parsed_get(h, ctypes.byref(fc), ctypes.byref(info))
+        assert info.consumed == raw.size and fc.num_dots == len(
+            z[name + "/dots"])
+        by, bx = -(-r["h"] // 8), -(-r["w"] // 8)
+        for c in range(3):
+            ac = np.ctypeslib.as_array(
+                ctypes.cast(fc.ac[c], ctypes.POINTER(ctypes.c_int16)),
+                (by * bx * 64,))
+            assert np.array_equal(ac.reshape(by, bx * 64),
+                                  z[name + "/passes"][0][c])
+        bad = raw[:raw.size // 2].copy()
+        assert lib.pikgpu_parse_frame(bad.ctypes.data, bad.size, r["w"],
+                                      r["h"], 2, ctypes.byref(h), msg,
+                                      128) == abi.EDATA
+        assert h.value
+    lib.pikgpu_parsed_free(h)
