@@ -7,6 +7,7 @@
 // smoothing, which run on the GPU (KDcDequant, KEpf<true>); the int16 DC
 // and the side information land in the frame-wide arrays pikgpu_frame
 // takes.
+#include <smmintrin.h>
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -78,21 +79,8 @@ inline uint16_t AbsResidual(int16_t c, int16_t p) {
   return uint16_t(d < 0 ? -d : d);
 }
 
-// Luma predictors from the neighbours n (above), w (left), l (above left)
-// and r (above right).
-inline void PredictY(int16_t n, int16_t w, int16_t l, int16_t r,
-                     int16_t p[8]) {
-  p[0] = Avg(Avg(n, w), r);
-  p[1] = Avg(w, n);
-  p[2] = Avg(n, r);
-  p[3] = Avg(w, l);
-  p[4] = Avg(n, l);
-  p[5] = w;
-  p[6] = ClampedGradient(n, w, l);
-  p[7] = n;
-}
-
-// Chroma predictors.
+// Chroma predictors from the neighbours n (above), w (left), l (above
+// left) and r (above right).
 inline void PredictXB(int16_t n, int16_t w, int16_t l, int16_t r,
                       int16_t p[8]) {
   p[0] = ClampedGradient(n, w, l);
@@ -105,66 +93,103 @@ inline void PredictXB(int16_t n, int16_t w, int16_t l, int16_t r,
   p[7] = Avg(Avg(w, r), n);
 }
 
-inline int ArgMin(const uint16_t c[8]) {
-  int best = 0;
-  for (int i = 1; i < 8; ++i) {
-    if (c[i] < c[best]) best = i;
+// Predictor i of PredictXB alone.
+inline int16_t PredictOneXB(int i, int16_t n, int16_t w, int16_t l,
+                            int16_t r) {
+  switch (i) {
+    case 0: return ClampedGradient(n, w, l);
+    case 1: return Avg(n, w);
+    case 2: return n;
+    case 3: return Avg(n, r);
+    case 4: return w;
+    case 5: return Avg(w, l);
+    case 6: return r;
+    default: return Avg(Avg(w, r), n);
   }
-  return best;
+}
+
+// The eight luma predictions as one vector of 16-bit lanes, lane i =
+// predictor i of PredictY: lanes 0..4 are averages of two operands (the
+// saturating sum halved), 5..7 the operand itself.
+__attribute__((target("sse4.1"))) inline __m128i PredictYv(int16_t n,
+                                                          int16_t w,
+                                                          int16_t l,
+                                                          int16_t r) {
+  const int16_t nw = Avg(n, w), g = ClampedGradient(n, w, l);
+  const __m128i a = _mm_setr_epi16(nw, w, n, w, n, w, g, n);
+  const __m128i b = _mm_setr_epi16(r, n, r, l, l, 0, 0, 0);
+  const __m128i avg = _mm_srai_epi16(_mm_adds_epi16(a, b), 1);
+  return _mm_blend_epi16(avg, a, 0xE0);
+}
+
+// AbsResidual of every lane: |c - p| of the saturated difference (the
+// 16-bit absolute value keeps -32768 as 0x8000).
+__attribute__((target("sse4.1"))) inline __m128i AbsResidualv(__m128i c,
+                                                             __m128i p) {
+  return _mm_abs_epi16(_mm_subs_epi16(c, p));
 }
 
 // One row of luma: `t` and `m` the two rows above (equal for row 1), `b`
 // the row being decoded from `res`. x = 0 is predicted by the pixel above,
 // x = 1 and the last x by the pixel to the left; in between the predictor
 // with the smallest error on the pixel above (predicted from its own
-// neighbours) plus on the pixel to the left.
-void ExpandRowY(int xs, const int16_t* res, const int16_t* t,
-                const int16_t* m, int16_t* b) {
+// neighbours) plus on the pixel to the left, costs summed with 16-bit
+// wrap-around and compared as unsigned values, the lowest index winning
+// ties (the minpos instruction the reference's default build uses).
+__attribute__((target("sse4.1"))) void ExpandRowY(int xs, const int16_t* res,
+                                                  const int16_t* t,
+                                                  const int16_t* m,
+                                                  int16_t* b) {
   b[0] = WrapAdd(m[0], res[0]);
   if (xs >= 2) b[1] = WrapAdd(b[0], res[1]);
   if (xs >= 4) {
-    int16_t pw[8];  // predictions for b[x - 1]
-    PredictY(m[1], b[0], m[0], m[2], pw);
+    // predictions for b[x - 1]
+    __m128i pw = PredictYv(m[1], b[0], m[0], m[2]);
+    alignas(16) int16_t pc[8];
     for (int x = 2; x < xs - 1; ++x) {
-      int16_t pn[8], pc[8];
-      PredictY(t[x], m[x - 1], t[x - 1], t[x + 1], pn);
-      uint16_t cost[8];
-      for (int i = 0; i < 8; ++i) {
-        cost[i] = uint16_t(AbsResidual(m[x], pn[i]) +
-                           AbsResidual(b[x - 1], pw[i]));
-      }
-      PredictY(m[x], b[x - 1], m[x - 1], m[x + 1], pc);
-      b[x] = WrapAdd(pc[ArgMin(cost)], res[x]);
-      for (int i = 0; i < 8; ++i) pw[i] = pc[i];
+      const __m128i pn = PredictYv(t[x], m[x - 1], t[x - 1], t[x + 1]);
+      const __m128i cost =
+          _mm_add_epi16(AbsResidualv(_mm_set1_epi16(m[x]), pn),
+                        AbsResidualv(_mm_set1_epi16(b[x - 1]), pw));
+      const int best = _mm_extract_epi16(_mm_minpos_epu16(cost), 1);
+      pw = PredictYv(m[x], b[x - 1], m[x - 1], m[x + 1]);
+      _mm_store_si128(reinterpret_cast<__m128i*>(pc), pw);
+      b[x] = WrapAdd(pc[best], res[x]);
     }
   }
   if (xs >= 2) b[xs - 1] = WrapAdd(b[xs - 2], res[xs - 1]);
 }
 
-// One row of chroma (X and B, the same predictor for both): the predictor
-// is chosen by its error on the co-located luma pixel, `ym` / `yb` the
-// decoded luma rows above and at the row.
-void ExpandRowXB(int xs, const int16_t* ym, const int16_t* yb,
-                 const int16_t* const res[2], const int16_t* const m[2],
-                 int16_t* const b[2]) {
-  for (int k = 0; k < 2; ++k) {
-    b[k][0] = WrapAdd(m[k][0], res[k][0]);
-    if (xs >= 2) b[k][1] = WrapAdd(b[k][0], res[k][1]);
-  }
+// The chroma predictor of every x of a row (both chroma channels use it):
+// the one with the smallest error on the co-located luma pixel, `ym` / `yb`
+// the decoded luma rows above and at the row.
+__attribute__((target("sse4.1"))) void ChromaPredictors(int xs,
+                                                        const int16_t* ym,
+                                                        const int16_t* yb,
+                                                        uint8_t* best) {
   for (int x = 2; x < xs - 1; ++x) {
-    int16_t py[8];
-    PredictXB(ym[x], yb[x - 1], ym[x - 1], ym[x + 1], py);
-    uint16_t cost[8];
-    for (int i = 0; i < 8; ++i) cost[i] = AbsResidual(yb[x], py[i]);
-    const int best = ArgMin(cost);
-    for (int k = 0; k < 2; ++k) {
-      int16_t pc[8];
-      PredictXB(m[k][x], b[k][x - 1], m[k][x - 1], m[k][x + 1], pc);
-      b[k][x] = WrapAdd(pc[best], res[k][x]);
-    }
+    int16_t p[8];
+    PredictXB(ym[x], yb[x - 1], ym[x - 1], ym[x + 1], p);
+    const __m128i pv = _mm_loadu_si128(reinterpret_cast<const __m128i*>(p));
+    const __m128i cost = AbsResidualv(_mm_set1_epi16(yb[x]), pv);
+    best[x] = uint8_t(_mm_extract_epi16(_mm_minpos_epu16(cost), 1));
   }
+}
+
+// One row of chroma (X and B): predictor best[x] of each pixel.
+void ExpandRowXB(int xs, const uint8_t* best, const int16_t* const res[2],
+                 const int16_t* const m[2], int16_t* const b[2]) {
   for (int k = 0; k < 2; ++k) {
-    if (xs >= 2) b[k][xs - 1] = WrapAdd(b[k][xs - 2], res[k][xs - 1]);
+    const int16_t* mk = m[k];
+    int16_t* bk = b[k];
+    bk[0] = WrapAdd(mk[0], res[k][0]);
+    if (xs >= 2) bk[1] = WrapAdd(bk[0], res[k][1]);
+    for (int x = 2; x < xs - 1; ++x) {
+      const int16_t p =
+          PredictOneXB(best[x], mk[x], bk[x - 1], mk[x - 1], mk[x + 1]);
+      bk[x] = WrapAdd(p, res[k][x]);
+    }
+    if (xs >= 2) bk[xs - 1] = WrapAdd(bk[xs - 2], res[k][xs - 1]);
   }
 }
 
@@ -190,12 +215,14 @@ void ExpandDc(int xs, int ys, const std::vector<int16_t>& q,
     ExpandRowY(xs, yq + size_t(y) * xs, t, yo + size_t(y - 1) * xs,
                yo + size_t(y) * xs);
   }
+  std::vector<uint8_t> best(size_t(std::max(xs, 1)), 0);
   for (int y = 1; y < ys; ++y) {
     const size_t my = size_t(y - 1) * xs, by = size_t(y) * xs;
     const int16_t* res[2] = {q.data() + by, q.data() + 2 * plane + by};
     const int16_t* m[2] = {out + my, out + 2 * plane + my};
     int16_t* b[2] = {out + by, out + 2 * plane + by};
-    ExpandRowXB(xs, yo + my, yo + by, res, m, b);
+    ChromaPredictors(xs, yo + my, yo + by, best.data());
+    ExpandRowXB(xs, best.data(), res, m, b);
   }
 }
 
