@@ -1841,8 +1841,8 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
 namespace {
 constexpr int kBranches = 32;
 
-typedef void (*BatchKernel)(const BatchFrame*, const BatchSpan*, int,
-                            uint32_t, const DeviceConsts*);
+typedef void (*BatchKernel)(const BatchFrame*, const BatchSpan*,
+                            const uint32_t*, uint32_t, const DeviceConsts*);
 
 // The phases in launch order: name, kernel, whether it can loop
 // (persistent), and its minimum workgroups per CU for the loop's grid.
@@ -1889,8 +1889,9 @@ struct pikgpu_batch {
   float last_ms = 0.0f;
   pikgpu_batch_schedule sched = {PIKGPU_BATCH_QUEUE, 1, {}};
   int num_cus = 256;
-  // queue schedule: the frames' parameters and the phases' span tables
-  DevBuf frames, spans;
+  // queue schedule: the frames' parameters, the phases' span tables and
+  // their workgroup -> span maps
+  DevBuf frames, spans, maps;
 };
 
 namespace {
@@ -2057,11 +2058,13 @@ int BuildQueueGraph(pikgpu_batch* b) {
     size_t first_span;
     int ns;
     uint32_t total;
+    size_t first_map;
   };
   std::vector<BatchSpan> spans;
+  std::vector<uint32_t> maps;
   std::vector<Launch> launches;
   for (int p = 0; p < PIKGPU_BATCH_PHASES; ++p) {
-    Launch L = {p, spans.size(), 0, 0};
+    Launch L = {p, spans.size(), 0, 0, maps.size()};
     uint32_t at = 0;
     for (int i : order) {
       BatchSpan sp;
@@ -2079,12 +2082,21 @@ int BuildQueueGraph(pikgpu_batch* b) {
       L.ns += 1;
     }
     L.total = at;
-    if (L.ns > 0) launches.push_back(L);
+    if (L.ns == 0) continue;
+    maps.resize(L.first_map + at, kNoSpan);
+    for (int j = 0; j < L.ns; ++j) {
+      const BatchSpan& sp = spans[L.first_span + j];
+      std::fill_n(maps.begin() + L.first_map + sp.first, sp.count,
+                  uint32_t(j));
+    }
+    launches.push_back(L);
   }
   int rc;
   if ((rc = GrowRaw(&b->frames, frames.size() * sizeof(BatchFrame))) ||
       (rc = GrowRaw(&b->spans, std::max<size_t>(1, spans.size()) *
-                                   sizeof(BatchSpan)))) {
+                                   sizeof(BatchSpan))) ||
+      (rc = GrowRaw(&b->maps,
+                    std::max<size_t>(1, maps.size()) * sizeof(uint32_t)))) {
     return BFail(b, rc, "batch tables: out of device memory");
   }
   BHIP_TRY(b, hipMemcpy(b->frames.p, frames.data(),
@@ -2094,9 +2106,13 @@ int BuildQueueGraph(pikgpu_batch* b) {
     BHIP_TRY(b, hipMemcpy(b->spans.p, spans.data(),
                           spans.size() * sizeof(BatchSpan),
                           hipMemcpyHostToDevice));
+    BHIP_TRY(b, hipMemcpy(b->maps.p, maps.data(),
+                          maps.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
   }
   const BatchFrame* F = static_cast<const BatchFrame*>(b->frames.p);
   const BatchSpan* S0 = static_cast<const BatchSpan*>(b->spans.p);
+  const uint32_t* M0 = static_cast<const uint32_t*>(b->maps.p);
   const DeviceConsts* K = b->items[0]->consts;
   BHIP_TRY(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
@@ -2109,9 +2125,9 @@ int BuildQueueGraph(pikgpu_batch* b) {
       grid = std::min<uint32_t>(grid, uint32_t(XcdGrid(k * b->num_cus)));
     }
     const BatchSpan* S = S0 + L.first_span;
-    int ns = L.ns;
+    const uint32_t* M = M0 + L.first_map;
     uint32_t total = L.total;
-    void* args[] = {&F, &S, &ns, &total, &K};
+    void* args[] = {&F, &S, &M, &total, &K};
     e = hipLaunchKernel(reinterpret_cast<const void*>(d.kernel), dim3(grid),
                         dim3(256), args, 0, b->main);
   }
@@ -2213,6 +2229,7 @@ void pikgpu_batch_destroy(pikgpu_batch* b) {
   }
   if (b->frames.p) (void)hipFree(b->frames.p);
   if (b->spans.p) (void)hipFree(b->spans.p);
+  if (b->maps.p) (void)hipFree(b->maps.p);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->t0) (void)hipEventDestroy(b->t0);
   if (b->t1) (void)hipEventDestroy(b->t1);

@@ -581,23 +581,12 @@ struct BatchSpan {
   int arg[kBatchArgs];
 };
 
-// The span holding launch workgroup w (spans sorted by `first`, the first
-// at 0); false for the idle workgroups between spans.
+// The span table's companion map: entry w is the index (into the launch's
+// spans) of the span holding launch workgroup w, or kNoSpan for the idle
+// workgroups between spans (one load instead of a search per workgroup).
 typedef const BatchSpan ConstSpan;
+constexpr uint32_t kNoSpan = 0xFFFFFFFFu;
 
-__device__ inline bool FindSpan(ConstSpan* S, int ns, uint32_t w, int* si) {
-  int lo = 0, hi = ns - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (S[mid].first <= w) {
-      lo = mid;
-    } else {
-      hi = mid - 1;
-    }
-  }
-  *si = lo;
-  return w - S[lo].first < S[lo].count;
-}
 __device__ inline VBlock SpanBlock(ConstSpan& s, uint32_t w) {
   const int l = int(w - s.first);
   VBlock v;
@@ -617,12 +606,12 @@ __device__ inline VBlock SpanBlock(ConstSpan& s, uint32_t w) {
 // frame's), and waits for the workgroup before the next one reuses LDS.
 #define PIKGPU_BATCH_LOOP(body)                                       \
   for (uint32_t w_ = blockIdx.x; w_ < total; w_ += gridDim.x) {       \
-    int si_ = 0;                                                      \
-    if (FindSpan(S, ns, w_, &si_)) {                                  \
+    const uint32_t si_ = M[w_];                                       \
+    if (si_ != kNoSpan) {                                             \
       ConstSpan& s = S[si_];                                          \
       const VBlock vb = SpanBlock(s, w_);                             \
-      const FrameParams P = F[s.frame].P;                             \
-      const FrameBuffers B = F[s.frame].B;                            \
+      const FrameParams& P = F[s.frame].P;                            \
+      const FrameBuffers& B = F[s.frame].B;                           \
       body;                                                           \
     }                                                                 \
     __syncthreads();                                                  \
@@ -635,12 +624,12 @@ __device__ inline VBlock SpanBlock(ConstSpan& s, uint32_t w) {
 #define PIKGPU_BATCH_ONE(body)                                        \
   {                                                                   \
     const uint32_t w_ = blockIdx.x;                                   \
-    int si_ = 0;                                                      \
-    if (w_ < total && FindSpan(S, ns, w_, &si_)) {                    \
+    const uint32_t si_ = w_ < total ? M[w_] : kNoSpan;                \
+    if (si_ != kNoSpan) {                                             \
       ConstSpan& s = S[si_];                                          \
       const VBlock vb = SpanBlock(s, w_);                             \
-      const FrameParams P = F[s.frame].P;                             \
-      const FrameBuffers B = F[s.frame].B;                            \
+      const FrameParams& P = F[s.frame].P;                            \
+      const FrameBuffers& B = F[s.frame].B;                           \
       body;                                                           \
     }                                                                 \
   }
@@ -648,7 +637,8 @@ __device__ inline VBlock SpanBlock(ConstSpan& s, uint32_t w) {
 // The batched kernels' common parameters.
 #define PIKGPU_BATCH_PARAMS                                             \
   const BatchFrame *__restrict__ F, const BatchSpan *__restrict__ S,    \
-      int ns, uint32_t total, const DeviceConsts *K
+      const uint32_t *__restrict__ M, uint32_t total,                   \
+      const DeviceConsts *K
 
 
 // ---------------------------------------------------------------------------

@@ -15,7 +15,8 @@
 namespace pikgpu {
 
 __device__ __forceinline__ void KBlocksBody(
-    FrameParams P, FrameBuffers B, const DeviceConsts* K, const VBlock& vb) {
+    const FrameParams& P, const FrameBuffers& B, const DeviceConsts* K,
+    const VBlock& vb) {
   // ~39 KB: four workgroups per CU. The IDCT's pixels reuse the HF
   // prediction's pixel buffer (dead once its transform is added).
   __shared__ float s_coef[3][kCoefPlane];
