@@ -89,6 +89,12 @@ struct pikgpu_ctx {
   hipEvent_t adj_ev[3] = {};
   hipEvent_t enc_ev[2] = {};
   float timings[kNumTimers] = {};
+  // Pinned host staging for strided host planes with short rows (packed
+  // here, then one linear copy): [0, stage_used) may still be in flight
+  // until stage_ev, recorded after the last staged copy, has completed.
+  void* stage = nullptr;
+  size_t stage_cap = 0, stage_used = 0;
+  hipEvent_t stage_ev = nullptr;
 };
 
 namespace {
@@ -144,11 +150,57 @@ void FreeAll(pikgpu_ctx* ctx) {
   }
 }
 
-// Copies a strided host plane into a dense device region.
+// Rows shorter than this are packed into the pinned staging buffer when
+// the host plane is strided; longer ones go row by row (per-row copy cost
+// is then small against the row's own transfer).
+constexpr size_t kStageRowMax = 32 << 10;
+
+// Makes the pinned staging buffer at least n bytes with nothing in flight.
+int StageReserve(pikgpu_ctx* ctx, size_t n) {
+  if (ctx->stage_used > 0) {
+    HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev));
+    ctx->stage_used = 0;
+  }
+  if (n > ctx->stage_cap) {
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_cap = 0;
+    HIP_TRY(ctx, hipHostMalloc(&ctx->stage, n, 0));
+    ctx->stage_cap = n;
+  }
+  return PIKGPU_OK;
+}
+
+// Packs a strided host plane into the context's pinned staging buffer and
+// queues one linear copy of it.
+int StageUpload(pikgpu_ctx* ctx, void* dst, const void* src,
+                size_t src_stride_b, size_t row_b, size_t rows) {
+  const size_t n = row_b * rows;
+  if (ctx->stage_used + n > ctx->stage_cap) {
+    const int rc = StageReserve(ctx, std::max<size_t>(n, size_t(4) << 20));
+    if (rc != PIKGPU_OK) return rc;
+  }
+  uint8_t* p = static_cast<uint8_t*>(ctx->stage) + ctx->stage_used;
+  const uint8_t* q = static_cast<const uint8_t*>(src);
+  for (size_t y = 0; y < rows; ++y) {
+    memcpy(p + y * row_b, q + y * src_stride_b, row_b);
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->stage_ev, ctx->stream));
+  ctx->stage_used += (n + 255) & ~size_t(255);
+  return PIKGPU_OK;
+}
+
+// Copies a strided host plane into a dense device region: one linear copy
+// when the plane is dense, packed through pinned staging when its rows are
+// short, else the 2-D copy.
 int Upload2D(pikgpu_ctx* ctx, void* dst, const void* src, size_t src_stride_b,
              size_t row_b, size_t rows) {
-  HIP_TRY(ctx, hipMemcpy2DAsync(dst, row_b, src, src_stride_b, row_b, rows,
-                                hipMemcpyHostToDevice, ctx->stream));
+  if (rows > 1 && src_stride_b != row_b && row_b < kStageRowMax) {
+    return StageUpload(ctx, dst, src, src_stride_b, row_b, rows);
+  }
+  HIP_TRY(ctx, Copy2D(dst, row_b, src, src_stride_b, row_b, rows,
+                      hipMemcpyHostToDevice, ctx->stream));
   return PIKGPU_OK;
 }
 
@@ -157,6 +209,16 @@ float ToF(double v) { return static_cast<float>(v); }
 }  // namespace
 
 namespace pikgpu {
+
+hipError_t Copy2D(void* dst, size_t dpitch, const void* src, size_t spitch,
+                  size_t row_b, size_t rows, hipMemcpyKind kind,
+                  hipStream_t stream) {
+  if (rows <= 1 || (dpitch == row_b && spitch == row_b)) {
+    return hipMemcpyAsync(dst, src, row_b * rows, kind, stream);
+  }
+  return hipMemcpy2DAsync(dst, dpitch, src, spitch, row_b, rows, kind,
+                          stream);
+}
 
 // Inverse of the opsin absorbance matrix as Inv3x3Matrix<float> computes it
 // (pik/linalg.h:153-170, pik/opsin_params.cc:13-26).
@@ -218,6 +280,11 @@ pikgpu_ctx* pikgpu_create(int device, const pikgpu_config* config) {
       return nullptr;
     }
   }
+  if (hipEventCreateWithFlags(&ctx->stage_ev, hipEventDisableTiming) !=
+      hipSuccess) {
+    delete ctx;
+    return nullptr;
+  }
   // Constant tables.
   DeviceConsts* h = new DeviceConsts;
   memset(h, 0, sizeof(*h));
@@ -255,6 +322,8 @@ void pikgpu_destroy(pikgpu_ctx* ctx) {
   for (hipEvent_t e : ctx->enc_ev) {
     if (e) (void)hipEventDestroy(e);
   }
+  if (ctx->stage_ev) (void)hipEventDestroy(ctx->stage_ev);
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -377,7 +446,11 @@ int ValidateFrame(const pikgpu_frame* f, std::string* err) {
           ox + s > bx || oy + s > by) {
         return VFail(err, PIKGPU_EINVAL, "misaligned multi-block transform");
       }
-      for (int k = 0; k < s * s; ++k) {
+      // A transform's first block checks the whole transform; every other
+      // block only that its first block is one of the same kind, whose own
+      // check then covers it.
+      const int kmax = idx == 0 ? s * s : 1;
+      for (int k = 0; k < kmax; ++k) {
         const uint8_t e =
             f->ac_strategy[(oy + k / s) * f->ac_strategy_stride + ox + k % s];
         if (e != ((raw << 4) | k)) {
@@ -1080,11 +1153,30 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
   }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const size_t oplane = size_t(ctx->B.opitch) * ys;
+  const size_t dpitch_b = size_t(ctx->B.opitch) * 4, row_b = xs * 4;
+  if (ys > 1 && (dpitch_b != row_b || out_stride_bytes != row_b) &&
+      row_b < kStageRowMax) {
+    // Short strided rows: the three planes (contiguous in device memory)
+    // in one copy to pinned staging, then row by row on the host.
+    int rc = StageReserve(ctx, 3 * oplane * 4);
+    if (rc != PIKGPU_OK) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->stage, ctx->B.out, 3 * oplane * 4,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = pikgpu_sync(ctx)) != PIKGPU_OK) return rc;
+    const uint8_t* p = static_cast<const uint8_t*>(ctx->stage);
+    for (int c = 0; c < 3; ++c) {
+      uint8_t* q = reinterpret_cast<uint8_t*>(out_planes[c]);
+      for (size_t y = 0; y < ys; ++y) {
+        memcpy(q + y * out_stride_bytes, p + (c * oplane) * 4 + y * dpitch_b,
+               row_b);
+      }
+    }
+    return PIKGPU_OK;
+  }
   for (int c = 0; c < 3; ++c) {
-    HIP_TRY(ctx, hipMemcpy2DAsync(out_planes[c], out_stride_bytes,
-                                  ctx->B.out + c * oplane,
-                                  size_t(ctx->B.opitch) * 4, xs * 4, ys,
-                                  hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, Copy2D(out_planes[c], out_stride_bytes,
+                        ctx->B.out + c * oplane, dpitch_b, row_b, ys,
+                        hipMemcpyDeviceToHost, ctx->stream));
   }
   return pikgpu_sync(ctx);
 }
@@ -1281,12 +1373,12 @@ int pikgpu_quantize(pikgpu_ctx* ctx, const float* const opsin[3],
   HIP_TRY(ctx, hipEventRecord(ctx->enc_ev[1], st));
   for (int c = 0; c < 3; ++c) {
     if (ac_out) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(ac_out[c], ac_stride * 2,
+      HIP_TRY(ctx, Copy2D(ac_out[c], ac_stride * 2,
                                     ac16 + c * nb * 64, row * 2, row * 2,
                                     size_t(P.by), hipMemcpyDeviceToHost, st));
     }
     if (dc_out) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(dc_out[c], dc_stride * 2, dc16 + c * nb,
+      HIP_TRY(ctx, Copy2D(dc_out[c], dc_stride * 2, dc16 + c * nb,
                                     size_t(P.bx) * 2, size_t(P.bx) * 2,
                                     size_t(P.by), hipMemcpyDeviceToHost, st));
     }
@@ -1328,7 +1420,7 @@ int pikgpu_download_srgb8(pikgpu_ctx* ctx, uint8_t* rgb, size_t stride_bytes) {
     return Fail(ctx, PIKGPU_EINVAL, "bad output buffer or stride");
   }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  HIP_TRY(ctx, hipMemcpy2DAsync(rgb, stride_bytes, ctx->srgb8.p,
+  HIP_TRY(ctx, Copy2D(rgb, stride_bytes, ctx->srgb8.p,
                                 size_t(ctx->spitch), xs * 3, ys,
                                 hipMemcpyDeviceToHost, ctx->stream));
   return pikgpu_sync(ctx);
@@ -1420,7 +1512,7 @@ int pikgpu_stage_dump(pikgpu_ctx* ctx, int stage, float* planes[3],
       src = B.out; pitch = B.opitch; plane = size_t(B.opitch) * P.ysize; break;
   }
   for (int c = 0; c < 3; ++c) {
-    HIP_TRY(ctx, hipMemcpy2DAsync(planes[c], stride_bytes, src + c * plane,
+    HIP_TRY(ctx, Copy2D(planes[c], stride_bytes, src + c * plane,
                                   pitch * 4, size_t(xs) * 4, ys,
                                   hipMemcpyDeviceToHost, ctx->stream));
   }
@@ -1590,16 +1682,16 @@ int pikgpu_fwd_adj(pikgpu_ctx* ctx, const pikgpu_adj_io* io) {
     }
     return PIKGPU_OK;
   }
-  HIP_TRY(ctx, hipMemcpy2DAsync(io->dl_dqf, io->dqf_stride * 4, A.dqf, bx * 4,
+  HIP_TRY(ctx, Copy2D(io->dl_dqf, io->dqf_stride * 4, A.dqf, bx * 4,
                                 bx * 4, by, hipMemcpyDeviceToHost, st));
   for (int c = 0; c < 3; ++c) {
     if (nlin) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(io->linear[c], io->linear_stride * 4,
+      HIP_TRY(ctx, Copy2D(io->linear[c], io->linear_stride * 4,
                                     A.linear[c], xs * 4, xs * 4, ys,
                                     hipMemcpyDeviceToHost, st));
     }
     if (ndc) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(io->dl_dcoeffs[c], io->dcoeffs_stride * 4,
+      HIP_TRY(ctx, Copy2D(io->dl_dcoeffs[c], io->dcoeffs_stride * 4,
                                     A.dcoef[c], bx * 64 * 4, bx * 64 * 4, by,
                                     hipMemcpyDeviceToHost, st));
     }
@@ -1795,18 +1887,18 @@ int pikgpu_search_quant_field(pikgpu_ctx* ctx,
   HIP_TRY(ctx, hipGetLastError());
   // outputs
   if (io->quant_field) {
-    HIP_TRY(ctx, hipMemcpy2DAsync(io->quant_field, io->quant_field_stride * 4,
+    HIP_TRY(ctx, Copy2D(io->quant_field, io->quant_field_stride * 4,
                                   qf, bx * 4, bx * 4, by,
                                   hipMemcpyDeviceToHost, st));
   }
   for (int c = 0; c < 3; ++c) {
     if (nac) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(io->ac[c], io->ac_stride * 2,
+      HIP_TRY(ctx, Copy2D(io->ac[c], io->ac_stride * 2,
                                     ac16 + c * nb * 64, bx * 128, bx * 128, by,
                                     hipMemcpyDeviceToHost, st));
     }
     if (ndc) {
-      HIP_TRY(ctx, hipMemcpy2DAsync(io->dc[c], io->dc_stride * 2,
+      HIP_TRY(ctx, Copy2D(io->dc[c], io->dc_stride * 2,
                                     dc16 + c * nb, bx * 2, bx * 2, by,
                                     hipMemcpyDeviceToHost, st));
     }

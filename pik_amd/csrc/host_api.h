@@ -36,6 +36,15 @@ void BinDots(const pikgpu_frame* f, int px, int py, std::vector<int32_t>* ids,
 // Inverse of the opsin absorbance matrix (Inv3x3Matrix<float>).
 void OpsinInverse(float out[9]);
 
+// A 2-D copy, issued as one linear copy when both sides' rows are
+// contiguous. The runtime moves a host-side 2-D copy row by row (from
+// pageable memory, one staged copy per row: a 4096x4096 frame's 512-row
+// side planes took ~30 ms that way, against ~2 ms for the whole upload as
+// linear copies).
+hipError_t Copy2D(void* dst, size_t dpitch, const void* src, size_t spitch,
+                  size_t row_b, size_t rows, hipMemcpyKind kind,
+                  hipStream_t stream);
+
 // The full-size reconstruction of a frame whose inputs are in device memory
 // (B), asynchronous on `st`; allocates nothing. `ev` (may be null) receives
 // events 0..7 between the kernel families.

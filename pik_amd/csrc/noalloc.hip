@@ -209,7 +209,7 @@ int NoAllocRun(const pikgpu_frame* f, float* const out_planes[3],
   } while (0)
   auto up2d = [&](const void* dst, const void* src, size_t spitch,
                   size_t row, size_t rows) {
-    return hipMemcpy2DAsync(const_cast<void*>(dst), row, src, spitch, row,
+    return Copy2D(const_cast<void*>(dst), row, src, spitch, row,
                             rows, hipMemcpyHostToDevice, st);
   };
   for (int c = 0; c < 3; ++c) {
@@ -253,7 +253,7 @@ int NoAllocRun(const pikgpu_frame* f, float* const out_planes[3],
   TRY(EnqueueFullSize(P, B, K, st, nullptr));
   const size_t oplane = size_t(B.opitch) * P.ysize;
   for (int c = 0; c < 3; ++c) {
-    TRY(hipMemcpy2DAsync(out_planes[c], out_stride_bytes, B.out + c * oplane,
+    TRY(Copy2D(out_planes[c], out_stride_bytes, B.out + c * oplane,
                          size_t(B.opitch) * 4, size_t(P.xsize) * 4, P.ysize,
                          hipMemcpyDeviceToHost, st));
   }

@@ -190,6 +190,62 @@ def test_rejects_inconsistent_strategies(frame):
     _expect(frame, abi.EINVAL, "misaligned|inconsistent", ac_strategy=acs)
 
 
+def _strategy_ok(acs):
+    """Whole-map restatement of the strategy rule (AcStrategyImage::
+    SetFromRaw, pik/ac_strategy.cc:1066-1099): every block names a valid
+    raw strategy and index, and belongs to a complete, aligned transform
+    inside the image whose blocks carry indices 0 .. s*s-1 in raster
+    order."""
+    by, bx = acs.shape
+    for y in range(by):
+        for x in range(bx):
+            raw, idx = int(acs[y, x]) >> 4, int(acs[y, x]) & 15
+            if raw > 8:
+                return False
+            s = synth.covered(raw)
+            if idx >= s * s:
+                return False
+            oy, ox = y - idx // s, x - idx % s
+            if oy < 0 or ox < 0 or oy % s or ox % s or oy + s > by or \
+                    ox + s > bx:
+                return False
+            for k in range(s * s):
+                if int(acs[oy + k // s, ox + k % s]) != (raw << 4) | k:
+                    return False
+    return True
+
+
+def test_strategy_check_against_whole_map_rule(frame):
+    """The library's check (each transform's blocks checked once, from its
+    first block) accepts and rejects exactly what the whole-map rule does,
+    on valid tilings with one or two entries changed."""
+    rng = np.random.default_rng(7)
+    by, bx = frame.ac_strategy.shape
+    base = frame.ac_strategy
+    kinds = [abi.ACS_DCT, abi.ACS_DCT16X16, abi.ACS_DCT32X32,
+             abi.ACS_DCT4X4, abi.ACS_IDENTITY]
+    seen = {True: 0, False: 0}
+    for trial in range(300):
+        acs = base.copy()
+        for _ in range(int(rng.integers(1, 3))):
+            y, x = int(rng.integers(0, by)), int(rng.integers(0, bx))
+            raw = kinds[int(rng.integers(0, len(kinds)))]
+            s = synth.covered(raw)
+            acs[y, x] = (raw << 4) | int(rng.integers(0, s * s))
+        want = _strategy_ok(acs)
+        fr = frame.copy()
+        fr.ac_strategy = acs
+        try:
+            validate(fr)
+            got = True
+        except PikGpuError as e:
+            assert e.code == abi.EINVAL
+            got = False
+        assert got == want, trial
+        seen[want] += 1
+    assert seen[True] > 10 and seen[False] > 100, seen
+
+
 def test_rejects_dequant_tables(frame):
     dm = frame.dequant_map.copy()
     dm[0, 0] = 200

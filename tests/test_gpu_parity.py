@@ -200,6 +200,29 @@ def test_padded_image_strides(rec):
     assert np.array_equal(pad, base[..., frame.xsize:])
 
 
+def test_padded_strides_wide_rows_and_repeated_uploads(rec):
+    """Padded rows on both sides of the host staging threshold: the AC rows
+    of a 2112-wide frame (33792 bytes) go as a 2-D copy, its short side
+    planes through the pinned staging buffer; three uploads in a row (the
+    staging buffer reused and wrapped) and the padded download."""
+    frame = synth.make_frame(2112, 96, seed=97, num_dots=2)
+    want, _, _ = checker_recon(frame, stages=False)
+    fr = frame.copy()
+    for name in ("ac", "dc", "quant_field", "ac_strategy", "sigma_lut_ids",
+                 "dequant_cf", "ytox_map", "ytob_map"):
+        setattr(fr, name, padded(getattr(frame, name)))
+    assert fr.ac.strides[-2] > 32 << 10
+    for k in range(3):
+        print("upload", k, flush=True)
+        rec.upload(fr)
+        rec.run()
+        out = padded(np.zeros((3, frame.ysize, frame.xsize), np.float32))
+        rec.download(out=out)
+        assert bit_equal_fraction(out, want) == 1.0
+        tight = rec.download()
+        assert bit_equal_fraction(tight, want) == 1.0
+
+
 def test_context_reuse_shrinking_and_growing(rec):
     """One context, frames of changing size: buffers are reused or grown and
     no state leaks from one frame into the next."""
