@@ -154,6 +154,8 @@ void FreeAll(pikgpu_ctx* ctx) {
 // the host plane is strided; longer ones go row by row (per-row copy cost
 // is then small against the row's own transfer).
 constexpr size_t kStageRowMax = 32 << 10;
+// Most bytes staged at once (larger planes go in chunks of rows).
+constexpr size_t kStageChunk = 16 << 20;
 
 // Makes the pinned staging buffer at least n bytes with nothing in flight.
 int StageReserve(pikgpu_ctx* ctx, size_t n) {
@@ -171,23 +173,28 @@ int StageReserve(pikgpu_ctx* ctx, size_t n) {
   return PIKGPU_OK;
 }
 
-// Packs a strided host plane into the context's pinned staging buffer and
-// queues one linear copy of it.
+// Packs a strided host plane (rows shorter than kStageRowMax) into the
+// context's pinned staging buffer and queues one linear copy per chunk of
+// at most kStageChunk bytes.
 int StageUpload(pikgpu_ctx* ctx, void* dst, const void* src,
                 size_t src_stride_b, size_t row_b, size_t rows) {
-  const size_t n = row_b * rows;
-  if (ctx->stage_used + n > ctx->stage_cap) {
-    const int rc = StageReserve(ctx, std::max<size_t>(n, size_t(4) << 20));
-    if (rc != PIKGPU_OK) return rc;
+  const size_t per = std::max<size_t>(1, kStageChunk / row_b);
+  for (size_t y0 = 0; y0 < rows; y0 += per) {
+    const size_t ny = std::min(per, rows - y0), n = row_b * ny;
+    if (ctx->stage_used + n > ctx->stage_cap) {
+      const int rc = StageReserve(ctx, kStageChunk);
+      if (rc != PIKGPU_OK) return rc;
+    }
+    uint8_t* p = static_cast<uint8_t*>(ctx->stage) + ctx->stage_used;
+    const uint8_t* q = static_cast<const uint8_t*>(src) + y0 * src_stride_b;
+    for (size_t y = 0; y < ny; ++y) {
+      memcpy(p + y * row_b, q + y * src_stride_b, row_b);
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(static_cast<uint8_t*>(dst) + y0 * row_b, p, n,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->stage_ev, ctx->stream));
+    ctx->stage_used += (n + 255) & ~size_t(255);
   }
-  uint8_t* p = static_cast<uint8_t*>(ctx->stage) + ctx->stage_used;
-  const uint8_t* q = static_cast<const uint8_t*>(src);
-  for (size_t y = 0; y < rows; ++y) {
-    memcpy(p + y * row_b, q + y * src_stride_b, row_b);
-  }
-  HIP_TRY(ctx, hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipEventRecord(ctx->stage_ev, ctx->stream));
-  ctx->stage_used += (n + 255) & ~size_t(255);
   return PIKGPU_OK;
 }
 
@@ -1156,19 +1163,24 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
   const size_t dpitch_b = size_t(ctx->B.opitch) * 4, row_b = xs * 4;
   if (ys > 1 && (dpitch_b != row_b || out_stride_bytes != row_b) &&
       row_b < kStageRowMax) {
-    // Short strided rows: the three planes (contiguous in device memory)
-    // in one copy to pinned staging, then row by row on the host.
-    int rc = StageReserve(ctx, 3 * oplane * 4);
+    // Short strided rows: chunks of whole device rows (contiguous) to
+    // pinned staging, then row by row on the host.
+    int rc = StageReserve(ctx, kStageChunk);
     if (rc != PIKGPU_OK) return rc;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->stage, ctx->B.out, 3 * oplane * 4,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    if ((rc = pikgpu_sync(ctx)) != PIKGPU_OK) return rc;
+    const size_t per = std::max<size_t>(1, kStageChunk / dpitch_b);
     const uint8_t* p = static_cast<const uint8_t*>(ctx->stage);
     for (int c = 0; c < 3; ++c) {
       uint8_t* q = reinterpret_cast<uint8_t*>(out_planes[c]);
-      for (size_t y = 0; y < ys; ++y) {
-        memcpy(q + y * out_stride_bytes, p + (c * oplane) * 4 + y * dpitch_b,
-               row_b);
+      const float* plane = ctx->B.out + c * oplane;
+      for (size_t y0 = 0; y0 < ys; y0 += per) {
+        const size_t ny = std::min(per, ys - y0);
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->stage, plane + y0 * (dpitch_b / 4),
+                                    ny * dpitch_b, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        if ((rc = pikgpu_sync(ctx)) != PIKGPU_OK) return rc;
+        for (size_t y = 0; y < ny; ++y) {
+          memcpy(q + (y0 + y) * out_stride_bytes, p + y * dpitch_b, row_b);
+        }
       }
     }
     return PIKGPU_OK;

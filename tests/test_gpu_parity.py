@@ -223,6 +223,26 @@ def test_padded_strides_wide_rows_and_repeated_uploads(rec):
         assert bit_equal_fraction(tight, want) == 1.0
 
 
+def test_padded_strides_in_chunks(rec):
+    """Padded planes larger than one staging chunk (16 MB): the AC planes
+    of a 2040x4160 frame (32640-byte rows, 17 MB each) and its output
+    planes (34 MB each) move in several chunks; the result equals the
+    tightly packed frame's bit for bit."""
+    frame = synth.make_frame(2040, 4160, seed=98, num_rects=20, num_dots=0)
+    want = rec.reconstruct(frame)
+    fr = frame.copy()
+    for name in ("ac", "dc", "quant_field", "ac_strategy", "sigma_lut_ids",
+                 "dequant_cf", "ytox_map", "ytob_map"):
+        setattr(fr, name, padded(getattr(frame, name)))
+    assert fr.ac.strides[-2] > fr.ac.shape[-1] * 2  # padded rows
+    assert fr.ac.shape[-1] * 2 < 32 << 10 and fr.ac[0].size * 2 > 16 << 20
+    rec.upload(fr)
+    rec.run()
+    out = padded(np.zeros((3, frame.ysize, frame.xsize), np.float32))
+    rec.download(out=out)
+    assert bit_equal_fraction(out, want) == 1.0
+
+
 def test_context_reuse_shrinking_and_growing(rec):
     """One context, frames of changing size: buffers are reused or grown and
     no state leaks from one frame into the next."""
