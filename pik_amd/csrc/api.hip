@@ -157,18 +157,21 @@ constexpr size_t kStageRowMax = 32 << 10;
 // Most bytes staged at once (larger planes go in chunks of rows).
 constexpr size_t kStageChunk = 16 << 20;
 
-// Makes the pinned staging buffer at least n bytes with nothing in flight.
+// Makes the pinned staging buffer at least n <= kStageChunk bytes with
+// nothing in flight; it grows by doubling up to kStageChunk, so a context
+// of small frames keeps a small one.
 int StageReserve(pikgpu_ctx* ctx, size_t n) {
   if (ctx->stage_used > 0) {
     HIP_TRY(ctx, hipEventSynchronize(ctx->stage_ev));
     ctx->stage_used = 0;
   }
   if (n > ctx->stage_cap) {
+    const size_t cap = std::min(kStageChunk, std::max(n, 2 * ctx->stage_cap));
     if (ctx->stage) (void)hipHostFree(ctx->stage);
     ctx->stage = nullptr;
     ctx->stage_cap = 0;
-    HIP_TRY(ctx, hipHostMalloc(&ctx->stage, n, 0));
-    ctx->stage_cap = n;
+    HIP_TRY(ctx, hipHostMalloc(&ctx->stage, cap, 0));
+    ctx->stage_cap = cap;
   }
   return PIKGPU_OK;
 }
@@ -182,7 +185,7 @@ int StageUpload(pikgpu_ctx* ctx, void* dst, const void* src,
   for (size_t y0 = 0; y0 < rows; y0 += per) {
     const size_t ny = std::min(per, rows - y0), n = row_b * ny;
     if (ctx->stage_used + n > ctx->stage_cap) {
-      const int rc = StageReserve(ctx, kStageChunk);
+      const int rc = StageReserve(ctx, n);
       if (rc != PIKGPU_OK) return rc;
     }
     uint8_t* p = static_cast<uint8_t*>(ctx->stage) + ctx->stage_used;
@@ -1165,9 +1168,9 @@ int pikgpu_download(pikgpu_ctx* ctx, float* out_planes[3],
       row_b < kStageRowMax) {
     // Short strided rows: chunks of whole device rows (contiguous) to
     // pinned staging, then row by row on the host.
-    int rc = StageReserve(ctx, kStageChunk);
-    if (rc != PIKGPU_OK) return rc;
     const size_t per = std::max<size_t>(1, kStageChunk / dpitch_b);
+    int rc = StageReserve(ctx, std::min(per, ys) * dpitch_b);
+    if (rc != PIKGPU_OK) return rc;
     const uint8_t* p = static_cast<const uint8_t*>(ctx->stage);
     for (int c = 0; c < 3; ++c) {
       uint8_t* q = reinterpret_cast<uint8_t*>(out_planes[c]);
