@@ -354,7 +354,7 @@ PIKGPU_API void* pikgpu_stream(pikgpu_ctx* ctx);
  * frame keeps its own device buffers; pikgpu_batch_run launches one hipGraph
  * holding, per phase of the reconstruction, one launch over the work of
  * every frame (see pikgpu_batch_schedule). Frames run at full size, linear
- * output, single pass. */
+ * output, with the passes pikgpu_batch_add_pass gives them. */
 typedef struct pikgpu_batch pikgpu_batch;
 
 PIKGPU_API pikgpu_batch* pikgpu_batch_create(int device,
@@ -365,8 +365,16 @@ PIKGPU_API const char* pikgpu_batch_last_error(const pikgpu_batch* batch);
  * reported as "frame i: <reason>". */
 PIKGPU_API int pikgpu_batch_upload(pikgpu_batch* batch,
                                    const pikgpu_frame* frames, int n);
+/* Adds an AC pass to frame i of the uploaded batch (multi-pass
+ * accumulation as pikgpu_frame_add_pass: Dequant<false>::DoAC of
+ * DecodeFromBitstream<false>, pik/pik_frame.cc:903-917,
+ * pik/compressed_image.cc:575-703); call once per further pass, in pass
+ * order, after pikgpu_batch_upload. The next run rebuilds the graph. */
+PIKGPU_API int pikgpu_batch_add_pass(pikgpu_batch* batch, int i,
+                                     const int16_t* const ac[3],
+                                     size_t ac_stride);
 /* Launches the reconstruction of every uploaded frame (asynchronous). The
- * graph is built on the first run after an upload. */
+ * graph is built on the first run after an upload or added pass. */
 PIKGPU_API int pikgpu_batch_run(pikgpu_batch* batch);
 PIKGPU_API int pikgpu_batch_sync(pikgpu_batch* batch);
 /* Device time of the last run in milliseconds (valid after a sync). */

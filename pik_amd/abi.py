@@ -297,6 +297,8 @@ def load():
         ctypes.POINTER(DcSectionC), ctypes.c_int, ctypes.c_char_p,
         ctypes.c_size_t]
     lib.pikgpu_batch_set_schedule.argtypes = [vp, ctypes.POINTER(ScheduleC)]
+    lib.pikgpu_batch_add_pass.argtypes = [vp, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_size_t]
     lib.pikgpu_batch_get_schedule.argtypes = [vp, ctypes.POINTER(ScheduleC)]
     lib.pikgpu_batch_phase_name.argtypes = [ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int)]
@@ -332,5 +334,5 @@ EXPORTED_SYMBOLS = [
     "pikgpu_decode_dc_groups", "pikgpu_parse_frame", "pikgpu_parsed_get",
     "pikgpu_parsed_pass_ac", "pikgpu_parsed_free", "pikgpu_decode_frame",
     "pikgpu_batch_set_schedule", "pikgpu_batch_get_schedule",
-    "pikgpu_batch_phase_name",
+    "pikgpu_batch_phase_name", "pikgpu_batch_add_pass",
 ]

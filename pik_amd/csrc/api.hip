@@ -2372,6 +2372,24 @@ int pikgpu_batch_upload(pikgpu_batch* b, const pikgpu_frame* frames, int n) {
   return PIKGPU_OK;
 }
 
+int pikgpu_batch_add_pass(pikgpu_batch* b, int i, const int16_t* const ac[3],
+                          size_t ac_stride) {
+  if (b == nullptr) return PIKGPU_EINVAL;
+  if (i < 0 || i >= b->n) return BFail(b, PIKGPU_EINVAL, "no such frame");
+  // a running graph may read the frame's pass buffer, which can move
+  BHIP_TRY(b, hipSetDevice(b->device));
+  BHIP_TRY(b, hipStreamSynchronize(b->main));
+  const int rc = pikgpu_frame_add_pass(b->items[i], ac, ac_stride);
+  if (rc != PIKGPU_OK) {
+    return BFail(b, rc, "frame " + std::to_string(i) + ": " +
+                            pikgpu_last_error(b->items[i]));
+  }
+  // the frame table (pass count, pass buffer) changed: new graph
+  DropGraph(b);
+  b->ran = false;
+  return PIKGPU_OK;
+}
+
 int pikgpu_batch_set_schedule(pikgpu_batch* b,
                               const pikgpu_batch_schedule* s) {
   if (b == nullptr) return PIKGPU_EINVAL;

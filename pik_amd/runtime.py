@@ -448,6 +448,22 @@ class BatchReconstructor:
         self._frames = frames
         self._keep = (arr, keep)
 
+    def add_pass(self, i, ac):
+        """Adds an AC pass (3 planes of int16, the frame's AC layout) to
+        frame i of the uploaded batch (pikgpu_batch_add_pass)."""
+        f = self._frames[i]
+        if len(ac) != 3:
+            raise ValueError("a pass has three planes")
+        want = (f.ysize_blocks, f.xsize_blocks * 64)
+        planes = [np.ascontiguousarray(a, np.int16) for a in ac]
+        for a in planes:
+            if a.shape != want:
+                raise ValueError("AC pass planes must be %s int16, got %s"
+                                 % (want, a.shape))
+        ptrs = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in planes])
+        self._check(self._lib.pikgpu_batch_add_pass(
+            self._b, int(i), ctypes.cast(ptrs, ctypes.c_void_p), want[1]))
+
     def run(self):
         self._check(self._lib.pikgpu_batch_run(self._b))
 

@@ -99,6 +99,49 @@ def test_branches(batch):
         assert np.array_equal(_bits(a), _bits(b)), i
 
 
+@pytest.mark.parametrize("mode", [abi.BATCH_QUEUE, abi.BATCH_BRANCHES])
+def test_multi_pass_frames(mode):
+    """Frames with added AC passes (pikgpu_batch_add_pass, 1 to 3 passes
+    per frame) decode as the single-frame path with the same passes does,
+    bit for bit; a pass added after a run takes effect on the next run."""
+    print("multi-pass batch, mode", mode, flush=True)
+    frames = [synth.make_frame(w, h, seed=300 + k, num_rects=6)
+              for k, (w, h) in enumerate([(264, 200), (520, 136), (72, 56),
+                                          (300, 300)])]
+    extra = [synth.make_extra_passes(fr, k % 3, seed=400 + k)
+             for k, fr in enumerate(frames)]
+    want = []
+    with Reconstructor(0) as r:
+        for fr, ex in zip(frames, extra):
+            r.upload(fr)
+            for p in ex:
+                r.add_pass(p)
+            r.run()
+            want.append(r.download())
+    with BatchReconstructor(0) as br:
+        br.set_schedule(mode=mode, order=0)
+        br.upload(frames)
+        for k, ex in enumerate(extra):
+            for p in ex:
+                br.add_pass(k, p)
+        br.run()
+        for k in range(len(frames)):
+            assert np.array_equal(_bits(br.download(k)), _bits(want[k])), k
+        # one more pass to frame 0 after the run
+        more = synth.make_extra_passes(frames[0], 1, seed=499)[0]
+        br.add_pass(0, more)
+        br.run()
+        got0 = br.download(0)
+        with pytest.raises(Exception):
+            br.add_pass(len(frames), more)
+    with Reconstructor(0) as r:
+        r.upload(frames[0])
+        for p in extra[0] + [more]:
+            r.add_pass(p)
+        r.run()
+        assert np.array_equal(_bits(got0), _bits(r.download()))
+
+
 def test_schedule_round_trip_and_refusals():
     with BatchReconstructor(0) as br:
         br.set_schedule(mode=abi.BATCH_QUEUE, order=2,
