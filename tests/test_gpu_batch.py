@@ -99,6 +99,37 @@ def test_branches(batch):
         assert np.array_equal(_bits(a), _bits(b)), i
 
 
+def test_bench_workload_frames():
+    """The frames of the bench's batch figure (pik_amd.autotune's
+    generator, seed 2: W, H ~ U[256, 2048], distance ~ U[0.5, 3.0]), 16
+    distinct ones each twice, under the tuned schedule: every output equals
+    the single-frame path's bit for bit, and the two largest equal the
+    oracle's."""
+    from oracle import port
+    from pik_amd.autotune import workload_frame
+    distinct = [workload_frame(i, 2) for i in range(16)]
+    frames = distinct + distinct
+    print("bench workload: %.1f MP" % (sum(f.xsize * f.ysize
+                                              for f in frames) / 1e6),
+          flush=True)
+    with BatchReconstructor(0) as br:
+        br.load_schedule()
+        br.upload(frames)
+        br.run()
+        outs = [br.download(i) for i in range(len(frames))]
+    with Reconstructor(0) as r:
+        for i, fr in enumerate(distinct):
+            want = r.reconstruct(fr)
+            assert np.array_equal(_bits(outs[i]), _bits(want)), i
+            assert np.array_equal(_bits(outs[i + 16]), _bits(want)), i
+    big = sorted(range(16), key=lambda i: -distinct[i].xsize *
+                 distinct[i].ysize)[:2]
+    for i in big:
+        print("oracle", distinct[i].xsize, distinct[i].ysize, flush=True)
+        want, _ = port.recon(distinct[i], stages=False)
+        assert np.array_equal(_bits(outs[i]), _bits(want)), i
+
+
 @pytest.mark.parametrize("mode", [abi.BATCH_QUEUE, abi.BATCH_BRANCHES])
 def test_multi_pass_frames(mode):
     """Frames with added AC passes (pikgpu_batch_add_pass, 1 to 3 passes
