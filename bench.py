@@ -78,6 +78,10 @@ BYTES_PER_PX = {
 }
 KERNEL_NAMES = {"epf_ar": "KEpf", "blocks_idct": "KBlocks",
                 "final_opsin": "KFinal"}
+# Their rows in the traffic profile (the image EPF is KEpf<false>; the DC
+# image's is KEpf<true>).
+PROFILE_ROWS = {"epf_ar": "KEpf<false>", "blocks_idct": "KBlocks",
+                "final_opsin": "KFinal"}
 # Whole-path algorithmic bytes per pixel: int16 AC (6) + f32 DC per block
 # (12/64) + i32 quant field per block (4/64) + strategy and sigma LUT id
 # bytes per block (2/64) + f32 linear out (12).
@@ -136,7 +140,8 @@ def traffic_from_profiles(kernel):
     path = os.path.join(ROOT, "profiles", "hbm_traffic_recon_4096.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(r"^\|\s*%s\s*\|(.*)\|\s*([0-9.eE+]+)\s*\|\s*$" % kernel)
+    pat = re.compile(r"^\|\s*%s\s*\|(.*)\|\s*([0-9.eE+]+)\s*\|\s*$"
+                     % re.escape(kernel))
     with open(path) as f:
         for line in f:
             m = pat.match(line.strip())
@@ -710,7 +715,7 @@ def main():
     roof = {"kernel": KERNEL_NAMES[dom], "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": traffic_from_profiles(KERNEL_NAMES[dom]),
+            "traffic": traffic_from_profiles(PROFILE_ROWS[dom]),
             "algorithmic_bytes": int(algo_bytes),
             "kernel_bytes": int(kernel_bytes),
             "avg_ms": round(fam_ms[dom], 4),
