@@ -7,7 +7,9 @@ Gaborish k750, EPF + AR and dot patches. The frame (quantized coefficients
 and side information, what the host holds after entropy decoding) is
 uploaded once; each timed step is one full device-resident reconstruction
 (DC dequant + DC EPF, LF predictions, dequant/CfL/HF prediction/IDCT, EPF +
-AR, dots, Gaborish, OpsinToLinear) into the linear Image3F in HBM.
+AR, dots, Gaborish, OpsinToLinear) into the linear Image3F in HBM; the K
+timed steps run back to back on the library's stream, the host
+synchronised before the first and after the last.
 
 Prints one JSON line. `roofline` covers the dominant kernel, timed with the
 library's HIP events on the stream it launches on; `cpu_baseline` times the
@@ -680,16 +682,26 @@ def main():
     for _ in range(a.warmup):
         R.run()
         R.sync()
-    fams = {}
+    # Timed: K reconstructions back to back on the library's stream (each
+    # runs whole, in stream order), the host synchronised before and after.
     R.sync()
     t0 = time.perf_counter()
+    for _ in range(a.steps):
+        R.run()
+    R.sync()
+    el = time.perf_counter() - t0
+    ms_step = el * 1000.0 / a.steps
+    # the last timed run's kernel-family times (HIP events on the library's
+    # stream): the roofline's launch duration
+    timed_ms = R.timings()
+    # Per-family device times averaged: the same K runs again, each
+    # synchronised so that its HIP events can be read (not timed).
+    fams = {}
     for _ in range(a.steps):
         R.run()
         R.sync()
         for k, v in R.timings().items():
             fams[k] = fams.get(k, 0.0) + v
-    el = time.perf_counter() - t0
-    ms_step = el * 1000.0 / a.steps
     fam_ms = {k: v / a.steps for k, v in fams.items()}
     value = mp * a.steps / el
     print("per-family ms/step: " + json.dumps(
@@ -711,16 +723,18 @@ def main():
     algo_bytes = PATH_BYTES_PER_PX * frame.xsize * frame.ysize
     kernel_bytes = BYTES_PER_PX[dom] * (pad_px if dom != "final_opsin"
                                         else frame.xsize * frame.ysize)
-    achieved = algo_bytes / (fam_ms[dom] * 1e-3) / 1e9
+    launch_ms = timed_ms.get(dom) or fam_ms[dom]
+    achieved = algo_bytes / (launch_ms * 1e-3) / 1e9
     roof = {"kernel": KERNEL_NAMES[dom], "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic_from_profiles(PROFILE_ROWS[dom]),
             "algorithmic_bytes": int(algo_bytes),
             "kernel_bytes": int(kernel_bytes),
-            "avg_ms": round(fam_ms[dom], 4),
+            "avg_ms": round(launch_ms, 4),
+            "avg_ms_over_steps": round(fam_ms[dom], 4),
             "limiter": "valu",
-            "valu": valu_roofline(KERNEL_NAMES[dom], fam_ms[dom])}
+            "valu": valu_roofline(KERNEL_NAMES[dom], launch_ms)}
 
     # the output, its 8-bit form, and the drop-in call with host copies
     gpu_out = R.download()
